@@ -1,0 +1,363 @@
+"""ctypes bindings of libfibrenet_amd.so (include/fibrenet.h).
+
+The library is the product: every step runs in its HIP kernels.  Loading
+fails loudly when the library has not been built; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import records as R
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfibrenet_amd.so")
+
+FN_OK = 0
+PREC = {"f64": 0, "f32": 1}
+ARG_BUFFER, ARG_INT, ARG_SETTINGS = 1, 2, 3
+KERNEL_NAMES = ["AdvancePosition", "ResetGrid", "ResetContactList", "FillGrid", "BuildContactList",
+                "ResolveContacts", "AdvanceVelocity", "ResetBeadEnergy", "ResolveSprings", "ResolveAngles"]
+
+# Every symbol include/fibrenet.h declares: (name, restype, argtypes).
+_c = ctypes
+_VP, _CP, _I, _LL = _c.c_void_p, _c.c_char_p, _c.c_int, _c.c_longlong
+SIGNATURES = [
+    ("fn_version", _CP, []),
+    ("fn_create", _I, [_I, _I, _c.POINTER(_VP)]),
+    ("fn_destroy", _I, [_VP]),
+    ("fn_last_error", _CP, [_VP]),
+    ("fn_set_log_callback", _I, [_VP, _VP, _VP]),
+    ("fn_register", _I, [_VP, _c.POINTER(_CP), _I]),
+    ("fn_buffer_create", _I, [_VP, _I, _I, _I, _VP, _c.POINTER(_I)]),
+    ("fn_buffer_write", _I, [_VP, _I, _VP, _I, _I]),
+    ("fn_buffer_read", _I, [_VP, _I, _VP]),
+    ("fn_buffer_info", _I, [_VP, _I, _c.POINTER(_I), _c.POINTER(_I), _c.POINTER(_I), _c.POINTER(_I)]),
+    ("fn_set_arg", _I, [_VP, _CP, _I, _I, _VP, _I]),
+    ("fn_set_range", _I, [_VP, _CP, _I]),
+    ("fn_launch", _I, [_VP, _CP]),
+    ("fn_finish", _I, [_VP]),
+    ("fn_run_steps", _I, [_VP, _I, _I, _I, _I]),
+    ("fn_export_cells", _I, [_VP, _VP]),
+    ("fn_export_pairs", _I, [_VP, _VP, _LL, _c.POINTER(_LL)]),
+    ("fn_export_breaks", _I, [_VP, _VP]),
+    ("fn_get_stats", _I, [_VP, _VP]),
+    ("fn_model_load", _I, [_CP, _c.POINTER(_VP)]),
+    ("fn_model_free", _I, [_VP]),
+    ("fn_model_error", _CP, []),
+    ("fn_model_counts", _I, [_VP] + [_c.POINTER(_I)] * 5),
+    ("fn_model_params", _I, [_VP] + [_c.POINTER(_I)] * 6),
+    ("fn_model_copy", _I, [_VP, _I, _VP]),
+    ("fn_sim_create", _I, [_I, _I, _CP, _c.POINTER(_VP)]),
+    ("fn_sim_destroy", _I, [_VP]),
+    ("fn_sim_engine", _VP, [_VP]),
+    ("fn_sim_buffer_id", _I, [_VP, _I]),
+    ("fn_sim_run_loop", _I, [_VP, _I, _I]),
+    ("fn_sim_minimize", _I, [_VP, _I]),
+    ("fn_gen_fibres", _I, [_c.c_ulonglong, _I, _I, _c.c_double, _c.c_double, _c.c_double, _I, _c.c_double,
+                           _c.c_double, _LL, _VP, _c.POINTER(_LL)]),
+]
+
+
+class FnStats(ctypes.Structure):
+    _fields_ = [("steps", _LL), ("rebuilds", _LL), ("list_entries", _LL), ("list_capacity", _LL),
+                ("hashed_beads", _I), ("max_cell_occupancy", _I), ("cells_over_capacity", _I),
+                ("max_list_length", _I), ("broken_springs", _I), ("img_aliases_pos", _I)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FibreNetError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("fibrenet error %d: %s" % (code, msg))
+        self.code = code
+
+
+_LIB = None
+
+
+def lib():
+    """The loaded library; raises if it has not been built (no fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libfibrenet_amd.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype, f.argtypes = res, args
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Model:
+    """Host records parsed from a FibreNet input file (fn_model_*)."""
+
+    def __init__(self, input_path):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.fn_model_load(os.fsencode(input_path), ctypes.byref(h))
+        if rc != FN_OK:
+            raise FibreNetError(rc, L.fn_model_error().decode())
+        try:
+            n = [ctypes.c_int() for _ in range(5)]
+            L.fn_model_counts(h, *[ctypes.byref(x) for x in n])
+            self.n_beads, self.n_springs, self.n_angles, self.n_pairs, self.n_cells = [x.value for x in n]
+            p = [ctypes.c_int() for _ in range(6)]
+            L.fn_model_params(h, *[ctypes.byref(x) for x in p])
+            (self.run_steps, self.dump_frequency, self.grid_update_freq, self.minimize_steps,
+             self.minimize_substeps, self.minimize_reset) = [x.value for x in p]
+            self.minimize_reset = bool(self.minimize_reset)
+            self.settings = np.zeros((), R.SETTINGS)
+            L.fn_model_copy(h, 0, _ptr(self.settings))
+            self.beads = self._copy(h, 1, R.BEAD, self.n_beads)
+            self.info = self._copy(h, 2, R.BEAD_INFO, self.n_beads)
+            self.springs = self._copy(h, 3, R.SPRING, self.n_springs)
+            self.angles = self._copy(h, 4, R.ANGLE, self.n_angles)
+            self.pairs = self._copy(h, 5, R.PAIR, self.n_pairs)
+            self.cell_nh = self._copy(h, 8, R.GRID_CELL_NH, self.n_cells)["n"]
+        finally:
+            L.fn_model_free(h)
+
+    @staticmethod
+    def _copy(h, kind, dtype, n):
+        a = np.zeros(n, dtype)
+        if n:
+            lib().fn_model_copy(h, kind, _ptr(a))
+        return a
+
+
+class Engine:
+    """One engine instance (fn_engine): buffers by id, kernels by name."""
+
+    def __init__(self, precision="f32", device=0, handle=None):
+        L = lib()
+        self._owned = handle is None
+        if handle is None:
+            h = ctypes.c_void_p()
+            rc = L.fn_create(int(device), PREC[precision], ctypes.byref(h))
+            if rc != FN_OK:
+                raise FibreNetError(rc, "fn_create failed (no HIP device?)")
+            handle = h.value
+        self.h = ctypes.c_void_p(handle)
+        self.precision = precision
+
+    def close(self):
+        if self._owned and self.h:
+            lib().fn_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != FN_OK:
+            raise FibreNetError(rc, lib().fn_last_error(self.h).decode())
+
+    # -- OpenclManager surface -----------------------------------------------------------
+    def register(self, names=KERNEL_NAMES):
+        arr = (ctypes.c_char_p * len(names))(*[n.encode() for n in names])
+        self._chk(lib().fn_register(self.h, arr, len(names)))
+
+    def create_buffer(self, kind, data=None, n=None, elem_size=None):
+        kind = R.KIND[kind] if isinstance(kind, str) else kind
+        es = elem_size or R.SIZE_OF_KIND[kind]
+        if data is not None:
+            data = np.ascontiguousarray(data)
+            n = len(data)
+        out = ctypes.c_int(-1)
+        self._chk(lib().fn_buffer_create(self.h, kind, es, int(n), _ptr(data) if data is not None else None,
+                                         ctypes.byref(out)))
+        return out.value
+
+    def buffer_info(self, bid):
+        v = [ctypes.c_int() for _ in range(4)]
+        self._chk(lib().fn_buffer_info(self.h, bid, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("size", "elements", "length", "struct_size"), [x.value for x in v]))
+
+    def write_buffer(self, bid, data):
+        info = self.buffer_info(bid)
+        data = np.ascontiguousarray(data)
+        self._chk(lib().fn_buffer_write(self.h, bid, _ptr(data), len(data), data.dtype.itemsize))
+        return info
+
+    def read_buffer(self, bid, dtype):
+        info = self.buffer_info(bid)
+        out = np.zeros(info["length"], dtype)
+        self._chk(lib().fn_buffer_read(self.h, bid, _ptr(out)))
+        return out[:info["elements"]]
+
+    def set_arg(self, kernel, idx, value):
+        if isinstance(value, np.ndarray) and value.dtype == R.SETTINGS:
+            v = np.ascontiguousarray(value.reshape(()))
+            self._chk(lib().fn_set_arg(self.h, kernel.encode(), idx, ARG_SETTINGS, _ptr(v), R.SETTINGS.itemsize))
+        elif isinstance(value, tuple) and value[0] == "buffer":
+            v = ctypes.c_int(value[1])
+            self._chk(lib().fn_set_arg(self.h, kernel.encode(), idx, ARG_BUFFER, ctypes.byref(v), 4))
+        else:
+            v = ctypes.c_int(int(value))
+            self._chk(lib().fn_set_arg(self.h, kernel.encode(), idx, ARG_INT, ctypes.byref(v), 4))
+
+    def set_range(self, kernel, n):
+        self._chk(lib().fn_set_range(self.h, kernel.encode(), int(n)))
+
+    def launch(self, kernel):
+        self._chk(lib().fn_launch(self.h, kernel.encode()))
+
+    def finish(self):
+        self._chk(lib().fn_finish(self.h))
+
+    def run_steps(self, n, grid_update_freq=8, springs=True, angles=True):
+        self._chk(lib().fn_run_steps(self.h, int(n), int(grid_update_freq), int(springs), int(angles)))
+
+    # -- exports ----------------------------------------------------------------------------
+    def export_cells(self, n_beads):
+        out = np.zeros(n_beads, np.int32)
+        self._chk(lib().fn_export_cells(self.h, _ptr(out)))
+        return out
+
+    def export_pairs(self):
+        n = ctypes.c_longlong()
+        self._chk(lib().fn_export_pairs(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros((max(n.value, 1), 2), np.int32)
+        self._chk(lib().fn_export_pairs(self.h, _ptr(out), n.value, ctypes.byref(n)))
+        return out[:n.value]
+
+    def export_breaks(self, n_springs):
+        out = np.zeros(max(n_springs, 1), np.int32)
+        self._chk(lib().fn_export_breaks(self.h, _ptr(out)))
+        return out[:n_springs]
+
+    def stats(self):
+        s = FnStats()
+        self._chk(lib().fn_get_stats(self.h, ctypes.byref(s)))
+        return s.as_dict()
+
+
+def bind_standard(eng, net):
+    """Uploads a network (generate.Network or Model) and binds every kernel
+    argument as SimulationManager::ReadKernelData does
+    (SimulationManager.cpp:1229-1304).  Returns the buffer ids."""
+    s = np.ascontiguousarray(net.settings.reshape(()))
+    C = int(s["cells_x"]) * int(s["cells_y"]) * int(s["cells_z"])
+    N = len(net.beads)
+    ids = {}
+    eng.register()
+    ids["head"] = eng.create_buffer("int", n=C)
+    ids["tail"] = eng.create_buffer("grid_cell", n=C)
+    cnh = np.zeros(len(net.cell_nh), R.GRID_CELL_NH)
+    cnh["n"] = net.cell_nh
+    ids["cellnh"] = eng.create_buffer("grid_cell_nh", cnh)
+    ids["pair"] = eng.create_buffer("pair", net.pairs)
+    ids["bead"] = eng.create_buffer("bead", net.beads)
+    ids["info"] = eng.create_buffer("bead_info", net.info)
+    ids["nh_head"] = eng.create_buffer("int", n=N)
+    ids["nh_tail"] = eng.create_buffer("particle_nh", n=N)
+    ids["coeff"] = eng.create_buffer("particle_nh", n=N)
+    if len(net.springs):
+        ids["spring"] = eng.create_buffer("spring", net.springs)
+    if len(net.angles):
+        ids["angle"] = eng.create_buffer("angle", net.angles)
+    B = lambda k: ("buffer", ids[k])
+    L = lambda k: eng.buffer_info(ids[k])["length"]
+    eng.set_arg("AdvancePosition", 0, B("bead")); eng.set_arg("AdvancePosition", 1, B("info"))
+    eng.set_arg("AdvancePosition", 2, s); eng.set_arg("AdvancePosition", 3, N)
+    eng.set_range("AdvancePosition", L("bead"))
+    if "spring" in ids:
+        eng.set_arg("ResolveSprings", 0, B("bead")); eng.set_arg("ResolveSprings", 1, B("spring"))
+        eng.set_arg("ResolveSprings", 2, len(net.springs)); eng.set_range("ResolveSprings", L("spring"))
+    if "angle" in ids:
+        eng.set_arg("ResolveAngles", 0, B("angle")); eng.set_arg("ResolveAngles", 1, B("spring"))
+        eng.set_arg("ResolveAngles", 2, B("bead")); eng.set_arg("ResolveAngles", 3, len(net.angles))
+        eng.set_range("ResolveAngles", L("angle"))
+    eng.set_arg("ResetGrid", 0, B("head")); eng.set_arg("ResetGrid", 1, C); eng.set_range("ResetGrid", L("head"))
+    eng.set_arg("ResetContactList", 0, B("nh_head")); eng.set_arg("ResetContactList", 1, N)
+    eng.set_range("ResetContactList", L("nh_head"))
+    for i, v in enumerate([B("bead"), B("head"), B("tail"), s, N]):
+        eng.set_arg("FillGrid", i, v)
+    eng.set_range("FillGrid", L("bead"))
+    for i, v in enumerate([B("head"), B("tail"), B("cellnh"), B("nh_head"), B("nh_tail"), B("bead"), B("info"),
+                           B("coeff"), B("pair"), s, C]):
+        eng.set_arg("BuildContactList", i, v)
+    eng.set_range("BuildContactList", L("head"))
+    for i, v in enumerate([B("nh_head"), B("nh_tail"), B("bead"), B("info"), B("coeff"), B("pair"), s, N]):
+        eng.set_arg("ResolveContacts", i, v)
+    eng.set_range("ResolveContacts", L("nh_head"))
+    for i, v in enumerate([B("bead"), B("info"), N, L("bead")]):
+        eng.set_arg("AdvanceVelocity", i, v)
+    eng.set_range("AdvanceVelocity", L("bead"))
+    eng.set_arg("ResetBeadEnergy", 0, B("bead")); eng.set_arg("ResetBeadEnergy", 1, N)
+    eng.set_range("ResetBeadEnergy", L("bead"))
+    return ids
+
+
+def run_loop_granular(eng, n_steps, grid_update_freq, springs=True, angles=True):
+    """SimulationManager::RunSimulationLoop through per-kernel launches with a
+    finish every step (SimulationManager.cpp:151-181)."""
+    cnt = 0
+    for _ in range(n_steps):
+        if cnt == grid_update_freq:
+            cnt = 0
+        eng.launch("AdvancePosition")
+        if springs:
+            eng.launch("ResolveSprings")
+        if angles:
+            eng.launch("ResolveAngles")
+        if cnt == 0:
+            for k in ("ResetGrid", "ResetContactList", "FillGrid", "BuildContactList"):
+                eng.launch(k)
+        eng.launch("ResolveContacts")
+        eng.launch("AdvanceVelocity")
+        eng.finish()
+        cnt += 1
+
+
+class Simulation:
+    """fn_sim: SimulationManager over the HIP engine, loaded from an input file."""
+
+    def __init__(self, input_path, precision="f32", device=0):
+        L = lib()
+        h = ctypes.c_void_p()
+        rc = L.fn_sim_create(int(device), PREC[precision], os.fsencode(input_path), ctypes.byref(h))
+        if rc != FN_OK:
+            raise FibreNetError(rc, L.fn_model_error().decode())
+        self.h = h
+        self.engine = Engine(precision, handle=L.fn_sim_engine(h))
+
+    def close(self):
+        if self.h:
+            lib().fn_sim_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def buffer_id(self, kind):
+        return lib().fn_sim_buffer_id(self.h, R.KIND[kind])
+
+    def run_loop(self, n, fast=True):
+        rc = lib().fn_sim_run_loop(self.h, int(n), int(bool(fast)))
+        if rc != FN_OK:
+            raise FibreNetError(rc, lib().fn_last_error(self.engine.h).decode())
+
+    def minimize(self, fast=True):
+        rc = lib().fn_sim_minimize(self.h, int(bool(fast)))
+        if rc != FN_OK:
+            raise FibreNetError(rc, lib().fn_last_error(self.engine.h).decode())
+
+    def beads(self):
+        return self.engine.read_buffer(self.buffer_id("bead"), R.BEAD)
+
+    def springs(self):
+        return self.engine.read_buffer(self.buffer_id("spring"), R.SPRING)

@@ -1,0 +1,1300 @@
+// engine.cpp - host side of the engine: buffers by id, kernels by name, the
+// fused hipGraph fast path, and the C ABI declared in include/fibrenet.h.
+//
+// Device state is structure-of-arrays (see kernels.h).  The reference's AoS
+// records are the wire format, converted on upload and read-back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "engine.h"
+#include "fibrenet.h"
+
+namespace {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCK(x)                                                                           \
+    do {                                                                                   \
+        hipError_t e_ = (x);                                                               \
+        if (e_ != hipSuccess) throw Error(FN_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+std::string fmt(const char *f, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, f);
+    vsnprintf(buf, sizeof buf, f, ap);
+    va_end(ap);
+    return buf;
+}
+
+template <typename X>
+struct DArr {
+    X *p = nullptr;
+    size_t n = 0;
+    DArr() = default;
+    DArr(const DArr &) = delete;
+    DArr &operator=(const DArr &) = delete;
+    ~DArr() { release(); }
+    void alloc(size_t k)
+    {
+        release();
+        if (k) HIPCK(hipMalloc(&p, k * sizeof(X)));
+        n = k;
+    }
+    void ensure(size_t k)
+    {
+        if (k > n) alloc(k);
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void put(const std::vector<X> &h, hipStream_t s)
+    {
+        ensure(h.size());
+        if (!h.empty()) HIPCK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(X), hipMemcpyHostToDevice, s));
+    }
+    void get(std::vector<X> &h, size_t k, hipStream_t s) const
+    {
+        h.resize(k);
+        if (k) HIPCK(hipMemcpyAsync(h.data(), p, k * sizeof(X), hipMemcpyDeviceToHost, s));
+    }
+};
+
+enum KernelId {
+    K_ADVANCE_POSITION, K_RESET_GRID, K_RESET_LIST, K_FILL_GRID, K_BUILD_LIST,
+    K_RESOLVE_CONTACTS, K_ADVANCE_VELOCITY, K_RESET_ENERGY, K_RESOLVE_SPRINGS, K_RESOLVE_ANGLES, K_COUNT
+};
+
+// Kernel names registered by SimulationManager::ReadKernelData (SimulationManager.cpp:1191-1200).
+const char *kKernelNames[K_COUNT] = {"AdvancePosition", "ResetGrid", "ResetContactList", "FillGrid",
+                                     "BuildContactList", "ResolveContacts", "AdvanceVelocity",
+                                     "ResetBeadEnergy", "ResolveSprings", "ResolveAngles"};
+
+// Argument contract of each kernel (kernel.cl signatures, bound at
+// SimulationManager.cpp:1236-1304): FN_ARG_* kind, and the buffer record kind.
+struct ArgSpec { int kind; int buf; };
+const std::vector<ArgSpec> kContract[K_COUNT] = {
+    /* AdvancePosition  */ {{FN_ARG_BUFFER, FN_BUF_BEAD}, {FN_ARG_BUFFER, FN_BUF_BEAD_INFO}, {FN_ARG_SETTINGS, 0},
+                            {FN_ARG_INT, 0}},
+    /* ResetGrid        */ {{FN_ARG_BUFFER, FN_BUF_INT}, {FN_ARG_INT, 0}},
+    /* ResetContactList */ {{FN_ARG_BUFFER, FN_BUF_INT}, {FN_ARG_INT, 0}},
+    /* FillGrid         */ {{FN_ARG_BUFFER, FN_BUF_BEAD}, {FN_ARG_BUFFER, FN_BUF_INT}, {FN_ARG_BUFFER, FN_BUF_GRID_CELL},
+                            {FN_ARG_SETTINGS, 0}, {FN_ARG_INT, 0}},
+    /* BuildContactList */ {{FN_ARG_BUFFER, FN_BUF_INT}, {FN_ARG_BUFFER, FN_BUF_GRID_CELL},
+                            {FN_ARG_BUFFER, FN_BUF_GRID_CELL_NH}, {FN_ARG_BUFFER, FN_BUF_INT},
+                            {FN_ARG_BUFFER, FN_BUF_PARTICLE_NH}, {FN_ARG_BUFFER, FN_BUF_BEAD},
+                            {FN_ARG_BUFFER, FN_BUF_BEAD_INFO}, {FN_ARG_BUFFER, FN_BUF_PARTICLE_NH},
+                            {FN_ARG_BUFFER, FN_BUF_PAIR}, {FN_ARG_SETTINGS, 0}, {FN_ARG_INT, 0}},
+    /* ResolveContacts  */ {{FN_ARG_BUFFER, FN_BUF_INT}, {FN_ARG_BUFFER, FN_BUF_PARTICLE_NH}, {FN_ARG_BUFFER, FN_BUF_BEAD},
+                            {FN_ARG_BUFFER, FN_BUF_BEAD_INFO}, {FN_ARG_BUFFER, FN_BUF_PARTICLE_NH},
+                            {FN_ARG_BUFFER, FN_BUF_PAIR}, {FN_ARG_SETTINGS, 0}, {FN_ARG_INT, 0}},
+    /* AdvanceVelocity  */ {{FN_ARG_BUFFER, FN_BUF_BEAD}, {FN_ARG_BUFFER, FN_BUF_BEAD_INFO}, {FN_ARG_INT, 0}, {FN_ARG_INT, 0}},
+    /* ResetBeadEnergy  */ {{FN_ARG_BUFFER, FN_BUF_BEAD}, {FN_ARG_INT, 0}},
+    /* ResolveSprings   */ {{FN_ARG_BUFFER, FN_BUF_BEAD}, {FN_ARG_BUFFER, FN_BUF_SPRING}, {FN_ARG_INT, 0}},
+    /* ResolveAngles    */ {{FN_ARG_BUFFER, FN_BUF_ANGLE}, {FN_ARG_BUFFER, FN_BUF_SPRING}, {FN_ARG_BUFFER, FN_BUF_BEAD},
+                            {FN_ARG_INT, 0}},
+};
+
+// Index of the argument holding the work-item count / settings of each kernel.
+const int kCountArg[K_COUNT] = {3, 1, 1, 4, 10, 7, 2, 1, 2, 3};
+const int kSettingsArg[K_COUNT] = {2, -1, -1, 3, 9, 6, -1, -1, -1, -1};
+
+int record_size(int kind)
+{
+    switch (kind) {
+    case FN_BUF_BEAD: return sizeof(fn_bead);
+    case FN_BUF_BEAD_INFO: return sizeof(fn_bead_info);
+    case FN_BUF_SPRING: return sizeof(fn_spring);
+    case FN_BUF_ANGLE: return sizeof(fn_angle);
+    case FN_BUF_PAIR: return sizeof(fn_pair);
+    case FN_BUF_INT: return sizeof(int);
+    case FN_BUF_GRID_CELL: return sizeof(fn_grid_cell);
+    case FN_BUF_GRID_CELL_NH: return sizeof(fn_grid_cell_nh);
+    case FN_BUF_PARTICLE_NH: return sizeof(fn_particle_nh);
+    default: return -1;
+    }
+}
+
+// OpenclManager.h:125 pads every buffer to a multiple of the work-group size (64).
+int pad64(int n) { return ((n + 63) / 64) * 64; }
+
+struct Buf { int kind, esize, elements, length; };
+
+struct Arg {
+    int kind = 0;
+    int ival = 0;
+    fn_settings s{};
+};
+
+struct KernelRec {
+    bool registered = false;
+    int range = 0;
+    std::map<int, Arg> args;
+};
+
+fnk::Box make_box(const fn_settings &s)
+{
+    fnk::Box b;
+    b.bc = s.bc_type;
+    b.cx = s.cells_x;
+    b.cy = s.cells_y;
+    b.cz = s.cells_z;
+    b.cxy = s.cells_xy;
+    b.types = s.types_num;
+    b.lx = s.box_size.x;
+    b.ly = s.box_size.y;
+    b.lz = s.box_size.z;
+    b.sx = s.grid_step.x;
+    b.sy = s.grid_step.y;
+    b.sz = s.grid_step.z;
+    b.wall_k = s.wall_contact_stiffness;
+    b.hbs = s.half_box_sq;
+    b.cutoff_sq = s.cutoff_sq;
+    b.damping = s.damping;
+    b.gx = s.gravity.x;
+    b.gy = s.gravity.y;
+    b.gz = s.gravity.z;
+    b.dt = s.dt;
+    return b;
+}
+
+}  // namespace
+
+// ----------------------------------------------------------------------------------------
+struct fn_engine {
+    virtual ~fn_engine() = default;
+    std::string err;
+    fn_log_fn log_fn = nullptr;
+    void *log_user = nullptr;
+    int device = 0;
+    int precision = 0;
+
+    void log(const std::string &m)
+    {
+        if (log_fn) log_fn(m.c_str(), log_user);
+        else fprintf(stderr, "fibrenet: %s\n", m.c_str());
+    }
+
+    virtual int create_buffer(int kind, int esize, int n, const void *host) = 0;
+    virtual void write_buffer(int id, const void *host, int n_padded, int esize) = 0;
+    virtual void read_buffer(int id, void *host) = 0;
+    virtual const Buf *buffer(int id) const = 0;
+    virtual void reg(const char *name) = 0;
+    virtual void set_arg(const char *kernel, int idx, int kind, const void *value, int size) = 0;
+    virtual void set_range(const char *kernel, int range) = 0;
+    virtual void launch(const char *kernel) = 0;
+    virtual void finish() = 0;
+    virtual void run_steps(int n, int guf, bool spr, bool ang) = 0;
+    virtual void export_cells(int *out) = 0;
+    virtual long long export_pairs(int *out, long long cap) = 0;
+    virtual void export_breaks(int *out) = 0;
+    virtual void stats(fn_stats *out) = 0;
+};
+
+namespace {
+
+template <typename T, typename P>
+class Engine final : public fn_engine {
+  public:
+    explicit Engine(int dev)
+    {
+        device = dev;
+        precision = std::is_same<T, double>::value ? FN_PREC_F64 : FN_PREC_F32;
+        HIPCK(hipSetDevice(dev));
+        HIPCK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+        step_.alloc(1);
+        stats_.alloc(fnk::kStatCount);
+        fnk::fill_int(step_.p, 0, 1, stream_);
+        fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
+        for (int k = 0; k < K_COUNT; ++k) kernels_[kKernelNames[k]] = KernelRec();
+        for (auto &r : role_) r = -1;
+    }
+    ~Engine() override
+    {
+        clear_graphs();
+        if (stream_) (void)hipStreamSynchronize(stream_);
+        if (stream_) (void)hipStreamDestroy(stream_);
+    }
+
+    // ---- buffers ------------------------------------------------------------------------
+    int create_buffer(int kind, int esize, int n, const void *host) override
+    {
+        const int rs = record_size(kind);
+        if (rs < 0) throw Error(FN_ERR_ARG, fmt("unknown buffer kind %d", kind));
+        if (esize != rs) throw Error(FN_ERR_MISMATCH, fmt("record size %d, kind %d expects %d", esize, kind, rs));
+        if (n <= 0) throw Error(FN_ERR_EMPTY, "buffer is empty");
+        const bool role_kind = kind == FN_BUF_BEAD || kind == FN_BUF_BEAD_INFO || kind == FN_BUF_SPRING ||
+                               kind == FN_BUF_ANGLE || kind == FN_BUF_PAIR || kind == FN_BUF_GRID_CELL_NH;
+        if (role_kind && role_[kind] >= 0)
+            throw Error(FN_ERR_UNSUPPORTED, fmt("a second buffer of record kind %d", kind));
+        Buf b{kind, esize, n, pad64(n)};
+        const int id = (int)bufs_.size();
+        bufs_.push_back(b);
+        if (role_kind) role_[kind] = id;
+        try {
+            upload(kind, n, host);
+        } catch (...) {
+            bufs_.pop_back();
+            if (role_kind) role_[kind] = -1;
+            throw;
+        }
+        return id;
+    }
+
+    void write_buffer(int id, const void *host, int n_padded, int esize) override
+    {
+        const Buf *b = buffer(id);
+        if (!b) throw Error(FN_ERR_ARG, fmt("no buffer %d", id));
+        if (!host || n_padded <= 0) throw Error(FN_ERR_EMPTY, "source buffer is empty");
+        if (b->length != n_padded || b->esize != esize)
+            throw Error(FN_ERR_MISMATCH, fmt("buffer %d holds %d x %d B, got %d x %d B", id, b->length, b->esize,
+                                             n_padded, esize));
+        upload(b->kind, b->elements, host);
+    }
+
+    void read_buffer(int id, void *host) override
+    {
+        const Buf *b = buffer(id);
+        if (!b) throw Error(FN_ERR_ARG, fmt("no buffer %d", id));
+        char *out = (char *)host;
+        memset(out, 0, (size_t)b->length * b->esize);
+        switch (b->kind) {
+        case FN_BUF_BEAD: download_beads((fn_bead *)out); break;
+        case FN_BUF_BEAD_INFO: memcpy(out, h_info_.data(), h_info_.size() * sizeof(fn_bead_info)); break;
+        case FN_BUF_SPRING: {
+            std::vector<int> st;
+            sp_status_.get(st, S_, stream_);
+            HIPCK(hipStreamSynchronize(stream_));
+            fn_spring *o = (fn_spring *)out;
+            for (int s = 0; s < S_; ++s) { o[s] = h_springs_[s]; o[s].status = st[s]; }
+            break;
+        }
+        case FN_BUF_ANGLE: {
+            std::vector<int> st;
+            an_status_.get(st, A_, stream_);
+            HIPCK(hipStreamSynchronize(stream_));
+            fn_angle *o = (fn_angle *)out;
+            for (int q = 0; q < A_; ++q) { o[q] = h_angles_[q]; o[q].status = st[q]; }
+            break;
+        }
+        case FN_BUF_PAIR: memcpy(out, h_pairs_.data(), h_pairs_.size() * sizeof(fn_pair)); break;
+        case FN_BUF_GRID_CELL_NH: memcpy(out, h_cellnh_.data(), h_cellnh_.size() * sizeof(fn_grid_cell_nh)); break;
+        default:
+            throw Error(FN_ERR_UNSUPPORTED, "the reference's grid/list scratch buffers are not materialised; "
+                                            "use fn_export_cells / fn_export_pairs");
+        }
+    }
+
+    const Buf *buffer(int id) const override
+    {
+        return (id >= 0 && id < (int)bufs_.size()) ? &bufs_[id] : nullptr;
+    }
+
+    // ---- kernels --------------------------------------------------------------------------
+    void reg(const char *name) override
+    {
+        auto it = kernels_.find(name ? name : "");
+        if (it == kernels_.end()) throw Error(FN_ERR_ARG, fmt("unable to create kernel: %s", name ? name : "(null)"));
+        it->second.registered = true;
+    }
+
+    KernelRec &kernel(const char *name)
+    {
+        auto it = kernels_.find(name ? name : "");
+        if (it == kernels_.end() || !it->second.registered)
+            throw Error(FN_ERR_ARG, fmt("kernel not registered: %s", name ? name : "(null)"));
+        return it->second;
+    }
+
+    static int kernel_id(const char *name)
+    {
+        for (int k = 0; k < K_COUNT; ++k)
+            if (!strcmp(name, kKernelNames[k])) return k;
+        return -1;
+    }
+
+    void set_arg(const char *name, int idx, int kind, const void *value, int size) override
+    {
+        KernelRec &kr = kernel(name);
+        const int kid = kernel_id(name);
+        const auto &c = kContract[kid];
+        if (idx < 0 || idx >= (int)c.size()) throw Error(FN_ERR_ARG, fmt("%s has no argument %d", name, idx));
+        if (c[idx].kind != kind) throw Error(FN_ERR_ARG, fmt("%s argument %d: wrong argument kind", name, idx));
+        if (!value) throw Error(FN_ERR_ARG, "null argument value");
+        Arg a;
+        a.kind = kind;
+        if (kind == FN_ARG_SETTINGS) {
+            if (size != (int)sizeof(fn_settings)) throw Error(FN_ERR_MISMATCH, "settings size");
+            memcpy(&a.s, value, sizeof(fn_settings));
+        } else {
+            if (size != (int)sizeof(int)) throw Error(FN_ERR_MISMATCH, "argument size");
+            a.ival = *(const int *)value;
+            if (kind == FN_ARG_BUFFER) {
+                const Buf *b = buffer(a.ival);
+                if (!b || b->kind != c[idx].buf)
+                    throw Error(FN_ERR_ARG, fmt("%s argument %d: buffer %d is not of record kind %d", name, idx,
+                                                a.ival, c[idx].buf));
+            }
+        }
+        kr.args[idx] = a;
+        clear_graphs();
+    }
+
+    void set_range(const char *name, int range) override { kernel(name).range = range; }
+
+    // Effective count of a launch: the bound count clipped to the work range.
+    int active(int kid, const KernelRec &kr)
+    {
+        const auto &c = kContract[kid];
+        for (int i = 0; i < (int)c.size(); ++i)
+            if (!kr.args.count(i)) throw Error(FN_ERR_ARG, fmt("%s: argument %d not set", kKernelNames[kid], i));
+        if (kr.range <= 0) throw Error(FN_ERR_ARG, fmt("%s: incorrect kernel work range", kKernelNames[kid]));
+        return std::min(kr.args.at(kCountArg[kid]).ival, kr.range);
+    }
+
+    const fn_settings &settings_of(int kid)
+    {
+        return kernels_.at(kKernelNames[kid]).args.at(kSettingsArg[kid]).s;
+    }
+
+    void launch(const char *name) override
+    {
+        KernelRec &kr = kernel(name);
+        const int kid = kernel_id(name);
+        const int n = active(kid, kr);
+        prepare();
+        switch (kid) {
+        case K_ADVANCE_POSITION: {
+            auto a = args_granular();
+            a.drift = make_box(kr.args.at(2).s);
+            check_alias(a.drift.bc);
+            a = args_granular();
+            a.drift = make_box(kr.args.at(2).s);
+            fnk::Launch<T, P>::advance_position(a, std::min(n, N_), stream_);
+            break;
+        }
+        case K_RESOLVE_SPRINGS: {
+            auto a = args_granular();
+            a.n_springs = std::min(n, S_);
+            if (S_ > 0 && a.n_springs > 0) fnk::Launch<T, P>::resolve_springs(a, stream_);
+            break;
+        }
+        case K_RESOLVE_ANGLES: {
+            auto a = args_granular();
+            a.n_angles = std::min(n, A_);
+            if (A_ > 0 && a.n_angles > 0) fnk::Launch<T, P>::resolve_angles(a, stream_);
+            break;
+        }
+        case K_RESET_GRID: grid_valid_ = false; break;
+        case K_RESET_LIST:
+            ensure_grid(nullptr);
+            fnk::fill_int(nb_start_.p, 0, N_ + 1, stream_);
+            break;
+        case K_FILL_GRID:
+            if (n != N_) throw Error(FN_ERR_UNSUPPORTED, "FillGrid over a partial bead range");
+            hash_and_sort(make_box(kr.args.at(3).s));
+            break;
+        case K_BUILD_LIST: {
+            const fn_settings &s = kr.args.at(9).s;
+            if (n != s.cells_x * s.cells_y * s.cells_z)
+                throw Error(FN_ERR_UNSUPPORTED, "BuildContactList over a partial grid");
+            if (!grid_valid_) throw Error(FN_ERR_ARG, "BuildContactList before FillGrid");
+            build_list(make_box(s), true);
+            break;
+        }
+        case K_RESOLVE_CONTACTS: {
+            if (n != N_) throw Error(FN_ERR_UNSUPPORTED, "ResolveContacts over a partial bead range");
+            ensure_grid(nullptr);
+            auto a = args_granular();
+            a.force = make_box(kr.args.at(6).s);
+            fnk::Launch<T, P>::resolve_contacts(a, stream_);
+            break;
+        }
+        case K_ADVANCE_VELOCITY: {
+            auto a = args_granular();
+            fnk::Launch<T, P>::advance_velocity(a, std::min(n, N_), stream_);
+            ++steps_;
+            break;
+        }
+        case K_RESET_ENERGY: {
+            auto a = args_granular();
+            fnk::Launch<T, P>::reset_energy(a, std::min(n, N_), stream_);
+            break;
+        }
+        }
+        HIPCK(hipGetLastError());
+    }
+
+    void finish() override { HIPCK(hipStreamSynchronize(stream_)); }
+
+    // ---- fast path --------------------------------------------------------------------------
+    void run_steps(int n, int guf, bool spr, bool ang) override
+    {
+        if (n <= 0) throw Error(FN_ERR_ARG, "loop size should be a positive number");
+        if (guf <= 0) throw Error(FN_ERR_ARG, "grid_update_freq should be positive");
+        for (int kid : {K_ADVANCE_POSITION, K_FILL_GRID, K_BUILD_LIST, K_RESOLVE_CONTACTS, K_ADVANCE_VELOCITY}) {
+            const KernelRec &kr = kernel(kKernelNames[kid]);
+            const int m = active(kid, kr);
+            const int want = kid == K_BUILD_LIST ? C_of(kr.args.at(9).s) : N_;
+            if (m != want) throw Error(FN_ERR_UNSUPPORTED, "fast path needs full-range kernels");
+        }
+        if (spr) {
+            const KernelRec &kr = kernel("ResolveSprings");
+            n_spr_ = std::min(active(K_RESOLVE_SPRINGS, kr), S_);
+        }
+        if (ang) {
+            const KernelRec &kr = kernel("ResolveAngles");
+            n_ang_ = std::min(active(K_RESOLVE_ANGLES, kr), A_);
+        }
+        spr = spr && S_ > 0 && n_spr_ > 0;
+        ang = ang && A_ > 0 && n_ang_ > 0;
+        prepare();
+        check_alias(settings_of(K_ADVANCE_POSITION).bc_type);
+        ensure_grid(&settings_of(K_FILL_GRID));
+        ensure_grid(&settings_of(K_BUILD_LIST));
+        // capacity of the contact list: grow before the run if the last rebuild came close
+        if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
+        fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
+        const int G = chunk_len(guf);
+        int done = 0;
+        while (done < n) {
+            const int len = std::min(G, n - done);
+            run_chunk(len, guf, spr, ang, done == 0, done + len == n);
+            done += len;
+        }
+        steps_ += n;
+        rebuilds_ += (n + guf - 1) / guf;
+        HIPCK(hipStreamSynchronize(stream_));
+        check_capacity();
+    }
+
+    // ---- exports --------------------------------------------------------------------------------
+    void export_cells(int *out) override
+    {
+        if (!grid_valid_) throw Error(FN_ERR_ARG, "no grid has been built");
+        HIPCK(hipMemcpyAsync(out, key_.p, sizeof(int) * N_, hipMemcpyDeviceToHost, stream_));
+        HIPCK(hipStreamSynchronize(stream_));
+    }
+
+    long long export_pairs(int *out, long long cap) override
+    {
+        std::vector<int> st, lst;
+        nb_start_.get(st, N_ + 1, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+        lst.resize(st[N_]);
+        if (st[N_]) HIPCK(hipMemcpyAsync(lst.data(), nb_list_.p, sizeof(int) * st[N_], hipMemcpyDeviceToHost, stream_));
+        HIPCK(hipStreamSynchronize(stream_));
+        std::vector<long long> keys;
+        for (int i = 0; i < N_; ++i)
+            for (int e = st[i]; e < st[i + 1]; ++e) {
+                const int j = lst[e] & fnk::kIdxMask;
+                if (i < j) keys.push_back((long long)i * N_ + j);
+            }
+        std::sort(keys.begin(), keys.end());
+        if (out) {
+            const long long m = std::min<long long>(cap, (long long)keys.size());
+            for (long long k = 0; k < m; ++k) {
+                out[2 * k] = (int)(keys[k] / N_);
+                out[2 * k + 1] = (int)(keys[k] % N_);
+            }
+        }
+        return (long long)keys.size();
+    }
+
+    void export_breaks(int *out) override
+    {
+        if (S_) HIPCK(hipMemcpyAsync(out, sp_break_.p, sizeof(int) * S_, hipMemcpyDeviceToHost, stream_));
+        HIPCK(hipStreamSynchronize(stream_));
+    }
+
+    void stats(fn_stats *o) override
+    {
+        int st[fnk::kStatCount];
+        HIPCK(hipMemcpyAsync(st, stats_.p, sizeof st, hipMemcpyDeviceToHost, stream_));
+        int tot = 0, hashed = 0;
+        if (nb_start_.p) HIPCK(hipMemcpyAsync(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        if (cell_start_.p) HIPCK(hipMemcpyAsync(&hashed, cell_start_.p + C_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        std::vector<int> ss;
+        sp_status_.get(ss, S_, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+        memset(o, 0, sizeof *o);
+        o->steps = steps_;
+        o->rebuilds = rebuilds_;
+        o->list_entries = tot;
+        o->list_capacity = cap_;
+        o->hashed_beads = hashed;
+        o->max_cell_occupancy = st[fnk::kStatMaxCell];
+        o->cells_over_capacity = st[fnk::kStatCellsOver];
+        o->max_list_length = st[fnk::kStatMaxList];
+        o->broken_springs = (int)std::count(ss.begin(), ss.end(), 0);
+        o->img_aliases_pos = alias_ ? 1 : 0;
+    }
+
+  private:
+    static int C_of(const fn_settings &s) { return s.cells_x * s.cells_y * s.cells_z; }
+
+    // ---- upload / download -------------------------------------------------------------------
+    template <typename X>
+    static void put3(DArr<X> *d, const std::vector<X> &x, const std::vector<X> &y, const std::vector<X> &z,
+                     hipStream_t s)
+    {
+        d[0].put(x, s);
+        d[1].put(y, s);
+        d[2].put(z, s);
+    }
+
+    void alloc_beads(int n)
+    {
+        clear_graphs();
+        N_ = n;
+        for (int b = 0; b < 2; ++b)
+            for (int c = 0; c < 3; ++c) {
+                pos_[b][c].alloc(n);
+                frc_[b][c].alloc(n);
+                img_[b][c].release();
+            }
+        for (int c = 0; c < 3; ++c) {
+            vel_[c].alloc(n);
+            vold_[c].alloc(n);
+        }
+        lam_.alloc(n); hlf_.alloc(n); hlf2_.alloc(n); rad_.alloc(n);
+        type_.alloc(n); tag_.alloc(n); loc_.alloc(n);
+        rad_d_.alloc(n); skin_d_.alloc(n);
+        alias_ = false;
+        csr_dirty_ = true;
+        grid_valid_ = false;
+    }
+
+    void upload(int kind, int n, const void *host)
+    {
+        HIPCK(hipSetDevice(device));
+        switch (kind) {
+        case FN_BUF_BEAD: {
+            std::vector<fn_bead> z;
+            const fn_bead *b = (const fn_bead *)host;
+            if (!b) { z.assign(n, fn_bead{}); b = z.data(); }
+            if (n != N_) alloc_beads(n);
+            upload_beads(b);
+            break;
+        }
+        case FN_BUF_BEAD_INFO: {
+            if (role_[FN_BUF_BEAD] < 0 || n != N_)
+                throw Error(FN_ERR_ARG, "BeadInfo buffer must follow the Bead buffer and match its length");
+            h_info_.assign(n, fn_bead_info{});
+            if (host) memcpy(h_info_.data(), host, sizeof(fn_bead_info) * n);
+            upload_info();
+            break;
+        }
+        case FN_BUF_SPRING:
+            h_springs_.assign(n, fn_spring{});
+            if (host) memcpy(h_springs_.data(), host, sizeof(fn_spring) * n);
+            S_ = n;
+            upload_springs();
+            break;
+        case FN_BUF_ANGLE:
+            h_angles_.assign(n, fn_angle{});
+            if (host) memcpy(h_angles_.data(), host, sizeof(fn_angle) * n);
+            A_ = n;
+            upload_angles();
+            break;
+        case FN_BUF_PAIR: {
+            h_pairs_.assign(n, fn_pair{});
+            if (host) memcpy(h_pairs_.data(), host, sizeof(fn_pair) * n);
+            std::vector<fnk::PairC> pc(n);
+            for (int k = 0; k < n; ++k) {
+                const fn_pair &p = h_pairs_[k];
+                pc[k] = fnk::PairC{p.type, p.c1, p.c2, p.c3, p.c4, p.c5, p.c6};
+            }
+            pairs_.put(pc, stream_);
+            clear_graphs();
+            break;
+        }
+        case FN_BUF_GRID_CELL_NH:
+            h_cellnh_.assign(n, fn_grid_cell_nh{});
+            if (host) memcpy(h_cellnh_.data(), host, sizeof(fn_grid_cell_nh) * n);
+            cellnh_checked_ = false;
+            break;
+        default: break;   // reference scratch (grid heads/tails, list heads/tails): not materialised
+        }
+        HIPCK(hipStreamSynchronize(stream_));
+    }
+
+    void upload_beads(const fn_bead *b)
+    {
+        const int n = N_;
+        std::vector<P> x(n), y(n), z(n), ix(n), iy(n), iz(n);
+        std::vector<T> a(n), bb(n), c(n);
+        bool eq = true;
+        for (int i = 0; i < n; ++i) {
+            x[i] = (P)b[i].position.x; y[i] = (P)b[i].position.y; z[i] = (P)b[i].position.z;
+            ix[i] = (P)b[i].image_position.x; iy[i] = (P)b[i].image_position.y; iz[i] = (P)b[i].image_position.z;
+            eq = eq && x[i] == ix[i] && y[i] == iy[i] && z[i] == iz[i];
+        }
+        pc_ = 0;
+        fc_ = 0;
+        put3(pos_[0], x, y, z, stream_);
+        pos_eq_img_ = eq;
+        if (alias_ && !eq) unalias();
+        if (!alias_) {
+            for (int k = 0; k < 2; ++k)
+                for (int q = 0; q < 3; ++q) img_[k][q].ensure(n);
+            put3(img_[0], ix, iy, iz, stream_);
+        }
+        auto field = [&](DArr<T> *d, const fn_vector3 fn_bead::*m) {
+            for (int i = 0; i < n; ++i) { a[i] = (T)(b[i].*m).x; bb[i] = (T)(b[i].*m).y; c[i] = (T)(b[i].*m).z; }
+            put3(d, a, bb, c, stream_);
+            HIPCK(hipStreamSynchronize(stream_));
+        };
+        field(vel_, &fn_bead::velocity);
+        field(vold_, &fn_bead::velocity_old);
+        field(frc_[0], &fn_bead::force);
+        field(frc_[1], &fn_bead::force_old);
+        HIPCK(hipStreamSynchronize(stream_));
+    }
+
+    void download_beads(fn_bead *o)
+    {
+        const int n = N_;
+        std::vector<P> x, y, z, ix, iy, iz;
+        auto get3p = [&](DArr<P> *d, std::vector<P> &a, std::vector<P> &b, std::vector<P> &c) {
+            d[0].get(a, n, stream_); d[1].get(b, n, stream_); d[2].get(c, n, stream_);
+        };
+        get3p(pos_[pc_], x, y, z);
+        get3p(alias_ ? pos_[pc_] : img_[pc_], ix, iy, iz);
+        std::vector<T> t[4][3];
+        DArr<T> *src[4] = {vel_, frc_[fc_], vold_, frc_[1 - fc_]};
+        for (int k = 0; k < 4; ++k)
+            for (int c = 0; c < 3; ++c) src[k][c].get(t[k][c], n, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+        for (int i = 0; i < n; ++i) {
+            fn_bead &b = o[i];
+            b.position = fn_vector3{(double)x[i], (double)y[i], (double)z[i], 0.0};
+            b.image_position = fn_vector3{(double)ix[i], (double)iy[i], (double)iz[i], 0.0};
+            fn_vector3 *dst[4] = {&b.velocity, &b.force, &b.velocity_old, &b.force_old};
+            for (int k = 0; k < 4; ++k)
+                *dst[k] = fn_vector3{(double)t[k][0][i], (double)t[k][1][i], (double)t[k][2][i], 0.0};
+        }
+    }
+
+    void upload_info()
+    {
+        const int n = N_;
+        std::vector<T> l(n), h(n), h2(n), r(n);
+        std::vector<int> ty(n), tg(n), lc(n);
+        std::vector<double> rd(n), sk(n);
+        for (int i = 0; i < n; ++i) {
+            const fn_bead_info &q = h_info_[i];
+            l[i] = (T)q.lambda_dt_mass_inv;
+            h[i] = (T)q.half_dt_mass_inv;
+            h2[i] = (T)q.half_dt2_mass_inv;
+            r[i] = (T)q.radius;
+            ty[i] = q.type;
+            tg[i] = q.tag;
+            lc[i] = q.loc_id;
+            rd[i] = q.radius;
+            sk[i] = q.skin_radius;
+        }
+        lam_.put(l, stream_); hlf_.put(h, stream_); hlf2_.put(h2, stream_); rad_.put(r, stream_);
+        type_.put(ty, stream_); tag_.put(tg, stream_); loc_.put(lc, stream_);
+        rad_d_.put(rd, stream_); skin_d_.put(sk, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+    }
+
+    void upload_springs()
+    {
+        const int n = S_;
+        std::vector<int> p1(n), p2(n), ty(n), st(n), br(n, -1);
+        std::vector<T> l0(n), k(n), c1(n), c2(n);
+        for (int s = 0; s < n; ++s) {
+            const fn_spring &q = h_springs_[s];
+            p1[s] = q.p1; p2[s] = q.p2; ty[s] = q.type; st[s] = q.status;
+            l0[s] = (T)q.rest_length; k[s] = (T)q.stiffness; c1[s] = (T)q.c1; c2[s] = (T)q.c2;
+        }
+        sp_p1_.put(p1, stream_); sp_p2_.put(p2, stream_); sp_type_.put(ty, stream_); sp_status_.put(st, stream_);
+        sp_break_.put(br, stream_);
+        sp_l0_.put(l0, stream_); sp_k_.put(k, stream_); sp_c1_.put(c1, stream_); sp_c2_.put(c2, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+        csr_dirty_ = true;
+        clear_graphs();
+    }
+
+    void upload_angles()
+    {
+        const int n = A_;
+        std::vector<int> b1(n), b2(n), b3(n), s1(n), s2(n), ty(n), st(n);
+        std::vector<T> th(n), k(n);
+        for (int q = 0; q < n; ++q) {
+            const fn_angle &a = h_angles_[q];
+            b1[q] = a.b1; b2[q] = a.b2; b3[q] = a.b3; s1[q] = a.s1; s2[q] = a.s2; ty[q] = a.type; st[q] = a.status;
+            th[q] = (T)a.angle; k[q] = (T)a.c1;
+        }
+        an_b1_.put(b1, stream_); an_b2_.put(b2, stream_); an_b3_.put(b3, stream_);
+        an_s1_.put(s1, stream_); an_s2_.put(s2, stream_); an_type_.put(ty, stream_); an_status_.put(st, stream_);
+        an_th0_.put(th, stream_); an_k_.put(k, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+        csr_dirty_ = true;
+        clear_graphs();
+    }
+
+    // Incidence lists bead -> springs and bead -> angles, ordered by bond
+    // index (then p1 before p2; b1, b3, b2) as the reference's sequential
+    // scatter adds them.
+    void build_csr()
+    {
+        const int n = N_;
+        std::vector<int> cnt(n + 1, 0);
+        for (const fn_spring &s : h_springs_) {
+            if (s.p1 < 0 || s.p1 >= n || s.p2 < 0 || s.p2 >= n) throw Error(FN_ERR_ARG, "spring bead index out of range");
+            cnt[s.p1]++; cnt[s.p2]++;
+        }
+        std::vector<int> st(n + 1, 0), ent(2 * (size_t)S_ + 1);
+        for (int i = 0; i < n; ++i) st[i + 1] = st[i] + cnt[i];
+        std::vector<int> cur(st.begin(), st.end() - 1);
+        for (int s = 0; s < S_; ++s) {
+            ent[cur[h_springs_[s].p1]++] = s << 1;
+            ent[cur[h_springs_[s].p2]++] = (s << 1) | 1;
+        }
+        sp_start_.put(st, stream_);
+        sp_ent_.put(ent, stream_);
+        std::fill(cnt.begin(), cnt.end(), 0);
+        for (const fn_angle &a : h_angles_) {
+            for (int b : {a.b1, a.b2, a.b3})
+                if (b < 0 || b >= n) throw Error(FN_ERR_ARG, "angle bead index out of range");
+            if (a.type == 1 && (a.s1 < 0 || a.s1 >= S_ || a.s2 < 0 || a.s2 >= S_))
+                throw Error(FN_ERR_ARG, "HR_FL angle spring index out of range");
+            cnt[a.b1]++; cnt[a.b2]++; cnt[a.b3]++;
+        }
+        std::vector<int> as(n + 1, 0), aent(3 * (size_t)A_ + 1);
+        for (int i = 0; i < n; ++i) as[i + 1] = as[i] + cnt[i];
+        std::vector<int> acur(as.begin(), as.end() - 1);
+        for (int q = 0; q < A_; ++q) {
+            aent[acur[h_angles_[q].b1]++] = (q << 2) | 0;
+            aent[acur[h_angles_[q].b3]++] = (q << 2) | 2;
+            aent[acur[h_angles_[q].b2]++] = (q << 2) | 1;
+        }
+        an_start_.put(as, stream_);
+        an_ent_.put(aent, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+        csr_dirty_ = false;
+        clear_graphs();
+    }
+
+    void prepare()
+    {
+        if (N_ <= 0 || role_[FN_BUF_BEAD_INFO] < 0) throw Error(FN_ERR_ARG, "bead and bead-info buffers are required");
+        if (role_[FN_BUF_PAIR] < 0) throw Error(FN_ERR_ARG, "pair table buffer is required");
+        if (csr_dirty_) build_csr();
+        if ((long long)N_ >= (1LL << fnk::kPidShift) || (int)h_pairs_.size() > (1 << (31 - fnk::kPidShift)))
+            throw Error(FN_ERR_UNSUPPORTED, "more than 2^26 beads or 32 type pairs");
+    }
+
+    // One array holds position and image when they are equal and the box is
+    // finite: both then receive the same increments forever (no periodic
+    // reset), so the reference's two fields stay bit-identical.
+    void check_alias(int bc)
+    {
+        const bool want = bc == 1 && pos_eq_img_;
+        if (want == alias_) return;
+        clear_graphs();
+        if (want) {
+            for (int k = 0; k < 2; ++k)
+                for (int q = 0; q < 3; ++q) img_[k][q].release();
+            alias_ = true;
+        } else {
+            unalias();
+        }
+    }
+
+    void unalias()
+    {
+        clear_graphs();
+        for (int k = 0; k < 2; ++k)
+            for (int q = 0; q < 3; ++q) {
+                img_[k][q].ensure(N_);
+                if (alias_) HIPCK(hipMemcpyAsync(img_[k][q].p, pos_[k][q].p, sizeof(P) * N_, hipMemcpyDeviceToDevice, stream_));
+            }
+        HIPCK(hipStreamSynchronize(stream_));
+        alias_ = false;
+        pos_eq_img_ = false;
+    }
+
+    fnk::Args<T, P> args_base()
+    {
+        fnk::Args<T, P> a{};
+        a.n = N_;
+        a.vx = vel_[0].p; a.vy = vel_[1].p; a.vz = vel_[2].p;
+        a.wx = vold_[0].p; a.wy = vold_[1].p; a.wz = vold_[2].p;
+        a.lam = lam_.p; a.hlf = hlf_.p; a.hlf2 = hlf2_.p;
+        a.type = type_.p; a.tag = tag_.p; a.loc = loc_.p; a.radius = rad_.p;
+        a.sp_start = sp_start_.p; a.sp_ent = sp_ent_.p; a.sp_p1 = sp_p1_.p; a.sp_p2 = sp_p2_.p;
+        a.sp_type = sp_type_.p; a.sp_status = sp_status_.p; a.sp_break = sp_break_.p;
+        a.sp_l0 = sp_l0_.p; a.sp_k = sp_k_.p; a.sp_c1 = sp_c1_.p; a.sp_c2 = sp_c2_.p;
+        a.n_springs = S_;
+        a.an_start = an_start_.p; a.an_ent = an_ent_.p; a.an_b1 = an_b1_.p; a.an_b2 = an_b2_.p; a.an_b3 = an_b3_.p;
+        a.an_s1 = an_s1_.p; a.an_s2 = an_s2_.p; a.an_type = an_type_.p; a.an_status = an_status_.p;
+        a.an_th0 = an_th0_.p; a.an_k = an_k_.p;
+        a.n_angles = A_;
+        a.nb_start = nb_start_.p; a.nb_list = nb_list_.p; a.pairs = pairs_.p;
+        a.step = step_.p;
+        return a;
+    }
+
+    void set_pos(fnk::Args<T, P> &a, int cur)
+    {
+        a.px = pos_[cur][0].p; a.py = pos_[cur][1].p; a.pz = pos_[cur][2].p;
+        DArr<P> *im = alias_ ? pos_[cur] : img_[cur];
+        a.ix = im[0].p; a.iy = im[1].p; a.iz = im[2].p;
+        a.qx = pos_[1 - cur][0].p; a.qy = pos_[1 - cur][1].p; a.qz = pos_[1 - cur][2].p;
+        DArr<P> *jm = alias_ ? pos_[1 - cur] : img_[1 - cur];
+        a.jx = jm[0].p; a.jy = jm[1].p; a.jz = jm[2].p;
+    }
+
+    fnk::Args<T, P> args_granular()
+    {
+        fnk::Args<T, P> a = args_base();
+        set_pos(a, pc_);
+        a.fx = frc_[fc_][0].p; a.fy = frc_[fc_][1].p; a.fz = frc_[fc_][2].p;
+        a.gx = frc_[1 - fc_][0].p; a.gy = frc_[1 - fc_][1].p; a.gz = frc_[1 - fc_][2].p;
+        const auto &kp = kernels_.at("AdvancePosition").args;
+        if (kp.count(2)) a.drift = make_box(kp.at(2).s);
+        const auto &kc = kernels_.at("ResolveContacts").args;
+        if (kc.count(6)) a.force = make_box(kc.at(6).s);
+        return a;
+    }
+
+    // ---- grid and contact list ---------------------------------------------------------------
+    void ensure_grid(const fn_settings *s)
+    {
+        nb_start_.ensure((size_t)N_ + 1);
+        ncnt_.ensure(N_);
+        key_.ensure(N_);
+        sorted_.ensure(N_);
+        if (!nb_list_.p) grow_list(std::max<long long>(32LL * N_, 1 << 20));
+        if (!nb_init_) {
+            fnk::fill_int(nb_start_.p, 0, N_ + 1, stream_);
+            nb_init_ = true;
+        }
+        if (s) {
+            const int C = C_of(*s);
+            if (C <= 0) throw Error(FN_ERR_ARG, "empty grid");
+            if (s->cells_x < 3 || s->cells_y < 3 || s->cells_z < 3)
+                throw Error(FN_ERR_UNSUPPORTED, "grid with fewer than three cells along an axis");
+            if ((long long)s->cells_x * s->cells_y != s->cells_xy) throw Error(FN_ERR_ARG, "cells_xy mismatch");
+            const int T = s->types_num;
+            if (T <= 0 || (long long)T * (T + 1) / 2 > (long long)h_pairs_.size())
+                throw Error(FN_ERR_ARG, "pair table smaller than types_num*(types_num+1)/2");
+            for (const fn_bead_info &q : h_info_)
+                if (q.type < 0 || q.type >= T) throw Error(FN_ERR_ARG, "bead type outside [0, types_num)");
+            if (C != C_) {
+                clear_graphs();
+                C_ = C;
+                cell_count_.alloc((size_t)C + 1);
+                cell_start_.alloc((size_t)C + 1);
+                cursor_.alloc((size_t)C + 1);
+            }
+            const int nb = std::max(fnk::scan_blocks_needed(std::max(C_, N_)), 1) + 1;
+            if ((int)scan_tmp_.n < nb) {
+                clear_graphs();
+                scan_tmp_.alloc(nb);
+            }
+            check_cell_nh(*s);
+        }
+    }
+
+    // The engine enumerates the standard neighbour stencil; a caller-supplied
+    // GridCellNh table must be the one SimulationManager.cpp:2193-2301 builds.
+    void check_cell_nh(const fn_settings &s)
+    {
+        if (cellnh_checked_ || h_cellnh_.empty()) return;
+        const int cx = s.cells_x, cy = s.cells_y, cz = s.cells_z;
+        if ((int)h_cellnh_.size() != cx * cy * cz) throw Error(FN_ERR_ARG, "GridCellNh table size differs from the grid");
+        auto nb = [&](int x, int y, int z) {
+            if (s.bc_type == 0) {
+                x = x < 0 ? cx - 1 : (x >= cx ? 0 : x);
+                y = y < 0 ? cy - 1 : (y >= cy ? 0 : y);
+                z = z < 0 ? cz - 1 : (z >= cz ? 0 : z);
+            } else if (s.bc_type == 1) {
+                if (x < 0 || x >= cx || y < 0 || y >= cy || z < 0 || z >= cz) return -1;
+            } else {
+                return -1;
+            }
+            return x + y * cx + z * cx * cy;
+        };
+        for (int z = 0; z < cz; ++z)
+            for (int y = 0; y < cy; ++y)
+                for (int x = 0; x < cx; ++x) {
+                    const int *t = h_cellnh_[x + y * cx + z * cx * cy].n;
+                    int k = 0;
+                    bool ok = true;
+                    for (int dx = -1; dx < 2; ++dx)
+                        for (int dy = -1; dy < 2; ++dy) ok = ok && t[k++] == nb(x + dx, y + dy, z - 1);
+                    for (int dy = -1; dy < 2; ++dy) ok = ok && t[k++] == nb(x + 1, y + dy, z);
+                    ok = ok && t[k++] == nb(x, y - 1, z);
+                    if (!ok) throw Error(FN_ERR_UNSUPPORTED, "GridCellNh table is not the standard half shell");
+                }
+        cellnh_checked_ = true;
+    }
+
+    void grow_list(long long want)
+    {
+        if (want > (1LL << 31) - 1) throw Error(FN_ERR_CAPACITY, "contact list larger than 2^31 entries");
+        if (want <= cap_ && nb_list_.p) return;
+        clear_graphs();
+        nb_list_.alloc((size_t)want);
+        cap_ = (int)want;
+    }
+
+    void hash_and_sort(const fnk::Box &b)
+    {
+        const fn_settings &s = settings_of(K_FILL_GRID);
+        ensure_grid(&s);
+        fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
+        fnk::Launch<T, P>::hash(pos_[pc_][0].p, pos_[pc_][1].p, pos_[pc_][2].p, N_, b, key_.p, cell_count_.p, stream_);
+        fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, stream_);
+        fnk::copy_int(cell_start_.p, cursor_.p, C_, stream_);
+        fnk::scatter(key_.p, N_, cursor_.p, sorted_.p, stream_);
+        fnk::sort_cells(cell_start_.p, sorted_.p, C_, stats_.p, stream_);
+        grid_valid_ = true;
+    }
+
+    // exact: count, then size the list on the host before filling (granular
+    // path); otherwise fill into the reserved capacity (fast path).
+    void build_list(const fnk::Box &b, bool exact)
+    {
+        ensure_grid(&settings_of(K_BUILD_LIST));
+        fnk::fill_int(ncnt_.p, 0, N_, stream_);
+        fnk::Launch<T, P>::pairs(false, pos_[pc_][0].p, pos_[pc_][1].p, pos_[pc_][2].p, key_.p, sorted_.p,
+                                 cell_start_.p, C_, N_, type_.p, tag_.p, loc_.p, rad_d_.p, skin_d_.p, pairs_.p, b,
+                                 ncnt_.p, nb_start_.p, nb_list_.p, cap_, stats_.p, stream_);
+        fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, stream_);
+        if (exact) {
+            int tot = 0;
+            HIPCK(hipMemcpyAsync(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+            HIPCK(hipStreamSynchronize(stream_));
+            if (tot > cap_) grow_list((long long)tot * 2);
+            last_total_ = tot;
+        }
+        fnk::Launch<T, P>::pairs(true, pos_[pc_][0].p, pos_[pc_][1].p, pos_[pc_][2].p, key_.p, sorted_.p,
+                                 cell_start_.p, C_, N_, type_.p, tag_.p, loc_.p, rad_d_.p, skin_d_.p, pairs_.p, b,
+                                 ncnt_.p, nb_start_.p, nb_list_.p, cap_, stats_.p, stream_);
+    }
+
+    void check_capacity()
+    {
+        int tot = 0;
+        HIPCK(hipMemcpy(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost));
+        last_total_ = std::max(last_total_, tot);
+        if (tot > cap_) {
+            grow_list((long long)tot * 2);
+            throw Error(FN_ERR_CAPACITY, fmt("contact list needed %d entries, %d were reserved; the run is invalid "
+                                             "and the capacity has been raised", tot, cap_));
+        }
+    }
+
+    // ---- fused chunks -----------------------------------------------------------------------------
+    int chunk_len(int guf) const
+    {
+        const char *e = getenv("FIBRENET_CHUNK");
+        int target = e ? atoi(e) : 128;
+        if (target < 1) target = 1;
+        return guf * std::max(1, target / guf);
+    }
+
+    void enqueue_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
+    {
+        const fnk::Box bd = make_box(settings_of(K_ADVANCE_POSITION));
+        const fnk::Box bf = make_box(settings_of(K_FILL_GRID));
+        const fnk::Box bl = make_box(settings_of(K_BUILD_LIST));
+        const fnk::Box bc = make_box(settings_of(K_RESOLVE_CONTACTS));
+        int pc = pc_, fc = fc_;
+        if (first) {
+            fnk::Args<T, P> a = args_base();
+            a.drift = bd;
+            set_pos(a, pc);
+            a.fx = frc_[fc][0].p; a.fy = frc_[fc][1].p; a.fz = frc_[fc][2].p;
+            fnk::Launch<T, P>::drift_first(a, stream_);
+            pc ^= 1;
+        }
+        for (int k = 0; k < len; ++k) {
+            if (k % guf == 0) {
+                fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
+                fnk::Launch<T, P>::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, bf, key_.p, cell_count_.p,
+                                        stream_);
+                fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, stream_);
+                fnk::copy_int(cell_start_.p, cursor_.p, C_, stream_);
+                fnk::scatter(key_.p, N_, cursor_.p, sorted_.p, stream_);
+                fnk::sort_cells(cell_start_.p, sorted_.p, C_, stats_.p, stream_);
+                fnk::fill_int(ncnt_.p, 0, N_, stream_);
+                for (int fill = 0; fill < 2; ++fill) {
+                    fnk::Launch<T, P>::pairs(fill, pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, key_.p, sorted_.p,
+                                             cell_start_.p, C_, N_, type_.p, tag_.p, loc_.p, rad_d_.p, skin_d_.p,
+                                             pairs_.p, bl, ncnt_.p, nb_start_.p, nb_list_.p, cap_, stats_.p, stream_);
+                    if (!fill) fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, stream_);
+                }
+            }
+            const bool is_last = last && k == len - 1;
+            fnk::Args<T, P> a = args_base();
+            a.drift = bd;
+            a.force = bc;
+            a.n_springs = n_spr_;
+            a.n_angles = n_ang_;
+            a.step_off = k;
+            set_pos(a, pc);
+            a.fx = frc_[fc][0].p; a.fy = frc_[fc][1].p; a.fz = frc_[fc][2].p;
+            a.gx = frc_[1 - fc][0].p; a.gy = frc_[1 - fc][1].p; a.gz = frc_[1 - fc][2].p;
+            fnk::Launch<T, P>::fused(a, spr, ang, !is_last, is_last, stream_);
+            fc ^= 1;
+            if (!is_last) pc ^= 1;
+        }
+        fnk::add_int(step_.p, len, stream_);
+    }
+
+    void run_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
+    {
+        const auto key = std::make_tuple(len, guf, (int)spr, (int)ang, (int)first, (int)last, pc_, fc_, n_spr_, n_ang_);
+        const bool use_graph = !getenv("FIBRENET_NO_GRAPH");
+        if (use_graph) {
+            auto it = graphs_.find(key);
+            if (it == graphs_.end()) {
+                hipGraph_t g;
+                HIPCK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+                enqueue_chunk(len, guf, spr, ang, first, last);
+                HIPCK(hipStreamEndCapture(stream_, &g));
+                hipGraphExec_t ex;
+                HIPCK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+                HIPCK(hipGraphDestroy(g));
+                it = graphs_.emplace(key, ex).first;
+            }
+            HIPCK(hipGraphLaunch(it->second, stream_));
+        } else {
+            enqueue_chunk(len, guf, spr, ang, first, last);
+        }
+        HIPCK(hipGetLastError());
+        const int flips_p = (first ? 1 : 0) + len - (last ? 1 : 0);
+        pc_ ^= (flips_p & 1);
+        fc_ ^= (len & 1);
+        grid_valid_ = true;
+    }
+
+    void clear_graphs()
+    {
+        for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+        graphs_.clear();
+    }
+
+    hipStream_t stream_ = nullptr;
+    std::vector<Buf> bufs_;
+    int role_[10];
+    std::map<std::string, KernelRec> kernels_;
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, hipGraphExec_t> graphs_;
+
+    int N_ = 0, S_ = 0, A_ = 0, C_ = 0;
+    int pc_ = 0, fc_ = 0;
+    int n_spr_ = 0, n_ang_ = 0;
+    bool alias_ = false, pos_eq_img_ = false, csr_dirty_ = true, grid_valid_ = false, nb_init_ = false;
+    bool cellnh_checked_ = false;
+    int cap_ = 0, last_total_ = 0;
+    long long steps_ = 0, rebuilds_ = 0;
+
+    DArr<P> pos_[2][3], img_[2][3];
+    DArr<T> vel_[3], vold_[3], frc_[2][3];
+    DArr<T> lam_, hlf_, hlf2_, rad_;
+    DArr<int> type_, tag_, loc_;
+    DArr<double> rad_d_, skin_d_;
+    DArr<int> sp_p1_, sp_p2_, sp_type_, sp_status_, sp_break_, sp_start_, sp_ent_;
+    DArr<T> sp_l0_, sp_k_, sp_c1_, sp_c2_;
+    DArr<int> an_b1_, an_b2_, an_b3_, an_s1_, an_s2_, an_type_, an_status_, an_start_, an_ent_;
+    DArr<T> an_th0_, an_k_;
+    DArr<fnk::PairC> pairs_;
+    DArr<int> key_, cell_count_, cell_start_, cursor_, sorted_, ncnt_, nb_start_, nb_list_, scan_tmp_, stats_, step_;
+
+    std::vector<fn_bead_info> h_info_;
+    std::vector<fn_spring> h_springs_;
+    std::vector<fn_angle> h_angles_;
+    std::vector<fn_pair> h_pairs_;
+    std::vector<fn_grid_cell_nh> h_cellnh_;
+};
+
+template <typename F>
+int guarded(fn_engine *eng, F &&f)
+{
+    if (!eng) return FN_ERR_ARG;
+    try {
+        HIPCK(hipSetDevice(eng->device));
+        f();
+        return FN_OK;
+    } catch (const Error &e) {
+        eng->err = e.what();
+        eng->log(e.what());
+        return e.code;
+    } catch (const std::exception &e) {
+        eng->err = e.what();
+        eng->log(e.what());
+        return FN_ERR_ARG;
+    }
+}
+
+}  // namespace
+
+// ---- C ABI ---------------------------------------------------------------------------------------
+extern "C" {
+
+const char *fn_version(void) { return "fibrenet_amd 0.1 (gfx950)"; }
+
+int fn_create(int device, int precision, fn_engine **out)
+{
+    if (!out) return FN_ERR_ARG;
+    *out = nullptr;
+    try {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return FN_ERR_HIP;
+        if (device < 0 || device >= n) return FN_ERR_ARG;
+        if (precision == FN_PREC_F64) *out = new Engine<double, double>(device);
+        else if (precision == FN_PREC_F32) *out = new Engine<float, double>(device);
+        else return FN_ERR_ARG;
+        return FN_OK;
+    } catch (const Error &e) {
+        fprintf(stderr, "fibrenet: %s\n", e.what());
+        return e.code;
+    } catch (...) {
+        return FN_ERR_HIP;
+    }
+}
+
+int fn_destroy(fn_engine *eng)
+{
+    if (!eng) return FN_ERR_ARG;
+    (void)hipSetDevice(eng->device);
+    delete eng;
+    return FN_OK;
+}
+
+const char *fn_last_error(fn_engine *eng) { return eng ? eng->err.c_str() : "null engine"; }
+
+int fn_set_log_callback(fn_engine *eng, fn_log_fn fn, void *user)
+{
+    if (!eng) return FN_ERR_ARG;
+    eng->log_fn = fn;
+    eng->log_user = user;
+    return FN_OK;
+}
+
+int fn_register(fn_engine *eng, const char *const *names, int n)
+{
+    return guarded(eng, [&] {
+        if (!names || n <= 0) throw Error(FN_ERR_ARG, "no kernel names");
+        for (int i = 0; i < n; ++i) eng->reg(names[i]);
+    });
+}
+
+int fn_buffer_create(fn_engine *eng, int kind, int elem_size, int n_elems, const void *host, int *id)
+{
+    return guarded(eng, [&] {
+        const int r = eng->create_buffer(kind, elem_size, n_elems, host);
+        if (id) *id = r;
+    });
+}
+
+int fn_buffer_write(fn_engine *eng, int id, const void *host, int n_padded, int elem_size)
+{
+    return guarded(eng, [&] { eng->write_buffer(id, host, n_padded, elem_size); });
+}
+
+int fn_buffer_read(fn_engine *eng, int id, void *host)
+{
+    return guarded(eng, [&] {
+        if (!host) throw Error(FN_ERR_ARG, "null destination");
+        eng->read_buffer(id, host);
+    });
+}
+
+int fn_buffer_info(fn_engine *eng, int id, int *size_bytes, int *elements, int *length, int *struct_size)
+{
+    return guarded(eng, [&] {
+        const Buf *b = eng->buffer(id);
+        if (!b) throw Error(FN_ERR_ARG, fmt("no buffer %d", id));
+        if (size_bytes) *size_bytes = b->elements * b->esize;
+        if (elements) *elements = b->elements;
+        if (length) *length = b->length;
+        if (struct_size) *struct_size = b->esize;
+    });
+}
+
+int fn_set_arg(fn_engine *eng, const char *kernel, int index, int arg_kind, const void *value, int size)
+{
+    return guarded(eng, [&] { eng->set_arg(kernel, index, arg_kind, value, size); });
+}
+
+int fn_set_range(fn_engine *eng, const char *kernel, int range)
+{
+    return guarded(eng, [&] { eng->set_range(kernel, range); });
+}
+
+int fn_launch(fn_engine *eng, const char *kernel)
+{
+    return guarded(eng, [&] { eng->launch(kernel); });
+}
+
+int fn_finish(fn_engine *eng)
+{
+    return guarded(eng, [&] { eng->finish(); });
+}
+
+int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_springs, int run_angles)
+{
+    return guarded(eng, [&] { eng->run_steps(n_steps, grid_update_freq, run_springs != 0, run_angles != 0); });
+}
+
+int fn_export_cells(fn_engine *eng, int *cell_of_bead)
+{
+    return guarded(eng, [&] {
+        if (!cell_of_bead) throw Error(FN_ERR_ARG, "null destination");
+        eng->export_cells(cell_of_bead);
+    });
+}
+
+int fn_export_pairs(fn_engine *eng, int *pairs, long long capacity, long long *n_pairs)
+{
+    return guarded(eng, [&] {
+        const long long n = eng->export_pairs(pairs, capacity);
+        if (n_pairs) *n_pairs = n;
+    });
+}
+
+int fn_export_breaks(fn_engine *eng, int *break_step)
+{
+    return guarded(eng, [&] {
+        if (!break_step) throw Error(FN_ERR_ARG, "null destination");
+        eng->export_breaks(break_step);
+    });
+}
+
+int fn_get_stats(fn_engine *eng, fn_stats *out)
+{
+    return guarded(eng, [&] {
+        if (!out) throw Error(FN_ERR_ARG, "null destination");
+        eng->stats(out);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,40 @@
+// engine.h - launchers of the device code (host-callable), shared by the
+// engine's host code and the kernel translation unit.
+#pragma once
+#include <hip/hip_runtime.h>
+#include "kernels.h"
+
+namespace fnk {
+
+// Slots of the device statistics array.
+enum { kStatMaxCell = 0, kStatCellsOver = 1, kStatMaxList = 2, kStatCount = 4 };
+
+template <typename T, typename P>
+struct Launch {
+    static void advance_position(const Args<T, P> &a, int n_active, hipStream_t s);
+    static void resolve_springs(const Args<T, P> &a, hipStream_t s);
+    static void resolve_angles(const Args<T, P> &a, hipStream_t s);
+    static void resolve_contacts(const Args<T, P> &a, hipStream_t s);
+    static void advance_velocity(const Args<T, P> &a, int n_active, hipStream_t s);
+    static void reset_energy(const Args<T, P> &a, int n_active, hipStream_t s);
+    static void drift_first(const Args<T, P> &a, hipStream_t s);
+    static void fused(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, hipStream_t s);
+    static void hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *count,
+                     hipStream_t s);
+    static void pairs(bool fill, const P *px, const P *py, const P *pz, const int *key, const int *sorted,
+                      const int *cell_start, int n_cells, int n, const int *type, const int *tag, const int *loc,
+                      const double *radius, const double *skin, const PairC *pairs, const Box &b, int *cnt,
+                      const int *nb_start, int *nb_list, int cap, int *stats, hipStream_t s);
+};
+
+void fill_int(int *p, int v, int n, hipStream_t s);
+void copy_int(const int *src, int *dst, int n, hipStream_t s);
+void add_int(int *p, int v, hipStream_t s);
+void scatter(const int *key, int n, int *cursor, int *sorted, hipStream_t s);
+void sort_cells(const int *start, int *sorted, int n_cells, int *stats, hipStream_t s);
+int scan_blocks_needed(int n);
+// out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total.  tmp holds
+// scan_blocks_needed(n) + 1 ints.
+void exclusive_scan(const int *in, int n, int *out, int *tmp, hipStream_t s);
+
+}  // namespace fnk

@@ -1,0 +1,188 @@
+/*
+ * fibrenet.h - C ABI of libfibrenet_amd, the MI355X (gfx950) engine for
+ * FibreNet's particle-dynamics step.
+ *
+ * The reference has no FFI: its device seam is the C++ class OpenclManager
+ * (OpenclManager.h:48-69), driven by SimulationManager through buffers
+ * addressed by int id, kernels addressed by name, and an in-order queue.  The
+ * functions below are that seam as a plain C ABI (opaque handle, plain
+ * pointers and sizes, no C++ or torch types), so that the C++ facade in
+ * fibrenet_manager.hpp, a ctypes/cgo/JNI binding, or FibreNet's own
+ * SimulationManager can drive the HIP engine.  Each declaration names the
+ * reference interface it replaces.
+ *
+ * Conventions: every function returns FN_OK (0) or a negative FN_ERR_* code
+ * and never throws; fn_last_error() gives the message of the last failure on
+ * a handle.  One host thread per handle; all launches are asynchronous on the
+ * handle's HIP stream and fn_finish() / the blocking reads and writes are the
+ * synchronisation points.
+ */
+#ifndef FIBRENET_H
+#define FIBRENET_H
+
+#include "fibrenet_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FN_OK 0
+#define FN_ERR_ARG -1          /* invalid argument or state                     */
+#define FN_ERR_HIP -2          /* HIP runtime failure                           */
+#define FN_ERR_MISMATCH -3     /* length / record size differs from the buffer  */
+#define FN_ERR_EMPTY -4        /* empty source, as CreateBuffer/WriteBuffer -2  */
+#define FN_ERR_UNSUPPORTED -5  /* a configuration the engine does not run       */
+#define FN_ERR_CAPACITY -6     /* contact list grew past its reserved capacity  */
+#define FN_ERR_LOAD -7         /* the input or model file is rejected           */
+
+/* Arithmetic of an engine instance. */
+#define FN_PREC_F64 0          /* fp64 state and arithmetic (parity build)      */
+#define FN_PREC_F32 1          /* fp32 forces/velocities, fp64 positions         */
+
+/* Record kinds of a buffer (the reference's element type T of CreateBuffer<T>). */
+#define FN_BUF_BEAD 1          /* fn_bead        (Bead)                         */
+#define FN_BUF_BEAD_INFO 2     /* fn_bead_info   (BeadInfo)                     */
+#define FN_BUF_SPRING 3        /* fn_spring      (SpringStructure)              */
+#define FN_BUF_ANGLE 4         /* fn_angle       (AngleBond)                    */
+#define FN_BUF_PAIR 5          /* fn_pair        (Pair)                         */
+#define FN_BUF_INT 6           /* int            (grid and list heads)          */
+#define FN_BUF_GRID_CELL 7     /* fn_grid_cell   (GridCell)                     */
+#define FN_BUF_GRID_CELL_NH 8  /* fn_grid_cell_nh(GridCellNh)                   */
+#define FN_BUF_PARTICLE_NH 9   /* fn_particle_nh (ParticleNh)                   */
+
+/* Kinds of a kernel argument (OpenclManager::SetKernelArg<T>). */
+#define FN_ARG_BUFFER 1        /* value = const int* buffer id                  */
+#define FN_ARG_INT 2           /* value = const int*                            */
+#define FN_ARG_SETTINGS 3      /* value = const fn_settings*, copied by value   */
+
+typedef struct fn_engine fn_engine;
+typedef struct fn_model fn_model;
+typedef struct fn_sim fn_sim;
+typedef void (*fn_log_fn)(const char *message, void *user);
+
+/* Counters of the engine's last rebuild and run. */
+typedef struct fn_stats {
+    long long steps;               /* steps advanced by fn_run_steps / launches */
+    long long rebuilds;            /* grid + contact-list rebuilds              */
+    long long list_entries;        /* two-sided contact-list entries (=2 pairs) */
+    long long list_capacity;       /* reserved entries                          */
+    int hashed_beads;              /* beads inside the grid at the last rebuild */
+    int max_cell_occupancy;        /* the reference drops beads beyond 64       */
+    int cells_over_capacity;       /* cells that the reference would truncate   */
+    int max_list_length;           /* longest two-sided list of one bead        */
+    int broken_springs;            /* springs with status 0                     */
+    int img_aliases_pos;           /* 1: one array holds position and image     */
+} fn_stats;
+
+/* Library version string. */
+const char *fn_version(void);
+
+/* OpenclManager::Initialize (OpenclManager.cpp:5-85): opens HIP device
+ * `device` (no interactive platform/device prompt) with one stream. */
+int fn_create(int device, int precision, fn_engine **out);
+/* OpenclManager::Close (OpenclManager.cpp:318-320); frees all device memory. */
+int fn_destroy(fn_engine *eng);
+/* Message of the last failure on `eng` (checkErr, OpenclManager.cpp:335-409). */
+const char *fn_last_error(fn_engine *eng);
+/* Log sink replacing log_stream (log_stream.h:10-67); NULL restores stderr. */
+int fn_set_log_callback(fn_engine *eng, fn_log_fn fn, void *user);
+
+/* OpenclManager::RegisterKernel (OpenclManager.cpp:87-113): every name must be
+ * one of FibreNet's ten step kernels; the engine's HIP code is prebuilt, so
+ * there is no source text to compile. */
+int fn_register(fn_engine *eng, const char *const *names, int n);
+
+/* OpenclManager::CreateBuffer<T>(std::vector<T>&, flags) (OpenclManager.h:121-146)
+ * and CreateBuffer<T>(int, flags) (OpenclManager.h:148-174).  `host` holds
+ * n_elems records of `kind` (NULL: zero-filled); the engine pads to a multiple
+ * of 64 records as the reference does.  *id receives the buffer id. */
+int fn_buffer_create(fn_engine *eng, int kind, int elem_size, int n_elems, const void *host, int *id);
+/* OpenclManager::WriteBuffer<T> (OpenclManager.h:100-119): blocking write of
+ * n_padded records; FN_ERR_MISMATCH when n_padded or elem_size differ. */
+int fn_buffer_write(fn_engine *eng, int id, const void *host, int n_padded, int elem_size);
+/* OpenclManager::ReadBuffer (OpenclManager.cpp:174-201): blocking read of the
+ * padded length x record size bytes into `host`.  Scratch buffers of the
+ * reference's grid and list (GridCell, ParticleNh, list heads) are not
+ * materialised and return FN_ERR_UNSUPPORTED; use the fn_export_* calls. */
+int fn_buffer_read(fn_engine *eng, int id, void *host);
+/* GetBufferSize / GetBufferElements / GetBufferLength / GetBufferStructureSize
+ * (OpenclManager.cpp:130-172); any pointer may be NULL. */
+int fn_buffer_info(fn_engine *eng, int id, int *size_bytes, int *elements, int *length, int *struct_size);
+
+/* OpenclManager::SetKernelArg<T> (OpenclManager.h:74-98): binds argument
+ * `index` of kernel `kernel` following the argument contract of
+ * SimulationManager.cpp:1236-1304; settings are captured by value. */
+int fn_set_arg(fn_engine *eng, const char *kernel, int index, int arg_kind, const void *value, int size);
+/* OpenclManager::SetKernelGlobalWorkRange (OpenclManager.cpp:299-316). */
+int fn_set_range(fn_engine *eng, const char *kernel, int range);
+/* OpenclManager::ExecuteKernel (OpenclManager.cpp:219-285): enqueues the
+ * engine's HIP implementation of the named kernel with its bound arguments. */
+int fn_launch(fn_engine *eng, const char *kernel);
+/* OpenclManager::RunKernels (OpenclManager.cpp:203-217): waits for the stream. */
+int fn_finish(fn_engine *eng);
+
+/* Fast path of SimulationManager::RunSimulationLoop (SimulationManager.cpp:151-181):
+ * n_steps steps with a rebuild at the first step and every grid_update_freq
+ * steps, as fused hipGraph-captured launches.  run_springs / run_angles mirror
+ * the caller's springs_num > 0 / angles_num > 0 tests.  Same results as the
+ * per-kernel launches; returns after the work is enqueued and checked. */
+int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_springs, int run_angles);
+
+/* Debug exports for parity checks (no reference counterpart). */
+/* Cell index of every bead at the last rebuild (-1: outside the grid). */
+int fn_export_cells(fn_engine *eng, int *cell_of_bead);
+/* Unordered contact pairs (i<j) of the current list, sorted, as 2*n ints. */
+int fn_export_pairs(fn_engine *eng, int *pairs, long long capacity, long long *n_pairs);
+/* Step index at which each spring broke (-1: intact or broken before upload). */
+int fn_export_breaks(fn_engine *eng, int *break_step);
+int fn_get_stats(fn_engine *eng, fn_stats *out);
+
+/* ---- loaders (SimulationManager::ReadInputFile, SimulationManager.cpp:1153-1171) ---- */
+
+/* Parses an input file and its model (and restart) file into host records,
+ * applying modif_bead / modif_spring; no device is touched.  Paths inside the
+ * input file are resolved relative to its directory when not absolute. */
+int fn_model_load(const char *input_path, fn_model **out);
+int fn_model_free(fn_model *m);
+const char *fn_model_error(void);
+/* Counts: beads, springs, angles, pair-table entries, cells. */
+int fn_model_counts(const fn_model *m, int *n_beads, int *n_springs, int *n_angles, int *n_pairs, int *n_cells);
+/* Run parameters: run, output_freq, grid_update_freq, minimize steps/substeps/reset. */
+int fn_model_params(const fn_model *m, int *run, int *output_freq, int *grid_update_freq, int *min_steps,
+                    int *min_substeps, int *min_reset);
+/* Copies the records of `kind` (FN_BUF_BEAD, _BEAD_INFO, _SPRING, _ANGLE,
+ * _PAIR, _GRID_CELL_NH) or the settings (kind 0) into `dst`. */
+int fn_model_copy(const fn_model *m, int kind, void *dst);
+
+/* ---- simulation host (SimulationManager) ---- */
+
+/* SimulationManager::Initialize (SimulationManager.cpp:8-51) without stdin:
+ * loads `input_path`, uploads through the buffer API and binds every kernel
+ * argument as ReadKernelData does (SimulationManager.cpp:1180-1307). */
+int fn_sim_create(int device, int precision, const char *input_path, fn_sim **out);
+int fn_sim_destroy(fn_sim *sim);
+fn_engine *fn_sim_engine(fn_sim *sim);
+/* Buffer id of a role: kind FN_BUF_BEAD, _BEAD_INFO, _SPRING, _ANGLE, _PAIR. */
+int fn_sim_buffer_id(fn_sim *sim, int kind);
+/* SimulationManager::RunSimulationLoop (SimulationManager.cpp:151-181).
+ * fast = 0: per-kernel launches with a finish every step, as the reference;
+ * fast = 1: the fused hipGraph path (fn_run_steps). */
+int fn_sim_run_loop(fn_sim *sim, int loop_size, int fast);
+/* SimulationManager::MinimizeSystemEnergy (SimulationManager.cpp:111-143). */
+int fn_sim_minimize(fn_sim *sim, int fast);
+
+/* ---- synthetic inputs (no reference counterpart) ---- */
+
+/* Places n_fibres straight fibres of n_beads beads (spacing apart) with
+ * uniformly random starts and directions; `finite` resamples fibres leaving
+ * [0,L]^3 and, with d_min > 0, fibres coming closer than d_min to an already
+ * placed bead.  out receives n_fibres*n_beads*3 unwrapped coordinates;
+ * *tries the number of fibres sampled (FN_ERR_CAPACITY past max_tries). */
+int fn_gen_fibres(unsigned long long seed, int n_fibres, int n_beads, double lx, double ly, double lz, int finite,
+                  double spacing, double d_min, long long max_tries, double *out, long long *tries);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FIBRENET_H */

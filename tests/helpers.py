@@ -1,0 +1,60 @@
+"""Shared builders for the parity tests: small networks of the two workloads
+and a driver running the HIP engine and the oracle side by side."""
+import numpy as np
+
+from fibrenet_amd import Engine, bind_standard, generate as G, records as R, run_loop_granular
+from oracle.oracle import Oracle
+
+
+def small_periodic(seed=11, n_fibres=60, n_beads=20, box=(30.0, 30.0, 12.0)):
+    return G.periodic_hs(n_fibres=n_fibres, n_beads=n_beads, box=box, seed=seed)
+
+
+def small_tensile(seed=12, n_fibres=50, n_beads=20, box=(60.0, 60.0, 30.0), weak=False):
+    net = G.tensile_lj(n_fibres=n_fibres, n_beads=n_beads, box=box, seed=seed, d_min=0.6)
+    if weak:   # low failure stress so that springs break within a few hundred steps
+        rs = np.random.Generator(np.random.PCG64(seed))
+        net.springs["c2"] = rs.integers(1, 4, size=len(net.springs)).astype(np.float64)
+        net.pull_v = 0.5
+        hi = net.info["type"] == 2
+        v = np.zeros((len(net.beads), 3))
+        v[hi, 0] = 0.5
+        R.set_vec(net.beads, "velocity", v)
+        R.set_vec(net.beads, "velocity_old", v)
+    return net
+
+
+def engine_for(net, precision):
+    eng = Engine(precision)
+    ids = bind_standard(eng, net)
+    return eng, ids
+
+
+def oracle_for(net):
+    return Oracle(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, net.cell_nh)
+
+
+def read_beads(eng, ids):
+    return eng.read_buffer(ids["bead"], R.BEAD)
+
+
+def run_engine(eng, net, n, fast=True):
+    spr, ang = len(net.springs) > 0, len(net.angles) > 0
+    if fast:
+        eng.run_steps(n, net.grid_update_freq, spr, ang)
+    else:
+        run_loop_granular(eng, n, net.grid_update_freq, spr, ang)
+
+
+def rel_err(a, b, floor=1.0):
+    """max |a-b| / max(floor, max|b|)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.max(np.abs(a - b)) / max(floor, float(np.max(np.abs(b))))) if a.size else 0.0
+
+
+FIELDS = ("position", "image_position", "velocity", "force", "velocity_old", "force_old")
+
+
+def field_errors(beads, ref):
+    return {f: rel_err(R.vec(beads, f), R.vec(ref, f)) for f in FIELDS}
