@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Benchmark of the FibreNet step on one MI355X.
+
+Workload (default): the tensile test with 2,000,000 beads - 20,000 straight
+fibres of 100 beads in a finite 1000x1000x50 box, HR_FL springs (k 100,
+E 1000, failure stress uniform in [50,150]), HR angles (k 10), Lennard-Jones
+contacts (eps 1, sigma 1, cutoff 2.5) between all three bead types, beads
+with x<5 clamped and x>995 pulled at +0.01, dt 5e-4, grid rebuild every 10
+steps.  A step is one pass of the fused step kernel over all beads (plus the
+grid rebuild every 10th step); inputs are resident in HBM when the timed
+region starts.
+
+Prints one JSON line: bead-steps/s of the timed steps, the roofline of the
+dominant kernel (the fused step kernel; its duration measured with HIP events
+on the engine's stream over the timed region) and a CPU baseline (the
+sequential fp64 oracle on the host, a bounded sample of the same network).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(net, pairs):
+    """Bytes per step of the step kernel by the per-unit figures: 100 B per
+    bead (position, image, velocity_old and force read and written once in
+    fp32, one mass coefficient), 16 B per HR spring, 28 B per HR_FL spring,
+    20 B per angle, 8 B per listed pair; plus, per rebuild, 36 B per bead
+    and 8 B per pair."""
+    import numpy as np
+    n = len(net.beads)
+    s_hr = int(np.sum(net.springs["type"] == 0))
+    s_fl = int(np.sum(net.springs["type"] == 1))
+    a = len(net.angles)
+    step = 100 * n + 16 * s_hr + 28 * s_fl + 20 * a + 8 * pairs
+    rebuild = 36 * n + 8 * pairs
+    return step, rebuild, dict(beads=n, springs_hr=s_hr, springs_hr_fl=s_fl, angles=a, pairs=pairs)
+
+
+def cpu_baseline(net, steps):
+    from oracle.oracle import Oracle
+    orc = Oracle(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, net.cell_nh)
+    t = time.perf_counter()
+    orc.run(steps, net.grid_update_freq)
+    dt = time.perf_counter() - t
+    return len(net.beads) * steps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--workload", default="tensile_lj", choices=["tensile_lj", "periodic_hs"])
+    ap.add_argument("--fibres", type=int, default=None)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-bytes", type=float, default=0,
+                    help="also run one device copy of this many bytes (counter calibration)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: the step is not sharded across GPUs in this build "
+              "(see the sharding design); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from fibrenet_amd import Engine, bind_standard, generate as G
+
+    t0 = time.perf_counter()
+    if args.workload == "tensile_lj":
+        net = G.tensile_lj(n_fibres=args.fibres or 20000)
+        desc = ("tensile_lj: %d fibres x 100 beads (%d beads), finite box 1000x1000x50, HR_FL springs + HR angles"
+                " + LJ contacts (cutoff 2.5), clamp/pull 0.01, dt 5e-4, grid_update_freq 10"
+                % (net.meta["n_fibres"], len(net.beads)))
+    else:
+        net = G.periodic_hs(n_fibres=args.fibres or 200)
+        desc = ("periodic_hs: %d fibres x 50 beads (%d beads), periodic box 100x100x20, HR springs + HR angles"
+                " + penalty contacts, dt 1e-3, grid_update_freq 8" % (net.meta["n_fibres"], len(net.beads)))
+    gen_s = time.perf_counter() - t0
+
+    eng = Engine(args.precision)
+    bind_standard(eng, net)
+    spr, ang = len(net.springs) > 0, len(net.angles) > 0
+    if args.warmup > 0:
+        eng.run_steps(args.warmup, net.grid_update_freq, spr, ang)
+    eng.finish()
+    print("bench: %s, warmup %d done, timing %d steps" % (desc, args.warmup, args.steps), file=sys.stderr,
+          flush=True)
+
+    # Timed region: HIP events around every fused step kernel and rebuild on the
+    # engine's stream (kernels launched directly, not through graphs).
+    eng.set_timing(True)
+    eng.finish()
+    t = time.perf_counter()
+    eng.run_steps(args.steps, net.grid_update_freq, spr, ang)   # returns after the stream has drained
+    wall = time.perf_counter() - t
+    tim = eng.timing()
+    st = eng.stats()
+    eng.set_timing(False)
+    # The same number of steps again through the hipGraph-captured chunks.
+    eng.run_steps(min(args.steps, 2 * net.grid_update_freq), net.grid_update_freq, spr, ang)   # capture
+    eng.finish()
+    t = time.perf_counter()
+    eng.run_steps(args.steps, net.grid_update_freq, spr, ang)
+    wall_graph = time.perf_counter() - t
+
+    n = len(net.beads)
+    pairs = st["list_entries"] // 2
+    b_step, b_rebuild, counts = algorithmic_bytes(net, pairs)
+    fused_ms = tim["fused_ms"] / max(tim["fused_launches"], 1)
+    rebuild_ms = tim["rebuild_ms"] / max(tim["rebuilds"], 1)
+    achieved = b_step / (fused_ms * 1e-3) / 1e9
+    value = n * args.steps / wall
+
+    copy = None
+    if args.copy_bytes > 0:
+        import torch
+        nb = int(args.copy_bytes) // 4
+        a = torch.empty(nb, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        copy = nb * 4
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "traffic_fused_step_%s_%s.json" % (args.workload, args.precision))
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        v, dt = cpu_baseline(net, args.cpu_steps)
+        cpu = {"value": v, "unit": "bead-steps/s", "cores": 1, "kind": "port",
+               "sample": "the same %d-bead network, %d steps (%d grid rebuilds) of the sequential fp64 oracle "
+                         "(restatement of kernel.cl), %.1f s" % (n, args.cpu_steps,
+                                                                (args.cpu_steps + net.grid_update_freq - 1)
+                                                                // net.grid_update_freq, dt)}
+
+    out = {
+        "metric": "bead-steps/s", "value": value, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 forces/velocities, f64 positions" if args.precision == "f32" else "f64",
+        "data": "synthetic (generated fibre network, seed %d)" % net.meta["seed"],
+        "config": {"workload": desc, "beads": n, "springs": len(net.springs), "angles": len(net.angles),
+                   "listed_pairs": pairs, "grid_update_freq": net.grid_update_freq, "precision": args.precision},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_fused_step", "algorithmic_bytes_per_launch": b_step,
+                     "avg_launch_ms": fused_ms, "launches_timed": tim["fused_launches"]},
+        "cpu_baseline": cpu,
+        "extra": {"graph_mode_bead_steps_per_s": n * args.steps / wall_graph,
+                  "graph_mode_ms_per_step": wall_graph * 1e3 / args.steps,
+                  "rebuild_ms": rebuild_ms, "rebuild_algorithmic_bytes": b_rebuild,
+                  "step_algorithmic_GBps_incl_rebuild": (b_step + b_rebuild / net.grid_update_freq)
+                  * args.steps / wall / 1e9, "counts": counts, "stats": st, "generate_s": gen_s,
+                  "gpu_over_cpu": (value / cpu["value"]) if cpu else None, "copy_bytes": copy},
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -42,6 +42,8 @@ SIGNATURES = [
     ("fn_export_pairs", _I, [_VP, _VP, _LL, _c.POINTER(_LL)]),
     ("fn_export_breaks", _I, [_VP, _VP]),
     ("fn_get_stats", _I, [_VP, _VP]),
+    ("fn_set_timing", _I, [_VP, _I]),
+    ("fn_get_timing", _I, [_VP, _VP]),
     ("fn_model_load", _I, [_CP, _c.POINTER(_VP)]),
     ("fn_model_free", _I, [_VP]),
     ("fn_model_error", _CP, []),
@@ -63,6 +65,13 @@ class FnStats(ctypes.Structure):
     _fields_ = [("steps", _LL), ("rebuilds", _LL), ("list_entries", _LL), ("list_capacity", _LL),
                 ("hashed_beads", _I), ("max_cell_occupancy", _I), ("cells_over_capacity", _I),
                 ("max_list_length", _I), ("broken_springs", _I), ("img_aliases_pos", _I)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FnTiming(ctypes.Structure):
+    _fields_ = [("fused_launches", _I), ("rebuilds", _I), ("fused_ms", _c.c_double), ("rebuild_ms", _c.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -241,12 +250,20 @@ class Engine:
         self._chk(lib().fn_get_stats(self.h, ctypes.byref(s)))
         return s.as_dict()
 
+    def set_timing(self, on):
+        self._chk(lib().fn_set_timing(self.h, int(bool(on))))
+
+    def timing(self):
+        t = FnTiming()
+        self._chk(lib().fn_get_timing(self.h, ctypes.byref(t)))
+        return t.as_dict()
+
 
 def bind_standard(eng, net):
     """Uploads a network (generate.Network or Model) and binds every kernel
     argument as SimulationManager::ReadKernelData does
     (SimulationManager.cpp:1229-1304).  Returns the buffer ids."""
-    s = np.ascontiguousarray(net.settings.reshape(()))
+    s = np.array(net.settings, R.SETTINGS).reshape(())
     C = int(s["cells_x"]) * int(s["cells_y"]) * int(s["cells_z"])
     N = len(net.beads)
     ids = {}

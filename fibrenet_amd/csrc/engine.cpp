@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <list>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -209,6 +210,8 @@ struct fn_engine {
     virtual long long export_pairs(int *out, long long cap) = 0;
     virtual void export_breaks(int *out) = 0;
     virtual void stats(fn_stats *out) = 0;
+    virtual void set_timing(bool on) = 0;
+    virtual void timing(fn_timing *out) = 0;
 };
 
 namespace {
@@ -472,6 +475,7 @@ class Engine final : public fn_engine {
         check_alias(settings_of(K_ADVANCE_POSITION).bc_type);
         ensure_grid(&settings_of(K_FILL_GRID));
         ensure_grid(&settings_of(K_BUILD_LIST));
+        ensure_scratch();
         // capacity of the contact list: grow before the run if the last rebuild came close
         if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
         fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
@@ -485,6 +489,7 @@ class Engine final : public fn_engine {
         steps_ += n;
         rebuilds_ += (n + guf - 1) / guf;
         HIPCK(hipStreamSynchronize(stream_));
+        collect_timing();
         check_capacity();
     }
 
@@ -550,8 +555,59 @@ class Engine final : public fn_engine {
         o->img_aliases_pos = alias_ ? 1 : 0;
     }
 
+    void set_timing(bool on) override
+    {
+        if (on != timing_) clear_graphs();
+        timing_ = on;
+        timing_acc_ = fn_timing{};
+    }
+
+    void timing(fn_timing *out) override { *out = timing_acc_; }
+
   private:
     static int C_of(const fn_settings &s) { return s.cells_x * s.cells_y * s.cells_z; }
+
+    // Events of one captured chunk: pairs around each fused kernel / rebuild.
+    struct Events {
+        std::vector<hipEvent_t> fused, rebuild;
+        void release()
+        {
+            for (auto e : fused) (void)hipEventDestroy(e);
+            for (auto e : rebuild) (void)hipEventDestroy(e);
+            fused.clear();
+            rebuild.clear();
+        }
+    };
+
+    static hipEvent_t new_event()
+    {
+        hipEvent_t e;
+        HIPCK(hipEventCreate(&e));
+        return e;
+    }
+
+    void mark(std::vector<hipEvent_t> *v)
+    {
+        if (!v) return;
+        v->push_back(new_event());
+        HIPCK(hipEventRecord(v->back(), stream_));
+    }
+
+    void accumulate(const Events &ev)
+    {
+        for (size_t k = 0; k + 1 < ev.fused.size(); k += 2) {
+            float ms = 0.f;
+            HIPCK(hipEventElapsedTime(&ms, ev.fused[k], ev.fused[k + 1]));
+            timing_acc_.fused_ms += ms;
+            timing_acc_.fused_launches++;
+        }
+        for (size_t k = 0; k + 1 < ev.rebuild.size(); k += 2) {
+            float ms = 0.f;
+            HIPCK(hipEventElapsedTime(&ms, ev.rebuild[k], ev.rebuild[k + 1]));
+            timing_acc_.rebuild_ms += ms;
+            timing_acc_.rebuilds++;
+        }
+    }
 
     // ---- upload / download -------------------------------------------------------------------
     template <typename X>
@@ -724,11 +780,12 @@ class Engine final : public fn_engine {
     {
         const int n = S_;
         std::vector<int> p1(n), p2(n), ty(n), st(n), br(n, -1);
-        std::vector<T> l0(n), k(n), c1(n), c2(n);
+        std::vector<T> k(n);
+        std::vector<double> l0(n), c1(n), c2(n);
         for (int s = 0; s < n; ++s) {
             const fn_spring &q = h_springs_[s];
             p1[s] = q.p1; p2[s] = q.p2; ty[s] = q.type; st[s] = q.status;
-            l0[s] = (T)q.rest_length; k[s] = (T)q.stiffness; c1[s] = (T)q.c1; c2[s] = (T)q.c2;
+            l0[s] = q.rest_length; k[s] = (T)q.stiffness; c1[s] = q.c1; c2[s] = q.c2;
         }
         sp_p1_.put(p1, stream_); sp_p2_.put(p2, stream_); sp_type_.put(ty, stream_); sp_status_.put(st, stream_);
         sp_break_.put(br, stream_);
@@ -769,13 +826,18 @@ class Engine final : public fn_engine {
         }
         std::vector<int> st(n + 1, 0), ent(2 * (size_t)S_ + 1);
         for (int i = 0; i < n; ++i) st[i + 1] = st[i] + cnt[i];
-        std::vector<int> cur(st.begin(), st.end() - 1);
+        std::vector<int> cur(st.begin(), st.end() - 1), twin(2 * (size_t)S_ + 1, -1);
         for (int s = 0; s < S_; ++s) {
-            ent[cur[h_springs_[s].p1]++] = s << 1;
-            ent[cur[h_springs_[s].p2]++] = (s << 1) | 1;
+            const int e1 = cur[h_springs_[s].p1]++;
+            ent[e1] = s << 1;
+            const int e2 = cur[h_springs_[s].p2]++;
+            ent[e2] = (s << 1) | 1;
+            twin[e1] = e1;
+            twin[e2] = e1;
         }
         sp_start_.put(st, stream_);
         sp_ent_.put(ent, stream_);
+        sp_twin_.put(twin, stream_);
         std::fill(cnt.begin(), cnt.end(), 0);
         for (const fn_angle &a : h_angles_) {
             for (int b : {a.b1, a.b2, a.b3})
@@ -786,14 +848,19 @@ class Engine final : public fn_engine {
         }
         std::vector<int> as(n + 1, 0), aent(3 * (size_t)A_ + 1);
         for (int i = 0; i < n; ++i) as[i + 1] = as[i] + cnt[i];
-        std::vector<int> acur(as.begin(), as.end() - 1);
+        std::vector<int> acur(as.begin(), as.end() - 1), owner(3 * (size_t)A_ + 1, -1);
         for (int q = 0; q < A_; ++q) {
-            aent[acur[h_angles_[q].b1]++] = (q << 2) | 0;
-            aent[acur[h_angles_[q].b3]++] = (q << 2) | 2;
-            aent[acur[h_angles_[q].b2]++] = (q << 2) | 1;
+            const int e1 = acur[h_angles_[q].b1]++;
+            aent[e1] = (q << 2) | 0;
+            const int e3 = acur[h_angles_[q].b3]++;
+            aent[e3] = (q << 2) | 2;
+            const int e2 = acur[h_angles_[q].b2]++;
+            aent[e2] = (q << 2) | 1;
+            owner[e1] = owner[e2] = owner[e3] = e2;
         }
         an_start_.put(as, stream_);
         an_ent_.put(aent, stream_);
+        an_owner_.put(owner, stream_);
         HIPCK(hipStreamSynchronize(stream_));
         csr_dirty_ = false;
         clear_graphs();
@@ -855,6 +922,8 @@ class Engine final : public fn_engine {
         a.an_th0 = an_th0_.p; a.an_k = an_k_.p;
         a.n_angles = A_;
         a.nb_start = nb_start_.p; a.nb_list = nb_list_.p; a.pairs = pairs_.p;
+        a.n_pairs = (int)h_pairs_.size();
+        a.sp_twin = sp_twin_.p; a.an_owner = an_owner_.p;
         a.step = step_.p;
         return a;
     }
@@ -900,11 +969,11 @@ class Engine final : public fn_engine {
             if (s->cells_x < 3 || s->cells_y < 3 || s->cells_z < 3)
                 throw Error(FN_ERR_UNSUPPORTED, "grid with fewer than three cells along an axis");
             if ((long long)s->cells_x * s->cells_y != s->cells_xy) throw Error(FN_ERR_ARG, "cells_xy mismatch");
-            const int T = s->types_num;
-            if (T <= 0 || (long long)T * (T + 1) / 2 > (long long)h_pairs_.size())
+            const int nt = s->types_num;
+            if (nt <= 0 || (long long)nt * (nt + 1) / 2 > (long long)h_pairs_.size())
                 throw Error(FN_ERR_ARG, "pair table smaller than types_num*(types_num+1)/2");
             for (const fn_bead_info &q : h_info_)
-                if (q.type < 0 || q.type >= T) throw Error(FN_ERR_ARG, "bead type outside [0, types_num)");
+                if (q.type < 0 || q.type >= nt) throw Error(FN_ERR_ARG, "bead type outside [0, types_num)");
             if (C != C_) {
                 clear_graphs();
                 C_ = C;
@@ -968,42 +1037,83 @@ class Engine final : public fn_engine {
     {
         const fn_settings &s = settings_of(K_FILL_GRID);
         ensure_grid(&s);
+        ensure_scratch();
         fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
         fnk::Launch<T, P>::hash(pos_[pc_][0].p, pos_[pc_][1].p, pos_[pc_][2].p, N_, b, key_.p, cell_count_.p, stream_);
         fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, stream_);
         fnk::copy_int(cell_start_.p, cursor_.p, C_, stream_);
         fnk::scatter(key_.p, N_, cursor_.p, sorted_.p, stream_);
-        fnk::sort_cells(cell_start_.p, sorted_.p, C_, stats_.p, stream_);
+        fnk::Launch<T, P>::gather_sorted(sorted_.p, cell_start_.p + C_, pos_[pc_][0].p, pos_[pc_][1].p,
+                                         pos_[pc_][2].p, type_.p, tag_.p, loc_.p, sorted_view(), N_, stream_);
         grid_valid_ = true;
     }
 
-    // exact: count, then size the list on the host before filling (granular
-    // path); otherwise fill into the reserved capacity (fast path).
+    fnk::Sorted sorted_view()
+    {
+        return fnk::Sorted{so_x_.p, so_y_.p, so_z_.p, so_tag_.p, so_loc_.p, so_type_.p, so_slot_.p};
+    }
+
+    void ensure_scratch()
+    {
+        so_x_.ensure(N_); so_y_.ensure(N_); so_z_.ensure(N_);
+        so_tag_.ensure(N_); so_loc_.ensure(N_); so_type_.ensure(N_); so_slot_.ensure(N_);
+        if (scratch_.n < (size_t)N_ * K_) {
+            clear_graphs();
+            scratch_.alloc((size_t)N_ * K_);
+        }
+    }
+
+    // Search, offsets, compaction.  exact (per-kernel path): the host checks
+    // the row and list capacities before compacting and grows them as needed.
+    void search_and_compact(const fnk::Box &b, bool exact)
+    {
+        for (;;) {
+            if (exact) fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
+            fnk::fill_int(ncnt_.p, 0, N_, stream_);
+            fnk::search(sorted_.p, cell_start_.p, C_, key_.p, sorted_view(), rad_d_.p, skin_d_.p, pairs_.p, b,
+                        ncnt_.p, scratch_.p, K_, stats_.p, N_, stream_);
+            fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, stream_);
+            if (exact) {
+                int tot = 0, over = 0;
+                HIPCK(hipMemcpyAsync(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+                HIPCK(hipMemcpyAsync(&over, stats_.p + fnk::kStatListOver, sizeof(int), hipMemcpyDeviceToHost, stream_));
+                HIPCK(hipStreamSynchronize(stream_));
+                if (over > 0) {
+                    grow_rows();
+                    continue;
+                }
+                if (tot > cap_) grow_list((long long)tot * 2);
+                last_total_ = tot;
+            }
+            fnk::compact(ncnt_.p, nb_start_.p, so_slot_.p, scratch_.p, K_, N_, nb_list_.p, cap_, stream_);
+            return;
+        }
+    }
+
+    void grow_rows()
+    {
+        K_ *= 2;
+        ensure_scratch();
+    }
+
     void build_list(const fnk::Box &b, bool exact)
     {
         ensure_grid(&settings_of(K_BUILD_LIST));
-        fnk::fill_int(ncnt_.p, 0, N_, stream_);
-        fnk::Launch<T, P>::pairs(false, pos_[pc_][0].p, pos_[pc_][1].p, pos_[pc_][2].p, key_.p, sorted_.p,
-                                 cell_start_.p, C_, N_, type_.p, tag_.p, loc_.p, rad_d_.p, skin_d_.p, pairs_.p, b,
-                                 ncnt_.p, nb_start_.p, nb_list_.p, cap_, stats_.p, stream_);
-        fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, stream_);
-        if (exact) {
-            int tot = 0;
-            HIPCK(hipMemcpyAsync(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
-            HIPCK(hipStreamSynchronize(stream_));
-            if (tot > cap_) grow_list((long long)tot * 2);
-            last_total_ = tot;
-        }
-        fnk::Launch<T, P>::pairs(true, pos_[pc_][0].p, pos_[pc_][1].p, pos_[pc_][2].p, key_.p, sorted_.p,
-                                 cell_start_.p, C_, N_, type_.p, tag_.p, loc_.p, rad_d_.p, skin_d_.p, pairs_.p, b,
-                                 ncnt_.p, nb_start_.p, nb_list_.p, cap_, stats_.p, stream_);
+        ensure_scratch();
+        search_and_compact(b, exact);
     }
 
     void check_capacity()
     {
-        int tot = 0;
+        int tot = 0, over = 0;
         HIPCK(hipMemcpy(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(&over, stats_.p + fnk::kStatListOver, sizeof(int), hipMemcpyDeviceToHost));
         last_total_ = std::max(last_total_, tot);
+        if (over > 0) {
+            grow_rows();
+            throw Error(FN_ERR_CAPACITY, fmt("a contact list outgrew its %d-entry search row; the run is invalid "
+                                             "and the row length has been doubled", K_ / 2));
+        }
         if (tot > cap_) {
             grow_list((long long)tot * 2);
             throw Error(FN_ERR_CAPACITY, fmt("contact list needed %d entries, %d were reserved; the run is invalid "
@@ -1020,7 +1130,7 @@ class Engine final : public fn_engine {
         return guf * std::max(1, target / guf);
     }
 
-    void enqueue_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
+    void enqueue_chunk(int len, int guf, bool spr, bool ang, bool first, bool last, Events *ev)
     {
         const fnk::Box bd = make_box(settings_of(K_ADVANCE_POSITION));
         const fnk::Box bf = make_box(settings_of(K_FILL_GRID));
@@ -1037,20 +1147,17 @@ class Engine final : public fn_engine {
         }
         for (int k = 0; k < len; ++k) {
             if (k % guf == 0) {
+                mark(ev ? &ev->rebuild : nullptr);
                 fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
                 fnk::Launch<T, P>::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, bf, key_.p, cell_count_.p,
                                         stream_);
                 fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, stream_);
                 fnk::copy_int(cell_start_.p, cursor_.p, C_, stream_);
                 fnk::scatter(key_.p, N_, cursor_.p, sorted_.p, stream_);
-                fnk::sort_cells(cell_start_.p, sorted_.p, C_, stats_.p, stream_);
-                fnk::fill_int(ncnt_.p, 0, N_, stream_);
-                for (int fill = 0; fill < 2; ++fill) {
-                    fnk::Launch<T, P>::pairs(fill, pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, key_.p, sorted_.p,
-                                             cell_start_.p, C_, N_, type_.p, tag_.p, loc_.p, rad_d_.p, skin_d_.p,
-                                             pairs_.p, bl, ncnt_.p, nb_start_.p, nb_list_.p, cap_, stats_.p, stream_);
-                    if (!fill) fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, stream_);
-                }
+                        fnk::Launch<T, P>::gather_sorted(sorted_.p, cell_start_.p + C_, pos_[pc][0].p, pos_[pc][1].p,
+                                                 pos_[pc][2].p, type_.p, tag_.p, loc_.p, sorted_view(), N_, stream_);
+                search_and_compact(bl, false);
+                mark(ev ? &ev->rebuild : nullptr);
             }
             const bool is_last = last && k == len - 1;
             fnk::Args<T, P> a = args_base();
@@ -1062,7 +1169,9 @@ class Engine final : public fn_engine {
             set_pos(a, pc);
             a.fx = frc_[fc][0].p; a.fy = frc_[fc][1].p; a.fz = frc_[fc][2].p;
             a.gx = frc_[1 - fc][0].p; a.gy = frc_[1 - fc][1].p; a.gz = frc_[1 - fc][2].p;
+            mark(ev ? &ev->fused : nullptr);
             fnk::Launch<T, P>::fused(a, spr, ang, !is_last, is_last, stream_);
+            mark(ev ? &ev->fused : nullptr);
             fc ^= 1;
             if (!is_last) pc ^= 1;
         }
@@ -1072,22 +1181,25 @@ class Engine final : public fn_engine {
     void run_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
     {
         const auto key = std::make_tuple(len, guf, (int)spr, (int)ang, (int)first, (int)last, pc_, fc_, n_spr_, n_ang_);
-        const bool use_graph = !getenv("FIBRENET_NO_GRAPH");
+        // Timed runs launch directly: HIP cannot time events recorded as graph nodes.
+        const bool use_graph = !getenv("FIBRENET_NO_GRAPH") && !timing_;
         if (use_graph) {
             auto it = graphs_.find(key);
             if (it == graphs_.end()) {
+                GraphRec rec;
                 hipGraph_t g;
                 HIPCK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-                enqueue_chunk(len, guf, spr, ang, first, last);
+                enqueue_chunk(len, guf, spr, ang, first, last, nullptr);
                 HIPCK(hipStreamEndCapture(stream_, &g));
-                hipGraphExec_t ex;
-                HIPCK(hipGraphInstantiate(&ex, g, nullptr, nullptr, 0));
+                HIPCK(hipGraphInstantiate(&rec.ex, g, nullptr, nullptr, 0));
                 HIPCK(hipGraphDestroy(g));
-                it = graphs_.emplace(key, ex).first;
+                it = graphs_.emplace(key, rec).first;
             }
-            HIPCK(hipGraphLaunch(it->second, stream_));
+            HIPCK(hipGraphLaunch(it->second.ex, stream_));
         } else {
-            enqueue_chunk(len, guf, spr, ang, first, last);
+            loose_.emplace_back();
+            enqueue_chunk(len, guf, spr, ang, first, last, timing_ ? &loose_.back() : nullptr);
+            if (timing_) used_.push_back(&loose_.back());
         }
         HIPCK(hipGetLastError());
         const int flips_p = (first ? 1 : 0) + len - (last ? 1 : 0);
@@ -1098,15 +1210,46 @@ class Engine final : public fn_engine {
 
     void clear_graphs()
     {
-        for (auto &kv : graphs_) (void)hipGraphExecDestroy(kv.second);
+        for (auto &kv : graphs_) {
+            (void)hipGraphExecDestroy(kv.second.ex);
+            kv.second.ev.release();
+        }
         graphs_.clear();
+        for (auto &e : loose_) e.release();
+        loose_.clear();
+        used_.clear();
     }
+
+    // After a synchronised run: fold the events of every chunk into the totals
+    // (a graph replayed several times holds the timings of its last replay).
+    void collect_timing()
+    {
+        if (!timing_) return;
+        std::vector<const Events *> seen;
+        for (const Events *e : used_)
+            if (std::find(seen.begin(), seen.end(), e) == seen.end()) {
+                seen.push_back(e);
+                accumulate(*e);
+            }
+        used_.clear();
+        for (auto &e : loose_) e.release();
+        loose_.clear();
+    }
+
+    struct GraphRec {
+        hipGraphExec_t ex = nullptr;
+        Events ev;
+    };
 
     hipStream_t stream_ = nullptr;
     std::vector<Buf> bufs_;
     int role_[10];
     std::map<std::string, KernelRec> kernels_;
-    std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, hipGraphExec_t> graphs_;
+    std::map<std::tuple<int, int, int, int, int, int, int, int, int, int>, GraphRec> graphs_;
+    std::list<Events> loose_;
+    std::vector<const Events *> used_;
+    bool timing_ = false;
+    fn_timing timing_acc_{};
 
     int N_ = 0, S_ = 0, A_ = 0, C_ = 0;
     int pc_ = 0, fc_ = 0;
@@ -1114,6 +1257,7 @@ class Engine final : public fn_engine {
     bool alias_ = false, pos_eq_img_ = false, csr_dirty_ = true, grid_valid_ = false, nb_init_ = false;
     bool cellnh_checked_ = false;
     int cap_ = 0, last_total_ = 0;
+    int K_ = 64;   // search-row length: entries one bead may collect before compaction
     long long steps_ = 0, rebuilds_ = 0;
 
     DArr<P> pos_[2][3], img_[2][3];
@@ -1121,12 +1265,15 @@ class Engine final : public fn_engine {
     DArr<T> lam_, hlf_, hlf2_, rad_;
     DArr<int> type_, tag_, loc_;
     DArr<double> rad_d_, skin_d_;
-    DArr<int> sp_p1_, sp_p2_, sp_type_, sp_status_, sp_break_, sp_start_, sp_ent_;
-    DArr<T> sp_l0_, sp_k_, sp_c1_, sp_c2_;
-    DArr<int> an_b1_, an_b2_, an_b3_, an_s1_, an_s2_, an_type_, an_status_, an_start_, an_ent_;
+    DArr<int> sp_p1_, sp_p2_, sp_type_, sp_status_, sp_break_, sp_start_, sp_ent_, sp_twin_;
+    DArr<double> sp_l0_, sp_c1_, sp_c2_;
+    DArr<T> sp_k_;
+    DArr<int> an_b1_, an_b2_, an_b3_, an_s1_, an_s2_, an_type_, an_status_, an_start_, an_ent_, an_owner_;
     DArr<T> an_th0_, an_k_;
     DArr<fnk::PairC> pairs_;
     DArr<int> key_, cell_count_, cell_start_, cursor_, sorted_, ncnt_, nb_start_, nb_list_, scan_tmp_, stats_, step_;
+    DArr<double> so_x_, so_y_, so_z_;
+    DArr<int> so_tag_, so_loc_, so_type_, so_slot_, scratch_;
 
     std::vector<fn_bead_info> h_info_;
     std::vector<fn_spring> h_springs_;
@@ -1286,6 +1433,19 @@ int fn_export_breaks(fn_engine *eng, int *break_step)
     return guarded(eng, [&] {
         if (!break_step) throw Error(FN_ERR_ARG, "null destination");
         eng->export_breaks(break_step);
+    });
+}
+
+int fn_set_timing(fn_engine *eng, int on)
+{
+    return guarded(eng, [&] { eng->set_timing(on != 0); });
+}
+
+int fn_get_timing(fn_engine *eng, fn_timing *out)
+{
+    return guarded(eng, [&] {
+        if (!out) throw Error(FN_ERR_ARG, "null destination");
+        eng->timing(out);
     });
 }
 

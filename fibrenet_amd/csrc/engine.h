@@ -7,7 +7,7 @@
 namespace fnk {
 
 // Slots of the device statistics array.
-enum { kStatMaxCell = 0, kStatCellsOver = 1, kStatMaxList = 2, kStatCount = 4 };
+enum { kStatMaxCell = 0, kStatCellsOver = 1, kStatMaxList = 2, kStatListOver = 3, kStatCount = 4 };
 
 template <typename T, typename P>
 struct Launch {
@@ -21,17 +21,23 @@ struct Launch {
     static void fused(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, hipStream_t s);
     static void hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *count,
                      hipStream_t s);
-    static void pairs(bool fill, const P *px, const P *py, const P *pz, const int *key, const int *sorted,
-                      const int *cell_start, int n_cells, int n, const int *type, const int *tag, const int *loc,
-                      const double *radius, const double *skin, const PairC *pairs, const Box &b, int *cnt,
-                      const int *nb_start, int *nb_list, int cap, int *stats, hipStream_t s);
+    static void gather_sorted(const int *sorted, const int *n_hashed, const P *px, const P *py, const P *pz,
+                              const int *type, const int *tag, const int *loc, const Sorted &so, int n,
+                              hipStream_t s);
 };
+
+// Neighbour search over the cell-sorted beads into scratch rows of K entries
+// per slot, then compaction into the bead-indexed CSR list.
+void search(const int *sorted, const int *cell_start, int n_cells, const int *key, const Sorted &so,
+            const double *radius, const double *skin, const PairC *pairs, const Box &b, int *cnt, int *scratch, int K,
+            int *stats, int n, hipStream_t s);
+void compact(const int *cnt, const int *nb_start, const int *slot_of, const int *scratch, int K, int n, int *nb_list,
+             int cap, hipStream_t s);
 
 void fill_int(int *p, int v, int n, hipStream_t s);
 void copy_int(const int *src, int *dst, int n, hipStream_t s);
 void add_int(int *p, int v, hipStream_t s);
 void scatter(const int *key, int n, int *cursor, int *sorted, hipStream_t s);
-void sort_cells(const int *start, int *sorted, int n_cells, int *stats, hipStream_t s);
 int scan_blocks_needed(int n);
 // out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total.  tmp holds
 // scan_blocks_needed(n) + 1 ints.

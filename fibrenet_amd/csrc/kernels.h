@@ -30,6 +30,13 @@ struct Box {
     double wall_k, hbs, cutoff_sq, damping, gx, gy, gz, dt;
 };
 
+// Bead data copied into cell order at a rebuild (the neighbour search reads
+// these contiguously) and the inverse permutation bead -> slot.
+struct Sorted {
+    double *x, *y, *z;
+    int *tag, *loc, *type, *slot_of;
+};
+
 // One pair-table entry (Pair.h:5-16): comparisons use the fp64 values.
 struct PairC {
     int type;
@@ -53,16 +60,21 @@ struct Args {
     // springs
     const int *sp_start, *sp_ent, *sp_p1, *sp_p2, *sp_type;
     int *sp_status, *sp_break;
-    const T *sp_l0, *sp_k, *sp_c1, *sp_c2;
+    const double *sp_l0, *sp_c1, *sp_c2;     // fp64: the failure test is decided as the reference decides it
+    const T *sp_k;
     int n_springs;
     // angles
     const int *an_start, *an_ent, *an_b1, *an_b2, *an_b3, *an_s1, *an_s2, *an_type;
     int *an_status;
     const T *an_th0, *an_k;
     int n_angles;
+    // index of the owner entry of the same bond: spring p2 entry -> p1 entry,
+    // angle b1/b3 entry -> b2 entry (bead-incidence lists)
+    const int *sp_twin, *an_owner;
     // contacts
     const int *nb_start, *nb_list;
     const PairC *pairs;
+    int n_pairs;
     // settings of AdvancePosition (integrator, walls) and ResolveContacts (minimum image)
     Box drift, force;
     int *step;                              // device step counter
@@ -162,21 +174,24 @@ __device__ __forceinline__ bool spring_eval(const Args<T, P> &a, int s, V3<T> &F
     dy -= a.iy[p2];
     dz -= a.iz[p2];
     const P d2 = dx * dx + dy * dy + dz * dz;
-    T l = (T)native_sqrt((double)d2);
-    T factor = a.sp_l0[s] - l;
-    if (l != (T)0) l = (T)1 / l;
-    T ux = (T)dx * l, uy = (T)dy * l, uz = (T)dz * l;
+    // length, extension and the HR_FL stress in fp64 from the float-rounded
+    // length (native_sqrt): the break decision is bit-exact in both builds
+    double l = (double)native_sqrt((double)d2);
+    const double factor = a.sp_l0[s] - l;
+    if (l != 0.0) l = 1.0 / l;
     if (ty == 1) {
-        T stress = -a.sp_c1[s] * factor * l;
+        const double stress = -a.sp_c1[s] * factor * l;
         if (fabs(stress) > a.sp_c2[s]) {
             broke = true;
             return false;
         }
     }
-    factor *= a.sp_k[s];
-    F.x = ux * factor;
-    F.y = uy * factor;
-    F.z = uz * factor;
+    const T il = (T)l;
+    const T ux = (T)dx * il, uy = (T)dy * il, uz = (T)dz * il;
+    const T fk = (T)factor * a.sp_k[s];
+    F.x = ux * fk;
+    F.y = uy * fk;
+    F.z = uz * fk;
     return true;
 }
 
@@ -193,44 +208,94 @@ __device__ __forceinline__ int spring_status_now(const Args<T, P> &a, int s)
 }
 
 // Angle bond forces on b1 (pa) and b3 (pc); b2 receives -pa-pc (kernel.cl:289-360).
+// Evaluated in fp64 in both builds and rounded to T at the end: the angle is
+// acos of the cosine, ill-conditioned at the straight fibre (d theta =
+// d cos / sin theta), and on an exactly straight segment the reference's
+// clamp of a cosine rounded below -1 to -0.999999 produces a finite force
+// whose direction comes from the fp64 cross product of the bond vectors.
 template <typename T, typename P>
-__device__ __forceinline__ void angle_eval(const Args<T, P> &a, int q, V3<T> &pa, V3<T> &pc)
+__device__ __forceinline__ void angle_eval(const Args<T, P> &a, int q, V3<T> &pa_out, V3<T> &pc_out)
 {
     const int b1 = a.an_b1[q], b2 = a.an_b2[q], b3 = a.an_b3[q];
     const P tx = a.ix[b2], ty = a.iy[b2], tz = a.iz[b2];
-    const T bax = (T)(a.ix[b1] - tx), bay = (T)(a.iy[b1] - ty), baz = (T)(a.iz[b1] - tz);
-    const T bcx = (T)(a.ix[b3] - tx), bcy = (T)(a.iy[b3] - ty), bcz = (T)(a.iz[b3] - tz);
-    const T cbx = bcx * (T)-1, cby = bcy * (T)-1, cbz = bcz * (T)-1;
-    const T l_ba = tsqrt(bax * bax + bay * bay + baz * baz);
-    const T l_bc = tsqrt(bcx * bcx + bcy * bcy + bcz * bcz);
-    T dotp = (bax * bcx + bay * bcy + baz * bcz) / (l_ba * l_bc);
-    if (dotp < (T)-1) dotp = (T)-0.999999;
-    if (dotp > (T)1) dotp = (T)0.999999;
-    const T ang = tacos(dotp);
-    const T mag = (T)-2 * a.an_k[q] * (ang - a.an_th0[q]);
-    const T fa = mag / l_ba;
-    const T fc = mag / l_bc;
-    const T cx = bay * bcz - baz * bcy;
-    const T cy = baz * bcx - bax * bcz;
-    const T cz = bax * bcy - bay * bcx;
-    pa.x = bay * cz - baz * cy;
-    pa.y = baz * cx - bax * cz;
-    pa.z = bax * cy - bay * cx;
-    T ln = tsqrt(pa.x * pa.x + pa.y * pa.y + pa.z * pa.z);
-    if (ln != (T)0) ln = (T)1 / ln;
+    const double bax = (double)(a.ix[b1] - tx), bay = (double)(a.iy[b1] - ty), baz = (double)(a.iz[b1] - tz);
+    const double bcx = (double)(a.ix[b3] - tx), bcy = (double)(a.iy[b3] - ty), bcz = (double)(a.iz[b3] - tz);
+    const double cbx = bcx * -1.0, cby = bcy * -1.0, cbz = bcz * -1.0;
+    const double l_ba = tsqrt(bax * bax + bay * bay + baz * baz);
+    const double l_bc = tsqrt(bcx * bcx + bcy * bcy + bcz * bcz);
+    double dotp = (bax * bcx + bay * bcy + baz * bcz) / (l_ba * l_bc);
+    if (dotp < -1.0) dotp = -0.999999;
+    if (dotp > 1.0) dotp = 0.999999;
+    const double ang = tacos(dotp);
+    const double mag = -2.0 * (double)a.an_k[q] * (ang - (double)a.an_th0[q]);
+    const double fa = mag / l_ba;
+    const double fc = mag / l_bc;
+    const double cx = bay * bcz - baz * bcy;
+    const double cy = baz * bcx - bax * bcz;
+    const double cz = bax * bcy - bay * bcx;
+    double pax = bay * cz - baz * cy, pay = baz * cx - bax * cz, paz = bax * cy - bay * cx;
+    double ln = tsqrt(pax * pax + pay * pay + paz * paz);
+    if (ln != 0.0) ln = 1.0 / ln;
     ln *= fa;
-    pa.x *= ln;
-    pa.y *= ln;
-    pa.z *= ln;
-    pc.x = cby * cz - cbz * cy;
-    pc.y = cbz * cx - cbx * cz;
-    pc.z = cbx * cy - cby * cx;
-    ln = tsqrt(pc.x * pc.x + pc.y * pc.y + pc.z * pc.z);
-    if (ln != (T)0) ln = (T)1 / ln;
+    pa_out.x = (T)(pax * ln);
+    pa_out.y = (T)(pay * ln);
+    pa_out.z = (T)(paz * ln);
+    double pcx = cby * cz - cbz * cy, pcy = cbz * cx - cbx * cz, pcz = cbx * cy - cby * cx;
+    ln = tsqrt(pcx * pcx + pcy * pcy + pcz * pcz);
+    if (ln != 0.0) ln = 1.0 / ln;
     ln *= fc;
-    pc.x *= ln;
-    pc.y *= ln;
-    pc.z *= ln;
+    pc_out.x = (T)(pcx * ln);
+    pc_out.y = (T)(pcy * ln);
+    pc_out.z = (T)(pcz * ln);
+}
+
+// One evaluated bond term as the beads of the bond consume it.
+template <typename T>
+struct SpTerm { T x, y, z; int on; };
+template <typename T>
+struct AnTerm { T ax, ay, az, cx, cy, cz; int on; };
+
+// Spring s evaluated by its owner (the p1 bead), which also records a break.
+template <typename T, typename P>
+__device__ __forceinline__ SpTerm<T> spring_owner(const Args<T, P> &a, int s)
+{
+    V3<T> F;
+    bool broke;
+    SpTerm<T> t;
+    t.on = spring_eval(a, s, F, broke) ? 1 : 0;
+    t.x = F.x; t.y = F.y; t.z = F.z;
+    if (broke) {
+        a.sp_status[s] = 0;
+        a.sp_break[s] = *a.step + a.step_off;
+    }
+    return t;
+}
+
+// Spring s evaluated by its p2 bead (no status write).
+template <typename T, typename P>
+__device__ __forceinline__ SpTerm<T> spring_partner(const Args<T, P> &a, int s)
+{
+    V3<T> F;
+    bool broke;
+    SpTerm<T> t;
+    t.on = spring_eval(a, s, F, broke) ? 1 : 0;
+    t.x = F.x; t.y = F.y; t.z = F.z;
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ void add_spring(V3<T> &f, const SpTerm<T> &t, bool p2)
+{
+    if (!t.on) return;
+    if (p2) {
+        f.x += t.x * (T)-1;
+        f.y += t.y * (T)-1;
+        f.z += t.z * (T)-1;
+    } else {
+        f.x += t.x;
+        f.y += t.y;
+        f.z += t.z;
+    }
 }
 
 // Adds the springs incident to bead i to f (kernel.cl:419-437 seen from bead i).
@@ -242,27 +307,58 @@ __device__ __forceinline__ void gather_springs(const Args<T, P> &a, int i, V3<T>
         const int ent = a.sp_ent[e];
         const int s = ent >> 1;
         if (s >= a.n_springs) continue;
-        V3<T> F;
-        bool broke;
-        if (spring_eval(a, s, F, broke)) {
-            if (ent & 1) {
-                f.x += F.x * (T)-1;
-                f.y += F.y * (T)-1;
-                f.z += F.z * (T)-1;
-            } else {
-                f.x += F.x;
-                f.y += F.y;
-                f.z += F.z;
-            }
-        } else if (broke && !(ent & 1)) {
-            a.sp_status[s] = 0;
-            a.sp_break[s] = *a.step + a.step_off;
-        }
+        add_spring(f, (ent & 1) ? spring_partner(a, s) : spring_owner(a, s), ent & 1);
     }
 }
 
-// Adds the angle bonds incident to bead i.  `recheck` re-derives this step's
-// spring statuses (fused path, where the spring pass runs in the same launch).
+// Angle q with this step's HR_FL rule (kernel.cl:274-286): an HR_FL angle dies
+// with either spring.  `recheck` re-derives this step's spring statuses (fused
+// path, where the spring pass runs in the same launch); the owner (b2) records
+// the death.
+template <typename T, typename P, bool recheck>
+__device__ __forceinline__ AnTerm<T> angle_term(const Args<T, P> &a, int q, bool owner)
+{
+    AnTerm<T> t;
+    int st = a.an_status[q];
+    if (a.an_type[q] == 1 && st == 1) {
+        const int s1 = a.an_s1[q], s2 = a.an_s2[q];
+        const int st1 = recheck ? spring_status_now(a, s1) : a.sp_status[s1];
+        const int st2 = recheck ? spring_status_now(a, s2) : a.sp_status[s2];
+        if (st1 == 0 || st2 == 0) {
+            st = 0;
+            if (owner) a.an_status[q] = 0;
+        }
+    }
+    t.on = st == 1 ? 1 : 0;
+    if (t.on) {
+        V3<T> pa, pc;
+        angle_eval(a, q, pa, pc);
+        t.ax = pa.x; t.ay = pa.y; t.az = pa.z;
+        t.cx = pc.x; t.cy = pc.y; t.cz = pc.z;
+    }
+    return t;
+}
+
+template <typename T>
+__device__ __forceinline__ void add_angle(V3<T> &f, const AnTerm<T> &t, int role)
+{
+    if (!t.on) return;
+    if (role == 0) {
+        f.x += t.ax;
+        f.y += t.ay;
+        f.z += t.az;
+    } else if (role == 2) {
+        f.x += t.cx;
+        f.y += t.cy;
+        f.z += t.cz;
+    } else {
+        f.x += -t.ax - t.cx;
+        f.y += -t.ay - t.cy;
+        f.z += -t.az - t.cz;
+    }
+}
+
+// Adds the angle bonds incident to bead i.
 template <typename T, typename P, bool recheck>
 __device__ __forceinline__ void gather_angles(const Args<T, P> &a, int i, V3<T> &f)
 {
@@ -272,32 +368,7 @@ __device__ __forceinline__ void gather_angles(const Args<T, P> &a, int i, V3<T> 
         const int q = ent >> 2;
         const int role = ent & 3;
         if (q >= a.n_angles) continue;
-        int st = a.an_status[q];
-        if (a.an_type[q] == 1 && st == 1) {
-            const int s1 = a.an_s1[q], s2 = a.an_s2[q];
-            const int st1 = recheck ? spring_status_now(a, s1) : a.sp_status[s1];
-            const int st2 = recheck ? spring_status_now(a, s2) : a.sp_status[s2];
-            if (st1 == 0 || st2 == 0) {
-                st = 0;
-                if (role == 1) a.an_status[q] = 0;
-            }
-        }
-        if (st != 1) continue;
-        V3<T> pa, pc;
-        angle_eval(a, q, pa, pc);
-        if (role == 0) {
-            f.x += pa.x;
-            f.y += pa.y;
-            f.z += pa.z;
-        } else if (role == 2) {
-            f.x += pc.x;
-            f.y += pc.y;
-            f.z += pc.z;
-        } else {
-            f.x += -pa.x - pc.x;
-            f.y += -pa.y - pc.y;
-            f.z += -pa.z - pc.z;
-        }
+        add_angle(f, angle_term<T, P, recheck>(a, q, role == 1), role);
     }
 }
 
@@ -305,7 +376,7 @@ __device__ __forceinline__ void gather_angles(const Args<T, P> &a, int i, V3<T> 
 // from either bead of a pair: the reaction on the partner is the exact
 // negation of the owner's term, so each side evaluates rij = p_i - p_j).
 template <typename T, typename P>
-__device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T> &f)
+__device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T> &f, const PairC *table)
 {
     const int e0 = a.nb_start[i], e1 = a.nb_start[i + 1];
     if (e0 == e1) return;
@@ -315,7 +386,7 @@ __device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T
     for (int e = e0; e < e1; ++e) {
         const int ent = a.nb_list[e];
         const int j = ent & kIdxMask;
-        const PairC pr = a.pairs[ent >> kPidShift];
+        const PairC pr = table[ent >> kPidShift];
         P rx = x1 - a.px[j], ry = y1 - a.py[j], rz = z1 - a.pz[j];
         P len = rx * rx + ry * ry + rz * rz;
         if (len > (P)b.hbs) {

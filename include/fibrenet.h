@@ -128,6 +128,19 @@ int fn_finish(fn_engine *eng);
  * per-kernel launches; returns after the work is enqueued and checked. */
 int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_springs, int run_angles);
 
+/* Kernel timing of the fast path (no reference counterpart): with timing on,
+ * fn_run_steps launches its kernels directly (not through graphs) and records
+ * HIP events around every fused step kernel and every grid rebuild on the
+ * engine's stream; fn_get_timing sums them since timing was switched on. */
+typedef struct fn_timing {
+    int fused_launches;            /* fused step kernels measured              */
+    int rebuilds;                  /* rebuild sequences measured               */
+    double fused_ms;               /* their summed durations                   */
+    double rebuild_ms;
+} fn_timing;
+int fn_set_timing(fn_engine *eng, int on);
+int fn_get_timing(fn_engine *eng, fn_timing *out);
+
 /* Debug exports for parity checks (no reference counterpart). */
 /* Cell index of every bead at the last rebuild (-1: outside the grid). */
 int fn_export_cells(fn_engine *eng, int *cell_of_bead);

@@ -3,10 +3,13 @@
 Tolerances, stated per build:
   f64  - every bead field within 1e-9 of the oracle, relative to the field's
          largest magnitude (floor 1.0).  The builds differ only in the order
-         contact forces are summed and in acos/pow ulps.
-  f32  - positions/images within 1e-5 (absolute, coordinates are O(10)),
-         velocities and forces within 2e-3 of the field's largest magnitude:
-         fp32 force arithmetic (ulp 6e-8) amplified by the stiff contacts.
+         contact forces are summed and in acos/pow ulps (measured: <= 3e-14).
+  f32  - one step from a shared state: forces within 2e-5 of the largest force
+         (fp32 arithmetic, ulp 6e-8, through LJ powers).  After 200 steps:
+         positions/images within 1e-5 absolute (coordinates are O(10)),
+         velocities within 2e-3 and forces within 1e-2 of the field's largest
+         magnitude - the fp32 trajectory drifts from the fp64 one by ~1e-6 in
+         position and LJ near-contacts (r ~ 0.6 sigma) turn that into force.
 Cell indices, contact-pair sets and spring-break steps are compared exactly.
 The fused (hipGraph) path and the per-kernel path are compared bit for bit.
 """
@@ -21,8 +24,8 @@ pytestmark = pytest.mark.gpu
 
 TOL = {"f64": {f: 1e-9 for f in ("position", "image_position", "velocity", "force", "velocity_old",
                                  "force_old")},
-       "f32": {"position": 1e-5, "image_position": 1e-5, "velocity": 2e-3, "force": 2e-3,
-               "velocity_old": 2e-3, "force_old": 2e-3}}
+       "f32": {"position": 1e-5, "image_position": 1e-5, "velocity": 2e-3, "force": 1e-2,
+               "velocity_old": 2e-3, "force_old": 1e-2}}
 
 WORKLOADS = {"periodic_hs": lambda: small_periodic(), "tensile_lj": lambda: small_tensile(),
              "tensile_lj_breaking": lambda: small_tensile(weak=True)}
@@ -81,6 +84,25 @@ def test_cells_pairs_breaks_exact_over_1000_steps(need_gpu, workload):
     assert np.array_equal(eng.export_breaks(len(net.springs)), orc.break_step)
     print(workload, "rebuilds", n_rebuilds, "breaks", int((orc.break_step >= 0).sum()),
           "pairs", len(orc.pair_set()))
+
+
+@pytest.mark.parametrize("workload", sorted(WORKLOADS))
+def test_f32_single_step_forces(need_gpu, workload):
+    """From a shared state (the oracle after 50 steps), one f32 step gives the
+    oracle's forces to fp32 rounding."""
+    net = WORKLOADS[workload]()
+    orc = oracle_for(net)
+    orc.run(50, net.grid_update_freq)
+    net.beads = orc.beads.copy()
+    net.springs = orc.springs.copy()
+    net.angles = orc.angles.copy()
+    eng, ids = engine_for(net, "f32")
+    run_engine(eng, net, 1, fast=True)
+    orc.run(1, net.grid_update_freq)
+    got = read_beads(eng, ids)
+    err = field_errors(got, orc.beads)
+    print(workload, err)
+    assert err["force"] <= 2e-5 and err["position"] <= 1e-12 + 1e-7
 
 
 def test_stagewise_f32_rebuild_exact(need_gpu):
