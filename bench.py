@@ -6,14 +6,15 @@ fibres of 100 beads in a finite 1000x1000x50 box, HR_FL springs (k 100,
 E 1000, failure stress uniform in [50,150]), HR angles (k 10), Lennard-Jones
 contacts (eps 1, sigma 1, cutoff 2.5) between all three bead types, beads
 with x<5 clamped and x>995 pulled at +0.01, dt 5e-4, grid rebuild every 10
-steps.  A step is one pass of the fused step kernel over all beads (plus the
-grid rebuild every 10th step); inputs are resident in HBM when the timed
-region starts.
+steps.  A step is the spring-term, angle-term and bead kernels over the whole
+network (plus the grid rebuild every 10th step); inputs are resident in HBM
+when the timed region starts.
 
 Prints one JSON line: bead-steps/s of the timed steps, the roofline of the
-dominant kernel (the fused step kernel; its duration measured with HIP events
-on the engine's stream over the timed region) and a CPU baseline (the
-sequential fp64 oracle on the host, a bounded sample of the same network).
+dominant kernel (the step kernel with the largest duration, measured with
+HIP events on the engine's stream over the timed region) and a CPU baseline
+(the sequential fp64 oracle on the host, a bounded sample of the same
+network).
 """
 import argparse
 import json
@@ -27,20 +28,37 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(net, pairs):
-    """Bytes per step of the step kernel by the per-unit figures: 100 B per
-    bead (position, image, velocity_old and force read and written once in
-    fp32, one mass coefficient), 16 B per HR spring, 28 B per HR_FL spring,
-    20 B per angle, 8 B per listed pair; plus, per rebuild, 36 B per bead
-    and 8 B per pair."""
+def algorithmic_bytes(net, pairs, tb, aliased):
+    """HBM bytes each launch of a step must move at least (every datum read or
+    written once; tb = bytes of a force/velocity component, positions fp64):
+
+    spring-term kernel: per spring p1, p2, type, status (16 B), rest length
+      (8 B), stiffness (tb), the two failure constants of an HR_FL spring
+      (16 B) and the term written (3 tb); per bead its image (24 B).
+    angle-term kernel: per angle b1, b2, b3, s1, s2, type, status (28 B),
+      rest angle and stiffness (2 tb), the two terms written (6 tb); per
+      spring its status (4 B); per bead its image (24 B).
+    bead kernel: per bead position read and next position written (48 B),
+      image read and written unless it is the position array (48 B), force
+      of the previous step, velocity_old read and written, this step's force
+      written, three mass coefficients (12 tb), three list offsets (12 B);
+      per spring two incidence entries and its term (8 B + 3 tb), per angle
+      three entries and its terms (12 B + 6 tb), per listed pair two list
+      entries (8 B).
+    rebuild: per bead position read, key, slot and the 32 B packed record
+      written (56 B), per listed pair two entries (8 B)."""
     import numpy as np
     n = len(net.beads)
     s_hr = int(np.sum(net.springs["type"] == 0))
     s_fl = int(np.sum(net.springs["type"] == 1))
+    s = s_hr + s_fl
     a = len(net.angles)
-    step = 100 * n + 16 * s_hr + 28 * s_fl + 20 * a + 8 * pairs
-    rebuild = 36 * n + 8 * pairs
-    return step, rebuild, dict(beads=n, springs_hr=s_hr, springs_hr_fl=s_fl, angles=a, pairs=pairs)
+    spring = (16 + 8 + tb + 3 * tb) * s + 16 * s_fl + 24 * n if s else 0
+    angle = (28 + 2 * tb + 6 * tb) * a + 4 * s + 24 * n if a else 0
+    bead = (48 + (0 if aliased else 48) + 12 * tb + 12) * n + (8 + 3 * tb) * s + (12 + 6 * tb) * a + 8 * pairs
+    rebuild = 56 * n + 8 * pairs
+    return (dict(spring=spring, angle=angle, bead=bead), rebuild,
+            dict(beads=n, springs_hr=s_hr, springs_hr_fl=s_fl, angles=a, pairs=pairs))
 
 
 def cpu_baseline(net, steps):
@@ -95,7 +113,7 @@ def main():
     print("bench: %s, warmup %d done, timing %d steps" % (desc, args.warmup, args.steps), file=sys.stderr,
           flush=True)
 
-    # Timed region: HIP events around every fused step kernel and rebuild on the
+    # Timed region: HIP events around every step kernel and rebuild on the
     # engine's stream (kernels launched directly, not through graphs).
     eng.set_timing(True)
     eng.finish()
@@ -114,10 +132,15 @@ def main():
 
     n = len(net.beads)
     pairs = st["list_entries"] // 2
-    b_step, b_rebuild, counts = algorithmic_bytes(net, pairs)
-    fused_ms = tim["fused_ms"] / max(tim["fused_launches"], 1)
+    aliased = bool(st["img_aliases_pos"])
+    b_kernel, b_rebuild, counts = algorithmic_bytes(net, pairs, 4 if args.precision == "f32" else 8, aliased)
+    n_t = max(tim["steps"], 1)
+    kern_ms = {"spring": tim["spring_ms"] / n_t, "angle": tim["angle_ms"] / n_t, "bead": tim["bead_ms"] / n_t}
+    names = {"spring": "k_spring_terms", "angle": "k_angle_terms", "bead": "k_bead_step"}
+    dom = max(kern_ms, key=kern_ms.get)
     rebuild_ms = tim["rebuild_ms"] / max(tim["rebuilds"], 1)
-    achieved = b_step / (fused_ms * 1e-3) / 1e9
+    achieved = b_kernel[dom] / (kern_ms[dom] * 1e-3) / 1e9
+    b_step = sum(b_kernel.values())
     value = n * args.steps / wall
 
     copy = None
@@ -133,7 +156,7 @@ def main():
         copy = nb * 4
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_fused_step_%s_%s.json" % (args.workload, args.precision))
+    tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (names[dom], args.workload, args.precision))
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
@@ -157,10 +180,14 @@ def main():
                    "listed_pairs": pairs, "grid_update_freq": net.grid_update_freq, "precision": args.precision},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "kernel": "k_fused_step", "algorithmic_bytes_per_launch": b_step,
-                     "avg_launch_ms": fused_ms, "launches_timed": tim["fused_launches"]},
+                     "kernel": names[dom], "algorithmic_bytes_per_launch": b_kernel[dom],
+                     "avg_launch_ms": kern_ms[dom], "launches_timed": tim["steps"]},
         "cpu_baseline": cpu,
-        "extra": {"graph_mode_bead_steps_per_s": n * args.steps / wall_graph,
+        "extra": {"kernels": {names[k]: {"avg_ms": kern_ms[k], "algorithmic_bytes": b_kernel[k],
+                                          "GBps": b_kernel[k] / (kern_ms[k] * 1e-3) / 1e9 if kern_ms[k] > 0 else None}
+                              for k in kern_ms},
+                  "step_kernels_ms": tim["step_ms"] / n_t,
+                  "graph_mode_bead_steps_per_s": n * args.steps / wall_graph,
                   "graph_mode_ms_per_step": wall_graph * 1e3 / args.steps,
                   "rebuild_ms": rebuild_ms, "rebuild_algorithmic_bytes": b_rebuild,
                   "step_algorithmic_GBps_incl_rebuild": (b_step + b_rebuild / net.grid_update_freq)

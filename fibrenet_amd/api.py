@@ -42,6 +42,8 @@ SIGNATURES = [
     ("fn_export_pairs", _I, [_VP, _VP, _LL, _c.POINTER(_LL)]),
     ("fn_export_breaks", _I, [_VP, _VP]),
     ("fn_get_stats", _I, [_VP, _VP]),
+    ("fn_set_option", _I, [_VP, _CP, _I]),
+    ("fn_get_option", _I, [_VP, _CP, _c.POINTER(_I)]),
     ("fn_set_timing", _I, [_VP, _I]),
     ("fn_get_timing", _I, [_VP, _VP]),
     ("fn_model_load", _I, [_CP, _c.POINTER(_VP)]),
@@ -71,7 +73,8 @@ class FnStats(ctypes.Structure):
 
 
 class FnTiming(ctypes.Structure):
-    _fields_ = [("fused_launches", _I), ("rebuilds", _I), ("fused_ms", _c.c_double), ("rebuild_ms", _c.c_double)]
+    _fields_ = [("steps", _I), ("rebuilds", _I), ("step_ms", _c.c_double), ("rebuild_ms", _c.c_double),
+                ("spring_ms", _c.c_double), ("angle_ms", _c.c_double), ("bead_ms", _c.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -249,6 +252,14 @@ class Engine:
         s = FnStats()
         self._chk(lib().fn_get_stats(self.h, ctypes.byref(s)))
         return s.as_dict()
+
+    def set_option(self, name, value):
+        self._chk(lib().fn_set_option(self.h, name.encode(), int(value)))
+
+    def get_option(self, name):
+        v = ctypes.c_int()
+        self._chk(lib().fn_get_option(self.h, name.encode(), ctypes.byref(v)))
+        return v.value
 
     def set_timing(self, on):
         self._chk(lib().fn_set_timing(self.h, int(bool(on))))

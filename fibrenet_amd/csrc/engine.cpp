@@ -211,6 +211,8 @@ struct fn_engine {
     virtual void export_breaks(int *out) = 0;
     virtual void stats(fn_stats *out) = 0;
     virtual void set_timing(bool on) = 0;
+    virtual void set_option(const std::string &name, int value) = 0;
+    virtual int get_option(const std::string &name) = 0;
     virtual void timing(fn_timing *out) = 0;
 };
 
@@ -476,6 +478,11 @@ class Engine final : public fn_engine {
         ensure_grid(&settings_of(K_FILL_GRID));
         ensure_grid(&settings_of(K_BUILD_LIST));
         ensure_scratch();
+        const double r2 = reach2(settings_of(K_BUILD_LIST));
+        if (r2 != reach2_) {
+            reach2_ = r2;
+            clear_graphs();
+        }
         // capacity of the contact list: grow before the run if the last rebuild came close
         if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
         fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
@@ -564,17 +571,33 @@ class Engine final : public fn_engine {
 
     void timing(fn_timing *out) override { *out = timing_acc_; }
 
+    void set_option(const std::string &name, int value) override
+    {
+        if (name == "chunk") chunk_ = value;
+        else if (name == "graphs") graphs_on_ = value != 0;
+        else throw Error(FN_ERR_ARG, "unknown option " + name);
+        clear_graphs();
+    }
+
+    int get_option(const std::string &name) override
+    {
+        if (name == "chunk") return chunk_;
+        if (name == "graphs") return graphs_on_ ? 1 : 0;
+        throw Error(FN_ERR_ARG, "unknown option " + name);
+    }
+
   private:
     static int C_of(const fn_settings &s) { return s.cells_x * s.cells_y * s.cells_z; }
 
-    // Events of one captured chunk: pairs around each fused kernel / rebuild.
+    // Events of one chunk: four per step (before the spring terms, before the
+    // angle terms, before the bead kernel, after it) and a pair per rebuild.
     struct Events {
-        std::vector<hipEvent_t> fused, rebuild;
+        std::vector<hipEvent_t> step, rebuild;
         void release()
         {
-            for (auto e : fused) (void)hipEventDestroy(e);
+            for (auto e : step) (void)hipEventDestroy(e);
             for (auto e : rebuild) (void)hipEventDestroy(e);
-            fused.clear();
+            step.clear();
             rebuild.clear();
         }
     };
@@ -595,11 +618,14 @@ class Engine final : public fn_engine {
 
     void accumulate(const Events &ev)
     {
-        for (size_t k = 0; k + 1 < ev.fused.size(); k += 2) {
-            float ms = 0.f;
-            HIPCK(hipEventElapsedTime(&ms, ev.fused[k], ev.fused[k + 1]));
-            timing_acc_.fused_ms += ms;
-            timing_acc_.fused_launches++;
+        for (size_t k = 0; k + 3 < ev.step.size(); k += 4) {
+            float ms[3] = {0.f, 0.f, 0.f};
+            for (int j = 0; j < 3; ++j) HIPCK(hipEventElapsedTime(&ms[j], ev.step[k + j], ev.step[k + j + 1]));
+            timing_acc_.spring_ms += ms[0];
+            timing_acc_.angle_ms += ms[1];
+            timing_acc_.bead_ms += ms[2];
+            timing_acc_.step_ms += (double)ms[0] + ms[1] + ms[2];
+            timing_acc_.steps++;
         }
         for (size_t k = 0; k + 1 < ev.rebuild.size(); k += 2) {
             float ms = 0.f;
@@ -774,6 +800,7 @@ class Engine final : public fn_engine {
         type_.put(ty, stream_); tag_.put(tg, stream_); loc_.put(lc, stream_);
         rad_d_.put(rd, stream_); skin_d_.put(sk, stream_);
         HIPCK(hipStreamSynchronize(stream_));
+        clear_graphs();
     }
 
     void upload_springs()
@@ -790,6 +817,7 @@ class Engine final : public fn_engine {
         sp_p1_.put(p1, stream_); sp_p2_.put(p2, stream_); sp_type_.put(ty, stream_); sp_status_.put(st, stream_);
         sp_break_.put(br, stream_);
         sp_l0_.put(l0, stream_); sp_k_.put(k, stream_); sp_c1_.put(c1, stream_); sp_c2_.put(c2, stream_);
+        for (auto &d : sp_f_) d.ensure(n);
         HIPCK(hipStreamSynchronize(stream_));
         csr_dirty_ = true;
         clear_graphs();
@@ -808,6 +836,8 @@ class Engine final : public fn_engine {
         an_b1_.put(b1, stream_); an_b2_.put(b2, stream_); an_b3_.put(b3, stream_);
         an_s1_.put(s1, stream_); an_s2_.put(s2, stream_); an_type_.put(ty, stream_); an_status_.put(st, stream_);
         an_th0_.put(th, stream_); an_k_.put(k, stream_);
+        for (auto &d : an_pa_) d.ensure(n);
+        for (auto &d : an_pc_) d.ensure(n);
         HIPCK(hipStreamSynchronize(stream_));
         csr_dirty_ = true;
         clear_graphs();
@@ -826,18 +856,13 @@ class Engine final : public fn_engine {
         }
         std::vector<int> st(n + 1, 0), ent(2 * (size_t)S_ + 1);
         for (int i = 0; i < n; ++i) st[i + 1] = st[i] + cnt[i];
-        std::vector<int> cur(st.begin(), st.end() - 1), twin(2 * (size_t)S_ + 1, -1);
+        std::vector<int> cur(st.begin(), st.end() - 1);
         for (int s = 0; s < S_; ++s) {
-            const int e1 = cur[h_springs_[s].p1]++;
-            ent[e1] = s << 1;
-            const int e2 = cur[h_springs_[s].p2]++;
-            ent[e2] = (s << 1) | 1;
-            twin[e1] = e1;
-            twin[e2] = e1;
+            ent[cur[h_springs_[s].p1]++] = s << 1;
+            ent[cur[h_springs_[s].p2]++] = (s << 1) | 1;
         }
         sp_start_.put(st, stream_);
         sp_ent_.put(ent, stream_);
-        sp_twin_.put(twin, stream_);
         std::fill(cnt.begin(), cnt.end(), 0);
         for (const fn_angle &a : h_angles_) {
             for (int b : {a.b1, a.b2, a.b3})
@@ -848,19 +873,14 @@ class Engine final : public fn_engine {
         }
         std::vector<int> as(n + 1, 0), aent(3 * (size_t)A_ + 1);
         for (int i = 0; i < n; ++i) as[i + 1] = as[i] + cnt[i];
-        std::vector<int> acur(as.begin(), as.end() - 1), owner(3 * (size_t)A_ + 1, -1);
+        std::vector<int> acur(as.begin(), as.end() - 1);
         for (int q = 0; q < A_; ++q) {
-            const int e1 = acur[h_angles_[q].b1]++;
-            aent[e1] = (q << 2) | 0;
-            const int e3 = acur[h_angles_[q].b3]++;
-            aent[e3] = (q << 2) | 2;
-            const int e2 = acur[h_angles_[q].b2]++;
-            aent[e2] = (q << 2) | 1;
-            owner[e1] = owner[e2] = owner[e3] = e2;
+            aent[acur[h_angles_[q].b1]++] = (q << 2) | 0;
+            aent[acur[h_angles_[q].b3]++] = (q << 2) | 2;
+            aent[acur[h_angles_[q].b2]++] = (q << 2) | 1;
         }
         an_start_.put(as, stream_);
         an_ent_.put(aent, stream_);
-        an_owner_.put(owner, stream_);
         HIPCK(hipStreamSynchronize(stream_));
         csr_dirty_ = false;
         clear_graphs();
@@ -923,7 +943,9 @@ class Engine final : public fn_engine {
         a.n_angles = A_;
         a.nb_start = nb_start_.p; a.nb_list = nb_list_.p; a.pairs = pairs_.p;
         a.n_pairs = (int)h_pairs_.size();
-        a.sp_twin = sp_twin_.p; a.an_owner = an_owner_.p;
+        a.sfx = sp_f_[0].p; a.sfy = sp_f_[1].p; a.sfz = sp_f_[2].p;
+        a.apx = an_pa_[0].p; a.apy = an_pa_[1].p; a.apz = an_pa_[2].p;
+        a.acx = an_pc_[0].p; a.acy = an_pc_[1].p; a.acz = an_pc_[2].p;
         a.step = step_.p;
         return a;
     }
@@ -1044,19 +1066,34 @@ class Engine final : public fn_engine {
         fnk::copy_int(cell_start_.p, cursor_.p, C_, stream_);
         fnk::scatter(key_.p, N_, cursor_.p, sorted_.p, stream_);
         fnk::Launch<T, P>::gather_sorted(sorted_.p, cell_start_.p + C_, pos_[pc_][0].p, pos_[pc_][1].p,
-                                         pos_[pc_][2].p, type_.p, tag_.p, loc_.p, sorted_view(), N_, stream_);
+                                         pos_[pc_][2].p, type_.p, tag_.p, loc_.p, key_.p, sorted_view(), N_, stream_);
         grid_valid_ = true;
     }
 
     fnk::Sorted sorted_view()
     {
-        return fnk::Sorted{so_x_.p, so_y_.p, so_z_.p, so_tag_.p, so_loc_.p, so_type_.p, so_slot_.p};
+        return fnk::Sorted{so_bead_.p, so_type_.p, so_cell_.p, so_slot_.p};
+    }
+
+    // Largest squared listing radius over the pair types in use: cutoff^2 for
+    // lj and st, (max radius + max skin)^2 for hs (kernel.cl:632-659).
+    double reach2(const fn_settings &s) const
+    {
+        double r2 = 0.0, maxr = 0.0, maxs = 0.0;
+        for (const fn_bead_info &q : h_info_) {
+            maxr = std::max(maxr, q.radius);
+            maxs = std::max(maxs, q.skin_radius);
+        }
+        for (const fn_pair &p : h_pairs_) {
+            if (p.type == 1 || p.type == 3) r2 = std::max(r2, s.cutoff_sq);
+            if (p.type == 2) r2 = std::max(r2, (maxr + maxs) * (maxr + maxs));
+        }
+        return r2;
     }
 
     void ensure_scratch()
     {
-        so_x_.ensure(N_); so_y_.ensure(N_); so_z_.ensure(N_);
-        so_tag_.ensure(N_); so_loc_.ensure(N_); so_type_.ensure(N_); so_slot_.ensure(N_);
+        so_bead_.ensure(N_); so_type_.ensure(N_); so_cell_.ensure(N_); so_slot_.ensure(N_);
         if (scratch_.n < (size_t)N_ * K_) {
             clear_graphs();
             scratch_.alloc((size_t)N_ * K_);
@@ -1070,7 +1107,7 @@ class Engine final : public fn_engine {
         for (;;) {
             if (exact) fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
             fnk::fill_int(ncnt_.p, 0, N_, stream_);
-            fnk::search(sorted_.p, cell_start_.p, C_, key_.p, sorted_view(), rad_d_.p, skin_d_.p, pairs_.p, b,
+            fnk::search(sorted_.p, cell_start_.p, C_, sorted_view(), rad_d_.p, skin_d_.p, pairs_.p, b, reach2_,
                         ncnt_.p, scratch_.p, K_, stats_.p, N_, stream_);
             fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, stream_);
             if (exact) {
@@ -1100,6 +1137,7 @@ class Engine final : public fn_engine {
     {
         ensure_grid(&settings_of(K_BUILD_LIST));
         ensure_scratch();
+        reach2_ = reach2(settings_of(K_BUILD_LIST));
         search_and_compact(b, exact);
     }
 
@@ -1124,9 +1162,7 @@ class Engine final : public fn_engine {
     // ---- fused chunks -----------------------------------------------------------------------------
     int chunk_len(int guf) const
     {
-        const char *e = getenv("FIBRENET_CHUNK");
-        int target = e ? atoi(e) : 128;
-        if (target < 1) target = 1;
+        const int target = chunk_ < 1 ? 1 : chunk_;
         return guf * std::max(1, target / guf);
     }
 
@@ -1155,7 +1191,8 @@ class Engine final : public fn_engine {
                 fnk::copy_int(cell_start_.p, cursor_.p, C_, stream_);
                 fnk::scatter(key_.p, N_, cursor_.p, sorted_.p, stream_);
                         fnk::Launch<T, P>::gather_sorted(sorted_.p, cell_start_.p + C_, pos_[pc][0].p, pos_[pc][1].p,
-                                                 pos_[pc][2].p, type_.p, tag_.p, loc_.p, sorted_view(), N_, stream_);
+                                                 pos_[pc][2].p, type_.p, tag_.p, loc_.p, key_.p, sorted_view(), N_,
+                                                 stream_);
                 search_and_compact(bl, false);
                 mark(ev ? &ev->rebuild : nullptr);
             }
@@ -1169,9 +1206,14 @@ class Engine final : public fn_engine {
             set_pos(a, pc);
             a.fx = frc_[fc][0].p; a.fy = frc_[fc][1].p; a.fz = frc_[fc][2].p;
             a.gx = frc_[1 - fc][0].p; a.gy = frc_[1 - fc][1].p; a.gz = frc_[1 - fc][2].p;
-            mark(ev ? &ev->fused : nullptr);
-            fnk::Launch<T, P>::fused(a, spr, ang, !is_last, is_last, stream_);
-            mark(ev ? &ev->fused : nullptr);
+            std::vector<hipEvent_t> *sv = ev ? &ev->step : nullptr;
+            mark(sv);
+            if (spr) fnk::Launch<T, P>::spring_terms(a, stream_);
+            mark(sv);
+            if (ang) fnk::Launch<T, P>::angle_terms(a, stream_);
+            mark(sv);
+            fnk::Launch<T, P>::bead_step(a, spr, ang, !is_last, is_last, stream_);
+            mark(sv);
             fc ^= 1;
             if (!is_last) pc ^= 1;
         }
@@ -1182,7 +1224,7 @@ class Engine final : public fn_engine {
     {
         const auto key = std::make_tuple(len, guf, (int)spr, (int)ang, (int)first, (int)last, pc_, fc_, n_spr_, n_ang_);
         // Timed runs launch directly: HIP cannot time events recorded as graph nodes.
-        const bool use_graph = !getenv("FIBRENET_NO_GRAPH") && !timing_;
+        const bool use_graph = graphs_on_ && !timing_;
         if (use_graph) {
             auto it = graphs_.find(key);
             if (it == graphs_.end()) {
@@ -1258,6 +1300,8 @@ class Engine final : public fn_engine {
     bool cellnh_checked_ = false;
     int cap_ = 0, last_total_ = 0;
     int K_ = 64;   // search-row length: entries one bead may collect before compaction
+    int chunk_ = 128;                                // steps per captured graph
+    bool graphs_on_ = true;
     long long steps_ = 0, rebuilds_ = 0;
 
     DArr<P> pos_[2][3], img_[2][3];
@@ -1265,15 +1309,17 @@ class Engine final : public fn_engine {
     DArr<T> lam_, hlf_, hlf2_, rad_;
     DArr<int> type_, tag_, loc_;
     DArr<double> rad_d_, skin_d_;
-    DArr<int> sp_p1_, sp_p2_, sp_type_, sp_status_, sp_break_, sp_start_, sp_ent_, sp_twin_;
+    DArr<int> sp_p1_, sp_p2_, sp_type_, sp_status_, sp_break_, sp_start_, sp_ent_;
     DArr<double> sp_l0_, sp_c1_, sp_c2_;
     DArr<T> sp_k_;
-    DArr<int> an_b1_, an_b2_, an_b3_, an_s1_, an_s2_, an_type_, an_status_, an_start_, an_ent_, an_owner_;
+    DArr<int> an_b1_, an_b2_, an_b3_, an_s1_, an_s2_, an_type_, an_status_, an_start_, an_ent_;
     DArr<T> an_th0_, an_k_;
+    DArr<T> sp_f_[3], an_pa_[3], an_pc_[3];   // per-bond terms of the current step
     DArr<fnk::PairC> pairs_;
     DArr<int> key_, cell_count_, cell_start_, cursor_, sorted_, ncnt_, nb_start_, nb_list_, scan_tmp_, stats_, step_;
-    DArr<double> so_x_, so_y_, so_z_;
-    DArr<int> so_tag_, so_loc_, so_type_, so_slot_, scratch_;
+    DArr<fnk::SortedBead> so_bead_;
+    DArr<int> so_type_, so_cell_, so_slot_, scratch_;
+    double reach2_ = 0.0;
 
     std::vector<fn_bead_info> h_info_;
     std::vector<fn_spring> h_springs_;
@@ -1433,6 +1479,19 @@ int fn_export_breaks(fn_engine *eng, int *break_step)
     return guarded(eng, [&] {
         if (!break_step) throw Error(FN_ERR_ARG, "null destination");
         eng->export_breaks(break_step);
+    });
+}
+
+int fn_set_option(fn_engine *eng, const char *name, int value)
+{
+    return guarded(eng, [&] { eng->set_option(name ? name : "", value); });
+}
+
+int fn_get_option(fn_engine *eng, const char *name, int *value)
+{
+    return guarded(eng, [&] {
+        const int v = eng->get_option(name ? name : "");
+        if (value) *value = v;
     });
 }
 

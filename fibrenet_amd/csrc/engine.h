@@ -12,24 +12,31 @@ enum { kStatMaxCell = 0, kStatCellsOver = 1, kStatMaxList = 2, kStatListOver = 3
 template <typename T, typename P>
 struct Launch {
     static void advance_position(const Args<T, P> &a, int n_active, hipStream_t s);
+    // bond terms of the step (a.n_springs / a.n_angles bonds)
+    static void spring_terms(const Args<T, P> &a, hipStream_t s);
+    static void angle_terms(const Args<T, P> &a, hipStream_t s);
+    // ResolveSprings / ResolveAngles: the terms, then their sum into f of each bead
     static void resolve_springs(const Args<T, P> &a, hipStream_t s);
     static void resolve_angles(const Args<T, P> &a, hipStream_t s);
     static void resolve_contacts(const Args<T, P> &a, hipStream_t s);
     static void advance_velocity(const Args<T, P> &a, int n_active, hipStream_t s);
     static void reset_energy(const Args<T, P> &a, int n_active, hipStream_t s);
     static void drift_first(const Args<T, P> &a, hipStream_t s);
-    static void fused(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, hipStream_t s);
+    // the bead kernel of the fast path (after spring_terms / angle_terms)
+    static void bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, hipStream_t s);
     static void hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *count,
                      hipStream_t s);
     static void gather_sorted(const int *sorted, const int *n_hashed, const P *px, const P *py, const P *pz,
-                              const int *type, const int *tag, const int *loc, const Sorted &so, int n,
-                              hipStream_t s);
+                              const int *type, const int *tag, const int *loc, const int *key, const Sorted &so,
+                              int n, hipStream_t s);
 };
 
 // Neighbour search over the cell-sorted beads into scratch rows of K entries
 // per slot, then compaction into the bead-indexed CSR list.
-void search(const int *sorted, const int *cell_start, int n_cells, const int *key, const Sorted &so,
-            const double *radius, const double *skin, const PairC *pairs, const Box &b, int *cnt, int *scratch, int K,
+// reach2: the largest squared listing radius of any pair type (a candidate
+// farther than that is rejected before its type is read).
+void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const double *radius,
+            const double *skin, const PairC *pairs, const Box &b, double reach2, int *cnt, int *scratch, int K,
             int *stats, int n, hipStream_t s);
 void compact(const int *cnt, const int *nb_start, const int *slot_of, const int *scratch, int K, int n, int *nb_list,
              int cap, hipStream_t s);

@@ -1,8 +1,14 @@
 // engine_kernels.hip - __global__ kernels of the FibreNet step (gfx950).
 //
 // Granular kernels reproduce one reference kernel each (ExecuteKernel by
-// name); the fused kernel does the whole step of one bead - forces, closing
-// kick and the next step's drift - so that a step is one pass over HBM.
+// name).  The fast path runs a step as three launches: spring terms (one
+// thread per spring), angle terms (one thread per angle) and the bead kernel,
+// which sums the terms, adds contacts, closes the kick and drifts the next
+// step, so that the bead state makes one pass over HBM per step.  Each launch
+// is a short chain of independent loads: the bond kernels read their beads
+// through the bond arrays, the bead kernel its incidence lists.
+#include <utility>
+
 #include "kernels.h"
 #include "engine.h"
 
@@ -36,25 +42,66 @@ __global__ void __launch_bounds__(kBlock) k_advance_position(Args<T, P> a, int n
     a.fx[i] = f.x; a.fy[i] = f.y; a.fz[i] = f.z;
 }
 
+// Spring terms of one step (kernel.cl:371-417), one thread per spring: the
+// force on p1 (zero for an inactive spring) and, for an HR_FL spring whose
+// stress exceeds its threshold, the status change and the break step.
 template <typename T, typename P>
-__global__ void __launch_bounds__(kBlock) k_resolve_springs(Args<T, P> a)
+__global__ void __launch_bounds__(kBlock) k_spring_terms(Args<T, P> a)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int s = xcd_block() * kBlock + threadIdx.x;
+    if (s >= a.n_springs) return;
+    V3<T> F;
+    bool broke;
+    const bool on = spring_eval(a, s, F, broke);
+    if (broke) {
+        a.sp_status[s] = 0;
+        a.sp_break[s] = *a.step + a.step_off;
+    }
+    a.sfx[s] = on ? F.x : (T)0;
+    a.sfy[s] = on ? F.y : (T)0;
+    a.sfz[s] = on ? F.z : (T)0;
+}
+
+// Angle terms of one step (kernel.cl:274-348), one thread per angle, after
+// the spring terms: an HR_FL angle dies with either of its springs; a live
+// angle writes its forces on b1 and b3 (b2 receives their negated sum).
+template <typename T, typename P>
+__global__ void __launch_bounds__(kBlock) k_angle_terms(Args<T, P> a)
+{
+    const int q = xcd_block() * kBlock + threadIdx.x;
+    if (q >= a.n_angles) return;
+    int st = a.an_status[q];
+    if (st == 1 && a.an_type[q] == 1 && (a.sp_status[a.an_s1[q]] == 0 || a.sp_status[a.an_s2[q]] == 0)) {
+        st = 0;
+        a.an_status[q] = 0;
+    }
+    V3<T> pa{(T)0, (T)0, (T)0}, pc{(T)0, (T)0, (T)0};
+    if (st == 1) angle_eval(a, q, pa, pc);
+    a.apx[q] = pa.x; a.apy[q] = pa.y; a.apz[q] = pa.z;
+    a.acx[q] = pc.x; a.acy[q] = pc.y; a.acz[q] = pc.z;
+}
+
+// ResolveSprings' accumulation seen from each bead: adds the spring terms.
+template <typename T, typename P>
+__global__ void __launch_bounds__(kBlock) k_gather_springs(Args<T, P> a)
+{
+    const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
     if (a.sp_start[i] == a.sp_start[i + 1]) return;
     V3<T> f{a.fx[i], a.fy[i], a.fz[i]};
-    gather_springs(a, i, f);
+    gather_spring_terms(a, i, f);
     a.fx[i] = f.x; a.fy[i] = f.y; a.fz[i] = f.z;
 }
 
+// ResolveAngles' accumulation seen from each bead: adds the angle terms.
 template <typename T, typename P>
-__global__ void __launch_bounds__(kBlock) k_resolve_angles(Args<T, P> a)
+__global__ void __launch_bounds__(kBlock) k_gather_angles(Args<T, P> a)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
     if (a.an_start[i] == a.an_start[i + 1]) return;
     V3<T> f{a.fx[i], a.fy[i], a.fz[i]};
-    gather_angles<T, P, false>(a, i, f);
+    gather_angle_terms(a, i, f);
     a.fx[i] = f.x; a.fy[i] = f.y; a.fz[i] = f.z;
 }
 
@@ -113,61 +160,24 @@ __global__ void __launch_bounds__(kBlock) k_drift_first(Args<T, P> a)
     if (a.jx != a.qx) { a.jx[i] = ix; a.jy[i] = iy; a.jz[i] = iz; }
 }
 
-// LDS slots for the bond terms of one block's beads (entries of the bead
-// incidence lists); a block whose lists are longer evaluates in-thread.
-constexpr int kSpCap = 640;
-constexpr int kAnCap = 800;
+// Pair-table entries kept in LDS by the bead kernel (the table of T types has
+// T(T+1)/2 entries; contact-list entries address at most 32).
 constexpr int kPairCap = 32;
 
-// One whole step of one bead: base force, springs, angles, contacts, closing
-// kick; with NEXT the drift of the following step, with LAST the predicted
-// velocity of this step for read-back.  Reads f of the previous step from
-// f*, writes this step's force to g*.
-//
-// Each bond is evaluated once per block by its owner bead (spring: p1, angle:
-// b2) into LDS; the other beads of the bond read the term there when the
-// owner is in the same block and evaluate it themselves otherwise.  The terms
-// are the same values either way and are added in incidence-list order, so
-// the result equals the per-kernel path bit for bit.
+// One whole step of one bead after the bond terms: base force, the spring and
+// angle terms in incidence-list order, contacts, closing kick; with NEXT the
+// drift of the following step, with LAST the predicted velocity of this step
+// for read-back.  Reads f of the previous step from f*, writes this step's
+// force to g*.  The terms and their summation order are those of the
+// per-kernel path, so both paths agree bit for bit.
 template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST>
-__global__ void __launch_bounds__(kBlock) k_fused_step(Args<T, P> a)
+__global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
 {
-    __shared__ SpTerm<T> sps[SPR ? kSpCap : 1];
-    __shared__ AnTerm<T> ans[ANG ? kAnCap : 1];
     __shared__ PairC prs[kPairCap];
-    const int b0 = xcd_block() * kBlock;
-    const int i = b0 + threadIdx.x;
-    const bool live = i < a.n;
-    const int bend = min(b0 + kBlock, a.n);
+    const int i = xcd_block() * kBlock + threadIdx.x;
     for (int k = threadIdx.x; k < a.n_pairs; k += kBlock) prs[k] = a.pairs[k];
-    int se0 = 0, se1 = 0, ae0 = 0, ae1 = 0;
-    bool sp_lds = false, an_lds = false;
-    if (SPR) {
-        se0 = a.sp_start[b0];
-        se1 = a.sp_start[bend];
-        sp_lds = se1 - se0 <= kSpCap;
-    }
-    if (ANG) {
-        ae0 = a.an_start[b0];
-        ae1 = a.an_start[bend];
-        an_lds = ae1 - ae0 <= kAnCap;
-    }
-    if (live) {
-        if (SPR && sp_lds)
-            for (int e = a.sp_start[i], e1 = a.sp_start[i + 1]; e < e1; ++e) {
-                const int ent = a.sp_ent[e];
-                const int s = ent >> 1;
-                if (!(ent & 1) && s < a.n_springs) sps[e - se0] = spring_owner(a, s);
-            }
-        if (ANG && an_lds)
-            for (int e = a.an_start[i], e1 = a.an_start[i + 1]; e < e1; ++e) {
-                const int ent = a.an_ent[e];
-                const int q = ent >> 2;
-                if ((ent & 3) == 1 && q < a.n_angles) ans[e - ae0] = angle_term<T, P, true>(a, q, true);
-            }
-    }
     __syncthreads();
-    if (!live) return;
+    if (i >= a.n) return;
     const T fpx = a.fx[i], fpy = a.fy[i], fpz = a.fz[i];
     const T wx = a.wx[i], wy = a.wy[i], wz = a.wz[i];
     const T lam = a.lam[i];
@@ -177,28 +187,8 @@ __global__ void __launch_bounds__(kBlock) k_fused_step(Args<T, P> a)
     vp.z = tfma(fpz, lam, wz);
     P px = a.px[i], py = a.py[i], pz = a.pz[i];
     V3<T> f = base_force<T, P>(a.drift, vp, px, py, pz);
-    if (SPR)
-        for (int e = a.sp_start[i], e1 = a.sp_start[i + 1]; e < e1; ++e) {
-            const int ent = a.sp_ent[e];
-            const int s = ent >> 1;
-            if (s >= a.n_springs) continue;
-            const bool p2 = ent & 1;
-            const int o = p2 ? a.sp_twin[e] : e;
-            const SpTerm<T> t = (sp_lds && o >= se0 && o < se1) ? sps[o - se0]
-                                                                  : (p2 ? spring_partner(a, s) : spring_owner(a, s));
-            add_spring(f, t, p2);
-        }
-    if (ANG)
-        for (int e = a.an_start[i], e1 = a.an_start[i + 1]; e < e1; ++e) {
-            const int ent = a.an_ent[e];
-            const int q = ent >> 2;
-            const int role = ent & 3;
-            if (q >= a.n_angles) continue;
-            const int o = role == 1 ? e : a.an_owner[e];
-            const AnTerm<T> t = (an_lds && o >= ae0 && o < ae1) ? ans[o - ae0]
-                                                                  : angle_term<T, P, true>(a, q, role == 1);
-            add_angle(f, t, role);
-        }
+    if (SPR) gather_spring_terms(a, i, f);
+    if (ANG) gather_angle_terms(a, i, f);
     gather_contacts(a, i, f, prs);
     a.gx[i] = f.x; a.gy[i] = f.y; a.gz[i] = f.z;
     const T h = a.hlf[i];
@@ -275,35 +265,39 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
-// Bead data in cell order, 36 B per bead: fp64 position and tag, loc, type.
+// Bead data in cell order, 36 B per bead: fp64 position, tag, loc, type.
 template <typename P>
 __global__ void __launch_bounds__(kBlock) k_gather_sorted(const int *sorted, const int *n_hashed, const P *px,
                                                           const P *py, const P *pz, const int *type, const int *tag,
-                                                          const int *loc, Sorted so)
+                                                          const int *loc, const int *key, Sorted so)
 {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= *n_hashed) return;
     const int i = sorted[s];
-    so.x[s] = (double)px[i];
-    so.y[s] = (double)py[i];
-    so.z[s] = (double)pz[i];
-    so.tag[s] = tag[i];
-    so.loc[s] = loc[i];
+    SortedBead b;
+    b.x = (double)px[i];
+    b.y = (double)py[i];
+    b.z = (double)pz[i];
+    b.tag = tag[i];
+    b.loc = loc[i];
+    so.bead[s] = b;
     so.type[s] = type[i];
+    so.cell[s] = key[i];
     so.slot_of[i] = s;
 }
 
-// One candidate range of cell-sorted slots [lo, hi) for the bead in slot s:
-// BuildContactList's predicate (kernel.cl:574-659) in fp64.
-__device__ __forceinline__ void scan_range(int lo, int hi, int s, double xi, double yi, double zi, int ti, int gi,
-                                           int li, double ri, double si, const Sorted &so, const int *sorted,
-                                           const double *radius, const double *skin, const PairC *pairs,
-                                           const Box &b, int *row, int K, int &k)
+// Scan of one candidate range of cell-sorted slots [lo, hi) for the bead in
+// slot s: BuildContactList's predicate (kernel.cl:574-659) in fp64.  The
+// distance is tested first against the largest listing radius of any pair
+// type (reach2), so that most candidates cost two loads.
+__device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBead &me, int ti, double ri, double si,
+                                           const Sorted &so, const int *sorted, const double *radius,
+                                           const double *skin, const PairC *pairs, const Box &b, double reach2,
+                                           int *row, int K, int &k)
 {
     for (int t = lo; t < hi; ++t) {
-        if (t == s) continue;
-        if (gi == so.tag[t] && abs(li - so.loc[t]) <= 1) continue;
-        double rx = xi - so.x[t], ry = yi - so.y[t], rz = zi - so.z[t];
+        const SortedBead o = so.bead[t];
+        double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
         double len = rx * rx + ry * ry + rz * rz;
         if (b.bc == 0 && len > b.hbs) {
             if (rx * rx > b.hbs) rx += (rx > 0.0) ? -b.lx : b.lx;
@@ -311,6 +305,8 @@ __device__ __forceinline__ void scan_range(int lo, int hi, int s, double xi, dou
             if (rz * rz > b.hbs) rz += (rz > 0.0) ? -b.lz : b.lz;
             len = rx * rx + ry * ry + rz * rz;
         }
+        if (!(len < reach2) || t == s) continue;
+        if (me.tag == o.tag && abs(me.loc - o.loc) <= 1) continue;
         const int pid = pair_index(ti, so.type[t], b.types);
         const int pt = pairs[pid].type;
         bool take = false;
@@ -334,22 +330,21 @@ __device__ __forceinline__ void scan_range(int lo, int hi, int s, double xi, dou
 // (SimulationManager.cpp:2210-2301) seen from both cells of a pair.  The three
 // cells of an x-row are one contiguous slot range unless the row wraps or is
 // clipped.  Up to K entries go to the slot's scratch row; cnt[bead] = count.
-__global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int *cell_start, int n_cells,
-                                                   const int *key, Sorted so, const double *radius, const double *skin,
-                                                   const PairC *pairs, Box b, int *cnt, int *scratch, int K,
-                                                   int *stats)
+__global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int *cell_start, int n_cells, Sorted so,
+                                                   const double *radius, const double *skin, const PairC *pairs,
+                                                   Box b, double reach2, int *cnt, int *scratch, int K, int *stats)
 {
     const int s = blockIdx.x * kBlock + threadIdx.x;
     int k = 0, occ = 0;
     if (s < cell_start[n_cells]) {
         const int i = sorted[s];
-        const int c = key[i];
+        const int c = so.cell[s];
         if (cell_start[c] == s) occ = cell_start[c + 1] - s;   // the cell's first slot reports its occupancy
         const int cz = c / b.cxy;
         const int cy = (c - cz * b.cxy) / b.cx;
         const int cx = c - cz * b.cxy - cy * b.cx;
-        const double xi = so.x[s], yi = so.y[s], zi = so.z[s];
-        const int ti = so.type[s], gi = so.tag[s], li = so.loc[s];
+        const SortedBead me = so.bead[s];
+        const int ti = so.type[s];
         const double ri = radius[i], si = skin[i];
         int *row = scratch + (size_t)s * K;
         for (int dz = -1; dz <= 1; ++dz) {
@@ -362,15 +357,15 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
                 else if (y < 0 || y >= b.cy) continue;
                 const int rbase = y * b.cx + z * b.cxy;
                 if (cx >= 1 && cx + 1 < b.cx) {
-                    scan_range(cell_start[rbase + cx - 1], cell_start[rbase + cx + 2], s, xi, yi, zi, ti, gi, li, ri,
-                               si, so, sorted, radius, skin, pairs, b, row, K, k);
+                    scan_range(cell_start[rbase + cx - 1], cell_start[rbase + cx + 2], s, me, ti, ri, si, so, sorted,
+                               radius, skin, pairs, b, reach2, row, K, k);
                 } else {
                     for (int dx = -1; dx <= 1; ++dx) {
                         int x = cx + dx;
                         if (b.bc == 0) x = x < 0 ? b.cx - 1 : (x >= b.cx ? 0 : x);
                         else if (x < 0 || x >= b.cx) continue;
-                        scan_range(cell_start[rbase + x], cell_start[rbase + x + 1], s, xi, yi, zi, ti, gi, li, ri,
-                                   si, so, sorted, radius, skin, pairs, b, row, K, k);
+                        scan_range(cell_start[rbase + x], cell_start[rbase + x + 1], s, me, ti, ri, si, so, sorted,
+                                   radius, skin, pairs, b, reach2, row, K, k);
                     }
                 }
             }
@@ -404,6 +399,24 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *n
     if (base + m > cap) return;   // capacity exceeded: the host reports it
     const int *row = scratch + (size_t)slot_of[i] * K;
     int *out = nb_list + base;
+    if (m <= 8) {
+        int v[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) v[k] = k < m ? row[k] : 0x7fffffff;
+#pragma unroll
+        for (int p = 0; p < 8; ++p)          // odd-even transposition sort of 8 registers
+#pragma unroll
+            for (int k = p & 1; k + 1 < 8; k += 2)
+                if (v[k] > v[k + 1]) {
+                    const int t = v[k];
+                    v[k] = v[k + 1];
+                    v[k + 1] = t;
+                }
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < m) out[k] = v[k];
+        return;
+    }
     for (int k = 0; k < m; ++k) {
         const int v = row[k];
         int j = k - 1;
@@ -502,14 +515,26 @@ void Launch<T, P>::advance_position(const Args<T, P> &a, int n_active, hipStream
     if (n_active > 0) k_advance_position<T, P><<<blocks(n_active), kBlock, 0, s>>>(a, n_active);
 }
 template <typename T, typename P>
+void Launch<T, P>::spring_terms(const Args<T, P> &a, hipStream_t s)
+{
+    if (a.n_springs > 0) k_spring_terms<T, P><<<blocks(a.n_springs), kBlock, 0, s>>>(a);
+}
+template <typename T, typename P>
+void Launch<T, P>::angle_terms(const Args<T, P> &a, hipStream_t s)
+{
+    if (a.n_angles > 0) k_angle_terms<T, P><<<blocks(a.n_angles), kBlock, 0, s>>>(a);
+}
+template <typename T, typename P>
 void Launch<T, P>::resolve_springs(const Args<T, P> &a, hipStream_t s)
 {
-    k_resolve_springs<T, P><<<blocks(a.n), kBlock, 0, s>>>(a);
+    spring_terms(a, s);
+    k_gather_springs<T, P><<<blocks(a.n), kBlock, 0, s>>>(a);
 }
 template <typename T, typename P>
 void Launch<T, P>::resolve_angles(const Args<T, P> &a, hipStream_t s)
 {
-    k_resolve_angles<T, P><<<blocks(a.n), kBlock, 0, s>>>(a);
+    angle_terms(a, s);
+    k_gather_angles<T, P><<<blocks(a.n), kBlock, 0, s>>>(a);
 }
 template <typename T, typename P>
 void Launch<T, P>::resolve_contacts(const Args<T, P> &a, hipStream_t s)
@@ -531,22 +556,25 @@ void Launch<T, P>::drift_first(const Args<T, P> &a, hipStream_t s)
 {
     k_drift_first<T, P><<<blocks(a.n), kBlock, 0, s>>>(a);
 }
-template <typename T, typename P>
-void Launch<T, P>::fused(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, hipStream_t s)
+template <typename T, typename P, int M>
+static void launch_fused_mask(const Args<T, P> &a, hipStream_t s)
 {
-    const int g = blocks(a.n);
-#define FN_FUSED(S_, A_, N_, L_) k_fused_step<T, P, S_, A_, N_, L_><<<g, kBlock, 0, s>>>(a)
-#define FN_FUSED_NL(S_, A_)                 \
-    if (next && last) FN_FUSED(S_, A_, true, true);       \
-    else if (next) FN_FUSED(S_, A_, true, false);         \
-    else if (last) FN_FUSED(S_, A_, false, true);         \
-    else FN_FUSED(S_, A_, false, false);
-    if (spr && ang) { FN_FUSED_NL(true, true) }
-    else if (spr) { FN_FUSED_NL(true, false) }
-    else if (ang) { FN_FUSED_NL(false, true) }
-    else { FN_FUSED_NL(false, false) }
-#undef FN_FUSED_NL
-#undef FN_FUSED
+    k_bead_step<T, P, (M & 1) != 0, (M & 2) != 0, (M & 4) != 0, (M & 8) != 0><<<blocks(a.n), kBlock, 0, s>>>(a);
+}
+
+template <typename T, typename P, int... M>
+static void dispatch_fused(const Args<T, P> &a, int mask, hipStream_t s, std::integer_sequence<int, M...>)
+{
+    using Fn = void (*)(const Args<T, P> &, hipStream_t);
+    static constexpr Fn table[] = {&launch_fused_mask<T, P, M>...};
+    table[mask](a, s);
+}
+
+template <typename T, typename P>
+void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, hipStream_t s)
+{
+    const int mask = (spr ? 1 : 0) | (ang ? 2 : 0) | (next ? 4 : 0) | (last ? 8 : 0);
+    dispatch_fused<T, P>(a, mask, s, std::make_integer_sequence<int, 16>{});
 }
 template <typename T, typename P>
 void Launch<T, P>::hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *count,
@@ -556,10 +584,10 @@ void Launch<T, P>::hash(const P *px, const P *py, const P *pz, int n, const Box 
 }
 template <typename T, typename P>
 void Launch<T, P>::gather_sorted(const int *sorted, const int *n_hashed, const P *px, const P *py, const P *pz,
-                                 const int *type, const int *tag, const int *loc, const Sorted &so, int n,
-                                 hipStream_t s)
+                                 const int *type, const int *tag, const int *loc, const int *key, const Sorted &so,
+                                 int n, hipStream_t s)
 {
-    k_gather_sorted<P><<<blocks(n), kBlock, 0, s>>>(sorted, n_hashed, px, py, pz, type, tag, loc, so);
+    k_gather_sorted<P><<<blocks(n), kBlock, 0, s>>>(sorted, n_hashed, px, py, pz, type, tag, loc, key, so);
 }
 
 template struct Launch<double, double>;
@@ -574,12 +602,12 @@ void copy_int(const int *src, int *dst, int n, hipStream_t s)
     if (n > 0) k_copy_int<<<blocks(n), kBlock, 0, s>>>(src, dst, n);
 }
 void add_int(int *p, int v, hipStream_t s) { k_add_int<<<1, 1, 0, s>>>(p, v); }
-void search(const int *sorted, const int *cell_start, int n_cells, const int *key, const Sorted &so,
-            const double *radius, const double *skin, const PairC *pairs, const Box &b, int *cnt, int *scratch, int K,
+void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const double *radius,
+            const double *skin, const PairC *pairs, const Box &b, double reach2, int *cnt, int *scratch, int K,
             int *stats, int n, hipStream_t s)
 {
-    k_search<<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, key, so, radius, skin, pairs, b, cnt, scratch,
-                                          K, stats);
+    k_search<<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, radius, skin, pairs, b, reach2, cnt,
+                                          scratch, K, stats);
 }
 void compact(const int *cnt, const int *nb_start, const int *slot_of, const int *scratch, int K, int n, int *nb_list,
              int cap, hipStream_t s)

@@ -4,11 +4,12 @@
 // velocities, forces and force arithmetic; P the type of positions (fp64 in
 // both builds, so that the drift of a pulled bead at x~1000 survives).
 //
-// Determinism: every bead gathers the terms acting on it (incident springs,
-// incident angles, its two-sided contact list) in a fixed order and owns its
-// own write; no float atomics.  A bond shared by two or three beads is
-// evaluated by each of them with identical operands and operation order, so
-// the partners see bit-identical, opposite contributions.
+// Determinism: each spring and each angle is evaluated once per step by one
+// thread, which writes its term (and owns any status change); every bead then
+// gathers the terms acting on it (incident springs, incident angles, its
+// two-sided contact list) in a fixed order and owns its own write; no float
+// atomics.  The two beads of a contact evaluate it with identical operands, so
+// they see bit-identical, opposite contributions.
 //
 // Operation order and rounding of each term follow kernel.cl so that the fp64
 // build reproduces the reference's arithmetic; mad() is an explicit fma() and
@@ -30,11 +31,19 @@ struct Box {
     double wall_k, hbs, cutoff_sq, damping, gx, gy, gz, dt;
 };
 
+// One bead in cell order: what the neighbour search reads per candidate, in
+// two 16-byte loads.
+struct alignas(16) SortedBead {
+    double x, y, z;
+    int tag, loc;
+};
+
 // Bead data copied into cell order at a rebuild (the neighbour search reads
-// these contiguously) and the inverse permutation bead -> slot.
+// these contiguously), the cell of each slot and the inverse permutation
+// bead -> slot.
 struct Sorted {
-    double *x, *y, *z;
-    int *tag, *loc, *type, *slot_of;
+    SortedBead *bead;
+    int *type, *cell, *slot_of;
 };
 
 // One pair-table entry (Pair.h:5-16): comparisons use the fp64 values.
@@ -68,9 +77,9 @@ struct Args {
     int *an_status;
     const T *an_th0, *an_k;
     int n_angles;
-    // index of the owner entry of the same bond: spring p2 entry -> p1 entry,
-    // angle b1/b3 entry -> b2 entry (bead-incidence lists)
-    const int *sp_twin, *an_owner;
+    // per-bond terms of the current step: spring force on p1; angle forces on b1, b3
+    T *sfx, *sfy, *sfz;
+    T *apx, *apy, *apz, *acx, *acy, *acz;
     // contacts
     const int *nb_start, *nb_list;
     const PairC *pairs;
@@ -195,24 +204,40 @@ __device__ __forceinline__ bool spring_eval(const Args<T, P> &a, int s, V3<T> &F
     return true;
 }
 
-// Status of spring s after this step's spring pass (for the HR_FL angle rule),
-// decided from the same evaluation every bead performs.
-template <typename T, typename P>
-__device__ __forceinline__ int spring_status_now(const Args<T, P> &a, int s)
+// Perpendicular force directions of an angle bond scaled to fa, fc
+// (kernel.cl:321-348) in the arithmetic type R.
+template <typename R, typename T>
+__device__ __forceinline__ void angle_dirs(R bax, R bay, R baz, R bcx, R bcy, R bcz, R fa, R fc, V3<T> &pa_out,
+                                           V3<T> &pc_out)
 {
-    if (a.sp_status[s] == 0) return 0;
-    V3<T> F;
-    bool broke;
-    spring_eval(a, s, F, broke);
-    return broke ? 0 : a.sp_status[s];
+    const R cbx = bcx * (R)-1, cby = bcy * (R)-1, cbz = bcz * (R)-1;
+    const R cx = bay * bcz - baz * bcy;
+    const R cy = baz * bcx - bax * bcz;
+    const R cz = bax * bcy - bay * bcx;
+    R pax = bay * cz - baz * cy, pay = baz * cx - bax * cz, paz = bax * cy - bay * cx;
+    R ln = tsqrt(pax * pax + pay * pay + paz * paz);
+    if (ln != (R)0) ln = (R)1 / ln;
+    ln *= fa;
+    pa_out.x = (T)(pax * ln);
+    pa_out.y = (T)(pay * ln);
+    pa_out.z = (T)(paz * ln);
+    R pcx = cby * cz - cbz * cy, pcy = cbz * cx - cbx * cz, pcz = cbx * cy - cby * cx;
+    ln = tsqrt(pcx * pcx + pcy * pcy + pcz * pcz);
+    if (ln != (R)0) ln = (R)1 / ln;
+    ln *= fc;
+    pc_out.x = (T)(pcx * ln);
+    pc_out.y = (T)(pcy * ln);
+    pc_out.z = (T)(pcz * ln);
 }
 
 // Angle bond forces on b1 (pa) and b3 (pc); b2 receives -pa-pc (kernel.cl:289-360).
-// Evaluated in fp64 in both builds and rounded to T at the end: the angle is
-// acos of the cosine, ill-conditioned at the straight fibre (d theta =
-// d cos / sin theta), and on an exactly straight segment the reference's
-// clamp of a cosine rounded below -1 to -0.999999 produces a finite force
-// whose direction comes from the fp64 cross product of the bond vectors.
+// The angle is acos of the cosine, ill-conditioned at the straight fibre
+// (d theta = d cos / sin theta), so cosine, lengths and acos are fp64 in both
+// builds.  The directions use T, except where the reference clamps a cosine
+// rounded beyond -1 or 1 (kernel.cl:311-312): on such an exactly straight
+// segment the force (|F| from the clamped 0.999999) points along the fp64
+// cross product of the bond vectors, which is rounding noise that only the
+// fp64 evaluation reproduces.
 template <typename T, typename P>
 __device__ __forceinline__ void angle_eval(const Args<T, P> &a, int q, V3<T> &pa_out, V3<T> &pc_out)
 {
@@ -220,155 +245,89 @@ __device__ __forceinline__ void angle_eval(const Args<T, P> &a, int q, V3<T> &pa
     const P tx = a.ix[b2], ty = a.iy[b2], tz = a.iz[b2];
     const double bax = (double)(a.ix[b1] - tx), bay = (double)(a.iy[b1] - ty), baz = (double)(a.iz[b1] - tz);
     const double bcx = (double)(a.ix[b3] - tx), bcy = (double)(a.iy[b3] - ty), bcz = (double)(a.iz[b3] - tz);
-    const double cbx = bcx * -1.0, cby = bcy * -1.0, cbz = bcz * -1.0;
+    if (sizeof(T) < sizeof(double)) {
+        // fp32 build away from the straight and the folded triple: the angle as
+        // atan2(|ba x bc|, ba . bc), well conditioned at any angle, and the
+        // perpendicular directions from the same cross product
+        // (|ba x (ba x bc)| = |ba| |ba x bc|).
+        const T fx = (T)bax, fy = (T)bay, fz = (T)baz, gx = (T)bcx, gy = (T)bcy, gz = (T)bcz;
+        const T cx = fy * gz - fz * gy, cy = fz * gx - fx * gz, cz = fx * gy - fy * gx;
+        const T c2 = cx * cx + cy * cy + cz * cz;
+        const T la2 = fx * fx + fy * fy + fz * fz, lc2 = gx * gx + gy * gy + gz * gz;
+        if (c2 > (T)1e-10 * la2 * lc2) {
+            const T cn = tsqrt(c2), l_ba = tsqrt(la2), l_bc = tsqrt(lc2);
+            const T ang = atan2f(cn, fx * gx + fy * gy + fz * gz);
+            const T mag = (T)-2 * a.an_k[q] * (ang - a.an_th0[q]);
+            const T sa = mag / (l_ba * l_ba * cn), sc = mag / (l_bc * l_bc * cn);
+            pa_out.x = (fy * cz - fz * cy) * sa;
+            pa_out.y = (fz * cx - fx * cz) * sa;
+            pa_out.z = (fx * cy - fy * cx) * sa;
+            pc_out.x = (gz * cy - gy * cz) * sc;
+            pc_out.y = (gx * cz - gz * cx) * sc;
+            pc_out.z = (gy * cx - gx * cy) * sc;
+            return;
+        }
+    }
     const double l_ba = tsqrt(bax * bax + bay * bay + baz * baz);
     const double l_bc = tsqrt(bcx * bcx + bcy * bcy + bcz * bcz);
     double dotp = (bax * bcx + bay * bcy + baz * bcz) / (l_ba * l_bc);
+    const bool clamped = dotp < -1.0 || dotp > 1.0;
     if (dotp < -1.0) dotp = -0.999999;
     if (dotp > 1.0) dotp = 0.999999;
     const double ang = tacos(dotp);
     const double mag = -2.0 * (double)a.an_k[q] * (ang - (double)a.an_th0[q]);
     const double fa = mag / l_ba;
     const double fc = mag / l_bc;
-    const double cx = bay * bcz - baz * bcy;
-    const double cy = baz * bcx - bax * bcz;
-    const double cz = bax * bcy - bay * bcx;
-    double pax = bay * cz - baz * cy, pay = baz * cx - bax * cz, paz = bax * cy - bay * cx;
-    double ln = tsqrt(pax * pax + pay * pay + paz * paz);
-    if (ln != 0.0) ln = 1.0 / ln;
-    ln *= fa;
-    pa_out.x = (T)(pax * ln);
-    pa_out.y = (T)(pay * ln);
-    pa_out.z = (T)(paz * ln);
-    double pcx = cby * cz - cbz * cy, pcy = cbz * cx - cbx * cz, pcz = cbx * cy - cby * cx;
-    ln = tsqrt(pcx * pcx + pcy * pcy + pcz * pcz);
-    if (ln != 0.0) ln = 1.0 / ln;
-    ln *= fc;
-    pc_out.x = (T)(pcx * ln);
-    pc_out.y = (T)(pcy * ln);
-    pc_out.z = (T)(pcz * ln);
+    if (sizeof(T) == sizeof(double) || clamped)
+        angle_dirs<double, T>(bax, bay, baz, bcx, bcy, bcz, fa, fc, pa_out, pc_out);
+    else
+        angle_dirs<T, T>((T)bax, (T)bay, (T)baz, (T)bcx, (T)bcy, (T)bcz, (T)fa, (T)fc, pa_out, pc_out);
 }
 
-// One evaluated bond term as the beads of the bond consume it.
-template <typename T>
-struct SpTerm { T x, y, z; int on; };
-template <typename T>
-struct AnTerm { T ax, ay, az, cx, cy, cz; int on; };
-
-// Spring s evaluated by its owner (the p1 bead), which also records a break.
+// Adds the spring terms incident to bead i (kernel.cl:426-436 seen from bead
+// i): +F as p1, -F as p2, in incidence-list (spring index) order.
 template <typename T, typename P>
-__device__ __forceinline__ SpTerm<T> spring_owner(const Args<T, P> &a, int s)
+__device__ __forceinline__ void gather_spring_terms(const Args<T, P> &a, int i, V3<T> &f)
 {
-    V3<T> F;
-    bool broke;
-    SpTerm<T> t;
-    t.on = spring_eval(a, s, F, broke) ? 1 : 0;
-    t.x = F.x; t.y = F.y; t.z = F.z;
-    if (broke) {
-        a.sp_status[s] = 0;
-        a.sp_break[s] = *a.step + a.step_off;
-    }
-    return t;
-}
-
-// Spring s evaluated by its p2 bead (no status write).
-template <typename T, typename P>
-__device__ __forceinline__ SpTerm<T> spring_partner(const Args<T, P> &a, int s)
-{
-    V3<T> F;
-    bool broke;
-    SpTerm<T> t;
-    t.on = spring_eval(a, s, F, broke) ? 1 : 0;
-    t.x = F.x; t.y = F.y; t.z = F.z;
-    return t;
-}
-
-template <typename T>
-__device__ __forceinline__ void add_spring(V3<T> &f, const SpTerm<T> &t, bool p2)
-{
-    if (!t.on) return;
-    if (p2) {
-        f.x += t.x * (T)-1;
-        f.y += t.y * (T)-1;
-        f.z += t.z * (T)-1;
-    } else {
-        f.x += t.x;
-        f.y += t.y;
-        f.z += t.z;
-    }
-}
-
-// Adds the springs incident to bead i to f (kernel.cl:419-437 seen from bead i).
-template <typename T, typename P>
-__device__ __forceinline__ void gather_springs(const Args<T, P> &a, int i, V3<T> &f)
-{
-    const int e0 = a.sp_start[i], e1 = a.sp_start[i + 1];
-    for (int e = e0; e < e1; ++e) {
+    for (int e = a.sp_start[i], e1 = a.sp_start[i + 1]; e < e1; ++e) {
         const int ent = a.sp_ent[e];
         const int s = ent >> 1;
         if (s >= a.n_springs) continue;
-        add_spring(f, (ent & 1) ? spring_partner(a, s) : spring_owner(a, s), ent & 1);
-    }
-}
-
-// Angle q with this step's HR_FL rule (kernel.cl:274-286): an HR_FL angle dies
-// with either spring.  `recheck` re-derives this step's spring statuses (fused
-// path, where the spring pass runs in the same launch); the owner (b2) records
-// the death.
-template <typename T, typename P, bool recheck>
-__device__ __forceinline__ AnTerm<T> angle_term(const Args<T, P> &a, int q, bool owner)
-{
-    AnTerm<T> t;
-    int st = a.an_status[q];
-    if (a.an_type[q] == 1 && st == 1) {
-        const int s1 = a.an_s1[q], s2 = a.an_s2[q];
-        const int st1 = recheck ? spring_status_now(a, s1) : a.sp_status[s1];
-        const int st2 = recheck ? spring_status_now(a, s2) : a.sp_status[s2];
-        if (st1 == 0 || st2 == 0) {
-            st = 0;
-            if (owner) a.an_status[q] = 0;
+        if (ent & 1) {
+            f.x += a.sfx[s] * (T)-1;
+            f.y += a.sfy[s] * (T)-1;
+            f.z += a.sfz[s] * (T)-1;
+        } else {
+            f.x += a.sfx[s];
+            f.y += a.sfy[s];
+            f.z += a.sfz[s];
         }
     }
-    t.on = st == 1 ? 1 : 0;
-    if (t.on) {
-        V3<T> pa, pc;
-        angle_eval(a, q, pa, pc);
-        t.ax = pa.x; t.ay = pa.y; t.az = pa.z;
-        t.cx = pc.x; t.cy = pc.y; t.cz = pc.z;
-    }
-    return t;
 }
 
-template <typename T>
-__device__ __forceinline__ void add_angle(V3<T> &f, const AnTerm<T> &t, int role)
+// Adds the angle terms incident to bead i (kernel.cl:350-360 seen from bead
+// i): pa as b1, pc as b3, -pa-pc as b2, in incidence-list order.
+template <typename T, typename P>
+__device__ __forceinline__ void gather_angle_terms(const Args<T, P> &a, int i, V3<T> &f)
 {
-    if (!t.on) return;
-    if (role == 0) {
-        f.x += t.ax;
-        f.y += t.ay;
-        f.z += t.az;
-    } else if (role == 2) {
-        f.x += t.cx;
-        f.y += t.cy;
-        f.z += t.cz;
-    } else {
-        f.x += -t.ax - t.cx;
-        f.y += -t.ay - t.cy;
-        f.z += -t.az - t.cz;
-    }
-}
-
-// Adds the angle bonds incident to bead i.
-template <typename T, typename P, bool recheck>
-__device__ __forceinline__ void gather_angles(const Args<T, P> &a, int i, V3<T> &f)
-{
-    const int e0 = a.an_start[i], e1 = a.an_start[i + 1];
-    for (int e = e0; e < e1; ++e) {
+    for (int e = a.an_start[i], e1 = a.an_start[i + 1]; e < e1; ++e) {
         const int ent = a.an_ent[e];
         const int q = ent >> 2;
         const int role = ent & 3;
         if (q >= a.n_angles) continue;
-        add_angle(f, angle_term<T, P, recheck>(a, q, role == 1), role);
+        if (role == 0) {
+            f.x += a.apx[q];
+            f.y += a.apy[q];
+            f.z += a.apz[q];
+        } else if (role == 2) {
+            f.x += a.acx[q];
+            f.y += a.acy[q];
+            f.z += a.acz[q];
+        } else {
+            f.x += -a.apx[q] - a.acx[q];
+            f.y += -a.apy[q] - a.acy[q];
+            f.z += -a.apz[q] - a.acz[q];
+        }
     }
 }
 

@@ -128,15 +128,26 @@ int fn_finish(fn_engine *eng);
  * per-kernel launches; returns after the work is enqueued and checked. */
 int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_springs, int run_angles);
 
+/* Launch options of the fast path (no reference counterpart):
+ *   "chunk"  n    steps captured per hipGraph (rounded to grid_update_freq)
+ *   "graphs" 1/0  launch through captured hipGraphs or directly
+ * All options give bit-identical results; they change only the speed. */
+int fn_set_option(fn_engine *eng, const char *name, int value);
+int fn_get_option(fn_engine *eng, const char *name, int *value);
+
 /* Kernel timing of the fast path (no reference counterpart): with timing on,
  * fn_run_steps launches its kernels directly (not through graphs) and records
- * HIP events around every fused step kernel and every grid rebuild on the
- * engine's stream; fn_get_timing sums them since timing was switched on. */
+ * HIP events around each of the three launches of every step (spring terms,
+ * angle terms, bead kernel) and around every grid rebuild on the engine's
+ * stream; fn_get_timing sums them since timing was switched on. */
 typedef struct fn_timing {
-    int fused_launches;            /* fused step kernels measured              */
+    int steps;                     /* steps measured                           */
     int rebuilds;                  /* rebuild sequences measured               */
-    double fused_ms;               /* their summed durations                   */
-    double rebuild_ms;
+    double step_ms;                /* summed durations of the step launches    */
+    double rebuild_ms;             /* summed durations of the rebuilds         */
+    double spring_ms;              /* of step_ms: the spring-term kernel       */
+    double angle_ms;               /*             the angle-term kernel        */
+    double bead_ms;                /*             the bead kernel              */
 } fn_timing;
 int fn_set_timing(fn_engine *eng, int on);
 int fn_get_timing(fn_engine *eng, fn_timing *out);
