@@ -401,13 +401,13 @@ class Engine final : public fn_engine {
         case K_RESOLVE_SPRINGS: {
             auto a = args_granular();
             a.n_springs = std::min(n, S_);
-            if (S_ > 0 && a.n_springs > 0) fnk::Launch<T, P>::resolve_springs(a, stream_);
+            if (S_ > 0 && a.n_springs > 0) fnk::Launch<T, P>::resolve_springs(a, S_, stream_);
             break;
         }
         case K_RESOLVE_ANGLES: {
             auto a = args_granular();
             a.n_angles = std::min(n, A_);
-            if (A_ > 0 && a.n_angles > 0) fnk::Launch<T, P>::resolve_angles(a, stream_);
+            if (A_ > 0 && a.n_angles > 0) fnk::Launch<T, P>::resolve_angles(a, A_, stream_);
             break;
         }
         case K_RESET_GRID: grid_valid_ = false; break;
@@ -574,6 +574,7 @@ class Engine final : public fn_engine {
     void set_option(const std::string &name, int value) override
     {
         if (name == "chunk") chunk_ = value;
+        else if (name == "chain") chain_on_ = value != 0;
         else if (name == "graphs") graphs_on_ = value != 0;
         else throw Error(FN_ERR_ARG, "unknown option " + name);
         clear_graphs();
@@ -582,6 +583,11 @@ class Engine final : public fn_engine {
     int get_option(const std::string &name) override
     {
         if (name == "chunk") return chunk_;
+        if (name == "chain") return chain_on_ ? 1 : 0;
+        if (name == "chain_layout") {
+            prepare();
+            return chain_ok_ ? 1 : 0;
+        }
         if (name == "graphs") return graphs_on_ ? 1 : 0;
         throw Error(FN_ERR_ARG, "unknown option " + name);
     }
@@ -817,7 +823,6 @@ class Engine final : public fn_engine {
         sp_p1_.put(p1, stream_); sp_p2_.put(p2, stream_); sp_type_.put(ty, stream_); sp_status_.put(st, stream_);
         sp_break_.put(br, stream_);
         sp_l0_.put(l0, stream_); sp_k_.put(k, stream_); sp_c1_.put(c1, stream_); sp_c2_.put(c2, stream_);
-        for (auto &d : sp_f_) d.ensure(n);
         HIPCK(hipStreamSynchronize(stream_));
         csr_dirty_ = true;
         clear_graphs();
@@ -836,16 +841,15 @@ class Engine final : public fn_engine {
         an_b1_.put(b1, stream_); an_b2_.put(b2, stream_); an_b3_.put(b3, stream_);
         an_s1_.put(s1, stream_); an_s2_.put(s2, stream_); an_type_.put(ty, stream_); an_status_.put(st, stream_);
         an_th0_.put(th, stream_); an_k_.put(k, stream_);
-        for (auto &d : an_pa_) d.ensure(n);
-        for (auto &d : an_pc_) d.ensure(n);
         HIPCK(hipStreamSynchronize(stream_));
         csr_dirty_ = true;
         clear_graphs();
     }
 
-    // Incidence lists bead -> springs and bead -> angles, ordered by bond
+    // Incidence slots bead -> springs and bead -> angles, ordered by bond
     // index (then p1 before p2; b1, b3, b2) as the reference's sequential
-    // scatter adds them.
+    // scatter adds them: the bond kernels write each term to its slot, the
+    // bead kernels sum a bead's slots in order.
     void build_csr()
     {
         const int n = N_;
@@ -854,15 +858,16 @@ class Engine final : public fn_engine {
             if (s.p1 < 0 || s.p1 >= n || s.p2 < 0 || s.p2 >= n) throw Error(FN_ERR_ARG, "spring bead index out of range");
             cnt[s.p1]++; cnt[s.p2]++;
         }
-        std::vector<int> st(n + 1, 0), ent(2 * (size_t)S_ + 1);
+        std::vector<int> st(n + 1, 0), slot(2 * (size_t)S_);
         for (int i = 0; i < n; ++i) st[i + 1] = st[i] + cnt[i];
         std::vector<int> cur(st.begin(), st.end() - 1);
         for (int s = 0; s < S_; ++s) {
-            ent[cur[h_springs_[s].p1]++] = s << 1;
-            ent[cur[h_springs_[s].p2]++] = (s << 1) | 1;
+            slot[2 * (size_t)s] = cur[h_springs_[s].p1]++;
+            slot[2 * (size_t)s + 1] = cur[h_springs_[s].p2]++;
         }
         sp_start_.put(st, stream_);
-        sp_ent_.put(ent, stream_);
+        sp_slot_.put(slot, stream_);
+        sp_term_.ensure(2 * (size_t)S_);
         std::fill(cnt.begin(), cnt.end(), 0);
         for (const fn_angle &a : h_angles_) {
             for (int b : {a.b1, a.b2, a.b3})
@@ -871,19 +876,62 @@ class Engine final : public fn_engine {
                 throw Error(FN_ERR_ARG, "HR_FL angle spring index out of range");
             cnt[a.b1]++; cnt[a.b2]++; cnt[a.b3]++;
         }
-        std::vector<int> as(n + 1, 0), aent(3 * (size_t)A_ + 1);
+        std::vector<int> as(n + 1, 0), aslot(3 * (size_t)A_);
         for (int i = 0; i < n; ++i) as[i + 1] = as[i] + cnt[i];
         std::vector<int> acur(as.begin(), as.end() - 1);
         for (int q = 0; q < A_; ++q) {
-            aent[acur[h_angles_[q].b1]++] = (q << 2) | 0;
-            aent[acur[h_angles_[q].b3]++] = (q << 2) | 2;
-            aent[acur[h_angles_[q].b2]++] = (q << 2) | 1;
+            aslot[3 * (size_t)q] = acur[h_angles_[q].b1]++;
+            aslot[3 * (size_t)q + 1] = acur[h_angles_[q].b3]++;
+            aslot[3 * (size_t)q + 2] = acur[h_angles_[q].b2]++;
         }
         an_start_.put(as, stream_);
-        an_ent_.put(aent, stream_);
+        an_slot_.put(aslot, stream_);
+        an_term_.ensure(3 * (size_t)A_);
+        build_chain();
         HIPCK(hipStreamSynchronize(stream_));
         csr_dirty_ = false;
         clear_graphs();
+    }
+
+    // Chain layout: every spring joins beads (i, i+1) and every angle is
+    // (i-1, i, i+1), at most one of each per bead, numbered along the chain
+    // (spring (i-1, i) before (i, i+1); angle centred on i-1 before the one on
+    // i before the one on i+1, the order of each bead's incidence list), and
+    // an HR_FL angle fails with the two chain springs of its centre bead.
+    void build_chain()
+    {
+        const int n = N_;
+        std::vector<int> cs(n, -1), ca(n, -1);
+        bool ok = true;
+        for (int s = 0; s < S_ && ok; ++s) {
+            const fn_spring &q = h_springs_[s];
+            if (q.p2 != q.p1 + 1 || cs[q.p1] >= 0) ok = false;
+            else cs[q.p1] = s;
+        }
+        for (int q = 0; q < A_ && ok; ++q) {
+            const fn_angle &g = h_angles_[q];
+            if (g.b2 != g.b1 + 1 || g.b3 != g.b2 + 1 || ca[g.b2] >= 0) ok = false;
+            else ca[g.b2] = q;
+        }
+        for (int i = 0; i < n && ok; ++i) {
+            if (i > 0 && cs[i - 1] >= 0 && cs[i] >= 0 && cs[i - 1] > cs[i]) ok = false;
+            int last = -1;
+            for (int k = i - 1; k <= i + 1; ++k)
+                if (k >= 0 && k < n && ca[k] >= 0) {
+                    if (ca[k] < last) ok = false;
+                    last = ca[k];
+                }
+            if (ca[i] >= 0 && h_angles_[ca[i]].type == 1) {
+                const fn_angle &g = h_angles_[ca[i]];
+                const int l = i > 0 ? cs[i - 1] : -1, r = cs[i];
+                if (l < 0 || r < 0 || !((g.s1 == l && g.s2 == r) || (g.s1 == r && g.s2 == l))) ok = false;
+            }
+        }
+        chain_ok_ = ok;
+        if (ok) {
+            ch_spring_.put(cs, stream_);
+            ch_angle_.put(ca, stream_);
+        }
     }
 
     void prepare()
@@ -933,19 +981,19 @@ class Engine final : public fn_engine {
         a.wx = vold_[0].p; a.wy = vold_[1].p; a.wz = vold_[2].p;
         a.lam = lam_.p; a.hlf = hlf_.p; a.hlf2 = hlf2_.p;
         a.type = type_.p; a.tag = tag_.p; a.loc = loc_.p; a.radius = rad_.p;
-        a.sp_start = sp_start_.p; a.sp_ent = sp_ent_.p; a.sp_p1 = sp_p1_.p; a.sp_p2 = sp_p2_.p;
+        a.sp_start = sp_start_.p; a.sp_p1 = sp_p1_.p; a.sp_p2 = sp_p2_.p;
         a.sp_type = sp_type_.p; a.sp_status = sp_status_.p; a.sp_break = sp_break_.p;
         a.sp_l0 = sp_l0_.p; a.sp_k = sp_k_.p; a.sp_c1 = sp_c1_.p; a.sp_c2 = sp_c2_.p;
         a.n_springs = S_;
-        a.an_start = an_start_.p; a.an_ent = an_ent_.p; a.an_b1 = an_b1_.p; a.an_b2 = an_b2_.p; a.an_b3 = an_b3_.p;
+        a.an_start = an_start_.p; a.an_b1 = an_b1_.p; a.an_b2 = an_b2_.p; a.an_b3 = an_b3_.p;
         a.an_s1 = an_s1_.p; a.an_s2 = an_s2_.p; a.an_type = an_type_.p; a.an_status = an_status_.p;
         a.an_th0 = an_th0_.p; a.an_k = an_k_.p;
         a.n_angles = A_;
         a.nb_start = nb_start_.p; a.nb_list = nb_list_.p; a.pairs = pairs_.p;
         a.n_pairs = (int)h_pairs_.size();
-        a.sfx = sp_f_[0].p; a.sfy = sp_f_[1].p; a.sfz = sp_f_[2].p;
-        a.apx = an_pa_[0].p; a.apy = an_pa_[1].p; a.apz = an_pa_[2].p;
-        a.acx = an_pc_[0].p; a.acy = an_pc_[1].p; a.acz = an_pc_[2].p;
+        a.sp_term = sp_term_.p; a.an_term = an_term_.p;
+        a.sp_slot = sp_slot_.p; a.an_slot = an_slot_.p;
+        a.ch_spring = ch_spring_.p; a.ch_angle = ch_angle_.p;
         a.step = step_.p;
         return a;
     }
@@ -979,6 +1027,7 @@ class Engine final : public fn_engine {
         nb_start_.ensure((size_t)N_ + 1);
         ncnt_.ensure(N_);
         key_.ensure(N_);
+        rank_.ensure(N_);
         sorted_.ensure(N_);
         if (!nb_list_.p) grow_list(std::max<long long>(32LL * N_, 1 << 20));
         if (!nb_init_) {
@@ -1001,7 +1050,6 @@ class Engine final : public fn_engine {
                 C_ = C;
                 cell_count_.alloc((size_t)C + 1);
                 cell_start_.alloc((size_t)C + 1);
-                cursor_.alloc((size_t)C + 1);
             }
             const int nb = std::max(fnk::scan_blocks_needed(std::max(C_, N_)), 1) + 1;
             if ((int)scan_tmp_.n < nb) {
@@ -1060,14 +1108,19 @@ class Engine final : public fn_engine {
         const fn_settings &s = settings_of(K_FILL_GRID);
         ensure_grid(&s);
         ensure_scratch();
-        fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
-        fnk::Launch<T, P>::hash(pos_[pc_][0].p, pos_[pc_][1].p, pos_[pc_][2].p, N_, b, key_.p, cell_count_.p, stream_);
-        fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, stream_);
-        fnk::copy_int(cell_start_.p, cursor_.p, C_, stream_);
-        fnk::scatter(key_.p, N_, cursor_.p, sorted_.p, stream_);
-        fnk::Launch<T, P>::gather_sorted(sorted_.p, cell_start_.p + C_, pos_[pc_][0].p, pos_[pc_][1].p,
-                                         pos_[pc_][2].p, type_.p, tag_.p, loc_.p, key_.p, sorted_view(), N_, stream_);
+        enqueue_sort(pc_, b);
         grid_valid_ = true;
+    }
+
+    // Counting sort of the beads (positions of buffer pc) into cells.
+    void enqueue_sort(int pc, const fnk::Box &b)
+    {
+        fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
+        fnk::Launch<T, P>::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, b, key_.p, rank_.p, cell_count_.p,
+                                stream_);
+        fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, stream_);
+        fnk::Launch<T, P>::place(key_.p, rank_.p, cell_start_.p, pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p,
+                                 type_.p, tag_.p, loc_.p, N_, sorted_.p, sorted_view(), stream_);
     }
 
     fnk::Sorted sorted_view()
@@ -1173,6 +1226,7 @@ class Engine final : public fn_engine {
         const fnk::Box bl = make_box(settings_of(K_BUILD_LIST));
         const fnk::Box bc = make_box(settings_of(K_RESOLVE_CONTACTS));
         int pc = pc_, fc = fc_;
+        const bool chain = use_chain();
         if (first) {
             fnk::Args<T, P> a = args_base();
             a.drift = bd;
@@ -1184,15 +1238,7 @@ class Engine final : public fn_engine {
         for (int k = 0; k < len; ++k) {
             if (k % guf == 0) {
                 mark(ev ? &ev->rebuild : nullptr);
-                fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
-                fnk::Launch<T, P>::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, bf, key_.p, cell_count_.p,
-                                        stream_);
-                fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, stream_);
-                fnk::copy_int(cell_start_.p, cursor_.p, C_, stream_);
-                fnk::scatter(key_.p, N_, cursor_.p, sorted_.p, stream_);
-                        fnk::Launch<T, P>::gather_sorted(sorted_.p, cell_start_.p + C_, pos_[pc][0].p, pos_[pc][1].p,
-                                                 pos_[pc][2].p, type_.p, tag_.p, loc_.p, key_.p, sorted_view(), N_,
-                                                 stream_);
+                enqueue_sort(pc, bf);
                 search_and_compact(bl, false);
                 mark(ev ? &ev->rebuild : nullptr);
             }
@@ -1208,17 +1254,19 @@ class Engine final : public fn_engine {
             a.gx = frc_[1 - fc][0].p; a.gy = frc_[1 - fc][1].p; a.gz = frc_[1 - fc][2].p;
             std::vector<hipEvent_t> *sv = ev ? &ev->step : nullptr;
             mark(sv);
-            if (spr) fnk::Launch<T, P>::spring_terms(a, stream_);
+            if (spr && !chain) fnk::Launch<T, P>::spring_terms(a, S_, stream_);
             mark(sv);
-            if (ang) fnk::Launch<T, P>::angle_terms(a, stream_);
+            if (ang && !chain) fnk::Launch<T, P>::angle_terms(a, A_, stream_);
             mark(sv);
-            fnk::Launch<T, P>::bead_step(a, spr, ang, !is_last, is_last, stream_);
+            fnk::Launch<T, P>::bead_step(a, spr, ang, !is_last, is_last, chain, stream_);
             mark(sv);
             fc ^= 1;
             if (!is_last) pc ^= 1;
         }
         fnk::add_int(step_.p, len, stream_);
     }
+
+    bool use_chain() const { return chain_on_ && chain_ok_ && n_spr_ == S_ && n_ang_ == A_; }
 
     void run_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
     {
@@ -1301,6 +1349,8 @@ class Engine final : public fn_engine {
     int cap_ = 0, last_total_ = 0;
     int K_ = 64;   // search-row length: entries one bead may collect before compaction
     int chunk_ = 128;                                // steps per captured graph
+    bool chain_on_ = true;                           // use the chain bead kernel where the layout allows
+    bool chain_ok_ = false;                          // the bonds follow the chain layout
     bool graphs_on_ = true;
     long long steps_ = 0, rebuilds_ = 0;
 
@@ -1309,14 +1359,15 @@ class Engine final : public fn_engine {
     DArr<T> lam_, hlf_, hlf2_, rad_;
     DArr<int> type_, tag_, loc_;
     DArr<double> rad_d_, skin_d_;
-    DArr<int> sp_p1_, sp_p2_, sp_type_, sp_status_, sp_break_, sp_start_, sp_ent_;
+    DArr<int> sp_p1_, sp_p2_, sp_type_, sp_status_, sp_break_, sp_start_, sp_slot_;
     DArr<double> sp_l0_, sp_c1_, sp_c2_;
     DArr<T> sp_k_;
-    DArr<int> an_b1_, an_b2_, an_b3_, an_s1_, an_s2_, an_type_, an_status_, an_start_, an_ent_;
+    DArr<int> an_b1_, an_b2_, an_b3_, an_s1_, an_s2_, an_type_, an_status_, an_start_, an_slot_;
     DArr<T> an_th0_, an_k_;
-    DArr<T> sp_f_[3], an_pa_[3], an_pc_[3];   // per-bond terms of the current step
+    DArr<fnk::V3<T>> sp_term_, an_term_;   // bond terms of the current step at the incidence slots
+    DArr<int> ch_spring_, ch_angle_;       // chain layout: spring (i, i+1), angle centred on i
     DArr<fnk::PairC> pairs_;
-    DArr<int> key_, cell_count_, cell_start_, cursor_, sorted_, ncnt_, nb_start_, nb_list_, scan_tmp_, stats_, step_;
+    DArr<int> key_, rank_, cell_count_, cell_start_, sorted_, ncnt_, nb_start_, nb_list_, scan_tmp_, stats_, step_;
     DArr<fnk::SortedBead> so_bead_;
     DArr<int> so_type_, so_cell_, so_slot_, scratch_;
     double reach2_ = 0.0;

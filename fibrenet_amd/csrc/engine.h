@@ -12,23 +12,28 @@ enum { kStatMaxCell = 0, kStatCellsOver = 1, kStatMaxList = 2, kStatListOver = 3
 template <typename T, typename P>
 struct Launch {
     static void advance_position(const Args<T, P> &a, int n_active, hipStream_t s);
-    // bond terms of the step (a.n_springs / a.n_angles bonds)
-    static void spring_terms(const Args<T, P> &a, hipStream_t s);
-    static void angle_terms(const Args<T, P> &a, hipStream_t s);
+    // bond terms of the step: the first a.n_springs / a.n_angles of the
+    // `total` uploaded bonds act, the others leave zero terms
+    static void spring_terms(const Args<T, P> &a, int total, hipStream_t s);
+    static void angle_terms(const Args<T, P> &a, int total, hipStream_t s);
     // ResolveSprings / ResolveAngles: the terms, then their sum into f of each bead
-    static void resolve_springs(const Args<T, P> &a, hipStream_t s);
-    static void resolve_angles(const Args<T, P> &a, hipStream_t s);
+    static void resolve_springs(const Args<T, P> &a, int total, hipStream_t s);
+    static void resolve_angles(const Args<T, P> &a, int total, hipStream_t s);
     static void resolve_contacts(const Args<T, P> &a, hipStream_t s);
     static void advance_velocity(const Args<T, P> &a, int n_active, hipStream_t s);
     static void reset_energy(const Args<T, P> &a, int n_active, hipStream_t s);
     static void drift_first(const Args<T, P> &a, hipStream_t s);
-    // the bead kernel of the fast path (after spring_terms / angle_terms)
-    static void bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, hipStream_t s);
-    static void hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *count,
+    // the bead kernel of the fast path: after spring_terms / angle_terms, or
+    // with chain (a.ch_spring / a.ch_angle layout) evaluating the bonds itself
+    static void bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, bool chain,
+                          hipStream_t s);
+    // counting sort of the beads by cell: hash (cell, rank in cell, counts),
+    // exclusive scan of the counts, place (slots, cell-ordered bead data)
+    static void hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *rank, int *count,
                      hipStream_t s);
-    static void gather_sorted(const int *sorted, const int *n_hashed, const P *px, const P *py, const P *pz,
-                              const int *type, const int *tag, const int *loc, const int *key, const Sorted &so,
-                              int n, hipStream_t s);
+    static void place(const int *key, const int *rank, const int *cell_start, const P *px, const P *py, const P *pz,
+                      const int *type, const int *tag, const int *loc, int n, int *sorted, const Sorted &so,
+                      hipStream_t s);
 };
 
 // Neighbour search over the cell-sorted beads into scratch rows of K entries
@@ -44,7 +49,6 @@ void compact(const int *cnt, const int *nb_start, const int *slot_of, const int 
 void fill_int(int *p, int v, int n, hipStream_t s);
 void copy_int(const int *src, int *dst, int n, hipStream_t s);
 void add_int(int *p, int v, hipStream_t s);
-void scatter(const int *key, int n, int *cursor, int *sorted, hipStream_t s);
 int scan_blocks_needed(int n);
 // out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total.  tmp holds
 // scan_blocks_needed(n) + 1 ints.

@@ -42,66 +42,63 @@ __global__ void __launch_bounds__(kBlock) k_advance_position(Args<T, P> a, int n
     a.fx[i] = f.x; a.fy[i] = f.y; a.fz[i] = f.z;
 }
 
-// Spring terms of one step (kernel.cl:371-417), one thread per spring: the
-// force on p1 (zero for an inactive spring) and, for an HR_FL spring whose
-// stress exceeds its threshold, the status change and the break step.
+// Spring terms of one step (kernel.cl:371-417), one thread per spring of the
+// `total` uploaded: +F at the p1 slot, -F at the p2 slot; for an HR_FL
+// spring whose stress exceeds its threshold, the status change and the break
+// step.  Springs that do not act this step (inactive, or beyond the
+// launched range a.n_springs) leave zero terms.
 template <typename T, typename P>
-__global__ void __launch_bounds__(kBlock) k_spring_terms(Args<T, P> a)
+__global__ void __launch_bounds__(kBlock) k_spring_terms(Args<T, P> a, int total)
 {
     const int s = xcd_block() * kBlock + threadIdx.x;
-    if (s >= a.n_springs) return;
-    V3<T> F;
-    bool broke;
-    const bool on = spring_eval(a, s, F, broke);
-    if (broke) {
-        a.sp_status[s] = 0;
-        a.sp_break[s] = *a.step + a.step_off;
+    if (s >= total) return;
+    V3<T> F{(T)0, (T)0, (T)0};
+    if (s < a.n_springs) {
+        bool broke;
+        if (!spring_eval(a, s, F, broke)) F = V3<T>{(T)0, (T)0, (T)0};
+        if (broke) {
+            a.sp_status[s] = 0;
+            a.sp_break[s] = *a.step + a.step_off;
+        }
     }
-    a.sfx[s] = on ? F.x : (T)0;
-    a.sfy[s] = on ? F.y : (T)0;
-    a.sfz[s] = on ? F.z : (T)0;
+    a.sp_term[a.sp_slot[2 * s]] = F;
+    a.sp_term[a.sp_slot[2 * s + 1]] = V3<T>{F.x * (T)-1, F.y * (T)-1, F.z * (T)-1};
 }
 
-// Angle terms of one step (kernel.cl:274-348), one thread per angle, after
-// the spring terms: an HR_FL angle dies with either of its springs; a live
-// angle writes its forces on b1 and b3 (b2 receives their negated sum).
+// Angle terms of one step (kernel.cl:274-360), one thread per angle of the
+// `total` uploaded, after the spring terms: an HR_FL angle dies with either
+// of its springs; a live angle puts pa at the b1 slot, pc at the b3 slot and
+// -pa-pc at the b2 slot.
 template <typename T, typename P>
-__global__ void __launch_bounds__(kBlock) k_angle_terms(Args<T, P> a)
+__global__ void __launch_bounds__(kBlock) k_angle_terms(Args<T, P> a, int total)
 {
     const int q = xcd_block() * kBlock + threadIdx.x;
-    if (q >= a.n_angles) return;
-    int st = a.an_status[q];
-    if (st == 1 && a.an_type[q] == 1 && (a.sp_status[a.an_s1[q]] == 0 || a.sp_status[a.an_s2[q]] == 0)) {
-        st = 0;
-        a.an_status[q] = 0;
-    }
+    if (q >= total) return;
     V3<T> pa{(T)0, (T)0, (T)0}, pc{(T)0, (T)0, (T)0};
-    if (st == 1) angle_eval(a, q, pa, pc);
-    a.apx[q] = pa.x; a.apy[q] = pa.y; a.apz[q] = pa.z;
-    a.acx[q] = pc.x; a.acy[q] = pc.y; a.acz[q] = pc.z;
+    if (q < a.n_angles) {
+        int st = a.an_status[q];
+        if (st == 1 && a.an_type[q] == 1 && (a.sp_status[a.an_s1[q]] == 0 || a.sp_status[a.an_s2[q]] == 0)) {
+            st = 0;
+            a.an_status[q] = 0;
+        }
+        if (st == 1) angle_eval(a, q, pa, pc);
+    }
+    a.an_term[a.an_slot[3 * q]] = pa;
+    a.an_term[a.an_slot[3 * q + 1]] = pc;
+    a.an_term[a.an_slot[3 * q + 2]] = V3<T>{-pa.x - pc.x, -pa.y - pc.y, -pa.z - pc.z};
 }
 
-// ResolveSprings' accumulation seen from each bead: adds the spring terms.
+// ResolveSprings' / ResolveAngles' accumulation seen from each bead: adds
+// the terms at the bead's slots of `start` to f.
 template <typename T, typename P>
-__global__ void __launch_bounds__(kBlock) k_gather_springs(Args<T, P> a)
+__global__ void __launch_bounds__(kBlock) k_gather_terms(Args<T, P> a, const int *start, const V3<T> *term)
 {
     const int i = xcd_block() * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    if (a.sp_start[i] == a.sp_start[i + 1]) return;
+    const int e0 = start[i], e1 = start[i + 1];
+    if (e0 == e1) return;
     V3<T> f{a.fx[i], a.fy[i], a.fz[i]};
-    gather_spring_terms(a, i, f);
-    a.fx[i] = f.x; a.fy[i] = f.y; a.fz[i] = f.z;
-}
-
-// ResolveAngles' accumulation seen from each bead: adds the angle terms.
-template <typename T, typename P>
-__global__ void __launch_bounds__(kBlock) k_gather_angles(Args<T, P> a)
-{
-    const int i = xcd_block() * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    if (a.an_start[i] == a.an_start[i + 1]) return;
-    V3<T> f{a.fx[i], a.fy[i], a.fz[i]};
-    gather_angle_terms(a, i, f);
+    add_terms(term, e0, e1, f);
     a.fx[i] = f.x; a.fy[i] = f.y; a.fz[i] = f.z;
 }
 
@@ -164,48 +161,188 @@ __global__ void __launch_bounds__(kBlock) k_drift_first(Args<T, P> a)
 // T(T+1)/2 entries; contact-list entries address at most 32).
 constexpr int kPairCap = 32;
 
-// One whole step of one bead after the bond terms: base force, the spring and
-// angle terms in incidence-list order, contacts, closing kick; with NEXT the
-// drift of the following step, with LAST the predicted velocity of this step
-// for read-back.  Reads f of the previous step from f*, writes this step's
-// force to g*.  The terms and their summation order are those of the
-// per-kernel path, so both paths agree bit for bit.
+// Bead state read at the start of the bead kernel.
+template <typename T, typename P>
+struct BeadState {
+    T fpx, fpy, fpz, wx, wy, wz, lam;
+    V3<T> vp;
+    P px, py, pz;
+};
+
+// Loads bead i and returns its base force: the predicted velocity of this
+// step (kernel.cl:200-205) gives damping, plus gravity and walls.
+template <typename T, typename P>
+__device__ __forceinline__ V3<T> bead_begin(const Args<T, P> &a, int i, BeadState<T, P> &b)
+{
+    b.fpx = a.fx[i]; b.fpy = a.fy[i]; b.fpz = a.fz[i];
+    b.wx = a.wx[i]; b.wy = a.wy[i]; b.wz = a.wz[i];
+    b.lam = a.lam[i];
+    b.vp.x = tfma(b.fpx, b.lam, b.wx);
+    b.vp.y = tfma(b.fpy, b.lam, b.wy);
+    b.vp.z = tfma(b.fpz, b.lam, b.wz);
+    b.px = a.px[i]; b.py = a.py[i]; b.pz = a.pz[i];
+    return base_force<T, P>(a.drift, b.vp, b.px, b.py, b.pz);
+}
+
+// Closing kick of the step (kernel.cl:444-464) and, with NEXT, the drift of
+// the next step from image (ix, iy, iz); with LAST the predicted velocity of
+// this step for read-back.
+template <typename T, typename P, bool NEXT, bool LAST>
+__device__ __forceinline__ void bead_end(const Args<T, P> &a, int i, const BeadState<T, P> &b, const V3<T> &f, P ix,
+                                         P iy, P iz)
+{
+    a.gx[i] = f.x; a.gy[i] = f.y; a.gz[i] = f.z;
+    const T h = a.hlf[i];
+    T sx = f.x, sy = f.y, sz = f.z;
+    sx += b.fpx;
+    sy += b.fpy;
+    sz += b.fpz;
+    const T nwx = tfma(sx, h, b.wx), nwy = tfma(sy, h, b.wy), nwz = tfma(sz, h, b.wz);
+    a.wx[i] = nwx; a.wy[i] = nwy; a.wz[i] = nwz;
+    if (LAST) { a.vx[i] = b.vp.x; a.vy[i] = b.vp.y; a.vz[i] = b.vp.z; }
+    if (NEXT) {
+        P px = b.px, py = b.py, pz = b.pz;
+        V3<T> vn;
+        drift(a.drift, f.x, f.y, f.z, nwx, nwy, nwz, b.lam, a.hlf2[i], px, py, pz, ix, iy, iz, vn);
+        a.qx[i] = px; a.qy[i] = py; a.qz[i] = pz;
+        if (a.jx != a.qx) { a.jx[i] = ix; a.jy[i] = iy; a.jz[i] = iz; }
+    }
+}
+
+// One whole step of one bead after the bond kernels: base force, the spring
+// and angle terms at its incidence slots, contacts, closing kick and next
+// drift.  Reads f of the previous step from f*, writes this step's force to
+// g*.  The terms and their summation order are those of the per-kernel
+// path, so both paths agree bit for bit.
 template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST>
 __global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
 {
     __shared__ PairC prs[kPairCap];
+    __shared__ V3<T> cterm[kBlock / 64][kWaveChunk];
     const int i = xcd_block() * kBlock + threadIdx.x;
     for (int k = threadIdx.x; k < a.n_pairs; k += kBlock) prs[k] = a.pairs[k];
     __syncthreads();
-    if (i >= a.n) return;
-    const T fpx = a.fx[i], fpy = a.fy[i], fpz = a.fz[i];
-    const T wx = a.wx[i], wy = a.wy[i], wz = a.wz[i];
-    const T lam = a.lam[i];
-    V3<T> vp;
-    vp.x = tfma(fpx, lam, wx);
-    vp.y = tfma(fpy, lam, wy);
-    vp.z = tfma(fpz, lam, wz);
-    P px = a.px[i], py = a.py[i], pz = a.pz[i];
-    V3<T> f = base_force<T, P>(a.drift, vp, px, py, pz);
-    if (SPR) gather_spring_terms(a, i, f);
-    if (ANG) gather_angle_terms(a, i, f);
-    gather_contacts(a, i, f, prs);
-    a.gx[i] = f.x; a.gy[i] = f.y; a.gz[i] = f.z;
-    const T h = a.hlf[i];
-    T sx = f.x, sy = f.y, sz = f.z;
-    sx += fpx;
-    sy += fpy;
-    sz += fpz;
-    const T nwx = tfma(sx, h, wx), nwy = tfma(sy, h, wy), nwz = tfma(sz, h, wz);
-    a.wx[i] = nwx; a.wy[i] = nwy; a.wz[i] = nwz;
-    if (LAST) { a.vx[i] = vp.x; a.vy[i] = vp.y; a.vz[i] = vp.z; }
-    if (NEXT) {
-        P ix = a.ix[i], iy = a.iy[i], iz = a.iz[i];
-        V3<T> vn;
-        drift(a.drift, f.x, f.y, f.z, nwx, nwy, nwz, lam, a.hlf2[i], px, py, pz, ix, iy, iz, vn);
-        a.qx[i] = px; a.qy[i] = py; a.qz[i] = pz;
-        if (a.jx != a.qx) { a.jx[i] = ix; a.jy[i] = iy; a.jz[i] = iz; }
+    const bool live = i < a.n;
+    const int ic = live ? i : a.n - 1;      // lanes past the end follow along for the wave-wide contacts
+    BeadState<T, P> b;
+    V3<T> f = bead_begin(a, ic, b);
+    if (SPR) add_terms(a.sp_term, a.sp_start[ic], a.sp_start[ic + 1], f);
+    if (ANG) add_terms(a.an_term, a.an_start[ic], a.an_start[ic + 1], f);
+    wave_contacts(a, ic, live, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6]);
+    if (!live) return;
+    if (NEXT) bead_end<T, P, NEXT, LAST>(a, i, b, f, a.ix[i], a.iy[i], a.iz[i]);
+    else bead_end<T, P, NEXT, LAST>(a, i, b, f, (P)0, (P)0, (P)0);
+}
+
+// Chain layout: a wave covers 64 consecutive beads and owns the middle 60;
+// two halo lanes on each side supply the images, spring terms and angle
+// terms that the owned beads need from their chain neighbours.
+constexpr int kHalo = 2;
+constexpr int kChainOwned = 64 - 2 * kHalo;
+
+// The bead kernel for networks whose bonds all follow the chain layout
+// (spring (i, i+1), angle (i-1, i, i+1), numbered along the chain).  Each
+// lane evaluates the spring to its right and the angle centred on it once;
+// the neighbours' images and the terms they need travel between adjacent
+// lanes by wave shuffles instead of through memory.  The terms, their zero
+// values for inactive bonds and their summation order are those of the
+// per-kernel path, so both paths agree bit for bit.  Only owned lanes write
+// bond statuses and bead state.
+template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST>
+__global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
+{
+    __shared__ PairC prs[kPairCap];
+    __shared__ V3<T> cterm[kBlock / 64][kWaveChunk];
+    for (int k = threadIdx.x; k < a.n_pairs; k += kBlock) prs[k] = a.pairs[k];
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int i = (xcd_block() * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kChainOwned + lane - kHalo;
+    const bool in = i >= 0 && i < a.n;
+    const bool own = in && lane >= kHalo && lane < 64 - kHalo;
+    const int ic = in ? i : (i < 0 ? 0 : a.n - 1);
+    P x = (P)0, y = (P)0, z = (P)0;
+    if (in) {
+        x = a.ix[i];
+        y = a.iy[i];
+        z = a.iz[i];
     }
+    // images of beads i + 1 and i - 1 (meaningless in the outermost lanes,
+    // whose results no owned lane uses)
+    const P xn = __shfl_down(x, 1, 64), yn = __shfl_down(y, 1, 64), zn = __shfl_down(z, 1, 64);
+    const P xp = __shfl_up(x, 1, 64), yp = __shfl_up(y, 1, 64), zp = __shfl_up(z, 1, 64);
+
+    // spring (i, i+1): its term and its status after this step's spring pass
+    const int sr = (in && (SPR || ANG)) ? a.ch_spring[i] : -1;
+    V3<T> fr{(T)0, (T)0, (T)0};
+    int str = 1;
+    if (sr >= 0) {
+        str = a.sp_status[sr];
+        if (SPR) {
+            bool broke;
+            if (!spring_force(a, sr, x - xn, y - yn, z - zn, fr, broke)) fr = V3<T>{(T)0, (T)0, (T)0};
+            if (broke) {
+                str = 0;
+                if (own) {
+                    a.sp_status[sr] = 0;
+                    a.sp_break[sr] = *a.step + a.step_off;
+                }
+            }
+        }
+    }
+    const int sl = __shfl_up(sr, 1, 64);
+    const int stl = __shfl_up(str, 1, 64);
+    const V3<T> fl{__shfl_up(fr.x, 1, 64), __shfl_up(fr.y, 1, 64), __shfl_up(fr.z, 1, 64)};
+
+    // angle (i-1, i, i+1) with this step's HR_FL rule (its springs are the
+    // two chain springs of bead i)
+    const int q = (in && ANG) ? a.ch_angle[i] : -1;
+    V3<T> pa{(T)0, (T)0, (T)0}, pc{(T)0, (T)0, (T)0};
+    if (q >= 0) {
+        int st = a.an_status[q];
+        if (st == 1 && a.an_type[q] == 1 && (stl == 0 || str == 0)) {
+            st = 0;
+            if (own) a.an_status[q] = 0;
+        }
+        if (st == 1) angle_force(a, q, xp, yp, zp, x, y, z, xn, yn, zn, pa, pc);
+    }
+    const int ql = __shfl_up(q, 1, 64), qn = __shfl_down(q, 1, 64);
+    const V3<T> pcl{__shfl_up(pc.x, 1, 64), __shfl_up(pc.y, 1, 64), __shfl_up(pc.z, 1, 64)};
+    const V3<T> pan{__shfl_down(pa.x, 1, 64), __shfl_down(pa.y, 1, 64), __shfl_down(pa.z, 1, 64)};
+
+    BeadState<T, P> b;
+    V3<T> f = bead_begin(a, ic, b);
+    if (SPR) {
+        if (sl >= 0) {          // bead i is p2 of spring (i-1, i)
+            f.x += fl.x * (T)-1;
+            f.y += fl.y * (T)-1;
+            f.z += fl.z * (T)-1;
+        }
+        if (sr >= 0) {          // p1 of spring (i, i+1)
+            f.x += fr.x;
+            f.y += fr.y;
+            f.z += fr.z;
+        }
+    }
+    if (ANG) {
+        if (ql >= 0) {          // b3 of angle (i-2, i-1, i)
+            f.x += pcl.x;
+            f.y += pcl.y;
+            f.z += pcl.z;
+        }
+        if (q >= 0) {           // b2 of angle (i-1, i, i+1)
+            f.x += -pa.x - pc.x;
+            f.y += -pa.y - pc.y;
+            f.z += -pa.z - pc.z;
+        }
+        if (qn >= 0) {          // b1 of angle (i, i+1, i+2)
+            f.x += pan.x;
+            f.y += pan.y;
+            f.z += pan.z;
+        }
+    }
+    wave_contacts(a, ic, own, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6]);
+    if (!own) return;
+    bead_end<T, P, NEXT, LAST>(a, i, b, f, x, y, z);
 }
 
 __global__ void k_add_int(int *p, int v) { *p += v; }
@@ -225,29 +362,24 @@ __global__ void __launch_bounds__(kBlock) k_copy_int(const int *src, int *dst, i
 // ---- grid rebuild ----------------------------------------------------------------
 
 // FillGrid's cell of a bead (kernel.cl:492-504) in fp64; -1 outside the grid.
+// The bead's rank among the beads of its cell comes from the count itself
+// (first pass of a counting sort).
 template <typename P>
 __global__ void __launch_bounds__(kBlock) k_hash(const P *px, const P *py, const P *pz, int n, Box b, int *key,
-                                                 int *count)
+                                                 int *rank, int *count)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int cx = (int)floor((double)px[i] / b.sx);
     const int cy = (int)floor((double)py[i] / b.sy);
     const int cz = (int)floor((double)pz[i] / b.sz);
-    int c = -1;
+    int c = -1, r = -1;
     if (cx >= 0 && cx < b.cx && cy >= 0 && cy < b.cy && cz >= 0 && cz < b.cz) {
         c = cx + cy * b.cx + cz * b.cxy;
-        atomicAdd(&count[c], 1);
+        r = atomicAdd(&count[c], 1);
     }
     key[i] = c;
-}
-
-__global__ void __launch_bounds__(kBlock) k_scatter(const int *key, int n, int *cursor, int *sorted)
-{
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int c = key[i];
-    if (c >= 0) sorted[atomicAdd(&cursor[c], 1)] = i;
+    rank[i] = r;
 }
 
 // Wave-wide max and sum (all 64 lanes take part), so that statistics cost one
@@ -265,15 +397,23 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
-// Bead data in cell order, 36 B per bead: fp64 position, tag, loc, type.
+// Second pass of the counting sort: each bead goes to slot cell_start[cell] +
+// rank and takes its data along (fp64 position, tag, loc, type; 40 B per
+// slot), read in bead order.  slot_of is the inverse permutation (-1 for a
+// bead outside the grid).
 template <typename P>
-__global__ void __launch_bounds__(kBlock) k_gather_sorted(const int *sorted, const int *n_hashed, const P *px,
-                                                          const P *py, const P *pz, const int *type, const int *tag,
-                                                          const int *loc, const int *key, Sorted so)
+__global__ void __launch_bounds__(kBlock) k_place(const int *key, const int *rank, const int *cell_start, const P *px,
+                                                  const P *py, const P *pz, const int *type, const int *tag,
+                                                  const int *loc, int n, int *sorted, Sorted so)
 {
-    const int s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= *n_hashed) return;
-    const int i = sorted[s];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int c = key[i];
+    if (c < 0) {
+        so.slot_of[i] = -1;
+        return;
+    }
+    const int s = cell_start[c] + rank[i];
     SortedBead b;
     b.x = (double)px[i];
     b.y = (double)py[i];
@@ -282,7 +422,8 @@ __global__ void __launch_bounds__(kBlock) k_gather_sorted(const int *sorted, con
     b.loc = loc[i];
     so.bead[s] = b;
     so.type[s] = type[i];
-    so.cell[s] = key[i];
+    so.cell[s] = c;
+    sorted[s] = i;
     so.slot_of[i] = s;
 }
 
@@ -515,26 +656,26 @@ void Launch<T, P>::advance_position(const Args<T, P> &a, int n_active, hipStream
     if (n_active > 0) k_advance_position<T, P><<<blocks(n_active), kBlock, 0, s>>>(a, n_active);
 }
 template <typename T, typename P>
-void Launch<T, P>::spring_terms(const Args<T, P> &a, hipStream_t s)
+void Launch<T, P>::spring_terms(const Args<T, P> &a, int total, hipStream_t s)
 {
-    if (a.n_springs > 0) k_spring_terms<T, P><<<blocks(a.n_springs), kBlock, 0, s>>>(a);
+    if (total > 0) k_spring_terms<T, P><<<blocks(total), kBlock, 0, s>>>(a, total);
 }
 template <typename T, typename P>
-void Launch<T, P>::angle_terms(const Args<T, P> &a, hipStream_t s)
+void Launch<T, P>::angle_terms(const Args<T, P> &a, int total, hipStream_t s)
 {
-    if (a.n_angles > 0) k_angle_terms<T, P><<<blocks(a.n_angles), kBlock, 0, s>>>(a);
+    if (total > 0) k_angle_terms<T, P><<<blocks(total), kBlock, 0, s>>>(a, total);
 }
 template <typename T, typename P>
-void Launch<T, P>::resolve_springs(const Args<T, P> &a, hipStream_t s)
+void Launch<T, P>::resolve_springs(const Args<T, P> &a, int total, hipStream_t s)
 {
-    spring_terms(a, s);
-    k_gather_springs<T, P><<<blocks(a.n), kBlock, 0, s>>>(a);
+    spring_terms(a, total, s);
+    k_gather_terms<T, P><<<blocks(a.n), kBlock, 0, s>>>(a, a.sp_start, a.sp_term);
 }
 template <typename T, typename P>
-void Launch<T, P>::resolve_angles(const Args<T, P> &a, hipStream_t s)
+void Launch<T, P>::resolve_angles(const Args<T, P> &a, int total, hipStream_t s)
 {
-    angle_terms(a, s);
-    k_gather_angles<T, P><<<blocks(a.n), kBlock, 0, s>>>(a);
+    angle_terms(a, total, s);
+    k_gather_terms<T, P><<<blocks(a.n), kBlock, 0, s>>>(a, a.an_start, a.an_term);
 }
 template <typename T, typename P>
 void Launch<T, P>::resolve_contacts(const Args<T, P> &a, hipStream_t s)
@@ -559,7 +700,13 @@ void Launch<T, P>::drift_first(const Args<T, P> &a, hipStream_t s)
 template <typename T, typename P, int M>
 static void launch_fused_mask(const Args<T, P> &a, hipStream_t s)
 {
-    k_bead_step<T, P, (M & 1) != 0, (M & 2) != 0, (M & 4) != 0, (M & 8) != 0><<<blocks(a.n), kBlock, 0, s>>>(a);
+    constexpr bool spr = (M & 1) != 0, ang = (M & 2) != 0, next = (M & 4) != 0, last = (M & 8) != 0;
+    if (M & 16) {
+        constexpr int per_block = kChainOwned * (kBlock / 64);
+        k_bead_step_chain<T, P, spr, ang, next, last><<<(a.n + per_block - 1) / per_block, kBlock, 0, s>>>(a);
+    } else {
+        k_bead_step<T, P, spr, ang, next, last><<<blocks(a.n), kBlock, 0, s>>>(a);
+    }
 }
 
 template <typename T, typename P, int... M>
@@ -571,23 +718,24 @@ static void dispatch_fused(const Args<T, P> &a, int mask, hipStream_t s, std::in
 }
 
 template <typename T, typename P>
-void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, hipStream_t s)
+void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, bool chain,
+                             hipStream_t s)
 {
-    const int mask = (spr ? 1 : 0) | (ang ? 2 : 0) | (next ? 4 : 0) | (last ? 8 : 0);
-    dispatch_fused<T, P>(a, mask, s, std::make_integer_sequence<int, 16>{});
+    const int mask = (spr ? 1 : 0) | (ang ? 2 : 0) | (next ? 4 : 0) | (last ? 8 : 0) | (chain ? 16 : 0);
+    dispatch_fused<T, P>(a, mask, s, std::make_integer_sequence<int, 32>{});
 }
 template <typename T, typename P>
-void Launch<T, P>::hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *count,
+void Launch<T, P>::hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *rank, int *count,
                         hipStream_t s)
 {
-    k_hash<P><<<blocks(n), kBlock, 0, s>>>(px, py, pz, n, b, key, count);
+    k_hash<P><<<blocks(n), kBlock, 0, s>>>(px, py, pz, n, b, key, rank, count);
 }
 template <typename T, typename P>
-void Launch<T, P>::gather_sorted(const int *sorted, const int *n_hashed, const P *px, const P *py, const P *pz,
-                                 const int *type, const int *tag, const int *loc, const int *key, const Sorted &so,
-                                 int n, hipStream_t s)
+void Launch<T, P>::place(const int *key, const int *rank, const int *cell_start, const P *px, const P *py,
+                         const P *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
+                         const Sorted &so, hipStream_t s)
 {
-    k_gather_sorted<P><<<blocks(n), kBlock, 0, s>>>(sorted, n_hashed, px, py, pz, type, tag, loc, key, so);
+    k_place<P><<<blocks(n), kBlock, 0, s>>>(key, rank, cell_start, px, py, pz, type, tag, loc, n, sorted, so);
 }
 
 template struct Launch<double, double>;
@@ -613,10 +761,6 @@ void compact(const int *cnt, const int *nb_start, const int *slot_of, const int 
              int cap, hipStream_t s)
 {
     k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, nb_start, slot_of, scratch, K, n, nb_list, cap);
-}
-void scatter(const int *key, int n, int *cursor, int *sorted, hipStream_t s)
-{
-    k_scatter<<<blocks(n), kBlock, 0, s>>>(key, n, cursor, sorted);
 }
 int scan_blocks_needed(int n) { return (n + kScanTile - 1) / kScanTile; }
 void exclusive_scan(const int *in, int n, int *out, int *tmp, hipStream_t s)

@@ -52,6 +52,9 @@ struct PairC {
     double c1, c2, c3, c4, c5, c6;
 };
 
+template <typename T>
+struct V3 { T x, y, z; };
+
 template <typename T, typename P>
 struct Args {
     int n;                                  // beads
@@ -67,19 +70,27 @@ struct Args {
     const int *type, *tag, *loc;
     const T *radius;
     // springs
-    const int *sp_start, *sp_ent, *sp_p1, *sp_p2, *sp_type;
+    const int *sp_start, *sp_p1, *sp_p2, *sp_type;
     int *sp_status, *sp_break;
     const double *sp_l0, *sp_c1, *sp_c2;     // fp64: the failure test is decided as the reference decides it
     const T *sp_k;
     int n_springs;
     // angles
-    const int *an_start, *an_ent, *an_b1, *an_b2, *an_b3, *an_s1, *an_s2, *an_type;
+    const int *an_start, *an_b1, *an_b2, *an_b3, *an_s1, *an_s2, *an_type;
     int *an_status;
     const T *an_th0, *an_k;
     int n_angles;
-    // per-bond terms of the current step: spring force on p1; angle forces on b1, b3
-    T *sfx, *sfy, *sfz;
-    T *apx, *apy, *apz, *acx, *acy, *acz;
+    // Bond terms of the current step, stored at the beads' incidence slots
+    // (bead i's spring terms are sp_term[sp_start[i] .. sp_start[i+1]), its
+    // angle terms an_term[an_start[i] .. an_start[i+1])).  sp_slot[2s+0/1]:
+    // slots of spring s at p1 / p2; an_slot[3q+0/1/2]: slots of angle q at
+    // b1 / b3 / b2.
+    V3<T> *sp_term, *an_term;
+    const int *sp_slot, *an_slot;
+    // Chain layout (fibres as consecutive bead runs): ch_spring[i] is the
+    // spring (p1 i, p2 i+1), ch_angle[i] the angle (b1 i-1, b2 i, b3 i+1),
+    // -1 where there is none.
+    const int *ch_spring, *ch_angle;
     // contacts
     const int *nb_start, *nb_list;
     const PairC *pairs;
@@ -110,9 +121,6 @@ __device__ __forceinline__ double tacos(double x) { return acos(x); }
 __device__ __forceinline__ float tacos(float x) { return acosf(x); }
 __device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
-
-template <typename T>
-struct V3 { T x, y, z; };
 
 // AdvancePosition's drift for one bead (kernel.cl:200-223 and 241-250).
 template <typename T, typename P>
@@ -168,20 +176,16 @@ __device__ __forceinline__ V3<T> base_force(const Box &b, const V3<T> &vp, P px,
     return f;
 }
 
-// Spring s evaluated from the current images (kernel.cl:371-417).  Returns
-// false when the spring is inactive or breaks now; otherwise F is the force on
-// p1 (p2 receives -F).  `broke` reports a break decided by this evaluation.
+// Spring s with image difference d = image(p1) - image(p2) (kernel.cl:371-417).
+// Returns false when the spring is inactive or breaks now; otherwise F is the
+// force on p1 (p2 receives -F).  `broke` reports a break decided by this
+// evaluation.
 template <typename T, typename P>
-__device__ __forceinline__ bool spring_eval(const Args<T, P> &a, int s, V3<T> &F, bool &broke)
+__device__ __forceinline__ bool spring_force(const Args<T, P> &a, int s, P dx, P dy, P dz, V3<T> &F, bool &broke)
 {
     broke = false;
     const int ty = a.sp_type[s];
     if (!(ty == 0 || ty == 1) || a.sp_status[s] != 1) return false;
-    const int p1 = a.sp_p1[s], p2 = a.sp_p2[s];
-    P dx = a.ix[p1], dy = a.iy[p1], dz = a.iz[p1];
-    dx -= a.ix[p2];
-    dy -= a.iy[p2];
-    dz -= a.iz[p2];
     const P d2 = dx * dx + dy * dy + dz * dz;
     // length, extension and the HR_FL stress in fp64 from the float-rounded
     // length (native_sqrt): the break decision is bit-exact in both builds
@@ -202,6 +206,14 @@ __device__ __forceinline__ bool spring_eval(const Args<T, P> &a, int s, V3<T> &F
     F.y = uy * fk;
     F.z = uz * fk;
     return true;
+}
+
+// Spring s evaluated from the current images of its beads.
+template <typename T, typename P>
+__device__ __forceinline__ bool spring_eval(const Args<T, P> &a, int s, V3<T> &F, bool &broke)
+{
+    const int p1 = a.sp_p1[s], p2 = a.sp_p2[s];
+    return spring_force(a, s, a.ix[p1] - a.ix[p2], a.iy[p1] - a.iy[p2], a.iz[p1] - a.iz[p2], F, broke);
 }
 
 // Perpendicular force directions of an angle bond scaled to fa, fc
@@ -238,13 +250,13 @@ __device__ __forceinline__ void angle_dirs(R bax, R bay, R baz, R bcx, R bcy, R 
 // segment the force (|F| from the clamped 0.999999) points along the fp64
 // cross product of the bond vectors, which is rounding noise that only the
 // fp64 evaluation reproduces.
+// Angle q from the images of b1 (x1..), b2 (tx..) and b3 (x3..).
 template <typename T, typename P>
-__device__ __forceinline__ void angle_eval(const Args<T, P> &a, int q, V3<T> &pa_out, V3<T> &pc_out)
+__device__ __forceinline__ void angle_force(const Args<T, P> &a, int q, P x1, P y1, P z1, P tx, P ty, P tz, P x3,
+                                            P y3, P z3, V3<T> &pa_out, V3<T> &pc_out)
 {
-    const int b1 = a.an_b1[q], b2 = a.an_b2[q], b3 = a.an_b3[q];
-    const P tx = a.ix[b2], ty = a.iy[b2], tz = a.iz[b2];
-    const double bax = (double)(a.ix[b1] - tx), bay = (double)(a.iy[b1] - ty), baz = (double)(a.iz[b1] - tz);
-    const double bcx = (double)(a.ix[b3] - tx), bcy = (double)(a.iy[b3] - ty), bcz = (double)(a.iz[b3] - tz);
+    const double bax = (double)(x1 - tx), bay = (double)(y1 - ty), baz = (double)(z1 - tz);
+    const double bcx = (double)(x3 - tx), bcy = (double)(y3 - ty), bcz = (double)(z3 - tz);
     if (sizeof(T) < sizeof(double)) {
         // fp32 build away from the straight and the folded triple: the angle as
         // atan2(|ba x bc|, ba . bc), well conditioned at any angle, and the
@@ -284,56 +296,83 @@ __device__ __forceinline__ void angle_eval(const Args<T, P> &a, int q, V3<T> &pa
         angle_dirs<T, T>((T)bax, (T)bay, (T)baz, (T)bcx, (T)bcy, (T)bcz, (T)fa, (T)fc, pa_out, pc_out);
 }
 
-// Adds the spring terms incident to bead i (kernel.cl:426-436 seen from bead
-// i): +F as p1, -F as p2, in incidence-list (spring index) order.
+// Angle q evaluated from the current images of its beads.
 template <typename T, typename P>
-__device__ __forceinline__ void gather_spring_terms(const Args<T, P> &a, int i, V3<T> &f)
+__device__ __forceinline__ void angle_eval(const Args<T, P> &a, int q, V3<T> &pa_out, V3<T> &pc_out)
 {
-    for (int e = a.sp_start[i], e1 = a.sp_start[i + 1]; e < e1; ++e) {
-        const int ent = a.sp_ent[e];
-        const int s = ent >> 1;
-        if (s >= a.n_springs) continue;
-        if (ent & 1) {
-            f.x += a.sfx[s] * (T)-1;
-            f.y += a.sfy[s] * (T)-1;
-            f.z += a.sfz[s] * (T)-1;
-        } else {
-            f.x += a.sfx[s];
-            f.y += a.sfy[s];
-            f.z += a.sfz[s];
-        }
+    const int b1 = a.an_b1[q], b2 = a.an_b2[q], b3 = a.an_b3[q];
+    angle_force(a, q, a.ix[b1], a.iy[b1], a.iz[b1], a.ix[b2], a.iy[b2], a.iz[b2], a.ix[b3], a.iy[b3], a.iz[b3],
+                pa_out, pc_out);
+}
+
+// Adds the bond terms stored at slots [e0, e1) in slot order (the order in
+// which the reference's bond kernels reach bead i: kernel.cl:350-360,
+// 426-436).
+template <typename T>
+__device__ __forceinline__ void add_terms(const V3<T> *term, int e0, int e1, V3<T> &f)
+{
+    for (int e = e0; e < e1; ++e) {
+        const V3<T> t = term[e];
+        f.x += t.x;
+        f.y += t.y;
+        f.z += t.z;
     }
 }
 
-// Adds the angle terms incident to bead i (kernel.cl:350-360 seen from bead
-// i): pa as b1, pc as b3, -pa-pc as b2, in incidence-list order.
+// Force on bead 1 (position x1.., radius r1) from the contact-list entry
+// `ent` (kernel.cl:833-1000 seen from either bead of a pair: the reaction on
+// the partner is the exact negation of the owner's term, so each side
+// evaluates rij = p_1 - p_j).  Returns false when the pair exerts no force
+// this step.
 template <typename T, typename P>
-__device__ __forceinline__ void gather_angle_terms(const Args<T, P> &a, int i, V3<T> &f)
+__device__ __forceinline__ bool contact_term(const Args<T, P> &a, P x1, P y1, P z1, T r1, int ent,
+                                             const PairC *table, V3<T> &t)
 {
-    for (int e = a.an_start[i], e1 = a.an_start[i + 1]; e < e1; ++e) {
-        const int ent = a.an_ent[e];
-        const int q = ent >> 2;
-        const int role = ent & 3;
-        if (q >= a.n_angles) continue;
-        if (role == 0) {
-            f.x += a.apx[q];
-            f.y += a.apy[q];
-            f.z += a.apz[q];
-        } else if (role == 2) {
-            f.x += a.acx[q];
-            f.y += a.acy[q];
-            f.z += a.acz[q];
-        } else {
-            f.x += -a.apx[q] - a.acx[q];
-            f.y += -a.apy[q] - a.acy[q];
-            f.z += -a.apz[q] - a.acz[q];
-        }
+    const int j = ent & kIdxMask;
+    const PairC pr = table[ent >> kPidShift];
+    const Box &b = a.force;
+    P rx = x1 - a.px[j], ry = y1 - a.py[j], rz = z1 - a.pz[j];
+    P len = rx * rx + ry * ry + rz * rz;
+    if (len > (P)b.hbs) {
+        if (rx * rx > (P)b.hbs) rx += (rx > (P)0) ? (P)-b.lx : (P)b.lx;
+        if (ry * ry > (P)b.hbs) ry += (ry > (P)0) ? (P)-b.ly : (P)b.ly;
+        if (rz * rz > (P)b.hbs) rz += (rz > (P)0) ? (P)-b.lz : (P)b.lz;
+        len = rx * rx + ry * ry + rz * rz;
     }
+    T coef;
+    T r;
+    if (pr.type == 1) {                           // Lennard-Jones, kernel.cl:875-919
+        if (!((double)len < b.cutoff_sq)) return false;
+        r = (T)native_sqrt((double)len);
+        if (!((double)r < pr.c3)) return false;
+        const T r2 = (T)len;
+        const T c1 = (T)pr.c1;
+        const T sigma2 = c1 * c1;
+        const T fr2 = sigma2 / r2;
+        const T fr6 = fr2 * fr2 * fr2;
+        coef = (T)48 * (T)pr.c2 * fr6 * (fr6 - (T)0.5) / r2;
+        coef -= (T)pr.c4;
+    } else if (pr.type == 2) {                    // penalty, kernel.cl:922-957
+        r = (T)native_sqrt((double)len);
+        coef = (r1 + a.radius[j]) - r;
+        if (!(coef > (T)0)) return false;
+        coef *= (T)pr.c1;
+    } else if (pr.type == 3) {                    // two-step harmonic, kernel.cl:960-999
+        if (!((double)len < pr.c4)) return false;
+        r = (T)native_sqrt((double)len);
+        coef = ((T)pr.c2 - r) * (T)pr.c1;
+        coef += ((T)pr.c6 - r) * (T)pr.c5;
+    } else {
+        return false;
+    }
+    const T inv = (r != (T)0) ? (T)1 / r : (T)0;
+    t.x = coef * ((T)rx * inv);
+    t.y = coef * ((T)ry * inv);
+    t.z = coef * ((T)rz * inv);
+    return true;
 }
 
-// Adds the contact forces of bead i's two-sided list (kernel.cl:833-1000 seen
-// from either bead of a pair: the reaction on the partner is the exact
-// negation of the owner's term, so each side evaluates rij = p_i - p_j).
+// Adds the contact forces of bead i's two-sided list in list order.
 template <typename T, typename P>
 __device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T> &f, const PairC *table)
 {
@@ -341,49 +380,75 @@ __device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T
     if (e0 == e1) return;
     const P x1 = a.px[i], y1 = a.py[i], z1 = a.pz[i];
     const T r1 = a.radius[i];
-    const Box &b = a.force;
     for (int e = e0; e < e1; ++e) {
-        const int ent = a.nb_list[e];
-        const int j = ent & kIdxMask;
-        const PairC pr = table[ent >> kPidShift];
-        P rx = x1 - a.px[j], ry = y1 - a.py[j], rz = z1 - a.pz[j];
-        P len = rx * rx + ry * ry + rz * rz;
-        if (len > (P)b.hbs) {
-            if (rx * rx > (P)b.hbs) rx += (rx > (P)0) ? (P)-b.lx : (P)b.lx;
-            if (ry * ry > (P)b.hbs) ry += (ry > (P)0) ? (P)-b.ly : (P)b.ly;
-            if (rz * rz > (P)b.hbs) rz += (rz > (P)0) ? (P)-b.lz : (P)b.lz;
-            len = rx * rx + ry * ry + rz * rz;
+        V3<T> t;
+        if (!contact_term(a, x1, y1, z1, r1, a.nb_list[e], table, t)) continue;
+        f.x += t.x;
+        f.y += t.y;
+        f.z += t.z;
+    }
+}
+
+// Contact-list entries evaluated per pass by one wave (LDS: kWaveChunk terms).
+constexpr int kWaveChunk = 256;
+
+// The same sums as gather_contacts, evaluated by the whole wave: the lists
+// of the wave's beads are laid end to end and each lane evaluates every
+// 64th entry, so a wave costs (its total entries / 64) rounds instead of
+// its longest list; the terms go through LDS and each bead adds its own in
+// list order.  An entry without force stores -0.0, which leaves any sum
+// unchanged, so the result equals gather_contacts bit for bit.  Every lane
+// of the wave must call this; `live` lanes own bead i at position x1...
+// `term` is this wave's LDS area of kWaveChunk entries.
+template <typename T, typename P>
+__device__ __forceinline__ void wave_contacts(const Args<T, P> &a, int i, bool live, P x1, P y1, P z1, V3<T> &f,
+                                              const PairC *table, V3<T> *term)
+{
+    const int lane = threadIdx.x & 63;
+    const int e0 = live ? a.nb_start[i] : 0;
+    const int cnt = live ? a.nb_start[i + 1] - e0 : 0;
+    const T r1 = live ? a.radius[i] : (T)0;
+    int incl = cnt;                                   // inclusive scan of the counts over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const int off = incl - cnt;
+    const int total = __shfl(incl, 63, 64);
+    for (int base = 0; base < total; base += kWaveChunk) {
+#pragma unroll 2
+        for (int r = 0; r < kWaveChunk; r += 64) {
+            const int k = base + r + lane;
+            int L = 0;                                // lane whose entries hold k: largest L with off_L <= k
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = L + step;
+                const int oc = __shfl(off, cand & 63, 64);
+                if (cand < 64 && oc <= k) L = cand;
+            }
+            const int eL = __shfl(e0, L, 64) + (k - __shfl(off, L, 64));
+            const P xL = __shfl(x1, L, 64), yL = __shfl(y1, L, 64), zL = __shfl(z1, L, 64);
+            const T rL = __shfl(r1, L, 64);
+            V3<T> t{(T)-0.0, (T)-0.0, (T)-0.0};
+            if (k < total && !contact_term(a, xL, yL, zL, rL, a.nb_list[eL], table, t))
+                t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
+            term[r + lane] = t;
         }
-        T coef;
-        T r;
-        if (pr.type == 1) {                           // Lennard-Jones, kernel.cl:875-919
-            if (!((double)len < b.cutoff_sq)) continue;
-            r = (T)native_sqrt((double)len);
-            if (!((double)r < pr.c3)) continue;
-            const T r2 = (T)len;
-            const T c1 = (T)pr.c1;
-            const T sigma2 = c1 * c1;
-            const T fr2 = sigma2 / r2;
-            const T fr6 = fr2 * fr2 * fr2;
-            coef = (T)48 * (T)pr.c2 * fr6 * (fr6 - (T)0.5) / r2;
-            coef -= (T)pr.c4;
-        } else if (pr.type == 2) {                    // penalty, kernel.cl:922-957
-            r = (T)native_sqrt((double)len);
-            coef = (r1 + a.radius[j]) - r;
-            if (!(coef > (T)0)) continue;
-            coef *= (T)pr.c1;
-        } else if (pr.type == 3) {                    // two-step harmonic, kernel.cl:960-999
-            if (!((double)len < pr.c4)) continue;
-            r = (T)native_sqrt((double)len);
-            coef = ((T)pr.c2 - r) * (T)pr.c1;
-            coef += ((T)pr.c6 - r) * (T)pr.c5;
-        } else {
-            continue;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int k0 = off > base ? off : base;
+        const int k1 = (off + cnt < base + kWaveChunk) ? off + cnt : base + kWaveChunk;
+        for (int k = k0; k < k1; ++k) {
+            const V3<T> t = term[k - base];
+            f.x += t.x;
+            f.y += t.y;
+            f.z += t.z;
         }
-        const T inv = (r != (T)0) ? (T)1 / r : (T)0;
-        f.x += coef * ((T)rx * inv);
-        f.y += coef * ((T)ry * inv);
-        f.z += coef * ((T)rz * inv);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 

@@ -130,6 +130,10 @@ int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_spri
 
 /* Launch options of the fast path (no reference counterpart):
  *   "chunk"  n    steps captured per hipGraph (rounded to grid_update_freq)
+ *   "chain"  1/0  bead kernel that evaluates chain bonds (spring (i, i+1),
+ *                 angle (i-1, i, i+1), numbered along the chain) in-wave,
+ *                 used when every bond follows that layout
+ *   "chain_layout" (read only) 1 when every bond follows the chain layout
  *   "graphs" 1/0  launch through captured hipGraphs or directly
  * All options give bit-identical results; they change only the speed. */
 int fn_set_option(fn_engine *eng, const char *name, int value);

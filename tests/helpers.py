@@ -24,6 +24,29 @@ def small_tensile(seed=12, n_fibres=50, n_beads=20, box=(60.0, 60.0, 30.0), weak
     return net
 
 
+def crosslinked(seed=13, n_links=60):
+    """The tensile network plus HR springs between beads of different fibres
+    (both index orders) and its angles listed in reverse: bonds outside the
+    chain layout, so the fast path sums bond terms through incidence slots."""
+    net = small_tensile(seed=seed, weak=True)
+    img = R.vec(net.beads, "image_position")
+    fib = np.asarray(net.info["tag"])
+    d = np.linalg.norm(img[:, None, :] - img[None, :, :], axis=2)
+    i, j = np.nonzero((d < 2.0) & (fib[:, None] != fib[None, :]))
+    keep = i < j
+    i, j = i[keep][:n_links], j[keep][:n_links]
+    extra = np.zeros(len(i), R.SPRING)
+    extra["type"] = 0
+    extra["status"] = 1
+    extra["p1"] = np.where(np.arange(len(i)) % 2 == 0, i, j)
+    extra["p2"] = np.where(np.arange(len(i)) % 2 == 0, j, i)
+    extra["rest_length"] = 0.9 * d[i, j]
+    extra["stiffness"] = 50.0
+    net.springs = np.concatenate([net.springs, extra])
+    net.angles = net.angles[::-1].copy()
+    return net
+
+
 def engine_for(net, precision):
     eng = Engine(precision)
     ids = bind_standard(eng, net)

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from fibrenet_amd import records as R
-from helpers import (engine_for, field_errors, oracle_for, read_beads, run_engine, small_periodic,
+from helpers import (crosslinked, engine_for, field_errors, oracle_for, read_beads, run_engine, small_periodic,
                      small_tensile)
 
 pytestmark = pytest.mark.gpu
@@ -28,7 +28,12 @@ TOL = {"f64": {f: 1e-9 for f in ("position", "image_position", "velocity", "forc
                "velocity_old": 2e-3, "force_old": 1e-2}}
 
 WORKLOADS = {"periodic_hs": lambda: small_periodic(), "tensile_lj": lambda: small_tensile(),
-             "tensile_lj_breaking": lambda: small_tensile(weak=True)}
+             "tensile_lj_breaking": lambda: small_tensile(weak=True), "crosslinked": lambda: crosslinked()}
+
+# Networks whose bonds follow the chain layout (fast path: bonds evaluated in
+# the bead kernel by wave shuffles) and one that does not (bond kernels and
+# incidence slots).
+CHAIN = {"periodic_hs": 1, "tensile_lj": 1, "tensile_lj_breaking": 1, "crosslinked": 0}
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -65,6 +70,31 @@ def test_fused_equals_per_kernel(need_gpu, workload, precision):
     assert np.array_equal(a.read_buffer(ia["spring"], R.SPRING)["status"],
                           b.read_buffer(ib["spring"], R.SPRING)["status"])
     assert np.array_equal(a.export_pairs(), b.export_pairs())
+
+
+@pytest.mark.parametrize("workload", sorted(WORKLOADS))
+def test_chain_layout_detected(need_gpu, workload):
+    net = WORKLOADS[workload]()
+    eng, _ = engine_for(net, "f64")
+    assert eng.get_option("chain_layout") == CHAIN[workload]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("workload", ["tensile_lj_breaking", "periodic_hs"])
+def test_chain_kernel_equals_slot_path(need_gpu, workload, precision):
+    """The in-wave chain bead kernel and the bond-kernel + slot path give the
+    same bytes (same terms, same summation order)."""
+    net = WORKLOADS[workload]()
+    a, ia = engine_for(net, precision)
+    b, ib = engine_for(net, precision)
+    b.set_option("chain", 0)
+    for n in (3, 150):
+        run_engine(a, net, n, fast=True)
+        run_engine(b, net, n, fast=True)
+    assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
+    for kind, rec in (("spring", R.SPRING), ("angle", R.ANGLE)):
+        assert a.read_buffer(ia[kind], rec).tobytes() == b.read_buffer(ib[kind], rec).tobytes()
+    assert np.array_equal(a.export_breaks(len(net.springs)), b.export_breaks(len(net.springs)))
 
 
 @pytest.mark.parametrize("workload", sorted(WORKLOADS))
