@@ -28,9 +28,21 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(net, pairs, tb, aliased):
+def algorithmic_bytes(net, pairs, tb, aliased, chain):
     """HBM bytes each launch of a step must move at least (every datum read or
-    written once; tb = bytes of a force/velocity component, positions fp64):
+    written once; tb = bytes of a force/velocity component, positions fp64).
+
+    chain layout (one launch per step, the bead kernel evaluating the bonds):
+      per bead position read and next position written (48 B), image read
+      and written unless it is the position array (48 B), force of the
+      previous step, velocity_old read and written, this step's force
+      written, three mass coefficients (12 tb), radius (tb), list offset and
+      count (8 B), chain spring and angle index (8 B); per spring type,
+      status, rest length (16 B), stiffness (tb) and the failure constants of
+      an HR_FL spring (16 B); per angle type, status (8 B), rest angle and
+      stiffness (2 tb); per listed pair two list entries (8 B).
+
+    otherwise (three launches per step):
 
     spring-term kernel: per spring p1, p2, type, status (16 B), rest length
       (8 B), stiffness (tb), the two failure constants of an HR_FL spring
@@ -53,10 +65,15 @@ def algorithmic_bytes(net, pairs, tb, aliased):
     s_fl = int(np.sum(net.springs["type"] == 1))
     s = s_hr + s_fl
     a = len(net.angles)
+    rebuild = 56 * n + 8 * pairs
+    if chain:
+        bead = ((48 + (0 if aliased else 48) + 13 * tb + 16) * n + (16 + tb) * s + 16 * s_fl
+                + (8 + 2 * tb) * a + 8 * pairs)
+        return (dict(spring=0, angle=0, bead=bead), rebuild,
+                dict(beads=n, springs_hr=s_hr, springs_hr_fl=s_fl, angles=a, pairs=pairs))
     spring = (16 + 8 + tb + 3 * tb) * s + 16 * s_fl + 24 * n if s else 0
     angle = (28 + 2 * tb + 6 * tb) * a + 4 * s + 24 * n if a else 0
     bead = (48 + (0 if aliased else 48) + 12 * tb + 12) * n + (8 + 3 * tb) * s + (12 + 6 * tb) * a + 8 * pairs
-    rebuild = 56 * n + 8 * pairs
     return (dict(spring=spring, angle=angle, bead=bead), rebuild,
             dict(beads=n, springs_hr=s_hr, springs_hr_fl=s_fl, angles=a, pairs=pairs))
 
@@ -78,7 +95,7 @@ def main():
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--workload", default="tensile_lj", choices=["tensile_lj", "periodic_hs"])
     ap.add_argument("--fibres", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-bytes", type=float, default=0,
                     help="also run one device copy of this many bytes (counter calibration)")
@@ -133,10 +150,14 @@ def main():
     n = len(net.beads)
     pairs = st["list_entries"] // 2
     aliased = bool(st["img_aliases_pos"])
-    b_kernel, b_rebuild, counts = algorithmic_bytes(net, pairs, 4 if args.precision == "f32" else 8, aliased)
+    chain = bool(eng.get_option("chain") and eng.get_option("chain_layout"))
+    b_kernel, b_rebuild, counts = algorithmic_bytes(net, pairs, 4 if args.precision == "f32" else 8, aliased, chain)
     n_t = max(tim["steps"], 1)
     kern_ms = {"spring": tim["spring_ms"] / n_t, "angle": tim["angle_ms"] / n_t, "bead": tim["bead_ms"] / n_t}
-    names = {"spring": "k_spring_terms", "angle": "k_angle_terms", "bead": "k_bead_step"}
+    names = {"spring": "k_spring_terms", "angle": "k_angle_terms",
+             "bead": "k_bead_step_chain" if chain else "k_bead_step"}
+    if chain:   # the bond kernels are not launched; their event pairs bracket nothing
+        kern_ms = {"bead": kern_ms["bead"]}
     dom = max(kern_ms, key=kern_ms.get)
     rebuild_ms = tim["rebuild_ms"] / max(tim["rebuilds"], 1)
     achieved = b_kernel[dom] / (kern_ms[dom] * 1e-3) / 1e9
@@ -144,16 +165,9 @@ def main():
     value = n * args.steps / wall
 
     copy = None
-    if args.copy_bytes > 0:
-        import torch
-        nb = int(args.copy_bytes) // 4
-        a = torch.empty(nb, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        a.fill_(1.0)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        copy = nb * 4
+    if args.copy_bytes > 0:   # one k_copy_probe launch of known size, for counter calibration
+        nb = (int(args.copy_bytes) + 15) // 16 * 16
+        copy = {"bytes": nb, "ms": eng.copy_probe(nb)}
 
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "traffic_%s_%s_%s.json" % (names[dom], args.workload, args.precision))
@@ -177,7 +191,8 @@ def main():
         "dtype": "f32 forces/velocities, f64 positions" if args.precision == "f32" else "f64",
         "data": "synthetic (generated fibre network, seed %d)" % net.meta["seed"],
         "config": {"workload": desc, "beads": n, "springs": len(net.springs), "angles": len(net.angles),
-                   "listed_pairs": pairs, "grid_update_freq": net.grid_update_freq, "precision": args.precision},
+                   "listed_pairs": pairs, "grid_update_freq": net.grid_update_freq, "precision": args.precision,
+                   "bond_path": "chain (in-wave)" if chain else "bond kernels + incidence slots"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": names[dom], "algorithmic_bytes_per_launch": b_kernel[dom],

@@ -46,6 +46,7 @@ SIGNATURES = [
     ("fn_get_option", _I, [_VP, _CP, _c.POINTER(_I)]),
     ("fn_set_timing", _I, [_VP, _I]),
     ("fn_get_timing", _I, [_VP, _VP]),
+    ("fn_copy_probe", _I, [_VP, _c.c_longlong, _VP]),
     ("fn_model_load", _I, [_CP, _c.POINTER(_VP)]),
     ("fn_model_free", _I, [_VP]),
     ("fn_model_error", _CP, []),
@@ -260,6 +261,12 @@ class Engine:
         v = ctypes.c_int()
         self._chk(lib().fn_get_option(self.h, name.encode(), ctypes.byref(v)))
         return v.value
+
+    def copy_probe(self, nbytes):
+        """Streaming device copy of nbytes; returns its duration in ms."""
+        ms = ctypes.c_double()
+        self._chk(lib().fn_copy_probe(self.h, int(nbytes), ctypes.byref(ms)))
+        return ms.value
 
     def set_timing(self, on):
         self._chk(lib().fn_set_timing(self.h, int(bool(on))))

@@ -214,6 +214,7 @@ struct fn_engine {
     virtual void set_option(const std::string &name, int value) = 0;
     virtual int get_option(const std::string &name) = 0;
     virtual void timing(fn_timing *out) = 0;
+    virtual void copy_probe(long long bytes, double *ms) = 0;
 };
 
 namespace {
@@ -570,6 +571,26 @@ class Engine final : public fn_engine {
     }
 
     void timing(fn_timing *out) override { *out = timing_acc_; }
+
+    void copy_probe(long long bytes, double *ms) override
+    {
+        const size_t n16 = ((size_t)bytes + 15) / 16;
+        DArr<int> src, dst;
+        src.alloc(n16 * 4);
+        dst.alloc(n16 * 4);
+        fnk::fill_int(src.p, 1, (int)std::min<size_t>(n16 * 4, (size_t)INT32_MAX), stream_);
+        hipEvent_t e0 = new_event(), e1 = new_event();
+        HIPCK(hipEventRecord(e0, stream_));
+        fnk::copy_probe(src.p, dst.p, n16, stream_);
+        HIPCK(hipEventRecord(e1, stream_));
+        HIPCK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIPCK(hipStreamSynchronize(stream_));
+    }
 
     void set_option(const std::string &name, int value) override
     {
@@ -1549,6 +1570,14 @@ int fn_get_option(fn_engine *eng, const char *name, int *value)
 int fn_set_timing(fn_engine *eng, int on)
 {
     return guarded(eng, [&] { eng->set_timing(on != 0); });
+}
+
+int fn_copy_probe(fn_engine *eng, long long bytes, double *ms)
+{
+    return guarded(eng, [&] {
+        if (bytes <= 0 || !ms) throw Error(FN_ERR_ARG, "copy probe needs a positive size and a destination");
+        eng->copy_probe(bytes, ms);
+    });
 }
 
 int fn_get_timing(fn_engine *eng, fn_timing *out)

@@ -49,6 +49,8 @@ void compact(const int *cnt, const int *nb_start, const int *slot_of, const int 
 void fill_int(int *p, int v, int n, hipStream_t s);
 void copy_int(const int *src, int *dst, int n, hipStream_t s);
 void add_int(int *p, int v, hipStream_t s);
+// Streaming copy of n16 16-byte words (the reference point for traffic counters).
+void copy_probe(const void *src, void *dst, size_t n16, hipStream_t s);
 int scan_blocks_needed(int n);
 // out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total.  tmp holds
 // scan_blocks_needed(n) + 1 ints.

@@ -750,6 +750,17 @@ void copy_int(const int *src, int *dst, int n, hipStream_t s)
     if (n > 0) k_copy_int<<<blocks(n), kBlock, 0, s>>>(src, dst, n);
 }
 void add_int(int *p, int v, hipStream_t s) { k_add_int<<<1, 1, 0, s>>>(p, v); }
+
+// Plain streaming copy, 16 B per lane and iteration, grid-stride.
+__global__ void __launch_bounds__(kBlock) k_copy_probe(const int4 *src, int4 *dst, size_t n)
+{
+    for (size_t k = (size_t)blockIdx.x * kBlock + threadIdx.x; k < n; k += (size_t)gridDim.x * kBlock) dst[k] = src[k];
+}
+
+void copy_probe(const void *src, void *dst, size_t n16, hipStream_t s)
+{
+    k_copy_probe<<<4096, kBlock, 0, s>>>((const int4 *)src, (int4 *)dst, n16);
+}
 void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const double *radius,
             const double *skin, const PairC *pairs, const Box &b, double reach2, int *cnt, int *scratch, int K,
             int *stats, int n, hipStream_t s)

@@ -156,6 +156,12 @@ typedef struct fn_timing {
 int fn_set_timing(fn_engine *eng, int on);
 int fn_get_timing(fn_engine *eng, fn_timing *out);
 
+/* Device-to-device copy of `bytes` (rounded up to 16) by a plain streaming
+ * kernel (k_copy_probe) on the engine's stream, between two buffers
+ * allocated for the call: the reference point that scales hardware traffic
+ * counters to bytes (no reference counterpart).  *ms receives its duration. */
+int fn_copy_probe(fn_engine *eng, long long bytes, double *ms);
+
 /* Debug exports for parity checks (no reference counterpart). */
 /* Cell index of every bead at the last rebuild (-1: outside the grid). */
 int fn_export_cells(fn_engine *eng, int *cell_of_bead);

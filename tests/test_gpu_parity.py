@@ -148,3 +148,59 @@ def test_stagewise_f32_rebuild_exact(need_gpu):
     eng.finish()
     assert np.array_equal(eng.export_cells(len(net.beads)), orc.cell_of_bead)
     assert np.array_equal(eng.export_pairs(), orc.pair_set())
+
+
+def _cadence_network():
+    """The oracle's cadence case (tests/test_oracle.py) as an engine network."""
+    from fibrenet_amd import generate as G
+    from test_oracle import _lj_coeffs, beads_of, pair, settings
+    s = settings(box=(30.0, 30.0, 30.0), unit=10.0, cutoff=2.5, dt=0.01)
+    b, q = beads_of([[10.0, 15.0, 15.0], [13.55, 15.0, 15.0]], s=s, mass=0.0)
+    for f in ("velocity", "velocity_old"):
+        b[f]["x"][1] = -10.0
+    net = G.Network()
+    net.beads, net.info, net.settings = b, q, s
+    net.springs, net.angles = np.zeros(0, R.SPRING), np.zeros(0, R.ANGLE)
+    net.pairs = np.array([pair("lj", **_lj_coeffs())], R.PAIR)
+    net.cell_nh = G.grid_neighbours(0, 3, 3, 3)
+    net.grid_update_freq = 8
+    return net
+
+
+@pytest.mark.parametrize("fast", [True, False])
+@pytest.mark.parametrize("calls", [(13,), (17,), (11, 2), (5, 8), (10, 3, 3)])
+def test_rebuild_cadence_matches_oracle(need_gpu, calls, fast):
+    """Rebuild at the first step of every call and every grid_update_freq steps
+    of it: the engine sees the approaching pair exactly when the oracle does."""
+    from test_oracle import cadence_case
+    net = _cadence_network()
+    eng, ids = engine_for(net, "f64")
+    for n in calls:
+        run_engine(eng, net, n, fast=fast)
+    f = R.vec(read_beads(eng, ids), "force")[0]
+    ref, _ = cadence_case(calls)
+    assert np.allclose(f, ref, rtol=1e-12, atol=0.0) and ((f == 0).all() == (ref == 0).all())
+
+
+@pytest.mark.parametrize("fast", [True, False])
+def test_simulation_minimize_then_run_matches_oracle(need_gpu, tmp_path, fast):
+    """fn_sim from files (SimulationManager's loaders), MinimizeSystemEnergy
+    with reset over 35 steps in chunks of 10 (the remainder dropped), then a
+    run of 37 steps: the f64 engine follows the oracle's MinimizeSystemEnergy
+    and RunSimulationLoop to the f64 tolerance."""
+    from fibrenet_amd import Model, Simulation, generate as G
+    net = small_tensile()
+    net.input_extra = net.input_extra + ["minimize_steps 35", "minimize_substeps 10", "minimize_reset yes"]
+    path = G.write_files(net, str(tmp_path), "tensile")
+    sim = Simulation(path, "f64")
+    sim.minimize(fast=fast)
+    sim.run_loop(37, fast=fast)
+    got = sim.engine.read_buffer(sim.buffer_id("bead"), R.BEAD)
+    m = Model(path)
+    orc = oracle_for(m)
+    orc.minimize(35, 10, True, m.grid_update_freq)
+    orc.run(37, m.grid_update_freq)
+    errs = field_errors(got, orc.beads)
+    for f, e in errs.items():
+        assert e <= TOL["f64"][f], (f, e)
+

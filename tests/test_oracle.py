@@ -147,6 +147,79 @@ def test_massless_bead_moves_at_constant_velocity():
     assert np.allclose(vec(o.beads, "position")[0], [10.0 + 0.01 * 100 * 0.01, 10.0, 10.0], atol=1e-13)
 
 
+def _lj_coeffs(eps=1.0, sig=1.0, rc=2.5):
+    fr6 = (sig * sig / (rc * rc)) ** 3
+    return dict(c1=sig, c2=eps, c3=rc, c4=48.0 * eps * fr6 * (fr6 - 0.5) / (rc * rc))
+
+
+def cadence_case(calls, g=8):
+    """Bead B (massless, x-velocity -10, dt 0.01) closes on bead A (massless,
+    at rest) from 3.55 away; after step k (0-based) they are 3.55 - 0.1 (k+1)
+    apart, inside the LJ cutoff 2.5 from step 10 on.  The pair is listed only
+    at a rebuild, so A feels a force at the end iff some rebuild at or after
+    step 10 happened.  Returns A's final force and the final distance."""
+    s = settings(box=(30.0, 30.0, 30.0), unit=10.0, cutoff=2.5, dt=0.01)
+    b, q = beads_of([[10.0, 15.0, 15.0], [13.55, 15.0, 15.0]], s=s, mass=0.0)
+    for f in ("velocity", "velocity_old"):
+        b[f]["x"][1] = -10.0
+    o = Oracle(b, q, springs_of([]), angles_of([]), np.array([pair("lj", **_lj_coeffs())], R.PAIR), s)
+    for n in calls:
+        o.run(n, g)
+    p = vec(o.beads, "position")
+    return vec(o.beads, "force")[0], p[1, 0] - p[0, 0]
+
+
+@pytest.mark.parametrize("calls, listed", [((13,), False),       # rebuilds at steps 0, 8: pair still out
+                                           ((17,), True),        # rebuild at step 16 lists it
+                                           ((11, 2), True),      # the second call rebuilds at its first step
+                                           ((5, 8), False),      # ... at global steps 0, 5 only
+                                           ((10, 3, 3), True)])
+def test_rebuild_cadence_restarts_per_call(calls, listed):
+    """RunSimulationLoop rebuilds at the first step of every call and then
+    every grid_update_freq steps of that call (SimulationManager.cpp:151-181,
+    Q13); a listed LJ pair then acts with the closed-form force."""
+    f, d = cadence_case(calls)
+    if not listed:
+        assert np.array_equal(f, [0.0, 0.0, 0.0])
+        return
+    c = _lj_coeffs()
+    h = 1e-6
+    F = lambda x: -(_lj_v(x + h, c["c2"], c["c1"]) - _lj_v(x - h, c["c2"], c["c1"])) / (2 * h)
+    mag = F(d) / d - F(c["c3"]) / c["c3"]
+    assert abs(d - (3.55 - 0.1 * sum(calls))) < 1e-12
+    assert np.allclose(f, [-mag, 0.0, 0.0], rtol=1e-6, atol=1e-12)
+
+
+def test_minimize_resets_each_chunk():
+    """MinimizeSystemEnergy with reset (SimulationManager.cpp:111-143): zeroed
+    forces and velocities before the loop and after each of steps // sub_steps
+    chunks (the remainder is dropped).  From rest with f = f_old = 0 a bead
+    under constant gravity g moves g dt^2 s (s - 1) / (2 m) per chunk of s
+    steps: the first drift sees no force, then each step adds one more
+    g dt^2 / m."""
+    g, dt, m, sub = -2.0, 0.01, 1.5, 10
+    s = settings(box=(50.0, 50.0, 50.0), dt=dt, gravity=(0.0, 0.0, g))
+    b, q = beads_of([[20.0, 20.0, 40.0]], s=s, mass=m, force=[0.0, 0.0, g])
+    o = Oracle(b, q, springs_of([]), angles_of([]), NOPAIR, s)
+    o.minimize(35, sub, True)
+    z = vec(o.beads, "position")[0, 2]
+    assert abs(z - (40.0 + 3 * g * dt * dt * sub * (sub - 1) / (2 * m))) < 1e-12
+    for f in ("velocity", "velocity_old", "force", "force_old"):
+        assert np.array_equal(vec(o.beads, f)[0], [0.0, 0.0, 0.0]), f
+
+
+def test_minimize_without_reset_is_plain_run():
+    """Without reset the chunks are plain steps (the file's force column seeds
+    the first drift, Q32): constant-force Velocity-Verlet is exact."""
+    g, dt, m = -2.0, 0.01, 1.5
+    s = settings(box=(50.0, 50.0, 50.0), dt=dt, gravity=(0.0, 0.0, g))
+    b, q = beads_of([[20.0, 20.0, 40.0]], s=s, mass=m, force=[0.0, 0.0, g])
+    o = Oracle(b, q, springs_of([]), angles_of([]), NOPAIR, s)
+    o.minimize(35, 10, False)
+    t = 30 * dt
+    assert abs(vec(o.beads, "position")[0, 2] - (40.0 + g * t * t / (2 * m))) < 1e-12
+
+
 def test_periodic_reset_and_walls():
     s = settings(box=(30.0, 30.0, 30.0), bc=0, dt=0.01)
     b, q = beads_of([[0.001, 29.999, 15.0]], vel=[-1.0, 1.0, 0.0], s=s)
