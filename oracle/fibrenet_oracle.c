@@ -25,7 +25,16 @@
  * Parity pins (tests/test_oracle.py): closed forms for the integrator, springs,
  * spring failure, angles, lj/hs/st contacts, cell hashing, periodic reset and
  * wall penalty; Newton's-third-law and momentum invariants; a brute-force
- * O(N^2) contact set over the same cell restriction; README fixtures.
+ * O(N^2) contact set over the same cell restriction; the rebuild cadence of
+ * or_run_loop (a pair is listed exactly at the first rebuild after it comes
+ * within the cutoff, the counter restarting with every call); the chunks and
+ * resets of or_minimize (closed-form displacement under constant gravity);
+ * README fixtures.
+ *
+ * Parity unpinned: the bit-level results of a particular OpenCL
+ * implementation - its acos and pow ulps, its native_sqrt binding, its
+ * atomic summation order and bucket slot order.  No reference test or
+ * fixture fixes them.
  */
 #include <math.h>
 #include <stdlib.h>
