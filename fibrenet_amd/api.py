@@ -66,6 +66,7 @@ SIGNATURES = [
 
 class FnStats(ctypes.Structure):
     _fields_ = [("steps", _LL), ("rebuilds", _LL), ("list_entries", _LL), ("list_capacity", _LL),
+                ("candidate_entries", _LL), ("candidate_builds", _I),
                 ("hashed_beads", _I), ("max_cell_occupancy", _I), ("cells_over_capacity", _I),
                 ("max_list_length", _I), ("broken_springs", _I), ("img_aliases_pos", _I)]
 

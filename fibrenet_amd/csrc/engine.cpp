@@ -414,7 +414,7 @@ class Engine final : public fn_engine {
         case K_RESET_GRID: grid_valid_ = false; break;
         case K_RESET_LIST:
             ensure_grid(nullptr);
-            fnk::fill_int(nb_start_.p, 0, N_ + 1, stream_);
+            fnk::fill_int(nb_cnt_.p, 0, N_, stream_);
             break;
         case K_FILL_GRID:
             if (n != N_) throw Error(FN_ERR_UNSUPPORTED, "FillGrid over a partial bead range");
@@ -425,7 +425,7 @@ class Engine final : public fn_engine {
             if (n != s.cells_x * s.cells_y * s.cells_z)
                 throw Error(FN_ERR_UNSUPPORTED, "BuildContactList over a partial grid");
             if (!grid_valid_) throw Error(FN_ERR_ARG, "BuildContactList before FillGrid");
-            build_list(make_box(s), true);
+            build_list(make_box(s));
             break;
         }
         case K_RESOLVE_CONTACTS: {
@@ -479,10 +479,10 @@ class Engine final : public fn_engine {
         ensure_grid(&settings_of(K_FILL_GRID));
         ensure_grid(&settings_of(K_BUILD_LIST));
         ensure_scratch();
-        const double r2 = reach2(settings_of(K_BUILD_LIST));
-        if (r2 != reach2_) {
-            reach2_ = r2;
-            clear_graphs();
+        geometry(settings_of(K_FILL_GRID), settings_of(K_BUILD_LIST));
+        if (cand_dirty_) {
+            fnk::fill_int(need_.p, 1, 1, stream_);
+            cand_dirty_ = false;
         }
         // capacity of the contact list: grow before the run if the last rebuild came close
         if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
@@ -495,10 +495,12 @@ class Engine final : public fn_engine {
             done += len;
         }
         steps_ += n;
-        rebuilds_ += (n + guf - 1) / guf;
+        const int rb = (n + guf - 1) / guf;
+        rebuilds_ += rb;
         HIPCK(hipStreamSynchronize(stream_));
         collect_timing();
         check_capacity();
+        tune_verlet(rb);
     }
 
     // ---- exports --------------------------------------------------------------------------------
@@ -511,15 +513,17 @@ class Engine final : public fn_engine {
 
     long long export_pairs(int *out, long long cap) override
     {
-        std::vector<int> st, lst;
+        std::vector<int> st, cnt, lst;
         nb_start_.get(st, N_ + 1, stream_);
+        nb_cnt_.get(cnt, N_, stream_);
         HIPCK(hipStreamSynchronize(stream_));
-        lst.resize(st[N_]);
-        if (st[N_]) HIPCK(hipMemcpyAsync(lst.data(), nb_list_.p, sizeof(int) * st[N_], hipMemcpyDeviceToHost, stream_));
+        const int used = std::min(st[N_], cap_);
+        lst.resize(used);
+        if (used) HIPCK(hipMemcpyAsync(lst.data(), nb_list_.p, sizeof(int) * used, hipMemcpyDeviceToHost, stream_));
         HIPCK(hipStreamSynchronize(stream_));
         std::vector<long long> keys;
         for (int i = 0; i < N_; ++i)
-            for (int e = st[i]; e < st[i + 1]; ++e) {
+            for (int e = st[i]; e < st[i] + cnt[i] && e < used; ++e) {
                 const int j = lst[e] & fnk::kIdxMask;
                 if (i < j) keys.push_back((long long)i * N_ + j);
             }
@@ -544,18 +548,23 @@ class Engine final : public fn_engine {
     {
         int st[fnk::kStatCount];
         HIPCK(hipMemcpyAsync(st, stats_.p, sizeof st, hipMemcpyDeviceToHost, stream_));
-        int tot = 0, hashed = 0;
-        if (nb_start_.p) HIPCK(hipMemcpyAsync(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
-        if (cell_start_.p) HIPCK(hipMemcpyAsync(&hashed, cell_start_.p + C_, sizeof(int), hipMemcpyDeviceToHost, stream_));
-        std::vector<int> ss;
+        int cand = 0;
+        if (nb_start_.p) HIPCK(hipMemcpyAsync(&cand, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        std::vector<int> ss, keys, cnt;
+        if (key_.p) key_.get(keys, N_, stream_);
+        if (nb_cnt_.p) nb_cnt_.get(cnt, N_, stream_);
         sp_status_.get(ss, S_, stream_);
         HIPCK(hipStreamSynchronize(stream_));
         memset(o, 0, sizeof *o);
         o->steps = steps_;
         o->rebuilds = rebuilds_;
-        o->list_entries = tot;
+        long long listed = 0;
+        for (int c : cnt) listed += c;
+        o->list_entries = listed;
         o->list_capacity = cap_;
-        o->hashed_beads = hashed;
+        o->candidate_entries = cand;
+        o->candidate_builds = st[fnk::kStatCandBuilds];
+        o->hashed_beads = (int)std::count_if(keys.begin(), keys.end(), [](int k) { return k >= 0; });
         o->max_cell_occupancy = st[fnk::kStatMaxCell];
         o->cells_over_capacity = st[fnk::kStatCellsOver];
         o->max_list_length = st[fnk::kStatMaxList];
@@ -596,6 +605,17 @@ class Engine final : public fn_engine {
     {
         if (name == "chunk") chunk_ = value;
         else if (name == "chain") chain_on_ = value != 0;
+        else if (name == "staged") staged_ = value != 0;
+        else if (name == "verlet") {
+            if (value < 0 || value > 2) throw Error(FN_ERR_ARG, "verlet is 0 (off), 1 (on) or 2 (auto)");
+            verlet_ = value;
+            verlet_trial_ = false;
+            verlet_runs_ = 0;
+        }
+        else if (name == "skin") {
+            if (value < 0 || value > 1000) throw Error(FN_ERR_ARG, "skin must lie in [0, 1000] permille of the reach");
+            skin_permille_ = value;
+        }
         else if (name == "graphs") graphs_on_ = value != 0;
         else throw Error(FN_ERR_ARG, "unknown option " + name);
         clear_graphs();
@@ -605,6 +625,9 @@ class Engine final : public fn_engine {
     {
         if (name == "chunk") return chunk_;
         if (name == "chain") return chain_on_ ? 1 : 0;
+        if (name == "staged") return staged_ ? 1 : 0;
+        if (name == "verlet") return verlet_;
+        if (name == "skin") return skin_permille_;
         if (name == "chain_layout") {
             prepare();
             return chain_ok_ ? 1 : 0;
@@ -735,6 +758,7 @@ class Engine final : public fn_engine {
                 pc[k] = fnk::PairC{p.type, p.c1, p.c2, p.c3, p.c4, p.c5, p.c6};
             }
             pairs_.put(pc, stream_);
+            cand_dirty_ = true;
             clear_graphs();
             break;
         }
@@ -779,6 +803,7 @@ class Engine final : public fn_engine {
         field(frc_[0], &fn_bead::force);
         field(frc_[1], &fn_bead::force_old);
         HIPCK(hipStreamSynchronize(stream_));
+        cand_dirty_ = true;
     }
 
     void download_beads(fn_bead *o)
@@ -827,6 +852,7 @@ class Engine final : public fn_engine {
         type_.put(ty, stream_); tag_.put(tg, stream_); loc_.put(lc, stream_);
         rad_d_.put(rd, stream_); skin_d_.put(sk, stream_);
         HIPCK(hipStreamSynchronize(stream_));
+        cand_dirty_ = true;
         clear_graphs();
     }
 
@@ -1010,7 +1036,7 @@ class Engine final : public fn_engine {
         a.an_s1 = an_s1_.p; a.an_s2 = an_s2_.p; a.an_type = an_type_.p; a.an_status = an_status_.p;
         a.an_th0 = an_th0_.p; a.an_k = an_k_.p;
         a.n_angles = A_;
-        a.nb_start = nb_start_.p; a.nb_list = nb_list_.p; a.pairs = pairs_.p;
+        a.nb_start = nb_start_.p; a.nb_cnt = nb_cnt_.p; a.nb_list = nb_list_.p; a.pairs = pairs_.p;
         a.n_pairs = (int)h_pairs_.size();
         a.sp_term = sp_term_.p; a.an_term = an_term_.p;
         a.sp_slot = sp_slot_.p; a.an_slot = an_slot_.p;
@@ -1046,13 +1072,20 @@ class Engine final : public fn_engine {
     void ensure_grid(const fn_settings *s)
     {
         nb_start_.ensure((size_t)N_ + 1);
+        nb_cnt_.ensure(N_);
         ncnt_.ensure(N_);
         key_.ensure(N_);
+        ckey_.ensure(N_);
         rank_.ensure(N_);
         sorted_.ensure(N_);
+        need_.ensure(1);
+        for (auto &r : ref_) r.ensure(N_);
         if (!nb_list_.p) grow_list(std::max<long long>(32LL * N_, 1 << 20));
         if (!nb_init_) {
             fnk::fill_int(nb_start_.p, 0, N_ + 1, stream_);
+            fnk::fill_int(nb_cnt_.p, 0, N_, stream_);
+            fnk::fill_int(key_.p, -1, N_, stream_);
+            fnk::fill_int(need_.p, 1, 1, stream_);
             nb_init_ = true;
         }
         if (s) {
@@ -1120,8 +1153,14 @@ class Engine final : public fn_engine {
         if (want > (1LL << 31) - 1) throw Error(FN_ERR_CAPACITY, "contact list larger than 2^31 entries");
         if (want <= cap_ && nb_list_.p) return;
         clear_graphs();
+        cand_.alloc((size_t)want);
         nb_list_.alloc((size_t)want);
         cap_ = (int)want;
+        // entries are only read where a build wrote them, but an overflowing
+        // build leaves gaps: keep every entry a valid (bead 0, pair 0) one
+        fnk::fill_int(cand_.p, 0, cap_, stream_);
+        fnk::fill_int(nb_list_.p, 0, cap_, stream_);
+        fnk::fill_int(need_.p, 1, 1, stream_);
     }
 
     void hash_and_sort(const fnk::Box &b)
@@ -1129,19 +1168,19 @@ class Engine final : public fn_engine {
         const fn_settings &s = settings_of(K_FILL_GRID);
         ensure_grid(&s);
         ensure_scratch();
-        enqueue_sort(pc_, b);
+        enqueue_sort(pc_, b, nullptr);
         grid_valid_ = true;
     }
 
     // Counting sort of the beads (positions of buffer pc) into cells.
-    void enqueue_sort(int pc, const fnk::Box &b)
+    void enqueue_sort(int pc, const fnk::Box &b, const int *gate)
     {
-        fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
-        fnk::Launch<T, P>::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, b, key_.p, rank_.p, cell_count_.p,
-                                stream_);
-        fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, stream_);
-        fnk::Launch<T, P>::place(key_.p, rank_.p, cell_start_.p, pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p,
-                                 type_.p, tag_.p, loc_.p, N_, sorted_.p, sorted_view(), stream_);
+        fnk::fill_gated(cell_count_.p, 0, C_ + 1, gate, stream_);
+        fnk::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, b, key_.p, ckey_.p, rank_.p, cell_count_.p, gate,
+                  stream_);
+        fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, gate, stream_);
+        fnk::place(key_.p, ckey_.p, rank_.p, cell_start_.p, pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, type_.p, tag_.p,
+                   loc_.p, N_, sorted_.p, sorted_view(), gate, stream_);
     }
 
     fnk::Sorted sorted_view()
@@ -1165,6 +1204,75 @@ class Engine final : public fn_engine {
         return r2;
     }
 
+    // Candidate radius and stencil for the grid and list settings.  Kept
+    // between rebuilds, the candidates are every pair within r = reach + skin
+    // (skin a fraction of reach), found over ceil(r / cell) cells per axis;
+    // that needs every pair within r seen at its minimum image (always in a
+    // finite box, in a periodic one when r < half the smallest side).
+    // Rebuilt at every rebuild, they are the pairs within reach in adjacent
+    // cells, as the reference lists them.
+    void geometry(const fn_settings &g, const fn_settings &l)
+    {
+        if (g.cells_x != l.cells_x || g.cells_y != l.cells_y || g.cells_z != l.cells_z ||
+            g.grid_step.x != l.grid_step.x || g.grid_step.y != l.grid_step.y || g.grid_step.z != l.grid_step.z)
+            throw Error(FN_ERR_UNSUPPORTED, "FillGrid and BuildContactList bound to different grids");
+        const double r2 = reach2(l);
+        const double reach = std::sqrt(r2);
+        const bool want = verlet_ == 1 || (verlet_ == 2 && verlet_trial_);
+        double skin = want ? reach * skin_permille_ / 1000.0 : 0.0;
+        const double lmin = std::min(l.box_size.x, std::min(l.box_size.y, l.box_size.z));
+        bool keep = want && skin > 0.0;
+        if (l.bc_type == 0 && !(reach + skin < 0.5 * lmin)) keep = false;
+        if (!keep) skin = 0.0;
+        const double r = reach + skin;
+        fnk::Stencil st{};
+        const double step[3] = {g.grid_step.x, g.grid_step.y, g.grid_step.z};
+        const int n[3] = {g.cells_x, g.cells_y, g.cells_z};
+        for (int k = 0; k < 3; ++k) {
+            const double q = (keep && step[k] > 0.0) ? std::ceil(r / step[k] * (1.0 + 1e-12)) : 1.0;
+            st.w[k] = (int)std::max(1.0, std::min(q, (double)n[k]));
+            st.all[k] = (l.bc_type == 0 && n[k] < 2 * st.w[k] + 1) ? 1 : 0;
+        }
+        fnk::Stencil ex{};
+        for (int k = 0; k < 3; ++k) {
+            ex.w[k] = 1;
+            ex.all[k] = 0;   // grids have at least three cells per axis
+        }
+        const double cand_r2 = r * r * (1.0 + 1e-12);
+        const double lim = 0.45 * skin;
+        if (r2 != reach2_ || cand_r2 != cand_r2_ || keep != keep_cand_ || memcmp(&st, &stencil_, sizeof st) != 0) {
+            clear_graphs();
+            fnk::fill_int(need_.p, 1, 1, stream_);
+        }
+        reach2_ = r2;
+        cand_r2_ = cand_r2;
+        keep_cand_ = keep;
+        lim2_ = lim * lim;
+        stencil_ = st;
+        stencil_exact_ = ex;
+    }
+
+    // verlet = 2: keeping the candidates pays only while most rebuilds find
+    // every bead within skin/2 of its snapshot.  Runs use the one-pass exact
+    // rebuild; every 16th run tries the kept candidates, and keeps them for
+    // the following runs while fewer than half of the rebuilds had to rebuild
+    // them.  The lists are the same either way.
+    void tune_verlet(int rebuilds)
+    {
+        if (verlet_ != 2) return;
+        if (!verlet_trial_) {
+            if (++verlet_runs_ >= 16) {
+                verlet_runs_ = 0;
+                verlet_trial_ = true;
+            }
+            return;
+        }
+        if (rebuilds < 4) return;
+        int builds = 0;
+        HIPCK(hipMemcpy(&builds, stats_.p + fnk::kStatCandBuilds, sizeof(int), hipMemcpyDeviceToHost));
+        if (2 * builds >= rebuilds) verlet_trial_ = false;
+    }
+
     void ensure_scratch()
     {
         so_bead_.ensure(N_); so_type_.ensure(N_); so_cell_.ensure(N_); so_slot_.ensure(N_);
@@ -1174,17 +1282,21 @@ class Engine final : public fn_engine {
         }
     }
 
-    // Search, offsets, compaction.  exact (per-kernel path): the host checks
-    // the row and list capacities before compacting and grows them as needed.
-    void search_and_compact(const fnk::Box &b, bool exact)
+    // Search, offsets, compaction.  final_list: the pairs BuildContactList
+    // lists (exact rule, reference stencil) into the contact list itself;
+    // otherwise the candidates within reach + skin into the candidate list.
+    // Gated by `gate`.  sync (per-kernel path, ungated): the host checks the
+    // row and list capacities before compacting and grows them as needed.
+    void search_and_compact(const fnk::Box &b, bool final_list, bool sync, const int *gate)
     {
         for (;;) {
-            if (exact) fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
-            fnk::fill_int(ncnt_.p, 0, N_, stream_);
-            fnk::search(sorted_.p, cell_start_.p, C_, sorted_view(), rad_d_.p, skin_d_.p, pairs_.p, b, reach2_,
-                        ncnt_.p, scratch_.p, K_, stats_.p, N_, stream_);
-            fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, stream_);
-            if (exact) {
+            if (sync) fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
+            fnk::fill_gated(ncnt_.p, 0, N_, gate, stream_);
+            fnk::search(sorted_.p, cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b,
+                        final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, final_list, ncnt_.p,
+                        scratch_.p, K_, stats_.p, N_, staged_, gate, stream_);
+            fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, gate, stream_);
+            if (sync) {
                 int tot = 0, over = 0;
                 HIPCK(hipMemcpyAsync(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
                 HIPCK(hipMemcpyAsync(&over, stats_.p + fnk::kStatListOver, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -1196,9 +1308,49 @@ class Engine final : public fn_engine {
                 if (tot > cap_) grow_list((long long)tot * 2);
                 last_total_ = tot;
             }
-            fnk::compact(ncnt_.p, nb_start_.p, so_slot_.p, scratch_.p, K_, N_, nb_list_.p, cap_, stream_);
+            fnk::compact(ncnt_.p, nb_start_.p, so_slot_.p, scratch_.p, K_, N_, final_list ? nb_list_.p : cand_.p, cap_,
+                         final_list ? nb_cnt_.p : nullptr, gate, stream_);
+            fnk::note_build(nb_start_.p + N_, stats_.p, gate, stream_);
             return;
         }
+    }
+
+    // The listing rule on the candidates at the positions of buffer pc.
+    void enqueue_filter(int pc, const fnk::Box &bg, const fnk::Box &bl)
+    {
+        fnk::FilterArgs f{};
+        f.px = pos_[pc][0].p; f.py = pos_[pc][1].p; f.pz = pos_[pc][2].p;
+        f.n = N_;
+        f.grid = bg;
+        f.list = bl;
+        f.cand_start = nb_start_.p; f.cand = cand_.p; f.cap = cap_;
+        f.pairs = pairs_.p;
+        f.n_pairs = (int)h_pairs_.size();
+        f.radius = rad_d_.p; f.skin = skin_d_.p;
+        f.list_out = nb_list_.p; f.cnt = nb_cnt_.p; f.key = key_.p; f.stats = stats_.p;
+        fnk::filter(f, stream_);
+    }
+
+    // One rebuild of the fast path at the positions of buffer pc.  With kept
+    // candidates: the displacement check, the gated candidate build with its
+    // snapshot, then the filter.  Otherwise the reference's rebuild in one
+    // pass: sort, exact search, compaction into the contact list (and the
+    // candidates are marked stale).
+    void enqueue_rebuild(int pc, const fnk::Box &bg, const fnk::Box &bl)
+    {
+        const double *p[3] = {pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p};
+        if (!keep_cand_) {
+            fnk::set_int(need_.p, 1, nullptr, stream_);
+            enqueue_sort(pc, bg, nullptr);
+            search_and_compact(bl, true, false, nullptr);
+            return;
+        }
+        fnk::disp_check(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, bl, lim2_, need_.p, stream_);
+        enqueue_sort(pc, bg, need_.p);
+        search_and_compact(bl, false, false, need_.p);
+        fnk::snapshot(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, need_.p, stream_);
+        fnk::set_int(need_.p, 0, need_.p, stream_);
+        enqueue_filter(pc, bg, bl);
     }
 
     void grow_rows()
@@ -1207,18 +1359,21 @@ class Engine final : public fn_engine {
         ensure_scratch();
     }
 
-    void build_list(const fnk::Box &b, bool exact)
+    // BuildContactList of the per-kernel path, after FillGrid's sort: the
+    // exact search into the contact list, sizes checked on the host.
+    void build_list(const fnk::Box &bl)
     {
         ensure_grid(&settings_of(K_BUILD_LIST));
         ensure_scratch();
-        reach2_ = reach2(settings_of(K_BUILD_LIST));
-        search_and_compact(b, exact);
+        geometry(settings_of(K_FILL_GRID), settings_of(K_BUILD_LIST));
+        fnk::set_int(need_.p, 1, nullptr, stream_);
+        search_and_compact(bl, true, true, nullptr);
     }
 
     void check_capacity()
     {
         int tot = 0, over = 0;
-        HIPCK(hipMemcpy(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(&tot, stats_.p + fnk::kStatCandMax, sizeof(int), hipMemcpyDeviceToHost));
         HIPCK(hipMemcpy(&over, stats_.p + fnk::kStatListOver, sizeof(int), hipMemcpyDeviceToHost));
         last_total_ = std::max(last_total_, tot);
         if (over > 0) {
@@ -1259,8 +1414,7 @@ class Engine final : public fn_engine {
         for (int k = 0; k < len; ++k) {
             if (k % guf == 0) {
                 mark(ev ? &ev->rebuild : nullptr);
-                enqueue_sort(pc, bf);
-                search_and_compact(bl, false);
+                enqueue_rebuild(pc, bf, bl);
                 mark(ev ? &ev->rebuild : nullptr);
             }
             const bool is_last = last && k == len - 1;
@@ -1372,6 +1526,14 @@ class Engine final : public fn_engine {
     int chunk_ = 128;                                // steps per captured graph
     bool chain_on_ = true;                           // use the chain bead kernel where the layout allows
     bool chain_ok_ = false;                          // the bonds follow the chain layout
+    int verlet_ = 2;                                 // candidates kept between rebuilds: 0 never, 1 always, 2 auto
+    bool verlet_trial_ = false;                      // auto: candidates currently kept
+    int verlet_runs_ = 0;                            // auto: runs since the last trial
+    bool staged_ = false;                            // neighbour search reads its block's neighbourhood from LDS
+    int skin_permille_ = 300;                        // skin as a fraction of the listing reach
+    bool keep_cand_ = false, cand_dirty_ = true;
+    double cand_r2_ = -1.0, lim2_ = 0.0;
+    fnk::Stencil stencil_{}, stencil_exact_{};
     bool graphs_on_ = true;
     long long steps_ = 0, rebuilds_ = 0;
 
@@ -1388,7 +1550,9 @@ class Engine final : public fn_engine {
     DArr<fnk::V3<T>> sp_term_, an_term_;   // bond terms of the current step at the incidence slots
     DArr<int> ch_spring_, ch_angle_;       // chain layout: spring (i, i+1), angle centred on i
     DArr<fnk::PairC> pairs_;
-    DArr<int> key_, rank_, cell_count_, cell_start_, sorted_, ncnt_, nb_start_, nb_list_, scan_tmp_, stats_, step_;
+    DArr<int> key_, ckey_, rank_, cell_count_, cell_start_, sorted_, ncnt_, nb_start_, nb_cnt_, cand_, nb_list_, scan_tmp_,
+        stats_, step_, need_;
+    DArr<double> ref_[3];   // positions at the last candidate build
     DArr<fnk::SortedBead> so_bead_;
     DArr<int> so_type_, so_cell_, so_slot_, scratch_;
     double reach2_ = 0.0;

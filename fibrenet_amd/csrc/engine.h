@@ -6,8 +6,35 @@
 
 namespace fnk {
 
-// Slots of the device statistics array.
-enum { kStatMaxCell = 0, kStatCellsOver = 1, kStatMaxList = 2, kStatListOver = 3, kStatCount = 4 };
+// Slots of the device statistics array: cell occupancy (at the last sort),
+// the longest listed and candidate lists, search rows that overflowed, the
+// candidate builds of the run and their largest total.
+enum {
+    kStatMaxCell = 0, kStatCellsOver = 1, kStatMaxList = 2, kStatListOver = 3, kStatMaxCand = 4,
+    kStatCandBuilds = 5, kStatCandMax = 6, kStatCount = 7
+};
+
+// Cells visited around a bead's cell by the candidate search, per axis: the
+// half-width w, or every cell of the axis once (a periodic axis with fewer
+// than 2w+1 cells).
+struct Stencil {
+    int w[3];
+    int all[3];
+};
+
+// Operands of the rebuild filter.
+struct FilterArgs {
+    const double *px, *py, *pz;     // positions
+    int n;
+    Box grid;                        // FillGrid's settings (cells, cell size)
+    Box list;                        // BuildContactList's settings (box, minimum image, cutoff)
+    const int *cand_start, *cand;    // candidate CSR
+    int cap;                         // entries reserved in cand and list_out
+    const PairC *pairs;
+    int n_pairs;
+    const double *radius, *skin;
+    int *list_out, *cnt, *key, *stats;
+};
 
 template <typename T, typename P>
 struct Launch {
@@ -27,33 +54,50 @@ struct Launch {
     // with chain (a.ch_spring / a.ch_angle layout) evaluating the bonds itself
     static void bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, bool chain,
                           hipStream_t s);
-    // counting sort of the beads by cell: hash (cell, rank in cell, counts),
-    // exclusive scan of the counts, place (slots, cell-ordered bead data)
-    static void hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *rank, int *count,
-                     hipStream_t s);
-    static void place(const int *key, const int *rank, const int *cell_start, const P *px, const P *py, const P *pz,
-                      const int *type, const int *tag, const int *loc, int n, int *sorted, const Sorted &so,
-                      hipStream_t s);
 };
 
-// Neighbour search over the cell-sorted beads into scratch rows of K entries
-// per slot, then compaction into the bead-indexed CSR list.
-// reach2: the largest squared listing radius of any pair type (a candidate
-// farther than that is rejected before its type is read).
-void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const double *radius,
-            const double *skin, const PairC *pairs, const Box &b, double reach2, int *cnt, int *scratch, int K,
-            int *stats, int n, hipStream_t s);
-void compact(const int *cnt, const int *nb_start, const int *slot_of, const int *scratch, int K, int n, int *nb_list,
-             int cap, hipStream_t s);
+// Rebuild.  Kernels taking `gate` do nothing unless *gate is nonzero (gate
+// NULL: always run): the candidate build runs only when the displacement
+// check asked for it.
+//
+// Counting sort of the beads by cell: hash writes FillGrid's cell (key, -1
+// outside the grid), the cell of the clamped position (ckey), the rank in
+// that cell and the counts; an exclusive scan of the counts; place puts the
+// beads' data into cell order.
+void hash(const double *px, const double *py, const double *pz, int n, const Box &b, int *key, int *ckey, int *rank,
+          int *count, const int *gate, hipStream_t s);
+void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
+           const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
+           const Sorted &so, const int *gate, hipStream_t s);
+// Partners over the stencil into scratch rows of K entries per slot: the
+// candidates within sqrt(r2), or (exact) the pairs BuildContactList lists;
+// staged: the neighbourhood of each block read through LDS.  Then
+// compaction into a bead-indexed CSR sorted per bead (with cnt_out: the final
+// list, its lengths and the listed total).
+void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
+            const double *radius, const double *skin, const Box &b, const Stencil &st, double r2, bool exact,
+            int *cnt, int *scratch, int K, int *stats, int n, bool staged, const int *gate, hipStream_t s);
+void compact(const int *cnt, const int *cand_start, const int *slot_of, const int *scratch, int K, int n, int *cand,
+             int cap, int *cnt_out, const int *gate, hipStream_t s);
+// *need = 1 when a bead moved more than sqrt(lim2) since the snapshot.
+void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
+                const double *rz, int n, const Box &b, double lim2, int *need, hipStream_t s);
+void snapshot(const double *px, const double *py, const double *pz, double *rx, double *ry, double *rz, int n,
+              const int *gate, hipStream_t s);
+// The listing rule on the candidates: contact list, counts and cells.
+void filter(const FilterArgs &f, hipStream_t s);
+// Counts a candidate build and keeps the largest candidate total of the run.
+void note_build(const int *total, int *stats, const int *gate, hipStream_t s);
 
 void fill_int(int *p, int v, int n, hipStream_t s);
-void copy_int(const int *src, int *dst, int n, hipStream_t s);
+void fill_gated(int *p, int v, int n, const int *gate, hipStream_t s);
+void set_int(int *p, int v, const int *gate, hipStream_t s);
 void add_int(int *p, int v, hipStream_t s);
 // Streaming copy of n16 16-byte words (the reference point for traffic counters).
 void copy_probe(const void *src, void *dst, size_t n16, hipStream_t s);
 int scan_blocks_needed(int n);
 // out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total.  tmp holds
 // scan_blocks_needed(n) + 1 ints.
-void exclusive_scan(const int *in, int n, int *out, int *tmp, hipStream_t s);
+void exclusive_scan(const int *in, int n, int *out, int *tmp, const int *gate, hipStream_t s);
 
 }  // namespace fnk

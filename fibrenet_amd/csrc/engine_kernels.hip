@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(kBlock) k_resolve_contacts(Args<T, P> a)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    if (a.nb_start[i] == a.nb_start[i + 1]) return;
+    if (a.nb_cnt[i] == 0) return;
     V3<T> f{a.fx[i], a.fy[i], a.fz[i]};
     gather_contacts(a, i, f, a.pairs);
     a.fx[i] = f.x; a.fy[i] = f.y; a.fz[i] = f.z;
@@ -224,11 +224,13 @@ __global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
     __syncthreads();
     const bool live = i < a.n;
     const int ic = live ? i : a.n - 1;      // lanes past the end follow along for the wave-wide contacts
+    const int e0 = a.nb_start[ic], cnt = live ? a.nb_cnt[ic] : 0;
+    const T r1 = a.radius[ic];
     BeadState<T, P> b;
     V3<T> f = bead_begin(a, ic, b);
     if (SPR) add_terms(a.sp_term, a.sp_start[ic], a.sp_start[ic + 1], f);
     if (ANG) add_terms(a.an_term, a.an_start[ic], a.an_start[ic + 1], f);
-    wave_contacts(a, ic, live, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6]);
+    wave_contacts(a, e0, cnt, r1, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6]);
     if (!live) return;
     if (NEXT) bead_end<T, P, NEXT, LAST>(a, i, b, f, a.ix[i], a.iy[i], a.iz[i]);
     else bead_end<T, P, NEXT, LAST>(a, i, b, f, (P)0, (P)0, (P)0);
@@ -260,26 +262,39 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
     const bool in = i >= 0 && i < a.n;
     const bool own = in && lane >= kHalo && lane < 64 - kHalo;
     const int ic = in ? i : (i < 0 ? 0 : a.n - 1);
+    // every load that depends on nothing else first: the bead's image, state,
+    // list offsets, and the indices of its chain bonds
     P x = (P)0, y = (P)0, z = (P)0;
     if (in) {
         x = a.ix[i];
         y = a.iy[i];
         z = a.iz[i];
     }
+    BeadState<T, P> b;
+    V3<T> f = bead_begin(a, ic, b);
+    const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
+    const T r1 = a.radius[ic];
+    const int sr = (in && (SPR || ANG)) ? a.ch_spring[i] : -1;
+    const int q = (in && ANG) ? a.ch_angle[i] : -1;
+    // then the bond parameters, in one round trip
+    SpringP<T> spp{};
+    AngleP<T> anp{};
+    if (sr >= 0) spp = load_spring(a, sr);
+    if (q >= 0) anp = load_angle(a, q);
+
     // images of beads i + 1 and i - 1 (meaningless in the outermost lanes,
     // whose results no owned lane uses)
     const P xn = __shfl_down(x, 1, 64), yn = __shfl_down(y, 1, 64), zn = __shfl_down(z, 1, 64);
     const P xp = __shfl_up(x, 1, 64), yp = __shfl_up(y, 1, 64), zp = __shfl_up(z, 1, 64);
 
     // spring (i, i+1): its term and its status after this step's spring pass
-    const int sr = (in && (SPR || ANG)) ? a.ch_spring[i] : -1;
     V3<T> fr{(T)0, (T)0, (T)0};
     int str = 1;
     if (sr >= 0) {
-        str = a.sp_status[sr];
+        str = spp.st;
         if (SPR) {
             bool broke;
-            if (!spring_force(a, sr, x - xn, y - yn, z - zn, fr, broke)) fr = V3<T>{(T)0, (T)0, (T)0};
+            if (!spring_force(spp, x - xn, y - yn, z - zn, fr, broke)) fr = V3<T>{(T)0, (T)0, (T)0};
             if (broke) {
                 str = 0;
                 if (own) {
@@ -295,22 +310,19 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
 
     // angle (i-1, i, i+1) with this step's HR_FL rule (its springs are the
     // two chain springs of bead i)
-    const int q = (in && ANG) ? a.ch_angle[i] : -1;
     V3<T> pa{(T)0, (T)0, (T)0}, pc{(T)0, (T)0, (T)0};
     if (q >= 0) {
-        int st = a.an_status[q];
-        if (st == 1 && a.an_type[q] == 1 && (stl == 0 || str == 0)) {
+        int st = anp.st;
+        if (st == 1 && anp.ty == 1 && (stl == 0 || str == 0)) {
             st = 0;
             if (own) a.an_status[q] = 0;
         }
-        if (st == 1) angle_force(a, q, xp, yp, zp, x, y, z, xn, yn, zn, pa, pc);
+        if (st == 1) angle_force(anp, xp, yp, zp, x, y, z, xn, yn, zn, pa, pc);
     }
     const int ql = __shfl_up(q, 1, 64), qn = __shfl_down(q, 1, 64);
     const V3<T> pcl{__shfl_up(pc.x, 1, 64), __shfl_up(pc.y, 1, 64), __shfl_up(pc.z, 1, 64)};
     const V3<T> pan{__shfl_down(pa.x, 1, 64), __shfl_down(pa.y, 1, 64), __shfl_down(pa.z, 1, 64)};
 
-    BeadState<T, P> b;
-    V3<T> f = bead_begin(a, ic, b);
     if (SPR) {
         if (sl >= 0) {          // bead i is p2 of spring (i-1, i)
             f.x += fl.x * (T)-1;
@@ -340,7 +352,7 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
             f.z += pan.z;
         }
     }
-    wave_contacts(a, ic, own, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6]);
+    wave_contacts(a, e0, cnt, r1, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6]);
     if (!own) return;
     bead_end<T, P, NEXT, LAST>(a, i, b, f, x, y, z);
 }
@@ -353,34 +365,93 @@ __global__ void __launch_bounds__(kBlock) k_fill_int(int *p, int v, int n)
     if (i < n) p[i] = v;
 }
 
-__global__ void __launch_bounds__(kBlock) k_copy_int(const int *src, int *dst, int n)
-{
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i < n) dst[i] = src[i];
-}
 
 // ---- grid rebuild ----------------------------------------------------------------
+//
+// A rebuild lists, for every bead, the partners BuildContactList would list
+// (kernel.cl:532-789): both beads inside the grid, in adjacent cells
+// (SimulationManager.cpp:2193-2301, seen from both cells), not bonded
+// neighbours of one fibre, and within the pair type's listing distance.
+// The engine keeps a candidate list - every pair within reach + skin, found by
+// a counting sort of the beads into the grid's cells and a search of the
+// surrounding cells - and filters it at each rebuild with that exact rule on
+// the current positions.  The candidates are rebuilt only when some bead has
+// moved more than skin/2 since they were built (any pair within reach now was
+// within reach + skin then), so the filtered list is the reference's list.
+// The candidate kernels are gated by a device flag so that the sequence can
+// be captured once and replayed.
 
-// FillGrid's cell of a bead (kernel.cl:492-504) in fp64; -1 outside the grid.
-// The bead's rank among the beads of its cell comes from the count itself
-// (first pass of a counting sort).
-template <typename P>
-__global__ void __launch_bounds__(kBlock) k_hash(const P *px, const P *py, const P *pz, int n, Box b, int *key,
-                                                 int *rank, int *count)
+__device__ __forceinline__ bool gated_off(const int *gate) { return gate && *gate == 0; }
+
+__global__ void __launch_bounds__(kBlock) k_fill_gated(int *p, int v, int n, const int *gate)
 {
+    if (gated_off(gate)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_set_int(int *p, int v, const int *gate)
+{
+    if (gated_off(gate)) return;
+    *p = v;
+}
+
+__global__ void k_note_build(const int *total, int *stats, const int *gate)
+{
+    if (gated_off(gate)) return;
+    stats[kStatCandBuilds] += 1;
+    stats[kStatCandMax] = max(stats[kStatCandMax], *total);
+}
+
+// FillGrid's cell of a bead (kernel.cl:492-504) in fp64 (-1 outside the grid)
+// into key; the cell of its position clamped into the box into ckey, with the
+// bead's rank among the beads of that cell (first pass of a counting sort).
+__global__ void __launch_bounds__(kBlock) k_hash(const double *px, const double *py, const double *pz, int n, Box b,
+                                                 int *key, int *ckey, int *rank, int *count, const int *gate)
+{
+    if (gated_off(gate)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int cx = (int)floor((double)px[i] / b.sx);
-    const int cy = (int)floor((double)py[i] / b.sy);
-    const int cz = (int)floor((double)pz[i] / b.sz);
-    int c = -1, r = -1;
-    if (cx >= 0 && cx < b.cx && cy >= 0 && cy < b.cy && cz >= 0 && cz < b.cz) {
-        c = cx + cy * b.cx + cz * b.cxy;
-        r = atomicAdd(&count[c], 1);
-    }
-    key[i] = c;
-    rank[i] = r;
+    const int cx = (int)floor(px[i] / b.sx);
+    const int cy = (int)floor(py[i] / b.sy);
+    const int cz = (int)floor(pz[i] / b.sz);
+    const bool in = cx >= 0 && cx < b.cx && cy >= 0 && cy < b.cy && cz >= 0 && cz < b.cz;
+    key[i] = in ? cx + cy * b.cx + cz * b.cxy : -1;
+    const int qx = min(max(cx, 0), b.cx - 1), qy = min(max(cy, 0), b.cy - 1), qz = min(max(cz, 0), b.cz - 1);
+    const int c = qx + qy * b.cx + qz * b.cxy;
+    ckey[i] = c;
+    rank[i] = atomicAdd(&count[c], 1);
 }
+
+// Second pass of the counting sort: each bead goes to slot cell_start[cell] +
+// rank with its data (fp64 position, tag, loc, type), read in bead order;
+// slot_of is the inverse permutation.  The slot's cell is stored as c for a
+// bead inside the grid and as ~c for one placed by its clamped position.
+__global__ void __launch_bounds__(kBlock) k_place(const int *key, const int *ckey, const int *rank,
+                                                  const int *cell_start, const double *px, const double *py,
+                                                  const double *pz, const int *type, const int *tag, const int *loc,
+                                                  int n, int *sorted, Sorted so, const int *gate)
+{
+    if (gated_off(gate)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int c = ckey[i];
+    const int s = cell_start[c] + rank[i];
+    SortedBead b;
+    b.x = px[i];
+    b.y = py[i];
+    b.z = pz[i];
+    b.tag = tag[i];
+    b.loc = loc[i];
+    so.bead[s] = b;
+    so.type[s] = type[i];
+    so.cell[s] = key[i] >= 0 ? c : ~c;
+    sorted[s] = i;
+    so.slot_of[i] = s;
+}
+
+// Cell of a sorted slot (see k_place) and whether its bead is inside the grid.
+__device__ __forceinline__ int slot_cell(int code) { return code >= 0 ? code : ~code; }
 
 // Wave-wide max and sum (all 64 lanes take part), so that statistics cost one
 // atomic per wave instead of one per thread on a single address.
@@ -397,116 +468,132 @@ __device__ __forceinline__ int wave_sum(int v)
     return v;
 }
 
-// Second pass of the counting sort: each bead goes to slot cell_start[cell] +
-// rank and takes its data along (fp64 position, tag, loc, type; 40 B per
-// slot), read in bead order.  slot_of is the inverse permutation (-1 for a
-// bead outside the grid).
-template <typename P>
-__global__ void __launch_bounds__(kBlock) k_place(const int *key, const int *rank, const int *cell_start, const P *px,
-                                                  const P *py, const P *pz, const int *type, const int *tag,
-                                                  const int *loc, int n, int *sorted, Sorted so)
+// Raises *p to v; the plain read first skips the atomic (which serialises on
+// one address across the whole grid) once the maximum has settled.
+__device__ __forceinline__ void raise_max(int *p, int v)
 {
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const int c = key[i];
-    if (c < 0) {
-        so.slot_of[i] = -1;
-        return;
-    }
-    const int s = cell_start[c] + rank[i];
-    SortedBead b;
-    b.x = (double)px[i];
-    b.y = (double)py[i];
-    b.z = (double)pz[i];
-    b.tag = tag[i];
-    b.loc = loc[i];
-    so.bead[s] = b;
-    so.type[s] = type[i];
-    so.cell[s] = c;
-    sorted[s] = i;
-    so.slot_of[i] = s;
+    if (v > *(volatile int *)p) atomicMax(p, v);
 }
 
-// Scan of one candidate range of cell-sorted slots [lo, hi) for the bead in
-// slot s: BuildContactList's predicate (kernel.cl:574-659) in fp64.  The
-// distance is tested first against the largest listing radius of any pair
-// type (reach2), so that most candidates cost two loads.
-__device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBead &me, int ti, double ri, double si,
-                                           const Sorted &so, const int *sorted, const double *radius,
-                                           const double *skin, const PairC *pairs, const Box &b, double reach2,
-                                           int *row, int K, int &k)
+// BuildContactList's distance (kernel.cl:600-628): minimum image per
+// component, only in a periodic box and only beyond halfBoxSq.
+__device__ __forceinline__ double list_len2(double rx, double ry, double rz, const Box &b)
 {
+    double len = rx * rx + ry * ry + rz * rz;
+    if (b.bc == 0 && len > b.hbs) {
+        if (rx * rx > b.hbs) rx += (rx > 0.0) ? -b.lx : b.lx;
+        if (ry * ry > b.hbs) ry += (ry > 0.0) ? -b.ly : b.ly;
+        if (rz * rz > b.hbs) rz += (rz > 0.0) ? -b.lz : b.lz;
+        len = rx * rx + ry * ry + rz * rz;
+    }
+    return len;
+}
+
+// Partners of the bead in slot s among slots [lo, hi): not a bonded
+// neighbour on the same fibre (kernel.cl:586-590), of a pair type that can
+// be listed, and within r2 (candidates) or - EXACT - within the pair type's
+// listing distance (kernel.cl:632-659; r2 is then the largest of them, a
+// cheap first rejection).  Slot t's record is src[t - shift] (global memory
+// or a staged copy in LDS).
+template <bool EXACT>
+__device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBead &me, int ti, double ri, double si,
+                                           const SortedBead *src, int shift, const Sorted &so, const int *sorted,
+                                           const PairC *pairs, const double *radius, const double *skin,
+                                           const Box &b, double r2, int *row, int K, int &k)
+{
+    if (lo >= hi) return;
+    SortedBead nxt = src[lo - shift];
     for (int t = lo; t < hi; ++t) {
-        const SortedBead o = so.bead[t];
-        double rx = me.x - o.x, ry = me.y - o.y, rz = me.z - o.z;
-        double len = rx * rx + ry * ry + rz * rz;
-        if (b.bc == 0 && len > b.hbs) {
-            if (rx * rx > b.hbs) rx += (rx > 0.0) ? -b.lx : b.lx;
-            if (ry * ry > b.hbs) ry += (ry > 0.0) ? -b.ly : b.ly;
-            if (rz * rz > b.hbs) rz += (rz > 0.0) ? -b.lz : b.lz;
-            len = rx * rx + ry * ry + rz * rz;
-        }
-        if (!(len < reach2) || t == s) continue;
+        const SortedBead o = nxt;                  // the next record is in flight while this one is tested
+        if (t + 1 < hi) nxt = src[t + 1 - shift];
+        const double len = list_len2(me.x - o.x, me.y - o.y, me.z - o.z, b);
+        if (!(len < r2) || t == s) continue;
         if (me.tag == o.tag && abs(me.loc - o.loc) <= 1) continue;
         const int pid = pair_index(ti, so.type[t], b.types);
         const int pt = pairs[pid].type;
-        bool take = false;
-        if (pt == 1 || pt == 3) take = len < b.cutoff_sq;
-        if (pt == 2) {
-            const int j = sorted[t];
-            const double rj = radius[j], sj = skin[j];
-            const double min_skin = (si > sj) ? sj : si;
-            const double max_r = (ri > rj) ? ri : rj;
-            const double th = max_r + min_skin;
-            take = len < th * th;
+        if (pt < 1 || pt > 3) continue;
+        if (EXACT) {
+            if (so.cell[t] < 0) continue;   // FillGrid does not hash a bead outside the grid
+            bool take = false;
+            if (pt == 1 || pt == 3) take = len < b.cutoff_sq;
+            if (pt == 2) {
+                const int j = sorted[t];
+                const double rj = radius[j], sj = skin[j];
+                const double min_skin = (si > sj) ? sj : si;
+                const double max_r = (ri > rj) ? ri : rj;
+                const double th = max_r + min_skin;
+                take = len < th * th;
+            }
+            if (!take) continue;
         }
-        if (!take) continue;
         if (k < K) row[k] = sorted[t] | (pid << kPidShift);
         ++k;
     }
 }
 
-// Contact candidates of the bead in slot s: its own cell and the 26
-// neighbours - the reference's own cell + 13-cell half shell
-// (SimulationManager.cpp:2210-2301) seen from both cells of a pair.  The three
-// cells of an x-row are one contiguous slot range unless the row wraps or is
+// Cells visited along one axis around cell c: c-w..c+w, wrapped (periodic) or
+// clipped (finite), or every cell once when `all`.
+struct AxisRange {
+    int c, n, w, all, periodic;
+    __device__ int first() const { return all ? 0 : -w; }
+    __device__ int last() const { return all ? n - 1 : w; }
+    // cell for step d, -1 when clipped
+    __device__ int at(int d) const
+    {
+        if (all) return d;
+        int x = c + d;
+        if (periodic) {
+            x = x < 0 ? x + n : (x >= n ? x - n : x);
+            return x;
+        }
+        return (x < 0 || x >= n) ? -1 : x;
+    }
+};
+
+// Candidate search of the bead in slot s over the stencil's cells; the cells
+// of one x-row are one contiguous slot range unless the row wraps or is
 // clipped.  Up to K entries go to the slot's scratch row; cnt[bead] = count.
+template <bool EXACT>
 __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int *cell_start, int n_cells, Sorted so,
-                                                   const double *radius, const double *skin, const PairC *pairs,
-                                                   Box b, double reach2, int *cnt, int *scratch, int K, int *stats)
+                                                   const PairC *pairs, const double *radius, const double *skin, Box b,
+                                                   Stencil st, double r2, int *cnt, int *scratch, int K, int *stats,
+                                                   const int *gate)
 {
+    if (gated_off(gate)) return;
     const int s = blockIdx.x * kBlock + threadIdx.x;
     int k = 0, occ = 0;
     if (s < cell_start[n_cells]) {
         const int i = sorted[s];
-        const int c = so.cell[s];
+        const int code = so.cell[s];
+        const int c = slot_cell(code);
         if (cell_start[c] == s) occ = cell_start[c + 1] - s;   // the cell's first slot reports its occupancy
         const int cz = c / b.cxy;
         const int cy = (c - cz * b.cxy) / b.cx;
         const int cx = c - cz * b.cxy - cy * b.cx;
         const SortedBead me = so.bead[s];
         const int ti = so.type[s];
-        const double ri = radius[i], si = skin[i];
+        const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
         int *row = scratch + (size_t)s * K;
-        for (int dz = -1; dz <= 1; ++dz) {
-            int z = cz + dz;
-            if (b.bc == 0) z = z < 0 ? b.cz - 1 : (z >= b.cz ? 0 : z);
-            else if (z < 0 || z >= b.cz) continue;
-            for (int dy = -1; dy <= 1; ++dy) {
-                int y = cy + dy;
-                if (b.bc == 0) y = y < 0 ? b.cy - 1 : (y >= b.cy ? 0 : y);
-                else if (y < 0 || y >= b.cy) continue;
+        const int per = b.bc == 0 ? 1 : 0;
+        const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, st.w[1], st.all[1], per},
+            az{cz, b.cz, st.w[2], st.all[2], per};
+        const bool row_run = !st.all[0] && cx - st.w[0] >= 0 && cx + st.w[0] < b.cx;
+        for (int dz = az.first(); dz <= az.last() && !(EXACT && code < 0); ++dz) {
+            const int z = az.at(dz);
+            if (z < 0) continue;
+            for (int dy = ay.first(); dy <= ay.last(); ++dy) {
+                const int y = ay.at(dy);
+                if (y < 0) continue;
                 const int rbase = y * b.cx + z * b.cxy;
-                if (cx >= 1 && cx + 1 < b.cx) {
-                    scan_range(cell_start[rbase + cx - 1], cell_start[rbase + cx + 2], s, me, ti, ri, si, so, sorted,
-                               radius, skin, pairs, b, reach2, row, K, k);
+                if (row_run) {
+                    scan_range<EXACT>(cell_start[rbase + cx - st.w[0]], cell_start[rbase + cx + st.w[0] + 1], s, me,
+                                      ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b, r2, row, K, k);
                 } else {
-                    for (int dx = -1; dx <= 1; ++dx) {
-                        int x = cx + dx;
-                        if (b.bc == 0) x = x < 0 ? b.cx - 1 : (x >= b.cx ? 0 : x);
-                        else if (x < 0 || x >= b.cx) continue;
-                        scan_range(cell_start[rbase + x], cell_start[rbase + x + 1], s, me, ti, ri, si, so, sorted,
-                                   radius, skin, pairs, b, reach2, row, K, k);
+                    for (int dx = ax.first(); dx <= ax.last(); ++dx) {
+                        const int x = ax.at(dx);
+                        if (x < 0) continue;
+                        scan_range<EXACT>(cell_start[rbase + x], cell_start[rbase + x + 1], s, me, ti, ri, si, so.bead,
+                                          0, so, sorted, pairs, radius, skin, b, r2, row, K, k);
                     }
                 }
             }
@@ -518,28 +605,156 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
     const int mocc = wave_max(occ);
     const int big = wave_sum(occ > 64 ? 1 : 0);
     if ((threadIdx.x & 63) == 0) {
-        if (mx > 0) atomicMax(&stats[kStatMaxList], mx);
+        raise_max(&stats[EXACT ? kStatMaxList : kStatMaxCand], mx);
         if (over > 0) atomicAdd(&stats[kStatListOver], over);
-        if (mocc > 0) atomicMax(&stats[kStatMaxCell], mocc);
+        raise_max(&stats[kStatMaxCell], mocc);
         if (big > 0) atomicAdd(&stats[kStatCellsOver], big);
     }
 }
 
-// Scratch rows (by slot) -> the bead-indexed CSR list the force kernels read,
-// each bead's entries in ascending order.  The order of beads inside a cell
-// follows the atomic scatter and is arbitrary; ordering the entries makes the
-// list, and so the contact-force summation order, deterministic.
-__global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *nb_start, const int *slot_of,
-                                                    const int *scratch, int K, int n, int *nb_list, int cap)
+// The same search with the block's neighbourhood staged in LDS.  A block of
+// kStageBlock consecutive slots covers cells [c_lo, c_hi]; for every (dy, dz)
+// of the stencil the cells its beads can reach in that row offset form one
+// contiguous cell (so slot) interval [c_lo - wx + off, c_hi + wx + off].
+// The block copies these intervals into LDS with coalesced loads (as many
+// as fit); each thread then reads a candidate range from LDS when the
+// range lies inside its staged interval, and from global memory otherwise
+// (rows that wrap around a periodic box, intervals that did not fit).
+constexpr int kStageBlock = 128;
+constexpr int kStageCap = 1248;       // records (32 B each) staged per block
+constexpr int kStageRows = 25;        // (2w+1)^2 for w <= 2
+
+template <bool EXACT>
+__global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted, const int *cell_start, int n_cells,
+                                                               Sorted so, const PairC *pairs, const double *radius,
+                                                               const double *skin, Box b, Stencil st, double r2,
+                                                               int *cnt, int *scratch, int K, int *stats,
+                                                               const int *gate)
 {
+    if (gated_off(gate)) return;
+    __shared__ SortedBead rec[kStageCap];
+    __shared__ int lo_q[kStageRows], hi_q[kStageRows], base_q[kStageRows + 1];
+    const int nh = cell_start[n_cells];
+    const int s0 = blockIdx.x * kStageBlock;
+    if (s0 >= nh) return;
+    const int s_end = min(s0 + kStageBlock, nh);
+    const int wy = st.w[1], wz = st.w[2], ny = 2 * wy + 1, nq = ny * (2 * wz + 1);
+    if ((int)threadIdx.x < nq) {          // one thread per row offset: the interval's slots
+        const int q = threadIdx.x;
+        const int c_lo = slot_cell(so.cell[s0]), c_hi = slot_cell(so.cell[s_end - 1]);
+        const int dy = q % ny - wy, dz = q / ny - wz;
+        const long long off = (long long)dy * b.cx + (long long)dz * b.cxy;
+        const long long a = max(0LL, (long long)c_lo - st.w[0] + off);
+        const long long e = min((long long)n_cells, (long long)c_hi + st.w[0] + 1 + off);
+        int lo = 0, hi = 0;
+        if (a < e) {
+            lo = cell_start[a];
+            hi = cell_start[e];
+        }
+        lo_q[q] = lo;
+        hi_q[q] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {                // staging offsets, as many intervals as fit
+        int base = 0;
+        for (int q = 0; q < nq; ++q) {
+            if (base + (hi_q[q] - lo_q[q]) > kStageCap) hi_q[q] = lo_q[q] = 0;   // read from global memory
+            base_q[q] = base;
+            base += hi_q[q] - lo_q[q];
+        }
+        base_q[nq] = base;
+    }
+    __syncthreads();
+    const int total = base_q[nq];
+    for (int k = threadIdx.x; k < total; k += kStageBlock) {
+        int q = 0;
+        while (k >= base_q[q + 1]) ++q;
+        rec[k] = so.bead[lo_q[q] + (k - base_q[q])];
+    }
+    __syncthreads();
+
+    const int s = s0 + threadIdx.x;
+    int k = 0, occ = 0;
+    if (s < s_end) {
+        const int i = sorted[s];
+        const int code = so.cell[s];
+        const int c = slot_cell(code);
+        if (cell_start[c] == s) occ = cell_start[c + 1] - s;
+        const int cz = c / b.cxy;
+        const int cy = (c - cz * b.cxy) / b.cx;
+        const int cx = c - cz * b.cxy - cy * b.cx;
+        const SortedBead me = so.bead[s];
+        const int ti = so.type[s];
+        const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
+        int *row = scratch + (size_t)s * K;
+        const int per = b.bc == 0 ? 1 : 0;
+        const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, wy, st.all[1], per},
+            az{cz, b.cz, wz, st.all[2], per};
+        const bool row_run = !st.all[0] && cx - st.w[0] >= 0 && cx + st.w[0] < b.cx;
+        for (int dz = az.first(); dz <= az.last() && !(EXACT && code < 0); ++dz) {
+            const int z = az.at(dz);
+            if (z < 0) continue;
+            for (int dy = ay.first(); dy <= ay.last(); ++dy) {
+                const int y = ay.at(dy);
+                if (y < 0) continue;
+                const int rbase = y * b.cx + z * b.cxy;
+                // staged interval of this row offset (none for axes visited whole)
+                const int q = (st.all[1] || st.all[2]) ? -1 : (dz + wz) * ny + (dy + wy);
+                const int slo = q >= 0 ? lo_q[q] : 0, shi = q >= 0 ? hi_q[q] : 0;
+                const int sbase = q >= 0 ? base_q[q] : 0;
+                auto scan = [&](int lo, int hi) {
+                    if (lo >= hi) return;
+                    if (lo >= slo && hi <= shi)
+                        scan_range<EXACT>(lo, hi, s, me, ti, ri, si, rec + sbase, slo, so, sorted, pairs, radius, skin,
+                                          b, r2, row, K, k);
+                    else
+                        scan_range<EXACT>(lo, hi, s, me, ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b,
+                                          r2, row, K, k);
+                };
+                if (row_run) {
+                    scan(cell_start[rbase + cx - st.w[0]], cell_start[rbase + cx + st.w[0] + 1]);
+                } else {
+                    for (int dx = ax.first(); dx <= ax.last(); ++dx) {
+                        const int x = ax.at(dx);
+                        if (x < 0) continue;
+                        scan(cell_start[rbase + x], cell_start[rbase + x + 1]);
+                    }
+                }
+            }
+        }
+        cnt[i] = k;
+    }
+    const int mx = wave_max(k);
+    const int over = wave_sum(k > K ? 1 : 0);
+    const int mocc = wave_max(occ);
+    const int big = wave_sum(occ > 64 ? 1 : 0);
+    if ((threadIdx.x & 63) == 0) {
+        raise_max(&stats[EXACT ? kStatMaxList : kStatMaxCand], mx);
+        if (over > 0) atomicAdd(&stats[kStatListOver], over);
+        raise_max(&stats[kStatMaxCell], mocc);
+        if (big > 0) atomicAdd(&stats[kStatCellsOver], big);
+    }
+}
+
+// Scratch rows (by slot) -> the bead-indexed candidate CSR, each bead's
+// entries in ascending order.  The order of beads inside a cell follows the
+// atomic counting sort and is arbitrary; ordering the entries makes the
+// lists, and so the contact-force summation order, deterministic.
+// With cnt_out (the final list): also the list length of every bead.
+__global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *cand_start, const int *slot_of,
+                                                    const int *scratch, int K, int n, int *cand, int cap, int *cnt_out,
+                                                    const int *gate)
+{
+    if (gated_off(gate)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
-    const int m = min(cnt[i], K);
+    int m = min(cnt[i], K);
+    if (cand_start[i] + m > cap) m = 0;   // capacity exceeded: the host reports the run invalid
+    if (cnt_out) cnt_out[i] = m;
     if (m == 0) return;
-    const int base = nb_start[i];
-    if (base + m > cap) return;   // capacity exceeded: the host reports it
+    const int base = cand_start[i];
     const int *row = scratch + (size_t)slot_of[i] * K;
-    int *out = nb_list + base;
+    int *out = cand + base;
     if (m <= 8) {
         int v[8];
 #pragma unroll
@@ -567,6 +782,133 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *n
         }
         out[j + 1] = v;
     }
+}
+
+// Sets *need when some bead has moved more than sqrt(lim2) from its position
+// at the last candidate build (minimum image in a periodic box; NaN counts as
+// moved).
+__global__ void __launch_bounds__(kBlock) k_disp(const double *px, const double *py, const double *pz,
+                                                 const double *rx, const double *ry, const double *rz, int n, Box b,
+                                                 double lim2, int *need)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    bool moved = false;
+    if (i < n) {
+        double dx = px[i] - rx[i], dy = py[i] - ry[i], dz = pz[i] - rz[i];
+        if (b.bc == 0) {
+            dx = dx > 0.5 * b.lx ? dx - b.lx : (dx < -0.5 * b.lx ? dx + b.lx : dx);
+            dy = dy > 0.5 * b.ly ? dy - b.ly : (dy < -0.5 * b.ly ? dy + b.ly : dy);
+            dz = dz > 0.5 * b.lz ? dz - b.lz : (dz < -0.5 * b.lz ? dz + b.lz : dz);
+        }
+        moved = !(dx * dx + dy * dy + dz * dz <= lim2);
+    }
+    if (__any(moved) && (threadIdx.x & 63) == 0) *need = 1;
+}
+
+__global__ void __launch_bounds__(kBlock) k_snapshot(const double *px, const double *py, const double *pz, double *rx,
+                                                     double *ry, double *rz, int n, const int *gate)
+{
+    if (gated_off(gate)) return;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    rx[i] = px[i];
+    ry[i] = py[i];
+    rz[i] = pz[i];
+}
+
+// The exact listing rule applied to the candidates: writes FillGrid's cell of
+// every bead to key, bead i's listed partners (in candidate order, so
+// ascending) to list[cand_start[i] ...] and their number to cnt[i].  The
+// candidates of a wave are evaluated round-robin by all its lanes (each lane
+// every 64th entry), the verdicts pass through LDS, and each bead then
+// writes its own accepted entries in order.
+__global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
+{
+    __shared__ int verdict[kBlock / 64][kWaveChunk];
+    int *acc = verdict[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < f.n;
+    const int ic = live ? i : f.n - 1;
+    const Box &g = f.grid;
+    const double x = f.px[ic], y = f.py[ic], z = f.pz[ic];
+    const int cx = (int)floor(x / g.sx), cy = (int)floor(y / g.sy), cz = (int)floor(z / g.sz);
+    const bool in = cx >= 0 && cx < g.cx && cy >= 0 && cy < g.cy && cz >= 0 && cz < g.cz;
+    if (live) f.key[i] = in ? cx + cy * g.cx + cz * g.cxy : -1;
+    const int e0 = f.cand_start[ic];
+    const int e1 = f.cand_start[ic + 1];
+    const int cnt = (live && in && e1 <= f.cap) ? e1 - e0 : 0;   // past the capacity: the host reports the run invalid
+    const double ri = f.radius[ic], si = f.skin[ic];
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    const int off = incl - cnt;
+    const int total = __shfl(incl, 63, 64);
+    const Box &b = f.list;
+    const bool per = b.bc == 0;
+    int kept = 0;
+    for (int base = 0; base < total; base += kWaveChunk) {
+        for (int r = 0; r < kWaveChunk; r += 64) {
+            const int k = base + r + lane;
+            int L = 0;
+#pragma unroll
+            for (int step = 32; step > 0; step >>= 1) {
+                const int cand = L + step;
+                if (cand < 64 && __shfl(off, cand & 63, 64) <= k) L = cand;
+            }
+            const int e = __shfl(e0, L, 64) + (k - __shfl(off, L, 64));
+            const double xL = __shfl(x, L, 64), yL = __shfl(y, L, 64), zL = __shfl(z, L, 64);
+            const int cxL = __shfl(cx, L, 64), cyL = __shfl(cy, L, 64), czL = __shfl(cz, L, 64);
+            const double riL = __shfl(ri, L, 64), siL = __shfl(si, L, 64);
+            int v = -1;
+            if (k < total) {
+                const int ent = f.cand[e];
+                const int j = ent & kIdxMask;
+                // entries of an overflowed build may be stale: never follow one outside the arrays
+                const bool valid = j < f.n && (ent >> kPidShift) < f.n_pairs;
+                const int jj = valid ? j : 0;
+                const double xj = f.px[jj], yj = f.py[jj], zj = f.pz[jj];
+                const int jx = (int)floor(xj / g.sx), jy = (int)floor(yj / g.sy), jz = (int)floor(zj / g.sz);
+                const int ax = abs(jx - cxL), ay = abs(jy - cyL), az = abs(jz - czL);
+                const bool jin = jx >= 0 && jx < g.cx && jy >= 0 && jy < g.cy && jz >= 0 && jz < g.cz;
+                const bool adj = (ax <= 1 || (per && ax == g.cx - 1)) && (ay <= 1 || (per && ay == g.cy - 1)) &&
+                                 (az <= 1 || (per && az == g.cz - 1));
+                if (valid && jin && adj) {
+                    const double len = list_len2(xL - xj, yL - yj, zL - zj, b);
+                    const int pt = f.pairs[ent >> kPidShift].type;
+                    bool take = false;
+                    if (pt == 1 || pt == 3) take = len < b.cutoff_sq;
+                    if (pt == 2) {                         // kernel.cl:632-659
+                        const double rj = f.radius[j], sj = f.skin[j];
+                        const double min_skin = (siL > sj) ? sj : siL;
+                        const double max_r = (riL > rj) ? riL : rj;
+                        const double th = max_r + min_skin;
+                        take = len < th * th;
+                    }
+                    if (take) v = ent;
+                }
+            }
+            acc[r + lane] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int k0 = off > base ? off : base;
+        const int k1 = (off + cnt < base + kWaveChunk) ? off + cnt : base + kWaveChunk;
+        for (int k = k0; k < k1; ++k) {
+            const int v = acc[k - base];
+            if (v >= 0) f.list_out[e0 + kept++] = v;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    if (live) f.cnt[i] = kept;
+    const int mx = wave_max(kept);
+    if (lane == 0) raise_max(&f.stats[kStatMaxList], mx);
 }
 
 // ---- device-wide exclusive scan (n ints -> n+1 offsets) ----------------------------
@@ -597,8 +939,9 @@ __device__ __forceinline__ int block_excl_scan(int v, int *lds, int &total)
     return off + x - v;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_reduce(const int *in, int n, int *bsum)
+__global__ void __launch_bounds__(kBlock) k_scan_reduce(const int *in, int n, int *bsum, const int *gate)
 {
+    if (gated_off(gate)) return;
     __shared__ int lds[kBlock / 64];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int s = 0;
@@ -610,8 +953,9 @@ __global__ void __launch_bounds__(kBlock) k_scan_reduce(const int *in, int n, in
 }
 
 // Exclusive scan of the block sums by one block; bsum[nb] receives the total.
-__global__ void __launch_bounds__(kBlock) k_scan_blocks(int *bsum, int nb)
+__global__ void __launch_bounds__(kBlock) k_scan_blocks(int *bsum, int nb, const int *gate)
 {
+    if (gated_off(gate)) return;
     __shared__ int lds[kBlock / 64];
     int carry = 0;
     for (int b0 = 0; b0 < nb; b0 += kBlock) {
@@ -625,8 +969,10 @@ __global__ void __launch_bounds__(kBlock) k_scan_blocks(int *bsum, int nb)
     if (threadIdx.x == 0) bsum[nb] = carry;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_apply(const int *in, int n, const int *bsum, int nb, int *out)
+__global__ void __launch_bounds__(kBlock) k_scan_apply(const int *in, int n, const int *bsum, int nb, int *out,
+                                                       const int *gate)
 {
+    if (gated_off(gate)) return;
     __shared__ int lds[kBlock / 64];
     const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int v[kScanItems];
@@ -724,30 +1070,12 @@ void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next,
     const int mask = (spr ? 1 : 0) | (ang ? 2 : 0) | (next ? 4 : 0) | (last ? 8 : 0) | (chain ? 16 : 0);
     dispatch_fused<T, P>(a, mask, s, std::make_integer_sequence<int, 32>{});
 }
-template <typename T, typename P>
-void Launch<T, P>::hash(const P *px, const P *py, const P *pz, int n, const Box &b, int *key, int *rank, int *count,
-                        hipStream_t s)
-{
-    k_hash<P><<<blocks(n), kBlock, 0, s>>>(px, py, pz, n, b, key, rank, count);
-}
-template <typename T, typename P>
-void Launch<T, P>::place(const int *key, const int *rank, const int *cell_start, const P *px, const P *py,
-                         const P *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
-                         const Sorted &so, hipStream_t s)
-{
-    k_place<P><<<blocks(n), kBlock, 0, s>>>(key, rank, cell_start, px, py, pz, type, tag, loc, n, sorted, so);
-}
-
 template struct Launch<double, double>;
 template struct Launch<float, double>;
 
 void fill_int(int *p, int v, int n, hipStream_t s)
 {
     if (n > 0) k_fill_int<<<blocks(n), kBlock, 0, s>>>(p, v, n);
-}
-void copy_int(const int *src, int *dst, int n, hipStream_t s)
-{
-    if (n > 0) k_copy_int<<<blocks(n), kBlock, 0, s>>>(src, dst, n);
 }
 void add_int(int *p, int v, hipStream_t s) { k_add_int<<<1, 1, 0, s>>>(p, v); }
 
@@ -761,25 +1089,73 @@ void copy_probe(const void *src, void *dst, size_t n16, hipStream_t s)
 {
     k_copy_probe<<<4096, kBlock, 0, s>>>((const int4 *)src, (int4 *)dst, n16);
 }
-void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const double *radius,
-            const double *skin, const PairC *pairs, const Box &b, double reach2, int *cnt, int *scratch, int K,
-            int *stats, int n, hipStream_t s)
+void fill_gated(int *p, int v, int n, const int *gate, hipStream_t s)
 {
-    k_search<<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, radius, skin, pairs, b, reach2, cnt,
-                                          scratch, K, stats);
+    if (n > 0) k_fill_gated<<<blocks(n), kBlock, 0, s>>>(p, v, n, gate);
 }
-void compact(const int *cnt, const int *nb_start, const int *slot_of, const int *scratch, int K, int n, int *nb_list,
-             int cap, hipStream_t s)
+void set_int(int *p, int v, const int *gate, hipStream_t s) { k_set_int<<<1, 1, 0, s>>>(p, v, gate); }
+void hash(const double *px, const double *py, const double *pz, int n, const Box &b, int *key, int *ckey, int *rank,
+          int *count, const int *gate, hipStream_t s)
 {
-    k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, nb_start, slot_of, scratch, K, n, nb_list, cap);
+    k_hash<<<blocks(n), kBlock, 0, s>>>(px, py, pz, n, b, key, ckey, rank, count, gate);
+}
+void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
+           const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
+           const Sorted &so, const int *gate, hipStream_t s)
+{
+    k_place<<<blocks(n), kBlock, 0, s>>>(key, ckey, rank, cell_start, px, py, pz, type, tag, loc, n, sorted, so,
+                                         gate);
+}
+template <bool EXACT>
+static void search_t(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
+                     const double *radius, const double *skin, const Box &b, const Stencil &st, double r2, int *cnt,
+                     int *scratch, int K, int *stats, int n, bool staged, const int *gate, hipStream_t s)
+{
+    if (staged && st.w[1] <= 2 && st.w[2] <= 2)
+        k_search_staged<EXACT><<<(n + kStageBlock - 1) / kStageBlock, kStageBlock, 0, s>>>(
+            sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2, cnt, scratch, K, stats, gate);
+    else
+        k_search<EXACT><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2,
+                                                     cnt, scratch, K, stats, gate);
+}
+void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
+            const double *radius, const double *skin, const Box &b, const Stencil &st, double r2, bool exact,
+            int *cnt, int *scratch, int K, int *stats, int n, bool staged, const int *gate, hipStream_t s)
+{
+    if (exact)
+        search_t<true>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2, cnt, scratch, K, stats, n,
+                       staged, gate, s);
+    else
+        search_t<false>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2, cnt, scratch, K, stats, n,
+                        staged, gate, s);
+}
+void compact(const int *cnt, const int *cand_start, const int *slot_of, const int *scratch, int K, int n, int *cand,
+             int cap, int *cnt_out, const int *gate, hipStream_t s)
+{
+    k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, slot_of, scratch, K, n, cand, cap, cnt_out, gate);
+}
+void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
+                const double *rz, int n, const Box &b, double lim2, int *need, hipStream_t s)
+{
+    k_disp<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, b, lim2, need);
+}
+void snapshot(const double *px, const double *py, const double *pz, double *rx, double *ry, double *rz, int n,
+              const int *gate, hipStream_t s)
+{
+    k_snapshot<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, gate);
+}
+void filter(const FilterArgs &f, hipStream_t s) { k_filter<<<blocks(f.n), kBlock, 0, s>>>(f); }
+void note_build(const int *total, int *stats, const int *gate, hipStream_t s)
+{
+    k_note_build<<<1, 1, 0, s>>>(total, stats, gate);
 }
 int scan_blocks_needed(int n) { return (n + kScanTile - 1) / kScanTile; }
-void exclusive_scan(const int *in, int n, int *out, int *tmp, hipStream_t s)
+void exclusive_scan(const int *in, int n, int *out, int *tmp, const int *gate, hipStream_t s)
 {
     const int nb = scan_blocks_needed(n > 0 ? n : 1);
-    k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, tmp);
-    k_scan_blocks<<<1, kBlock, 0, s>>>(tmp, nb);
-    k_scan_apply<<<nb, kBlock, 0, s>>>(in, n, tmp, nb, out);
+    k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, tmp, gate);
+    k_scan_blocks<<<1, kBlock, 0, s>>>(tmp, nb, gate);
+    k_scan_apply<<<nb, kBlock, 0, s>>>(in, n, tmp, nb, out, gate);
 }
 
 }  // namespace fnk

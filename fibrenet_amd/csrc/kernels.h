@@ -39,7 +39,8 @@ struct alignas(16) SortedBead {
 };
 
 // Bead data copied into cell order at a rebuild (the neighbour search reads
-// these contiguously), the cell of each slot and the inverse permutation
+// these contiguously), the cell of each slot (c, or ~c for a bead outside
+// the grid, placed by its clamped position) and the inverse permutation
 // bead -> slot.
 struct Sorted {
     SortedBead *bead;
@@ -91,8 +92,8 @@ struct Args {
     // spring (p1 i, p2 i+1), ch_angle[i] the angle (b1 i-1, b2 i, b3 i+1),
     // -1 where there is none.
     const int *ch_spring, *ch_angle;
-    // contacts
-    const int *nb_start, *nb_list;
+    // contacts: bead i's partners are nb_list[nb_start[i] .. nb_start[i] + nb_cnt[i])
+    const int *nb_start, *nb_cnt, *nb_list;
     const PairC *pairs;
     int n_pairs;
     // settings of AdvancePosition (integrator, walls) and ResolveContacts (minimum image)
@@ -176,32 +177,46 @@ __device__ __forceinline__ V3<T> base_force(const Box &b, const V3<T> &vp, P px,
     return f;
 }
 
-// Spring s with image difference d = image(p1) - image(p2) (kernel.cl:371-417).
-// Returns false when the spring is inactive or breaks now; otherwise F is the
-// force on p1 (p2 receives -F).  `broke` reports a break decided by this
-// evaluation.
+// Parameters of one spring, loaded together (one round trip to memory).
+template <typename T>
+struct SpringP {
+    int ty, st;
+    double l0, c1, c2;
+    T k;
+};
+
 template <typename T, typename P>
-__device__ __forceinline__ bool spring_force(const Args<T, P> &a, int s, P dx, P dy, P dz, V3<T> &F, bool &broke)
+__device__ __forceinline__ SpringP<T> load_spring(const Args<T, P> &a, int s)
+{
+    return SpringP<T>{a.sp_type[s], a.sp_status[s], a.sp_l0[s], a.sp_c1[s], a.sp_c2[s], a.sp_k[s]};
+}
+
+// Spring with parameters p and image difference d = image(p1) - image(p2)
+// (kernel.cl:371-417).  Returns false when the spring is inactive or breaks
+// now; otherwise F is the force on p1 (p2 receives -F).  `broke` reports a
+// break decided by this evaluation.
+template <typename T, typename P>
+__device__ __forceinline__ bool spring_force(const SpringP<T> &p, P dx, P dy, P dz, V3<T> &F, bool &broke)
 {
     broke = false;
-    const int ty = a.sp_type[s];
-    if (!(ty == 0 || ty == 1) || a.sp_status[s] != 1) return false;
+    const int ty = p.ty;
+    if (!(ty == 0 || ty == 1) || p.st != 1) return false;
     const P d2 = dx * dx + dy * dy + dz * dz;
     // length, extension and the HR_FL stress in fp64 from the float-rounded
     // length (native_sqrt): the break decision is bit-exact in both builds
     double l = (double)native_sqrt((double)d2);
-    const double factor = a.sp_l0[s] - l;
+    const double factor = p.l0 - l;
     if (l != 0.0) l = 1.0 / l;
     if (ty == 1) {
-        const double stress = -a.sp_c1[s] * factor * l;
-        if (fabs(stress) > a.sp_c2[s]) {
+        const double stress = -p.c1 * factor * l;
+        if (fabs(stress) > p.c2) {
             broke = true;
             return false;
         }
     }
     const T il = (T)l;
     const T ux = (T)dx * il, uy = (T)dy * il, uz = (T)dz * il;
-    const T fk = (T)factor * a.sp_k[s];
+    const T fk = (T)factor * p.k;
     F.x = ux * fk;
     F.y = uy * fk;
     F.z = uz * fk;
@@ -213,7 +228,7 @@ template <typename T, typename P>
 __device__ __forceinline__ bool spring_eval(const Args<T, P> &a, int s, V3<T> &F, bool &broke)
 {
     const int p1 = a.sp_p1[s], p2 = a.sp_p2[s];
-    return spring_force(a, s, a.ix[p1] - a.ix[p2], a.iy[p1] - a.iy[p2], a.iz[p1] - a.iz[p2], F, broke);
+    return spring_force(load_spring(a, s), a.ix[p1] - a.ix[p2], a.iy[p1] - a.iy[p2], a.iz[p1] - a.iz[p2], F, broke);
 }
 
 // Perpendicular force directions of an angle bond scaled to fa, fc
@@ -250,10 +265,23 @@ __device__ __forceinline__ void angle_dirs(R bax, R bay, R baz, R bcx, R bcy, R 
 // segment the force (|F| from the clamped 0.999999) points along the fp64
 // cross product of the bond vectors, which is rounding noise that only the
 // fp64 evaluation reproduces.
-// Angle q from the images of b1 (x1..), b2 (tx..) and b3 (x3..).
+// Parameters of one angle, loaded together.
+template <typename T>
+struct AngleP {
+    int st, ty;
+    T th0, k;
+};
+
 template <typename T, typename P>
-__device__ __forceinline__ void angle_force(const Args<T, P> &a, int q, P x1, P y1, P z1, P tx, P ty, P tz, P x3,
-                                            P y3, P z3, V3<T> &pa_out, V3<T> &pc_out)
+__device__ __forceinline__ AngleP<T> load_angle(const Args<T, P> &a, int q)
+{
+    return AngleP<T>{a.an_status[q], a.an_type[q], a.an_th0[q], a.an_k[q]};
+}
+
+// Angle with parameters g from the images of b1 (x1..), b2 (tx..) and b3 (x3..).
+template <typename T, typename P>
+__device__ __forceinline__ void angle_force(const AngleP<T> &g, P x1, P y1, P z1, P tx, P ty, P tz, P x3, P y3,
+                                            P z3, V3<T> &pa_out, V3<T> &pc_out)
 {
     const double bax = (double)(x1 - tx), bay = (double)(y1 - ty), baz = (double)(z1 - tz);
     const double bcx = (double)(x3 - tx), bcy = (double)(y3 - ty), bcz = (double)(z3 - tz);
@@ -269,7 +297,7 @@ __device__ __forceinline__ void angle_force(const Args<T, P> &a, int q, P x1, P 
         if (c2 > (T)1e-10 * la2 * lc2) {
             const T cn = tsqrt(c2), l_ba = tsqrt(la2), l_bc = tsqrt(lc2);
             const T ang = atan2f(cn, fx * gx + fy * gy + fz * gz);
-            const T mag = (T)-2 * a.an_k[q] * (ang - a.an_th0[q]);
+            const T mag = (T)-2 * g.k * (ang - g.th0);
             const T sa = mag / (l_ba * l_ba * cn), sc = mag / (l_bc * l_bc * cn);
             pa_out.x = (fy * cz - fz * cy) * sa;
             pa_out.y = (fz * cx - fx * cz) * sa;
@@ -287,7 +315,7 @@ __device__ __forceinline__ void angle_force(const Args<T, P> &a, int q, P x1, P 
     if (dotp < -1.0) dotp = -0.999999;
     if (dotp > 1.0) dotp = 0.999999;
     const double ang = tacos(dotp);
-    const double mag = -2.0 * (double)a.an_k[q] * (ang - (double)a.an_th0[q]);
+    const double mag = -2.0 * (double)g.k * (ang - (double)g.th0);
     const double fa = mag / l_ba;
     const double fc = mag / l_bc;
     if (sizeof(T) == sizeof(double) || clamped)
@@ -301,8 +329,8 @@ template <typename T, typename P>
 __device__ __forceinline__ void angle_eval(const Args<T, P> &a, int q, V3<T> &pa_out, V3<T> &pc_out)
 {
     const int b1 = a.an_b1[q], b2 = a.an_b2[q], b3 = a.an_b3[q];
-    angle_force(a, q, a.ix[b1], a.iy[b1], a.iz[b1], a.ix[b2], a.iy[b2], a.iz[b2], a.ix[b3], a.iy[b3], a.iz[b3],
-                pa_out, pc_out);
+    angle_force(load_angle(a, q), a.ix[b1], a.iy[b1], a.iz[b1], a.ix[b2], a.iy[b2], a.iz[b2], a.ix[b3], a.iy[b3],
+                a.iz[b3], pa_out, pc_out);
 }
 
 // Adds the bond terms stored at slots [e0, e1) in slot order (the order in
@@ -376,7 +404,7 @@ __device__ __forceinline__ bool contact_term(const Args<T, P> &a, P x1, P y1, P 
 template <typename T, typename P>
 __device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T> &f, const PairC *table)
 {
-    const int e0 = a.nb_start[i], e1 = a.nb_start[i + 1];
+    const int e0 = a.nb_start[i], e1 = e0 + a.nb_cnt[i];
     if (e0 == e1) return;
     const P x1 = a.px[i], y1 = a.py[i], z1 = a.pz[i];
     const T r1 = a.radius[i];
@@ -398,16 +426,14 @@ constexpr int kWaveChunk = 256;
 // its longest list; the terms go through LDS and each bead adds its own in
 // list order.  An entry without force stores -0.0, which leaves any sum
 // unchanged, so the result equals gather_contacts bit for bit.  Every lane
-// of the wave must call this; `live` lanes own bead i at position x1...
-// `term` is this wave's LDS area of kWaveChunk entries.
+// of the wave must call this; a live lane's bead is at x1.. with radius r1
+// and list entries [e0, e0 + cnt) (cnt 0 for other lanes).  `term` is this
+// wave's LDS area of kWaveChunk entries.
 template <typename T, typename P>
-__device__ __forceinline__ void wave_contacts(const Args<T, P> &a, int i, bool live, P x1, P y1, P z1, V3<T> &f,
+__device__ __forceinline__ void wave_contacts(const Args<T, P> &a, int e0, int cnt, T r1, P x1, P y1, P z1, V3<T> &f,
                                               const PairC *table, V3<T> *term)
 {
     const int lane = threadIdx.x & 63;
-    const int e0 = live ? a.nb_start[i] : 0;
-    const int cnt = live ? a.nb_start[i + 1] - e0 : 0;
-    const T r1 = live ? a.radius[i] : (T)0;
     int incl = cnt;                                   // inclusive scan of the counts over the wave
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {

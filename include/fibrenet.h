@@ -66,6 +66,8 @@ typedef struct fn_stats {
     long long rebuilds;            /* grid + contact-list rebuilds              */
     long long list_entries;        /* two-sided contact-list entries (=2 pairs) */
     long long list_capacity;       /* reserved entries                          */
+    long long candidate_entries;   /* two-sided candidate entries (reach+skin)  */
+    int candidate_builds;          /* candidate builds during the last run      */
     int hashed_beads;              /* beads inside the grid at the last rebuild */
     int max_cell_occupancy;        /* the reference drops beads beyond 64       */
     int cells_over_capacity;       /* cells that the reference would truncate   */
@@ -134,6 +136,13 @@ int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_spri
  *                 angle (i-1, i, i+1), numbered along the chain) in-wave,
  *                 used when every bond follows that layout
  *   "chain_layout" (read only) 1 when every bond follows the chain layout
+ *   "verlet" 0/1/2 keep the contact candidates (pairs within reach + skin)
+ *                 between rebuilds while every bead stays within skin/2 of
+ *                 its position at their build: never, always, or auto
+ *                 (default: every 16th run tries it and keeps it while it
+ *                 saves candidate builds)
+ *   "staged" 1/0  neighbour search reads each block's neighbourhood from LDS
+ *   "skin"   n    the skin in permille of the largest listing distance (300)
  *   "graphs" 1/0  launch through captured hipGraphs or directly
  * All options give bit-identical results; they change only the speed. */
 int fn_set_option(fn_engine *eng, const char *name, int value);

@@ -97,11 +97,20 @@ def test_chain_kernel_equals_slot_path(need_gpu, workload, precision):
     assert np.array_equal(a.export_breaks(len(net.springs)), b.export_breaks(len(net.springs)))
 
 
+# Candidate-list settings: kept with the default skin (30% of the reach), kept
+# with a tiny skin (rebuilt at most rebuilds), rebuilt every time, and the
+# automatic choice.
+VERLET = {"on": {"verlet": 1}, "skin5": {"verlet": 1, "skin": 5}, "off": {"verlet": 0}, "auto": {}}
+
+
+@pytest.mark.parametrize("verlet", sorted(VERLET))
 @pytest.mark.parametrize("workload", sorted(WORKLOADS))
-def test_cells_pairs_breaks_exact_over_1000_steps(need_gpu, workload):
+def test_cells_pairs_breaks_exact_over_1000_steps(need_gpu, workload, verlet):
     """f64 build against the sequential oracle, every rebuild of 1,000 steps."""
     net = WORKLOADS[workload]()
     eng, ids = engine_for(net, "f64")
+    for k, v in VERLET[verlet].items():
+        eng.set_option(k, v)
     orc = oracle_for(net)
     g = net.grid_update_freq
     n_rebuilds = 0
@@ -203,4 +212,28 @@ def test_simulation_minimize_then_run_matches_oracle(need_gpu, tmp_path, fast):
     errs = field_errors(got, orc.beads)
     for f, e in errs.items():
         assert e <= TOL["f64"][f], (f, e)
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("workload", sorted(WORKLOADS))
+def test_candidate_list_is_exact(need_gpu, workload, precision):
+    """Keeping the skin-padded candidates between rebuilds gives the same bytes
+    as rebuilding them at every rebuild, and the candidates are kept for at
+    least some rebuilds."""
+    net = WORKLOADS[workload]()
+    a, ia = engine_for(net, precision)
+    b, ib = engine_for(net, precision)
+    a.set_option("verlet", 1)
+    b.set_option("verlet", 0)
+    builds = 0
+    for n in (7, 93, 100):
+        run_engine(a, net, n, fast=True)
+        run_engine(b, net, n, fast=True)
+        builds += a.stats()["candidate_builds"]
+        assert np.array_equal(a.export_pairs(), b.export_pairs())
+    assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
+    rebuilds = a.stats()["rebuilds"]
+    print(workload, precision, "candidate builds", builds, "of", rebuilds, "rebuilds")
+    assert 1 <= builds < rebuilds
+    assert b.stats()["candidate_builds"] >= 1
 
