@@ -659,11 +659,14 @@ class Engine final : public fn_engine {
         return e;
     }
 
+    // Records a timing event on the stream; under graph capture as an
+    // external event node, which records each time the graph runs.
     void mark(std::vector<hipEvent_t> *v)
     {
         if (!v) return;
         v->push_back(new_event());
-        HIPCK(hipEventRecord(v->back(), stream_));
+        if (capturing_) HIPCK(hipEventRecordWithFlags(v->back(), stream_, hipEventRecordExternal));
+        else HIPCK(hipEventRecord(v->back(), stream_));
     }
 
     void accumulate(const Events &ev)
@@ -1446,21 +1449,25 @@ class Engine final : public fn_engine {
     void run_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
     {
         const auto key = std::make_tuple(len, guf, (int)spr, (int)ang, (int)first, (int)last, pc_, fc_, n_spr_, n_ang_);
-        // Timed runs launch directly: HIP cannot time events recorded as graph nodes.
-        const bool use_graph = graphs_on_ && !timing_;
-        if (use_graph) {
+        if (graphs_on_) {
             auto it = graphs_.find(key);
             if (it == graphs_.end()) {
                 GraphRec rec;
                 hipGraph_t g;
                 HIPCK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-                enqueue_chunk(len, guf, spr, ang, first, last, nullptr);
+                capturing_ = true;
+                enqueue_chunk(len, guf, spr, ang, first, last, timing_ ? &rec.ev : nullptr);
+                capturing_ = false;
                 HIPCK(hipStreamEndCapture(stream_, &g));
                 HIPCK(hipGraphInstantiate(&rec.ex, g, nullptr, nullptr, 0));
                 HIPCK(hipGraphDestroy(g));
                 it = graphs_.emplace(key, rec).first;
             }
             HIPCK(hipGraphLaunch(it->second.ex, stream_));
+            if (timing_) {   // the graph's event nodes hold this replay's times until the next replay
+                HIPCK(hipStreamSynchronize(stream_));
+                accumulate(it->second.ev);
+            }
         } else {
             loose_.emplace_back();
             enqueue_chunk(len, guf, spr, ang, first, last, timing_ ? &loose_.back() : nullptr);
@@ -1514,6 +1521,7 @@ class Engine final : public fn_engine {
     std::list<Events> loose_;
     std::vector<const Events *> used_;
     bool timing_ = false;
+    bool capturing_ = false;
     fn_timing timing_acc_{};
 
     int N_ = 0, S_ = 0, A_ = 0, C_ = 0;

@@ -149,11 +149,12 @@ int fn_set_option(fn_engine *eng, const char *name, int value);
 int fn_get_option(fn_engine *eng, const char *name, int *value);
 
 /* Kernel timing of the fast path (no reference counterpart): with timing on,
- * fn_run_steps launches its kernels directly (not through graphs) and records
- * HIP events around each launch slot of every step (spring terms, angle
- * terms, bead kernel; with the chain bead kernel the first two slots are
- * empty) and around every grid rebuild on the engine's stream;
- * fn_get_timing sums them since timing was switched on. */
+ * fn_run_steps records HIP events around each launch slot of every step
+ * (spring terms, angle terms, bead kernel; with the chain bead kernel the
+ * first two slots are empty) and around every grid rebuild on the engine's
+ * stream - as event nodes of the captured graphs (the stream is synchronised
+ * after each graph to read them) or around direct launches when graphs are
+ * off; fn_get_timing sums them since timing was switched on. */
 typedef struct fn_timing {
     int steps;                     /* steps measured                           */
     int rebuilds;                  /* rebuild sequences measured               */
