@@ -6,8 +6,8 @@ OpenclManager/SimulationManager.  This package holds the ctypes bindings and
 the synthetic-network generator.
 """
 from . import records
-from .api import (Engine, FibreNetError, KERNEL_NAMES, LIB_PATH, Model, Simulation, bind_standard, lib,
+from .api import (Engine, Ensemble, FibreNetError, KERNEL_NAMES, LIB_PATH, Model, Simulation, bind_standard, lib,
                   run_loop_granular)
 
-__all__ = ["Engine", "FibreNetError", "KERNEL_NAMES", "LIB_PATH", "Model", "Simulation", "bind_standard",
-           "lib", "records", "run_loop_granular"]
+__all__ = ["Engine", "Ensemble", "FibreNetError", "KERNEL_NAMES", "LIB_PATH", "Model", "Simulation",
+           "bind_standard", "lib", "records", "run_loop_granular"]

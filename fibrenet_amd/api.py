@@ -47,6 +47,8 @@ SIGNATURES = [
     ("fn_set_timing", _I, [_VP, _I]),
     ("fn_get_timing", _I, [_VP, _VP]),
     ("fn_copy_probe", _I, [_VP, _c.c_longlong, _VP]),
+    ("fn_batch_create", _I, [_I, _I, _I, _VP, _VP]),
+    ("fn_batch_read", _I, [_VP, _I, _I, _VP]),
     ("fn_model_load", _I, [_CP, _c.POINTER(_VP)]),
     ("fn_model_free", _I, [_VP]),
     ("fn_model_error", _CP, []),
@@ -80,6 +82,12 @@ class FnTiming(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class FnNetwork(ctypes.Structure):
+    """fn_network: one network of a batched ensemble (host records + settings)."""
+    _fields_ = [("beads", _VP), ("info", _VP), ("n_beads", _I), ("springs", _VP), ("n_springs", _I),
+                ("angles", _VP), ("n_angles", _I), ("pairs", _VP), ("n_pairs", _I), ("settings", _VP)]
 
 
 class FibreNetError(RuntimeError):
@@ -397,3 +405,47 @@ class Simulation:
 
     def springs(self):
         return self.engine.read_buffer(self.buffer_id("spring"), R.SPRING)
+
+
+class Ensemble(Engine):
+    """A batched ensemble (fn_batch_create): independent networks - each a
+    generate.Network or Model with its own records and settings - advanced
+    together by run_steps, one launch per step for all of them."""
+
+    _KINDS = {"bead": (R.KIND["bead"], R.BEAD), "spring": (R.KIND["spring"], R.SPRING),
+              "angle": (R.KIND["angle"], R.ANGLE)}
+
+    def __init__(self, nets, precision="f32", device=0):
+        self._keep = []
+        arr = (FnNetwork * len(nets))()
+        self.counts = []
+        for k, net in enumerate(nets):
+            b = np.ascontiguousarray(net.beads, R.BEAD)
+            q = np.ascontiguousarray(net.info, R.BEAD_INFO)
+            sp = np.ascontiguousarray(net.springs, R.SPRING)
+            an = np.ascontiguousarray(net.angles, R.ANGLE)
+            pr = np.ascontiguousarray(net.pairs, R.PAIR)
+            st = np.array(net.settings, R.SETTINGS).reshape(())
+            self._keep += [b, q, sp, an, pr, st]
+            ptr = lambda a: a.ctypes.data if a.size else None
+            arr[k] = FnNetwork(ptr(b), ptr(q), len(b), ptr(sp), len(sp), ptr(an), len(an), ptr(pr), len(pr),
+                               st.ctypes.data)
+            self.counts.append({"bead": len(b), "spring": len(sp), "angle": len(an)})
+        h = ctypes.c_void_p()
+        L = lib()
+        rc = L.fn_batch_create(int(device), PREC[precision], len(nets), arr, ctypes.byref(h))
+        if rc != FN_OK:
+            msg = L.fn_last_error(h).decode() if h.value else "fn_batch_create failed (no HIP device?)"
+            if h.value:
+                L.fn_destroy(h)
+            raise FibreNetError(rc, msg)
+        super().__init__(precision, device, handle=h.value)
+        self._owned = True
+
+    def read(self, k, kind="bead"):
+        code, rec = self._KINDS[kind]
+        out = np.zeros(self.counts[k][kind], rec)
+        if len(out):
+            self._chk(lib().fn_batch_read(self.h, int(k), code, out.ctypes.data))
+        return out
+

@@ -155,7 +155,7 @@ struct KernelRec {
 
 fnk::Box make_box(const fn_settings &s)
 {
-    fnk::Box b;
+    fnk::Box b{};
     b.bc = s.bc_type;
     b.cx = s.cells_x;
     b.cy = s.cells_y;
@@ -215,6 +215,8 @@ struct fn_engine {
     virtual int get_option(const std::string &name) = 0;
     virtual void timing(fn_timing *out) = 0;
     virtual void copy_probe(long long bytes, double *ms) = 0;
+    virtual void load_ensemble(const fn_network *nets, int n) = 0;
+    virtual void ensemble_read(int net, int kind, void *out) = 0;
 };
 
 namespace {
@@ -458,6 +460,11 @@ class Engine final : public fn_engine {
     {
         if (n <= 0) throw Error(FN_ERR_ARG, "loop size should be a positive number");
         if (guf <= 0) throw Error(FN_ERR_ARG, "grid_update_freq should be positive");
+        if (!ens_boxes_.empty()) {
+            prepare_ensemble_run(spr, ang);
+            run_chunks(n, guf, spr && S_ > 0, ang && A_ > 0);
+            return;
+        }
         for (int kid : {K_ADVANCE_POSITION, K_FILL_GRID, K_BUILD_LIST, K_RESOLVE_CONTACTS, K_ADVANCE_VELOCITY}) {
             const KernelRec &kr = kernel(kKernelNames[kid]);
             const int m = active(kid, kr);
@@ -484,6 +491,12 @@ class Engine final : public fn_engine {
             fnk::fill_int(need_.p, 1, 1, stream_);
             cand_dirty_ = false;
         }
+        run_chunks(n, guf, spr, ang);
+    }
+
+    // The chunks of a fast-path run, then the checks after it.
+    void run_chunks(int n, int guf, bool spr, bool ang)
+    {
         // capacity of the contact list: grow before the run if the last rebuild came close
         if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
         fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
@@ -989,7 +1002,8 @@ class Engine final : public fn_engine {
         if (N_ <= 0 || role_[FN_BUF_BEAD_INFO] < 0) throw Error(FN_ERR_ARG, "bead and bead-info buffers are required");
         if (role_[FN_BUF_PAIR] < 0) throw Error(FN_ERR_ARG, "pair table buffer is required");
         if (csr_dirty_) build_csr();
-        if ((long long)N_ >= (1LL << fnk::kPidShift) || (int)h_pairs_.size() > (1 << (31 - fnk::kPidShift)))
+        if ((long long)N_ >= (1LL << fnk::kPidShift) ||
+            (ens_boxes_.empty() && (int)h_pairs_.size() > (1 << (31 - fnk::kPidShift))))
             throw Error(FN_ERR_UNSUPPORTED, "more than 2^26 beads or 32 type pairs");
     }
 
@@ -1044,6 +1058,8 @@ class Engine final : public fn_engine {
         a.sp_term = sp_term_.p; a.an_term = an_term_.p;
         a.sp_slot = sp_slot_.p; a.an_slot = an_slot_.p;
         a.ch_spring = ch_spring_.p; a.ch_angle = ch_angle_.p;
+        a.net = ens_net();
+        a.boxes = boxes_.p;
         a.step = step_.p;
         return a;
     }
@@ -1179,7 +1195,8 @@ class Engine final : public fn_engine {
     void enqueue_sort(int pc, const fnk::Box &b, const int *gate)
     {
         fnk::fill_gated(cell_count_.p, 0, C_ + 1, gate, stream_);
-        fnk::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, b, key_.p, ckey_.p, rank_.p, cell_count_.p, gate,
+        fnk::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, b, ens_net(), boxes_.p, key_.p, ckey_.p, rank_.p,
+                  cell_count_.p, gate,
                   stream_);
         fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, gate, stream_);
         fnk::place(key_.p, ckey_.p, rank_.p, cell_start_.p, pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, type_.p, tag_.p,
@@ -1295,9 +1312,9 @@ class Engine final : public fn_engine {
         for (;;) {
             if (sync) fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
             fnk::fill_gated(ncnt_.p, 0, N_, gate, stream_);
-            fnk::search(sorted_.p, cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b,
-                        final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, final_list, ncnt_.p,
-                        scratch_.p, K_, stats_.p, N_, staged_, gate, stream_);
+            fnk::search(sorted_.p, cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b, ens_net(),
+                        boxes_.p, final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, final_list,
+                        ncnt_.p, scratch_.p, K_, stats_.p, N_, staged_, gate, stream_);
             fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, gate, stream_);
             if (sync) {
                 int tot = 0, over = 0;
@@ -1400,10 +1417,10 @@ class Engine final : public fn_engine {
 
     void enqueue_chunk(int len, int guf, bool spr, bool ang, bool first, bool last, Events *ev)
     {
-        const fnk::Box bd = make_box(settings_of(K_ADVANCE_POSITION));
-        const fnk::Box bf = make_box(settings_of(K_FILL_GRID));
-        const fnk::Box bl = make_box(settings_of(K_BUILD_LIST));
-        const fnk::Box bc = make_box(settings_of(K_RESOLVE_CONTACTS));
+        const fnk::Box bd = kbox(K_ADVANCE_POSITION);
+        const fnk::Box bf = kbox(K_FILL_GRID);
+        const fnk::Box bl = kbox(K_BUILD_LIST);
+        const fnk::Box bc = kbox(K_RESOLVE_CONTACTS);
         int pc = pc_, fc = fc_;
         const bool chain = use_chain();
         if (first) {
@@ -1445,6 +1462,188 @@ class Engine final : public fn_engine {
     }
 
     bool use_chain() const { return chain_on_ && chain_ok_ && n_spr_ == S_ && n_ang_ == A_; }
+
+    // Settings of a kernel as a Box; in an ensemble every kernel reads its
+    // network's Box and this one (the first network's) only fills the slot.
+    fnk::Box kbox(int kid)
+    {
+        if (!ens_boxes_.empty()) return ens_boxes_[0];
+        return make_box(settings_of(kid));
+    }
+
+    // ---- ensemble ---------------------------------------------------------------------------
+    // Independent networks in one engine: beads, bonds and pair tables
+    // concatenated (bond indices shifted), every bead tagged with its network,
+    // every network with its own Box (box, grid, integrator constants) whose
+    // cells are numbered after the previous networks' and whose pair entries
+    // start at pair_base.  Contacts never cross networks: a bead only searches
+    // its own network's cells.
+    void load_ensemble(const fn_network *nets, int n) override
+    {
+        if (n <= 0 || !nets) throw Error(FN_ERR_ARG, "an ensemble needs at least one network");
+        if (!bufs_.empty()) throw Error(FN_ERR_ARG, "an ensemble is loaded into an engine without buffers");
+        std::vector<fn_bead> beads;
+        std::vector<fn_bead_info> info;
+        std::vector<fn_spring> sp;
+        std::vector<fn_angle> an;
+        std::vector<fn_pair> pr;
+        std::vector<fnk::Box> boxes;
+        std::vector<int> net;
+        std::vector<fn_settings> sets;
+        std::vector<int> b0{0}, s0{0}, a0{0};
+        long long cells = 0;
+        bool finite = true;
+        for (int k = 0; k < n; ++k) {
+            const fn_network &w = nets[k];
+            if (w.n_beads <= 0 || !w.beads || !w.info || !w.settings || w.n_pairs <= 0 || !w.pairs)
+                throw Error(FN_ERR_ARG, fmt("network %d: beads, bead info, pair table and settings are required", k));
+            if (w.n_springs < 0 || w.n_angles < 0 || (w.n_springs && !w.springs) || (w.n_angles && !w.angles))
+                throw Error(FN_ERR_ARG, fmt("network %d: bond counts and records disagree", k));
+            const fn_settings &st = *w.settings;
+            const int nt = st.types_num;
+            if (st.cells_x < 3 || st.cells_y < 3 || st.cells_z < 3 || (long long)st.cells_x * st.cells_y != st.cells_xy)
+                throw Error(FN_ERR_UNSUPPORTED, fmt("network %d: grid with fewer than three cells along an axis", k));
+            if (nt <= 0 || (long long)nt * (nt + 1) / 2 > w.n_pairs || w.n_pairs > (1 << (31 - fnk::kPidShift)))
+                throw Error(FN_ERR_ARG, fmt("network %d: pair table does not match types_num", k));
+            const int bb = (int)beads.size(), sb = (int)sp.size(), pb = (int)pr.size();
+            for (int i = 0; i < w.n_beads; ++i) {
+                if (w.info[i].type < 0 || w.info[i].type >= nt)
+                    throw Error(FN_ERR_ARG, fmt("network %d: bead type outside [0, types_num)", k));
+                beads.push_back(w.beads[i]);
+                info.push_back(w.info[i]);
+                net.push_back(k);
+            }
+            for (int q = 0; q < w.n_springs; ++q) {
+                fn_spring r = w.springs[q];
+                if (r.p1 < 0 || r.p1 >= w.n_beads || r.p2 < 0 || r.p2 >= w.n_beads)
+                    throw Error(FN_ERR_ARG, fmt("network %d: spring bead index out of range", k));
+                r.p1 += bb;
+                r.p2 += bb;
+                sp.push_back(r);
+            }
+            for (int q = 0; q < w.n_angles; ++q) {
+                fn_angle r = w.angles[q];
+                if (r.b1 < 0 || r.b1 >= w.n_beads || r.b2 < 0 || r.b2 >= w.n_beads || r.b3 < 0 || r.b3 >= w.n_beads)
+                    throw Error(FN_ERR_ARG, fmt("network %d: angle bead index out of range", k));
+                if (r.type == 1 && (r.s1 < 0 || r.s1 >= w.n_springs || r.s2 < 0 || r.s2 >= w.n_springs))
+                    throw Error(FN_ERR_ARG, fmt("network %d: HR_FL angle spring index out of range", k));
+                r.b1 += bb;
+                r.b2 += bb;
+                r.b3 += bb;
+                r.s1 += sb;
+                r.s2 += sb;
+                an.push_back(r);
+            }
+            for (int q = 0; q < w.n_pairs; ++q) pr.push_back(w.pairs[q]);
+            fnk::Box bx = make_box(st);
+            bx.cell_base = (int)cells;
+            bx.pair_base = pb;
+            cells += (long long)st.cells_xy * st.cells_z;
+            if (cells >= (1LL << 31) - 1) throw Error(FN_ERR_CAPACITY, "more than 2^31 grid cells in the ensemble");
+            boxes.push_back(bx);
+            sets.push_back(st);
+            finite = finite && st.bc_type == 1;
+            b0.push_back((int)beads.size());
+            s0.push_back((int)sp.size());
+            a0.push_back((int)an.size());
+        }
+        create_buffer(FN_BUF_BEAD, sizeof(fn_bead), (int)beads.size(), beads.data());
+        create_buffer(FN_BUF_BEAD_INFO, sizeof(fn_bead_info), (int)info.size(), info.data());
+        if (!sp.empty()) create_buffer(FN_BUF_SPRING, sizeof(fn_spring), (int)sp.size(), sp.data());
+        if (!an.empty()) create_buffer(FN_BUF_ANGLE, sizeof(fn_angle), (int)an.size(), an.data());
+        create_buffer(FN_BUF_PAIR, sizeof(fn_pair), (int)pr.size(), pr.data());
+        ens_boxes_ = boxes;
+        ens_sets_ = sets;
+        ens_b0_ = b0;
+        ens_s0_ = s0;
+        ens_a0_ = a0;
+        ens_finite_ = finite;
+        boxes_.put(boxes, stream_);
+        net_.put(net, stream_);
+        C_ = (int)cells;
+        cell_count_.alloc((size_t)C_ + 1);
+        cell_start_.alloc((size_t)C_ + 1);
+        HIPCK(hipStreamSynchronize(stream_));
+        clear_graphs();
+    }
+
+    void prepare_ensemble_run(bool spr, bool ang)
+    {
+        (void)spr;
+        (void)ang;
+        prepare();
+        n_spr_ = S_;
+        n_ang_ = A_;
+        check_alias(ens_finite_ ? 1 : 0);
+        ensure_grid(nullptr);
+        const int nb = std::max(fnk::scan_blocks_needed(std::max(C_, N_)), 1) + 1;
+        if ((int)scan_tmp_.n < nb) {
+            clear_graphs();
+            scan_tmp_.alloc(nb);
+        }
+        ensure_scratch();
+        // one rebuild path (the exact search over every network's own 3x3x3
+        // stencil); the prefilter distance covers every network's reach
+        double r2 = 0.0;
+        for (const fn_settings &st : ens_sets_) r2 = std::max(r2, reach2(st));
+        fnk::Stencil ex{};
+        for (int k = 0; k < 3; ++k) ex.w[k] = 1;
+        if (r2 != reach2_ || keep_cand_) clear_graphs();
+        reach2_ = r2;
+        keep_cand_ = false;
+        stencil_exact_ = ex;
+    }
+
+    const int *ens_net() const { return ens_boxes_.empty() ? nullptr : net_.p; }
+
+    // Records of one network, with its bond indices shifted back.
+    void ensemble_read(int k, int kind, void *out) override
+    {
+        if (ens_boxes_.empty()) throw Error(FN_ERR_ARG, "the engine holds no ensemble");
+        if (k < 0 || k >= (int)ens_boxes_.size()) throw Error(FN_ERR_ARG, fmt("no network %d", k));
+        const int b0 = ens_b0_[k], b1 = ens_b0_[k + 1], s0 = ens_s0_[k], s1 = ens_s0_[k + 1];
+        const int a0 = ens_a0_[k], a1 = ens_a0_[k + 1];
+        switch (kind) {
+        case FN_BUF_BEAD: {
+            std::vector<fn_bead> all((size_t)pad64(N_));
+            download_beads(all.data());
+            memcpy(out, all.data() + b0, sizeof(fn_bead) * (size_t)(b1 - b0));
+            break;
+        }
+        case FN_BUF_SPRING: {
+            std::vector<int> st;
+            sp_status_.get(st, S_, stream_);
+            HIPCK(hipStreamSynchronize(stream_));
+            fn_spring *o = (fn_spring *)out;
+            for (int q = s0; q < s1; ++q) {
+                fn_spring r = h_springs_[q];
+                r.status = st[q];
+                r.p1 -= b0;
+                r.p2 -= b0;
+                o[q - s0] = r;
+            }
+            break;
+        }
+        case FN_BUF_ANGLE: {
+            std::vector<int> st;
+            an_status_.get(st, A_, stream_);
+            HIPCK(hipStreamSynchronize(stream_));
+            fn_angle *o = (fn_angle *)out;
+            for (int q = a0; q < a1; ++q) {
+                fn_angle r = h_angles_[q];
+                r.status = st[q];
+                r.b1 -= b0;
+                r.b2 -= b0;
+                r.b3 -= b0;
+                r.s1 -= s0;
+                r.s2 -= s0;
+                o[q - a0] = r;
+            }
+            break;
+        }
+        default: throw Error(FN_ERR_ARG, "ensemble records are beads, springs or angles");
+        }
+    }
 
     void run_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
     {
@@ -1561,6 +1760,14 @@ class Engine final : public fn_engine {
     DArr<int> key_, ckey_, rank_, cell_count_, cell_start_, sorted_, ncnt_, nb_start_, nb_cnt_, cand_, nb_list_, scan_tmp_,
         stats_, step_, need_;
     DArr<double> ref_[3];   // positions at the last candidate build
+    // ensemble: Box and settings of every network, first bead / spring / angle
+    // of every network (and the totals), network of every bead
+    std::vector<fnk::Box> ens_boxes_;
+    std::vector<fn_settings> ens_sets_;
+    std::vector<int> ens_b0_, ens_s0_, ens_a0_;
+    bool ens_finite_ = false;
+    DArr<fnk::Box> boxes_;
+    DArr<int> net_;
     DArr<fnk::SortedBead> so_bead_;
     DArr<int> so_type_, so_cell_, so_slot_, scratch_;
     double reach2_ = 0.0;
@@ -1749,6 +1956,22 @@ int fn_copy_probe(fn_engine *eng, long long bytes, double *ms)
     return guarded(eng, [&] {
         if (bytes <= 0 || !ms) throw Error(FN_ERR_ARG, "copy probe needs a positive size and a destination");
         eng->copy_probe(bytes, ms);
+    });
+}
+
+int fn_batch_create(int device, int precision, int n_networks, const fn_network *nets, fn_engine **out)
+{
+    const int rc = fn_create(device, precision, out);
+    if (rc != FN_OK) return rc;
+    const int lr = guarded(*out, [&] { (*out)->load_ensemble(nets, n_networks); });
+    return lr;
+}
+
+int fn_batch_read(fn_engine *eng, int network, int kind, void *out)
+{
+    return guarded(eng, [&] {
+        if (!out) throw Error(FN_ERR_ARG, "null destination");
+        eng->ensemble_read(network, kind, out);
     });
 }
 

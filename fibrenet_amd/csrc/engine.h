@@ -64,8 +64,10 @@ struct Launch {
 // outside the grid), the cell of the clamped position (ckey), the rank in
 // that cell and the counts; an exclusive scan of the counts; place puts the
 // beads' data into cell order.
-void hash(const double *px, const double *py, const double *pz, int n, const Box &b, int *key, int *ckey, int *rank,
-          int *count, const int *gate, hipStream_t s);
+// (net, boxes: the ensemble's network of each bead and Box of each network,
+// or NULL for one network described by b)
+void hash(const double *px, const double *py, const double *pz, int n, const Box &b, const int *net,
+          const Box *boxes, int *key, int *ckey, int *rank, int *count, const int *gate, hipStream_t s);
 void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
            const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
            const Sorted &so, const int *gate, hipStream_t s);
@@ -75,8 +77,9 @@ void place(const int *key, const int *ckey, const int *rank, const int *cell_sta
 // compaction into a bead-indexed CSR sorted per bead (with cnt_out: the final
 // list, its lengths and the listed total).
 void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
-            const double *radius, const double *skin, const Box &b, const Stencil &st, double r2, bool exact,
-            int *cnt, int *scratch, int K, int *stats, int n, bool staged, const int *gate, hipStream_t s);
+            const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
+            const Stencil &st, double r2, bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
+            const int *gate, hipStream_t s);
 void compact(const int *cnt, const int *cand_start, const int *slot_of, const int *scratch, int K, int n, int *cand,
              int cap, int *cnt_out, const int *gate, hipStream_t s);
 // *need = 1 when a bead moved more than sqrt(lim2) since the snapshot.

@@ -142,8 +142,16 @@ __global__ void __launch_bounds__(kBlock) k_reset_energy(Args<T, P> a, int n_act
     a.wx[i] = a.wy[i] = a.wz[i] = (T)0;
 }
 
+// The Box of bead i's network (ENS) or the single network's.
+template <bool ENS, typename T, typename P>
+__device__ __forceinline__ const Box &drift_box(const Args<T, P> &a, int i)
+{
+    if constexpr (ENS) return a.boxes[a.net[i]];
+    else return a.drift;
+}
+
 // First drift of a fused chunk: positions of the step into the other buffer.
-template <typename T, typename P>
+template <typename T, typename P, bool ENS>
 __global__ void __launch_bounds__(kBlock) k_drift_first(Args<T, P> a)
 {
     const int i = xcd_block() * kBlock + threadIdx.x;
@@ -151,8 +159,8 @@ __global__ void __launch_bounds__(kBlock) k_drift_first(Args<T, P> a)
     P px = a.px[i], py = a.py[i], pz = a.pz[i];
     P ix = a.ix[i], iy = a.iy[i], iz = a.iz[i];
     V3<T> vp;
-    drift(a.drift, a.fx[i], a.fy[i], a.fz[i], a.wx[i], a.wy[i], a.wz[i], a.lam[i], a.hlf2[i], px, py, pz, ix, iy,
-          iz, vp);
+    drift(drift_box<ENS>(a, i), a.fx[i], a.fy[i], a.fz[i], a.wx[i], a.wy[i], a.wz[i], a.lam[i], a.hlf2[i], px, py,
+          pz, ix, iy, iz, vp);
     a.qx[i] = px; a.qy[i] = py; a.qz[i] = pz;
     if (a.jx != a.qx) { a.jx[i] = ix; a.jy[i] = iy; a.jz[i] = iz; }
 }
@@ -172,7 +180,7 @@ struct BeadState {
 // Loads bead i and returns its base force: the predicted velocity of this
 // step (kernel.cl:200-205) gives damping, plus gravity and walls.
 template <typename T, typename P>
-__device__ __forceinline__ V3<T> bead_begin(const Args<T, P> &a, int i, BeadState<T, P> &b)
+__device__ __forceinline__ V3<T> bead_begin(const Args<T, P> &a, const Box &bx, int i, BeadState<T, P> &b)
 {
     b.fpx = a.fx[i]; b.fpy = a.fy[i]; b.fpz = a.fz[i];
     b.wx = a.wx[i]; b.wy = a.wy[i]; b.wz = a.wz[i];
@@ -181,15 +189,15 @@ __device__ __forceinline__ V3<T> bead_begin(const Args<T, P> &a, int i, BeadStat
     b.vp.y = tfma(b.fpy, b.lam, b.wy);
     b.vp.z = tfma(b.fpz, b.lam, b.wz);
     b.px = a.px[i]; b.py = a.py[i]; b.pz = a.pz[i];
-    return base_force<T, P>(a.drift, b.vp, b.px, b.py, b.pz);
+    return base_force<T, P>(bx, b.vp, b.px, b.py, b.pz);
 }
 
 // Closing kick of the step (kernel.cl:444-464) and, with NEXT, the drift of
 // the next step from image (ix, iy, iz); with LAST the predicted velocity of
 // this step for read-back.
 template <typename T, typename P, bool NEXT, bool LAST>
-__device__ __forceinline__ void bead_end(const Args<T, P> &a, int i, const BeadState<T, P> &b, const V3<T> &f, P ix,
-                                         P iy, P iz)
+__device__ __forceinline__ void bead_end(const Args<T, P> &a, const Box &bx, int i, const BeadState<T, P> &b,
+                                         const V3<T> &f, P ix, P iy, P iz)
 {
     a.gx[i] = f.x; a.gy[i] = f.y; a.gz[i] = f.z;
     const T h = a.hlf[i];
@@ -203,7 +211,7 @@ __device__ __forceinline__ void bead_end(const Args<T, P> &a, int i, const BeadS
     if (NEXT) {
         P px = b.px, py = b.py, pz = b.pz;
         V3<T> vn;
-        drift(a.drift, f.x, f.y, f.z, nwx, nwy, nwz, b.lam, a.hlf2[i], px, py, pz, ix, iy, iz, vn);
+        drift(bx, f.x, f.y, f.z, nwx, nwy, nwz, b.lam, a.hlf2[i], px, py, pz, ix, iy, iz, vn);
         a.qx[i] = px; a.qy[i] = py; a.qz[i] = pz;
         if (a.jx != a.qx) { a.jx[i] = ix; a.jy[i] = iy; a.jz[i] = iz; }
     }
@@ -214,26 +222,28 @@ __device__ __forceinline__ void bead_end(const Args<T, P> &a, int i, const BeadS
 // drift.  Reads f of the previous step from f*, writes this step's force to
 // g*.  The terms and their summation order are those of the per-kernel
 // path, so both paths agree bit for bit.
-template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST>
+template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS>
 __global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
 {
     __shared__ PairC prs[kPairCap];
     __shared__ V3<T> cterm[kBlock / 64][kWaveChunk];
     const int i = xcd_block() * kBlock + threadIdx.x;
-    for (int k = threadIdx.x; k < a.n_pairs; k += kBlock) prs[k] = a.pairs[k];
+    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = a.pairs[k];
     __syncthreads();
     const bool live = i < a.n;
     const int ic = live ? i : a.n - 1;      // lanes past the end follow along for the wave-wide contacts
     const int e0 = a.nb_start[ic], cnt = live ? a.nb_cnt[ic] : 0;
     const T r1 = a.radius[ic];
+    const Box &bx = drift_box<ENS>(a, ic);
     BeadState<T, P> b;
-    V3<T> f = bead_begin(a, ic, b);
+    V3<T> f = bead_begin(a, bx, ic, b);
     if (SPR) add_terms(a.sp_term, a.sp_start[ic], a.sp_start[ic + 1], f);
     if (ANG) add_terms(a.an_term, a.an_start[ic], a.an_start[ic + 1], f);
-    wave_contacts(a, e0, cnt, r1, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6]);
+    wave_contacts<T, P, ENS>(a, e0, cnt, r1, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6],
+                             ENS ? a.net[ic] : 0);
     if (!live) return;
-    if (NEXT) bead_end<T, P, NEXT, LAST>(a, i, b, f, a.ix[i], a.iy[i], a.iz[i]);
-    else bead_end<T, P, NEXT, LAST>(a, i, b, f, (P)0, (P)0, (P)0);
+    if (NEXT) bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, a.ix[i], a.iy[i], a.iz[i]);
+    else bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, (P)0, (P)0, (P)0);
 }
 
 // Chain layout: a wave covers 64 consecutive beads and owns the middle 60;
@@ -250,12 +260,12 @@ constexpr int kChainOwned = 64 - 2 * kHalo;
 // values for inactive bonds and their summation order are those of the
 // per-kernel path, so both paths agree bit for bit.  Only owned lanes write
 // bond statuses and bead state.
-template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST>
+template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS>
 __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
 {
     __shared__ PairC prs[kPairCap];
     __shared__ V3<T> cterm[kBlock / 64][kWaveChunk];
-    for (int k = threadIdx.x; k < a.n_pairs; k += kBlock) prs[k] = a.pairs[k];
+    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = a.pairs[k];
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int i = (xcd_block() * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kChainOwned + lane - kHalo;
@@ -270,8 +280,9 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
         y = a.iy[i];
         z = a.iz[i];
     }
+    const Box &bx = drift_box<ENS>(a, ic);
     BeadState<T, P> b;
-    V3<T> f = bead_begin(a, ic, b);
+    V3<T> f = bead_begin(a, bx, ic, b);
     const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
     const T r1 = a.radius[ic];
     const int sr = (in && (SPR || ANG)) ? a.ch_spring[i] : -1;
@@ -352,9 +363,10 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
             f.z += pan.z;
         }
     }
-    wave_contacts(a, e0, cnt, r1, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6]);
+    wave_contacts<T, P, ENS>(a, e0, cnt, r1, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6],
+                             ENS ? a.net[ic] : 0);
     if (!own) return;
-    bead_end<T, P, NEXT, LAST>(a, i, b, f, x, y, z);
+    bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, x, y, z);
 }
 
 __global__ void k_add_int(int *p, int v) { *p += v; }
@@ -383,6 +395,14 @@ __global__ void __launch_bounds__(kBlock) k_fill_int(int *p, int v, int n)
 
 __device__ __forceinline__ bool gated_off(const int *gate) { return gate && *gate == 0; }
 
+// The Box of bead i's network in an ensemble, else the single network's.
+template <bool ENS>
+__device__ __forceinline__ const Box &net_box(const Box &single, const Box *boxes, const int *net, int i)
+{
+    if constexpr (ENS) return boxes[net[i]];
+    else return single;
+}
+
 __global__ void __launch_bounds__(kBlock) k_fill_gated(int *p, int v, int n, const int *gate)
 {
     if (gated_off(gate)) return;
@@ -406,19 +426,24 @@ __global__ void k_note_build(const int *total, int *stats, const int *gate)
 // FillGrid's cell of a bead (kernel.cl:492-504) in fp64 (-1 outside the grid)
 // into key; the cell of its position clamped into the box into ckey, with the
 // bead's rank among the beads of that cell (first pass of a counting sort).
-__global__ void __launch_bounds__(kBlock) k_hash(const double *px, const double *py, const double *pz, int n, Box b,
-                                                 int *key, int *ckey, int *rank, int *count, const int *gate)
+// In an ensemble the cells are those of the bead's network, numbered from its
+// Box's cell_base.
+template <bool ENS>
+__global__ void __launch_bounds__(kBlock) k_hash(const double *px, const double *py, const double *pz, int n, Box bs,
+                                                 const int *net, const Box *boxes, int *key, int *ckey, int *rank,
+                                                 int *count, const int *gate)
 {
     if (gated_off(gate)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
+    const Box &b = net_box<ENS>(bs, boxes, net, i);
     const int cx = (int)floor(px[i] / b.sx);
     const int cy = (int)floor(py[i] / b.sy);
     const int cz = (int)floor(pz[i] / b.sz);
     const bool in = cx >= 0 && cx < b.cx && cy >= 0 && cy < b.cy && cz >= 0 && cz < b.cz;
-    key[i] = in ? cx + cy * b.cx + cz * b.cxy : -1;
+    key[i] = in ? b.cell_base + cx + cy * b.cx + cz * b.cxy : -1;
     const int qx = min(max(cx, 0), b.cx - 1), qy = min(max(cy, 0), b.cy - 1), qz = min(max(cz, 0), b.cz - 1);
-    const int c = qx + qy * b.cx + qz * b.cxy;
+    const int c = b.cell_base + qx + qy * b.cx + qz * b.cxy;
     ckey[i] = c;
     rank[i] = atomicAdd(&count[c], 1);
 }
@@ -553,20 +578,23 @@ struct AxisRange {
 // Candidate search of the bead in slot s over the stencil's cells; the cells
 // of one x-row are one contiguous slot range unless the row wraps or is
 // clipped.  Up to K entries go to the slot's scratch row; cnt[bead] = count.
-template <bool EXACT>
+template <bool EXACT, bool ENS>
 __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int *cell_start, int n_cells, Sorted so,
-                                                   const PairC *pairs, const double *radius, const double *skin, Box b,
-                                                   Stencil st, double r2, int *cnt, int *scratch, int K, int *stats,
-                                                   const int *gate)
+                                                   const PairC *pairs_all, const double *radius, const double *skin,
+                                                   Box bs, const int *net, const Box *boxes, Stencil st, double r2,
+                                                   int *cnt, int *scratch, int K, int *stats, const int *gate)
 {
     if (gated_off(gate)) return;
-    const int s = blockIdx.x * kBlock + threadIdx.x;
+    const int s = xcd_block() * kBlock + threadIdx.x;   // neighbouring slot ranges share one XCD's L2
     int k = 0, occ = 0;
     if (s < cell_start[n_cells]) {
         const int i = sorted[s];
+        const Box &b = net_box<ENS>(bs, boxes, net, i);
+        const PairC *pairs = pairs_all + b.pair_base;
         const int code = so.cell[s];
-        const int c = slot_cell(code);
-        if (cell_start[c] == s) occ = cell_start[c + 1] - s;   // the cell's first slot reports its occupancy
+        const int cg = slot_cell(code);
+        if (cell_start[cg] == s) occ = cell_start[cg + 1] - s;   // the cell's first slot reports its occupancy
+        const int c = cg - b.cell_base;                          // cell within the bead's network
         const int cz = c / b.cxy;
         const int cy = (c - cz * b.cxy) / b.cx;
         const int cx = c - cz * b.cxy - cy * b.cx;
@@ -584,7 +612,7 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
             for (int dy = ay.first(); dy <= ay.last(); ++dy) {
                 const int y = ay.at(dy);
                 if (y < 0) continue;
-                const int rbase = y * b.cx + z * b.cxy;
+                const int rbase = b.cell_base + y * b.cx + z * b.cxy;
                 if (row_run) {
                     scan_range<EXACT>(cell_start[rbase + cx - st.w[0]], cell_start[rbase + cx + st.w[0] + 1], s, me,
                                       ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b, r2, row, K, k);
@@ -635,7 +663,7 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
     __shared__ SortedBead rec[kStageCap];
     __shared__ int lo_q[kStageRows], hi_q[kStageRows], base_q[kStageRows + 1];
     const int nh = cell_start[n_cells];
-    const int s0 = blockIdx.x * kStageBlock;
+    const int s0 = xcd_block() * kStageBlock;
     if (s0 >= nh) return;
     const int s_end = min(s0 + kStageBlock, nh);
     const int wy = st.w[1], wz = st.w[2], ny = 2 * wy + 1, nq = ny * (2 * wz + 1);
@@ -827,7 +855,7 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     __shared__ int verdict[kBlock / 64][kWaveChunk];
     int *acc = verdict[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int i = xcd_block() * kBlock + threadIdx.x;
     const bool live = i < f.n;
     const int ic = live ? i : f.n - 1;
     const Box &g = f.grid;
@@ -1041,17 +1069,19 @@ void Launch<T, P>::reset_energy(const Args<T, P> &a, int n_active, hipStream_t s
 template <typename T, typename P>
 void Launch<T, P>::drift_first(const Args<T, P> &a, hipStream_t s)
 {
-    k_drift_first<T, P><<<blocks(a.n), kBlock, 0, s>>>(a);
+    if (a.net) k_drift_first<T, P, true><<<blocks(a.n), kBlock, 0, s>>>(a);
+    else k_drift_first<T, P, false><<<blocks(a.n), kBlock, 0, s>>>(a);
 }
 template <typename T, typename P, int M>
 static void launch_fused_mask(const Args<T, P> &a, hipStream_t s)
 {
     constexpr bool spr = (M & 1) != 0, ang = (M & 2) != 0, next = (M & 4) != 0, last = (M & 8) != 0;
+    constexpr bool ens = (M & 32) != 0;
     if (M & 16) {
         constexpr int per_block = kChainOwned * (kBlock / 64);
-        k_bead_step_chain<T, P, spr, ang, next, last><<<(a.n + per_block - 1) / per_block, kBlock, 0, s>>>(a);
+        k_bead_step_chain<T, P, spr, ang, next, last, ens><<<(a.n + per_block - 1) / per_block, kBlock, 0, s>>>(a);
     } else {
-        k_bead_step<T, P, spr, ang, next, last><<<blocks(a.n), kBlock, 0, s>>>(a);
+        k_bead_step<T, P, spr, ang, next, last, ens><<<blocks(a.n), kBlock, 0, s>>>(a);
     }
 }
 
@@ -1067,8 +1097,9 @@ template <typename T, typename P>
 void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, bool chain,
                              hipStream_t s)
 {
-    const int mask = (spr ? 1 : 0) | (ang ? 2 : 0) | (next ? 4 : 0) | (last ? 8 : 0) | (chain ? 16 : 0);
-    dispatch_fused<T, P>(a, mask, s, std::make_integer_sequence<int, 32>{});
+    const int mask =
+        (spr ? 1 : 0) | (ang ? 2 : 0) | (next ? 4 : 0) | (last ? 8 : 0) | (chain ? 16 : 0) | (a.net ? 32 : 0);
+    dispatch_fused<T, P>(a, mask, s, std::make_integer_sequence<int, 64>{});
 }
 template struct Launch<double, double>;
 template struct Launch<float, double>;
@@ -1094,10 +1125,11 @@ void fill_gated(int *p, int v, int n, const int *gate, hipStream_t s)
     if (n > 0) k_fill_gated<<<blocks(n), kBlock, 0, s>>>(p, v, n, gate);
 }
 void set_int(int *p, int v, const int *gate, hipStream_t s) { k_set_int<<<1, 1, 0, s>>>(p, v, gate); }
-void hash(const double *px, const double *py, const double *pz, int n, const Box &b, int *key, int *ckey, int *rank,
-          int *count, const int *gate, hipStream_t s)
+void hash(const double *px, const double *py, const double *pz, int n, const Box &b, const int *net,
+          const Box *boxes, int *key, int *ckey, int *rank, int *count, const int *gate, hipStream_t s)
 {
-    k_hash<<<blocks(n), kBlock, 0, s>>>(px, py, pz, n, b, key, ckey, rank, count, gate);
+    if (net) k_hash<true><<<blocks(n), kBlock, 0, s>>>(px, py, pz, n, b, net, boxes, key, ckey, rank, count, gate);
+    else k_hash<false><<<blocks(n), kBlock, 0, s>>>(px, py, pz, n, b, nullptr, nullptr, key, ckey, rank, count, gate);
 }
 void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
            const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
@@ -1108,26 +1140,31 @@ void place(const int *key, const int *ckey, const int *rank, const int *cell_sta
 }
 template <bool EXACT>
 static void search_t(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
-                     const double *radius, const double *skin, const Box &b, const Stencil &st, double r2, int *cnt,
-                     int *scratch, int K, int *stats, int n, bool staged, const int *gate, hipStream_t s)
+                     const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
+                     const Stencil &st, double r2, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
+                     const int *gate, hipStream_t s)
 {
-    if (staged && st.w[1] <= 2 && st.w[2] <= 2)
+    if (net)
+        k_search<EXACT, true><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b,
+                                                           net, boxes, st, r2, cnt, scratch, K, stats, gate);
+    else if (staged && st.w[1] <= 2 && st.w[2] <= 2)
         k_search_staged<EXACT><<<(n + kStageBlock - 1) / kStageBlock, kStageBlock, 0, s>>>(
             sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2, cnt, scratch, K, stats, gate);
     else
-        k_search<EXACT><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2,
-                                                     cnt, scratch, K, stats, gate);
+        k_search<EXACT, false><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b,
+                                                            nullptr, nullptr, st, r2, cnt, scratch, K, stats, gate);
 }
 void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
-            const double *radius, const double *skin, const Box &b, const Stencil &st, double r2, bool exact,
-            int *cnt, int *scratch, int K, int *stats, int n, bool staged, const int *gate, hipStream_t s)
+            const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
+            const Stencil &st, double r2, bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
+            const int *gate, hipStream_t s)
 {
     if (exact)
-        search_t<true>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2, cnt, scratch, K, stats, n,
-                       staged, gate, s);
+        search_t<true>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K,
+                       stats, n, staged, gate, s);
     else
-        search_t<false>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2, cnt, scratch, K, stats, n,
-                        staged, gate, s);
+        search_t<false>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K,
+                        stats, n, staged, gate, s);
 }
 void compact(const int *cnt, const int *cand_start, const int *slot_of, const int *scratch, int K, int n, int *cand,
              int cap, int *cnt_out, const int *gate, hipStream_t s)

@@ -21,11 +21,14 @@ namespace fnk {
 
 constexpr int kBlock = 256;
 
-// Grid / box parameters taken from one bound copy of SimulationSettings.
+// Grid / box parameters taken from one bound copy of SimulationSettings.  In
+// an ensemble every network has its own, with the offsets of its cells in the
+// ensemble's cell numbering and of its entries in the concatenated pair table.
 struct Box {
     int bc;                 // 0 periodic, 1 finite
     int cx, cy, cz, cxy;    // grid cells
     int types;              // number of bead types
+    int cell_base, pair_base;
     double lx, ly, lz;      // box size
     double sx, sy, sz;      // grid step
     double wall_k, hbs, cutoff_sq, damping, gx, gy, gz, dt;
@@ -98,6 +101,9 @@ struct Args {
     int n_pairs;
     // settings of AdvancePosition (integrator, walls) and ResolveContacts (minimum image)
     Box drift, force;
+    // ensemble: the network of every bead and the Box of every network
+    const int *net;
+    const Box *boxes;
     int *step;                              // device step counter
     int step_off;
 };
@@ -353,12 +359,11 @@ __device__ __forceinline__ void add_terms(const V3<T> *term, int e0, int e1, V3<
 // evaluates rij = p_1 - p_j).  Returns false when the pair exerts no force
 // this step.
 template <typename T, typename P>
-__device__ __forceinline__ bool contact_term(const Args<T, P> &a, P x1, P y1, P z1, T r1, int ent,
+__device__ __forceinline__ bool contact_term(const Args<T, P> &a, const Box &b, P x1, P y1, P z1, T r1, int ent,
                                              const PairC *table, V3<T> &t)
 {
     const int j = ent & kIdxMask;
     const PairC pr = table[ent >> kPidShift];
-    const Box &b = a.force;
     P rx = x1 - a.px[j], ry = y1 - a.py[j], rz = z1 - a.pz[j];
     P len = rx * rx + ry * ry + rz * rz;
     if (len > (P)b.hbs) {
@@ -410,7 +415,7 @@ __device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T
     const T r1 = a.radius[i];
     for (int e = e0; e < e1; ++e) {
         V3<T> t;
-        if (!contact_term(a, x1, y1, z1, r1, a.nb_list[e], table, t)) continue;
+        if (!contact_term(a, a.force, x1, y1, z1, r1, a.nb_list[e], table, t)) continue;
         f.x += t.x;
         f.y += t.y;
         f.z += t.z;
@@ -428,10 +433,11 @@ constexpr int kWaveChunk = 256;
 // unchanged, so the result equals gather_contacts bit for bit.  Every lane
 // of the wave must call this; a live lane's bead is at x1.. with radius r1
 // and list entries [e0, e0 + cnt) (cnt 0 for other lanes).  `term` is this
-// wave's LDS area of kWaveChunk entries.
-template <typename T, typename P>
+// wave's LDS area of kWaveChunk entries.  ENS: the lane's bead belongs to
+// network `netw` (its Box and pair table); otherwise a.force and `table`.
+template <typename T, typename P, bool ENS>
 __device__ __forceinline__ void wave_contacts(const Args<T, P> &a, int e0, int cnt, T r1, P x1, P y1, P z1, V3<T> &f,
-                                              const PairC *table, V3<T> *term)
+                                              const PairC *table, V3<T> *term, int netw = 0)
 {
     const int lane = threadIdx.x & 63;
     int incl = cnt;                                   // inclusive scan of the counts over the wave
@@ -457,8 +463,14 @@ __device__ __forceinline__ void wave_contacts(const Args<T, P> &a, int e0, int c
             const P xL = __shfl(x1, L, 64), yL = __shfl(y1, L, 64), zL = __shfl(z1, L, 64);
             const T rL = __shfl(r1, L, 64);
             V3<T> t{(T)-0.0, (T)-0.0, (T)-0.0};
-            if (k < total && !contact_term(a, xL, yL, zL, rL, a.nb_list[eL], table, t))
-                t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
+            if (ENS) {
+                const Box &bk = a.boxes[__shfl(netw, L, 64)];
+                if (k < total && !contact_term(a, bk, xL, yL, zL, rL, a.nb_list[eL], a.pairs + bk.pair_base, t))
+                    t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
+            } else {
+                if (k < total && !contact_term(a, a.force, xL, yL, zL, rL, a.nb_list[eL], table, t))
+                    t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
+            }
             term[r + lane] = t;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -185,7 +185,8 @@ def _chain_bonds(n_fibres, n_beads, spring_type, k, l0, e_mod=None, sigma_f=None
 
 
 def periodic_hs(n_fibres=200, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B0, dt=1e-3, steps=10000,
-                spring_k=100.0, angle_k=10.0, contact_k=1000.0, radius=0.5, mass=1.0, grid_update_freq=8):
+                spring_k=100.0, angle_k=10.0, contact_k=1000.0, radius=0.5, mass=1.0, grid_update_freq=8,
+                cell_table=True):
     net = Network()
     s = make_settings(box, 0, 1.0, 1.0, dt, 0.5, wall_k=100.0)
     xyz, tries = place_fibres(seed, n_fibres, n_beads, box, finite=False)
@@ -209,7 +210,8 @@ def periodic_hs(n_fibres=200, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B0,
     net.springs, net.angles = _chain_bonds(n_fibres, n_beads, 0, spring_k, 1.0, angle_k=angle_k)
     net.pairs = np.array([hs_pair(0, 0, contact_k)], R.PAIR)
     net.settings = s
-    net.cell_nh = grid_neighbours(0, int(s["cells_x"]), int(s["cells_y"]), int(s["cells_z"]))
+    if cell_table:   # the GridCellNh buffer of the per-kernel path; an ensemble does not use it
+        net.cell_nh = grid_neighbours(0, int(s["cells_x"]), int(s["cells_y"]), int(s["cells_z"]))
     net.grid_update_freq = grid_update_freq
     net.run_steps = steps
     net.type_names = ["C1"]
@@ -321,3 +323,26 @@ def write_files(net, directory, name="model"):
     with open(ipath, "w") as fh:
         fh.write("\n".join(inp) + "\n")
     return ipath
+
+
+def ensemble(n_networks=512, seed=0xF1B0, fibres=(20, 400), lengths=(20, 80), side=(60.0, 200.0),
+             volume_fraction=0.05, stiffness=(30.0, 300.0), dt=1e-3):
+    """Independent periodic networks of uneven size: per network a fibre count
+    and a fibre length uniform in the given ranges, a cubic box whose side
+    gives the volume fraction (clamped to `side`), a spring stiffness
+    log-uniform in `stiffness`; angles and penalty contacts as in
+    periodic_hs.  Network i uses the seed seed + 2000 + i."""
+    rs = np.random.Generator(np.random.PCG64(seed))
+    nets = []
+    for i in range(n_networks):
+        nf = int(rs.integers(fibres[0], fibres[1] + 1))
+        nb = int(rs.integers(lengths[0], lengths[1] + 1))
+        n = nf * nb
+        L = float(np.clip((n * (np.pi / 6.0) / volume_fraction) ** (1.0 / 3.0), side[0], side[1]))
+        k = float(np.exp(rs.uniform(np.log(stiffness[0]), np.log(stiffness[1]))))
+        net = periodic_hs(n_fibres=nf, n_beads=nb, box=(L, L, L), seed=seed + 2000 + i, dt=dt, spring_k=k,
+                          cell_table=False)
+        net.meta.update(network=i, spring_k=k, side=L)
+        nets.append(net)
+    return nets
+

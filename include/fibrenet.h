@@ -173,6 +173,31 @@ int fn_get_timing(fn_engine *eng, fn_timing *out);
  * counters to bytes (no reference counterpart).  *ms receives its duration. */
 int fn_copy_probe(fn_engine *eng, long long bytes, double *ms);
 
+/* ---- batched ensemble (no reference counterpart) ----
+ * Independent networks, each with its own records and settings (box, grid,
+ * pair table, integrator constants), advanced together: every fn_run_steps
+ * step is one launch over all of them, and contacts never cross networks.
+ * Bond indices are local to their network.  The ensemble uses the fast path
+ * only (fn_run_steps); fn_get_stats and fn_export_* cover all networks with
+ * global bead indices (network k's beads follow networks 0..k-1). */
+typedef struct fn_network {
+    const fn_bead *beads;          /* n_beads records                          */
+    const fn_bead_info *info;      /* n_beads records                          */
+    int n_beads;
+    const fn_spring *springs;      /* may be NULL when n_springs is 0          */
+    int n_springs;
+    const fn_angle *angles;        /* may be NULL when n_angles is 0           */
+    int n_angles;
+    const fn_pair *pairs;          /* the network's pair table (<= 32 entries) */
+    int n_pairs;
+    const fn_settings *settings;   /* box, grid and integrator of the network  */
+} fn_network;
+/* Opens device `device` and loads the networks into one engine. */
+int fn_batch_create(int device, int precision, int n_networks, const fn_network *nets, fn_engine **out);
+/* Records of network `network`: kind FN_BUF_BEAD, FN_BUF_SPRING or
+ * FN_BUF_ANGLE (bond indices local to the network). */
+int fn_batch_read(fn_engine *eng, int network, int kind, void *out);
+
 /* Debug exports for parity checks (no reference counterpart). */
 /* Cell index of every bead at the last rebuild (-1: outside the grid). */
 int fn_export_cells(fn_engine *eng, int *cell_of_bead);

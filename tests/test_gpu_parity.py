@@ -13,6 +13,8 @@ Tolerances, stated per build:
 Cell indices, contact-pair sets and spring-break steps are compared exactly.
 The fused (hipGraph) path and the per-kernel path are compared bit for bit.
 """
+import copy
+
 import numpy as np
 import pytest
 
@@ -236,4 +238,46 @@ def test_candidate_list_is_exact(need_gpu, workload, precision):
     print(workload, precision, "candidate builds", builds, "of", rebuilds, "rebuilds")
     assert 1 <= builds < rebuilds
     assert b.stats()["candidate_builds"] >= 1
+
+
+def _ensemble_members():
+    """Networks of uneven size and kind: periodic penalty networks in
+    different boxes, the tensile network (finite box, LJ, breaking springs)
+    and the crosslinked one (bonds outside the chain layout)."""
+    from fibrenet_amd import generate as G
+    nets = [G.periodic_hs(n_fibres=f, n_beads=b, box=(L, L, 0.5 * L), seed=30 + f, spring_k=k)
+            for f, b, L, k in ((12, 15, 16.0, 60.0), (40, 22, 30.0, 150.0), (5, 9, 9.0, 100.0))]
+    return nets + [small_tensile(weak=True), crosslinked()]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+def test_ensemble_equals_plain_runs(need_gpu, precision):
+    """a batched ensemble reproduces each network's own run bit for bit:
+    same boxes, pair tables and integrator constants per network, contacts
+    only inside a network, the same terms in the same order."""
+    from fibrenet_amd import Ensemble
+    nets = _ensemble_members()
+    guf = 8
+    ens = Ensemble(nets, precision)
+    for n in (5, 60):
+        ens.run_steps(n, guf, True, True)
+    for k, net in enumerate(nets):
+        eng, ids = engine_for(net, precision)
+        for n in (5, 60):
+            eng.run_steps(n, guf, len(net.springs) > 0, len(net.angles) > 0)
+        assert ens.read(k, "bead").tobytes() == read_beads(eng, ids).tobytes(), k
+        assert np.array_equal(ens.read(k, "spring")["status"], eng.read_buffer(ids["spring"], R.SPRING)["status"])
+        assert np.array_equal(ens.read(k, "angle")["status"], eng.read_buffer(ids["angle"], R.ANGLE)["status"])
+    st = ens.stats()
+    assert st["steps"] == 65 and st["list_entries"] > 0
+
+
+def test_ensemble_rejects_bad_networks(need_gpu):
+    from fibrenet_amd import Ensemble, FibreNetError
+    net = _ensemble_members()[0]
+    bad = copy.deepcopy(net)
+    bad.springs = bad.springs.copy()
+    bad.springs["p2"][0] = len(bad.beads) + 3
+    with pytest.raises(FibreNetError):
+        Ensemble([net, bad], "f64")
 
