@@ -78,6 +78,50 @@ def algorithmic_bytes(net, pairs, tb, aliased, chain):
             dict(beads=n, springs_hr=s_hr, springs_hr_fl=s_fl, angles=a, pairs=pairs))
 
 
+def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8):
+    """Batched ensemble of config-2 networks against looping the plain engine
+    over the same networks: bead-steps/s of each (Sum N_i per step)."""
+    from fibrenet_amd import Engine, Ensemble, bind_standard, generate as G
+    t0 = time.perf_counter()
+    nets = G.ensemble(n_networks)
+    gen_s = time.perf_counter() - t0
+    n_tot = sum(len(n.beads) for n in nets)
+    ens = Ensemble(nets, precision)
+    ens.run_steps(warmup, guf, True, True)
+    ens.run_steps(steps, guf, True, True)          # captures the graphs of this run length
+    ens.finish()
+    t = time.perf_counter()
+    ens.run_steps(steps, guf, True, True)
+    wall = time.perf_counter() - t
+    st = ens.stats()
+    ens.close()
+    # the plain engine, one network after the other (graphs captured first)
+    engines = []
+    for net in nets:
+        net.cell_nh = G.grid_neighbours(0, int(net.settings["cells_x"]), int(net.settings["cells_y"]),
+                                        int(net.settings["cells_z"]))
+        e = Engine(precision)
+        bind_standard(e, net)
+        e.run_steps(loop_steps, guf, True, True)
+        engines.append(e)
+    for e in engines:
+        e.finish()
+    t = time.perf_counter()
+    for e in engines:
+        e.run_steps(loop_steps, guf, True, True)
+    for e in engines:
+        e.finish()
+    wall_loop = time.perf_counter() - t
+    for e in engines:
+        e.close()
+    v_ens = n_tot * steps / wall
+    v_loop = n_tot * loop_steps / wall_loop
+    return {"networks": n_networks, "beads": n_tot, "steps": steps, "ms_per_step": wall * 1e3 / steps,
+            "bead_steps_per_s": v_ens, "loop_steps": loop_steps, "loop_bead_steps_per_s": v_loop,
+            "speedup_over_loop": v_ens / v_loop, "listed_pairs": st["list_entries"] // 2,
+            "grid_update_freq": guf, "generate_s": gen_s}
+
+
 def cpu_baseline(net, steps):
     from oracle.oracle import Oracle
     orc = Oracle(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, net.cell_nh)
@@ -93,7 +137,11 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--workload", default="tensile_lj", choices=["tensile_lj", "periodic_hs"])
+    ap.add_argument("--workload", default="tensile_lj", choices=["tensile_lj", "periodic_hs", "ensemble"])
+    ap.add_argument("--networks", type=int, default=512, help="ensemble: number of networks")
+    ap.add_argument("--loop-steps", type=int, default=40, help="ensemble: steps of the looped plain engine")
+    ap.add_argument("--ensemble-networks", type=int, default=128,
+                    help="networks of the ensemble comparison reported beside the plain workload (0: none)")
     ap.add_argument("--fibres", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -108,6 +156,21 @@ def main():
         sys.exit(2)
 
     from fibrenet_amd import Engine, bind_standard, generate as G
+
+    if args.workload == "ensemble":
+        e = ensemble_bench(args.networks, args.steps, args.warmup, args.loop_steps, args.precision)
+        out = {"metric": "bead-steps/s (batched ensemble)", "value": e["bead_steps_per_s"], "unit": "bead-steps/s",
+               "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": e["ms_per_step"],
+               "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "f32 forces/velocities, f64 positions" if args.precision == "f32" else "f64",
+               "data": "synthetic (generated periodic fibre networks)",
+               "config": {"workload": "ensemble: %d periodic networks, fibres U[20,400] x beads U[20,80], cubic "
+                                      "boxes of side 60-200 at volume fraction 0.05, spring k log-uniform "
+                                      "[30,300], penalty contacts, dt 1e-3" % args.networks,
+                          "beads": e["beads"], "precision": args.precision},
+               "extra": e}
+        print(json.dumps(out))
+        return
 
     t0 = time.perf_counter()
     if args.workload == "tensile_lj":
@@ -178,6 +241,13 @@ def main():
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
+    ens = None
+    if args.ensemble_networks > 0:
+        try:
+            ens = ensemble_bench(args.ensemble_networks, 400, 40, args.loop_steps, args.precision)
+        except Exception as exc:          # the plain measurement stands on its own
+            ens = {"error": repr(exc)}
+
     cpu = None
     if not args.no_cpu_baseline:
         v, dt = cpu_baseline(net, args.cpu_steps)
@@ -210,7 +280,8 @@ def main():
                   "rebuild_ms": rebuild_ms, "rebuild_algorithmic_bytes": b_rebuild,
                   "step_algorithmic_GBps_incl_rebuild": (b_step + b_rebuild / net.grid_update_freq)
                   * args.steps / wall / 1e9, "counts": counts, "stats": st, "generate_s": gen_s,
-                  "gpu_over_cpu": (value / cpu["value"]) if cpu else None, "copy_bytes": copy},
+                  "gpu_over_cpu": (value / cpu["value"]) if cpu else None, "copy_bytes": copy,
+                  "ensemble": ens},
     }
     print(json.dumps(out))
 

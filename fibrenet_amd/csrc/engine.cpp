@@ -868,6 +868,13 @@ class Engine final : public fn_engine {
         type_.put(ty, stream_); tag_.put(tg, stream_); loc_.put(lc, stream_);
         rad_d_.put(rd, stream_); skin_d_.put(sk, stream_);
         HIPCK(hipStreamSynchronize(stream_));
+        max_radius_ = 0.0;
+        max_skin_ = 0.0;
+        types_checked_ = -1;
+        for (const fn_bead_info &q : h_info_) {
+            max_radius_ = std::max(max_radius_, q.radius);
+            max_skin_ = std::max(max_skin_, q.skin_radius);
+        }
         cand_dirty_ = true;
         clear_graphs();
     }
@@ -1116,8 +1123,11 @@ class Engine final : public fn_engine {
             const int nt = s->types_num;
             if (nt <= 0 || (long long)nt * (nt + 1) / 2 > (long long)h_pairs_.size())
                 throw Error(FN_ERR_ARG, "pair table smaller than types_num*(types_num+1)/2");
-            for (const fn_bead_info &q : h_info_)
-                if (q.type < 0 || q.type >= nt) throw Error(FN_ERR_ARG, "bead type outside [0, types_num)");
+            if (types_checked_ != nt) {   // once per upload of the bead info and types_num
+                for (const fn_bead_info &q : h_info_)
+                    if (q.type < 0 || q.type >= nt) throw Error(FN_ERR_ARG, "bead type outside [0, types_num)");
+                types_checked_ = nt;
+            }
             if (C != C_) {
                 clear_graphs();
                 C_ = C;
@@ -1212,14 +1222,10 @@ class Engine final : public fn_engine {
     // lj and st, (max radius + max skin)^2 for hs (kernel.cl:632-659).
     double reach2(const fn_settings &s) const
     {
-        double r2 = 0.0, maxr = 0.0, maxs = 0.0;
-        for (const fn_bead_info &q : h_info_) {
-            maxr = std::max(maxr, q.radius);
-            maxs = std::max(maxs, q.skin_radius);
-        }
+        double r2 = 0.0;
         for (const fn_pair &p : h_pairs_) {
             if (p.type == 1 || p.type == 3) r2 = std::max(r2, s.cutoff_sq);
-            if (p.type == 2) r2 = std::max(r2, (maxr + maxs) * (maxr + maxs));
+            if (p.type == 2) r2 = std::max(r2, (max_radius_ + max_skin_) * (max_radius_ + max_skin_));
         }
         return r2;
     }
@@ -1584,8 +1590,11 @@ class Engine final : public fn_engine {
         ensure_scratch();
         // one rebuild path (the exact search over every network's own 3x3x3
         // stencil); the prefilter distance covers every network's reach
-        double r2 = 0.0;
-        for (const fn_settings &st : ens_sets_) r2 = std::max(r2, reach2(st));
+        double r2 = 0.0, cut2 = 0.0;
+        for (const fn_settings &st : ens_sets_) cut2 = std::max(cut2, st.cutoff_sq);
+        fn_settings any{};
+        any.cutoff_sq = cut2;
+        r2 = reach2(any);
         fnk::Stencil ex{};
         for (int k = 0; k < 3; ++k) ex.w[k] = 1;
         if (r2 != reach2_ || keep_cand_) clear_graphs();
@@ -1762,6 +1771,8 @@ class Engine final : public fn_engine {
     DArr<double> ref_[3];   // positions at the last candidate build
     // ensemble: Box and settings of every network, first bead / spring / angle
     // of every network (and the totals), network of every bead
+    double max_radius_ = 0.0, max_skin_ = 0.0;   // over all beads: the listing reach of hs pairs
+    int types_checked_ = -1;                     // types_num the bead types were last checked against
     std::vector<fnk::Box> ens_boxes_;
     std::vector<fn_settings> ens_sets_;
     std::vector<int> ens_b0_, ens_s0_, ens_a0_;
