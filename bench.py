@@ -122,6 +122,29 @@ def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8):
             "grid_update_freq": guf, "generate_s": gen_s}
 
 
+def adjoint_bench(eng, net, steps, plain_ms, every):
+    """Adjoint cost on the benchmark network, from the engine's current
+    state: the checkpointing forward over `steps` steps sampling the clamp's
+    reaction force 20 times, then the backward sweep (segment recompute,
+    per-step records, adjoint kernel) with dL/d(sample) = 1.  Ratios are
+    against the plain forward's ms per step."""
+    import numpy as np
+    group = (np.asarray(net.info["type"]) == 1).astype(np.int32)
+    samples = np.unique(np.linspace(steps // 20 - 1, steps - 1, 20).astype(np.int32))
+    eng.finish()
+    t0 = time.perf_counter()
+    eng.adjoint_forward(steps, group, samples, net.grid_update_freq, True, True, every)
+    t1 = time.perf_counter()
+    g = eng.adjoint_backward(np.ones(len(samples)), len(net.springs), len(net.angles), len(net.pairs))
+    t2 = time.perf_counter()
+    fwd, bwd = (t1 - t0) * 1e3 / steps, (t2 - t1) * 1e3 / steps
+    return {"steps": steps, "checkpoint_every": every or "auto", "samples": len(samples),
+            "forward_ms_per_step": fwd, "backward_ms_per_step": bwd,
+            "backward_over_forward": bwd / plain_ms, "total_over_forward": (fwd + bwd) / plain_ms,
+            "d_spring_scale_sum": float(g["spring"].sum()), "d_angle_scale_sum": float(g["angle"].sum()),
+            "d_contact_scale": [float(x) for x in g["pair"]]}
+
+
 def cpu_baseline(net, steps):
     from oracle.oracle import Oracle
     orc = Oracle(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, net.cell_nh)
@@ -144,6 +167,9 @@ def main():
                     help="networks of the ensemble comparison reported beside the plain workload (0: none)")
     ap.add_argument("--fibres", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the adjoint cost measurement on the same network (0: none)")
+    ap.add_argument("--checkpoint-every", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-bytes", type=float, default=0,
                     help="also run one device copy of this many bytes (counter calibration)")
@@ -241,6 +267,13 @@ def main():
         with open(tfile) as fh:
             traffic = json.load(fh).get("bytes_per_launch")
 
+    adj = None
+    if args.adjoint_steps > 0:
+        try:
+            adj = adjoint_bench(eng, net, args.adjoint_steps, wall * 1e3 / args.steps, args.checkpoint_every)
+        except Exception as exc:          # the plain measurement stands on its own
+            adj = {"error": repr(exc)}
+
     ens = None
     if args.ensemble_networks > 0:
         try:
@@ -281,7 +314,7 @@ def main():
                   "step_algorithmic_GBps_incl_rebuild": (b_step + b_rebuild / net.grid_update_freq)
                   * args.steps / wall / 1e9, "counts": counts, "stats": st, "generate_s": gen_s,
                   "gpu_over_cpu": (value / cpu["value"]) if cpu else None, "copy_bytes": copy,
-                  "ensemble": ens},
+                  "ensemble": ens, "adjoint": adj},
     }
     print(json.dumps(out))
 

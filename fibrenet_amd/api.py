@@ -49,6 +49,8 @@ SIGNATURES = [
     ("fn_copy_probe", _I, [_VP, _c.c_longlong, _VP]),
     ("fn_batch_create", _I, [_I, _I, _I, _VP, _VP]),
     ("fn_batch_read", _I, [_VP, _I, _I, _VP]),
+    ("fn_adjoint_forward", _I, [_VP, _VP, _VP, _VP, _I, _VP]),
+    ("fn_adjoint_backward", _I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     ("fn_model_load", _I, [_CP, _c.POINTER(_VP)]),
     ("fn_model_free", _I, [_VP]),
     ("fn_model_error", _CP, []),
@@ -88,6 +90,12 @@ class FnNetwork(ctypes.Structure):
     """fn_network: one network of a batched ensemble (host records + settings)."""
     _fields_ = [("beads", _VP), ("info", _VP), ("n_beads", _I), ("springs", _VP), ("n_springs", _I),
                 ("angles", _VP), ("n_angles", _I), ("pairs", _VP), ("n_pairs", _I), ("settings", _VP)]
+
+
+class FnAdjointSpec(ctypes.Structure):
+    """fn_adjoint_spec: the run an adjoint forward records."""
+    _fields_ = [("n_steps", _I), ("grid_update_freq", _I), ("run_springs", _I), ("run_angles", _I),
+                ("checkpoint_every", _I)]
 
 
 class FibreNetError(RuntimeError):
@@ -206,8 +214,15 @@ class Engine:
         return dict(zip(("size", "elements", "length", "struct_size"), [x.value for x in v]))
 
     def write_buffer(self, bid, data):
+        """OpenclManager::WriteBuffer: the records padded to the buffer's
+        length (unpadded records of the buffer's element count are padded
+        here with zero records, as CreateBuffer pads)."""
         info = self.buffer_info(bid)
         data = np.ascontiguousarray(data)
+        if len(data) == info["elements"] and info["length"] > len(data):
+            padded = np.zeros(info["length"], data.dtype)
+            padded[:len(data)] = data
+            data = padded
         self._chk(lib().fn_buffer_write(self.h, bid, _ptr(data), len(data), data.dtype.itemsize))
         return info
 
@@ -276,6 +291,37 @@ class Engine:
         ms = ctypes.c_double()
         self._chk(lib().fn_copy_probe(self.h, int(nbytes), ctypes.byref(ms)))
         return ms.value
+
+    # -- adjoint ------------------------------------------------------------------------------
+    def adjoint_forward(self, n, group, sample_steps, grid_update_freq=8, springs=True, angles=True,
+                        checkpoint_every=0):
+        """Runs n steps as run_steps does, recording checkpoints; returns the
+        x-force summed over the beads flagged in `group` at each of
+        `sample_steps` (strictly increasing, in [0, n))."""
+        g = np.ascontiguousarray(group, np.int32)
+        st = np.ascontiguousarray(sample_steps, np.int32)
+        out = np.zeros(max(len(st), 1), np.float64)
+        spec = FnAdjointSpec(int(n), int(grid_update_freq), int(bool(springs)), int(bool(angles)),
+                             int(checkpoint_every))
+        self._chk(lib().fn_adjoint_forward(self.h, ctypes.byref(spec), _ptr(g), _ptr(st), len(st), _ptr(out)))
+        self._adj_beads = len(g)
+        return out[:len(st)]
+
+    def adjoint_backward(self, dloss, n_springs, n_angles, n_pairs):
+        """Derivatives of a loss with dL/d(sample) = dloss through the last
+        adjoint_forward run: per-bond and per-pair-type force-scale
+        derivatives and those of the initial positions and velocity_old."""
+        d = np.ascontiguousarray(dloss, np.float64)
+        n = getattr(self, "_adj_beads", 0)
+        out = {"spring": np.zeros(max(n_springs, 1)), "angle": np.zeros(max(n_angles, 1)),
+               "pair": np.zeros(max(n_pairs, 1)), "position": np.zeros((n, 3)), "velocity_old": np.zeros((n, 3))}
+        self._chk(lib().fn_adjoint_backward(self.h, _ptr(d) if d.size else None, _ptr(out["spring"]),
+                                            _ptr(out["angle"]), _ptr(out["pair"]), _ptr(out["position"]),
+                                            _ptr(out["velocity_old"])))
+        out["spring"] = out["spring"][:n_springs]
+        out["angle"] = out["angle"][:n_angles]
+        out["pair"] = out["pair"][:n_pairs]
+        return out
 
     def set_timing(self, on):
         self._chk(lib().fn_set_timing(self.h, int(bool(on))))

@@ -1,0 +1,339 @@
+// adjoint.hip - reverse-mode derivative of the FibreNet step (gfx950).
+//
+// The step as the engine runs it (per bead, P = position, W = velocity_old,
+// F' = force of the previous step, coefficients lam = lambda*dt/m,
+// h = (dt/2)/m, h2 = (dt^2/2)/m):
+//     x' = x + dt W + h2 F'                 drift (kernel.cl:209-223)
+//     V  = W + lam F'                       predicted velocity (kernel.cl:205-207)
+//     F  = D V + G(x')                      damping, gravity, walls, bonds, contacts
+//     W' = W + h (F + F')                   kick (kernel.cl:459-461)
+// With the contact lists and bond statuses of the forward run held fixed,
+// the adjoint of one step, given the adjoints X, Wb of x', W' and Fb of F
+// (Fb already holding the loss seed and h Wb), is
+//     X  += J(x') Fb                        J = dG/dx (symmetric: every force
+//                                            is central or derives from a
+//                                            potential)
+//     Vb  = D Fb
+//     F'b = h Wb + lam Vb + h2 X
+//     Wb' = Wb + Vb + dt X
+// and the adjoint of a force scale s (force = s * unit force) gains Fb . F.
+// J(x') Fb is the directional derivative of every force along Fb, evaluated
+// here with dual numbers in the same per-bead gathers as the forces.
+//
+// Arithmetic: fp64 in both builds.  Force values (the scale adjoints) use
+// the forward's own rounding where it matters: spring and contact lengths
+// are the float-rounded native_sqrt of the forward (the extension l0 - l and
+// the overlap r1 + r2 - r are differences of nearly equal numbers), angle
+// values come from the forward's angle_force.  Derivatives are those of
+// the exact expressions at the recorded positions.
+#include "kernels.h"
+#include "engine.h"
+
+namespace fnk {
+
+// A dual number v + d eps (eps^2 = 0).
+struct Dual {
+    double v, d;
+};
+__device__ __forceinline__ Dual cst(double v) { return Dual{v, 0.0}; }
+__device__ __forceinline__ Dual operator+(Dual a, Dual b) { return Dual{a.v + b.v, a.d + b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual{a.v - b.v, a.d - b.d}; }
+__device__ __forceinline__ Dual operator-(Dual a) { return Dual{-a.v, -a.d}; }
+__device__ __forceinline__ Dual operator*(Dual a, Dual b) { return Dual{a.v * b.v, a.v * b.d + a.d * b.v}; }
+__device__ __forceinline__ Dual operator*(double s, Dual a) { return Dual{s * a.v, s * a.d}; }
+__device__ __forceinline__ Dual operator/(Dual a, Dual b)
+{
+    const double q = a.v / b.v;
+    return Dual{q, (a.d - q * b.d) / b.v};
+}
+__device__ __forceinline__ Dual dsqrt(Dual a)
+{
+    const double r = sqrt(a.v);
+    return Dual{r, r > 0.0 ? a.d / (2.0 * r) : 0.0};
+}
+__device__ __forceinline__ Dual datan2(Dual y, Dual x)
+{
+    const double n = x.v * x.v + y.v * y.v;
+    return Dual{atan2(y.v, x.v), n > 0.0 ? (x.v * y.d - y.v * x.d) / n : 0.0};
+}
+// |r| whose value is the forward's float-rounded length (native_sqrt) and
+// whose derivative is the exact one.
+__device__ __forceinline__ Dual flen(Dual r2)
+{
+    const double e = sqrt(r2.v);
+    return Dual{(double)native_sqrt(r2.v), e > 0.0 ? r2.d / (2.0 * e) : 0.0};
+}
+
+struct DV3 {
+    Dual x, y, z;
+};
+__device__ __forceinline__ DV3 dcross(const DV3 &a, const DV3 &b)
+{
+    return DV3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ Dual ddot(const DV3 &a, const DV3 &b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ DV3 dscale(Dual s, const DV3 &a) { return DV3{s * a.x, s * a.y, s * a.z}; }
+
+// Spring force on p1 for d = x_p1 - x_p2: k (l0 - l) d / l with the
+// forward's rounded length (kernel.cl:384-437).
+__device__ __forceinline__ DV3 spring_dual(double k, double l0, const DV3 &d)
+{
+    const Dual l = flen(ddot(d, d));
+    if (l.v == 0.0) return DV3{cst(0), cst(0), cst(0)};
+    return dscale((k * (cst(l0) - l)) / l, d);
+}
+
+// Angle forces on b1 (pa) and b3 (pc) from ba = x1 - x2, bc = x3 - x2: the
+// negative gradient of k (theta - theta0)^2 (kernel.cl:289-348 in exact
+// arithmetic).  With c = ba x bc and phi = pi - theta = atan2(|c|, -ba.bc)
+// the force on b1 is
+//     [2k phi/|c| - 2k (pi - theta0)/|c|] (ba x c) / |ba|^2
+// and on b3 the same with (c x bc) / |bc|^2.  phi/|c| is a smooth function
+// of |c|^2 (atan(t)/t is even in t), summed as its series for a nearly
+// straight triple, so the derivative stays accurate where |c| is rounding
+// noise; the second term, zero for theta0 = pi, is singular there and is
+// kept only where |c|^2 > 1e-10 |ba|^2 |bc|^2 (below that the forward's
+// own angle is dominated by rounding).
+__device__ __forceinline__ void angle_dual(double k, double th0, const DV3 &ba, const DV3 &bc, DV3 &pa, DV3 &pc)
+{
+    const DV3 none{cst(0), cst(0), cst(0)};
+    pa = pc = none;
+    const DV3 c = dcross(ba, bc);
+    const Dual c2 = ddot(c, c), la2 = ddot(ba, ba), lc2 = ddot(bc, bc), dp = ddot(ba, bc);
+    if (la2.v == 0.0 || lc2.v == 0.0) return;
+    Dual g;                                    // phi / |c|
+    if (dp.v < 0.0) {
+        const Dual m = -dp;
+        const Dual u = c2 / (m * m);           // tan^2 phi
+        Dual at;                               // atan(t) / t, t^2 = u
+        if (u.v < 1e-4) {
+            at = cst(1.0) + u * (cst(-1.0 / 3.0) + u * (cst(0.2) + u * (cst(-1.0 / 7.0) + u * cst(1.0 / 9.0))));
+        } else {
+            const Dual t = dsqrt(u);
+            at = datan2(t, cst(1.0)) / t;
+        }
+        g = at / m;
+    } else {
+        if (c2.v == 0.0) return;               // folded onto itself: no direction
+        const Dual cn = dsqrt(c2);
+        g = datan2(cn, -dp) / cn;
+    }
+    Dual coef = (2.0 * k) * g;
+    if (c2.v > 1e-10 * la2.v * lc2.v) coef = coef - cst(2.0 * k * (M_PI - th0)) / dsqrt(c2);
+    pa = dscale(coef / la2, dcross(ba, c));
+    pc = dscale(coef / lc2, dcross(c, bc));
+}
+
+// Central contact force on bead 1 for r = x1 - x2 (minimum image applied)
+// as the forward evaluates it (contact_force): lj |F|/r along r (kernel.cl:
+// 875-919), penalty (922-957), two-step harmonic (960-999), with the
+// forward's rounded length.  Called only for pairs that exerted a force.
+__device__ __forceinline__ DV3 contact_dual(const PairC &pr, double r1, double r2, const DV3 &r)
+{
+    const Dual len2 = ddot(r, r);
+    const Dual rr = flen(len2);
+    Dual coef;
+    if (pr.type == 1) {
+        const Dual fr2 = cst(pr.c1 * pr.c1) / len2;
+        const Dual fr6 = fr2 * fr2 * fr2;
+        coef = (48.0 * pr.c2) * fr6 * (fr6 - cst(0.5)) / len2 - cst(pr.c4);
+    } else if (pr.type == 2) {
+        coef = pr.c1 * (cst(r1 + r2) - rr);
+    } else {
+        coef = pr.c1 * (cst(pr.c2) - rr) + pr.c5 * (cst(pr.c6) - rr);
+    }
+    if (rr.v == 0.0) return DV3{cst(0), cst(0), cst(0)};
+    return dscale(coef / rr, r);
+}
+
+__device__ __forceinline__ void add_d(double3 &z, const DV3 &f, double sgn)
+{
+    z.x += sgn * f.x.d;
+    z.y += sgn * f.y.d;
+    z.z += sgn * f.z.d;
+}
+__device__ __forceinline__ double dot_v(const double3 &u, const DV3 &f)
+{
+    return u.x * f.x.v + u.y * f.y.v + u.z * f.z.v;
+}
+__device__ __forceinline__ DV3 dvec(double3 x, double3 u)
+{
+    return DV3{Dual{x.x, u.x}, Dual{x.y, u.y}, Dual{x.z, u.z}};
+}
+__device__ __forceinline__ double3 sub3(double3 a, double3 b) { return double3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+
+// One adjoint step for every bead (thread per bead).  Reads the positions
+// and images at which the step's forces were evaluated, the bond statuses
+// after the step, the step's contact list and Fb (adjoint of this step's
+// force, seed and h Wb included); updates X, Wb, writes Fb of the previous
+// step (seed and h Wb of that step included) to fb_next, and adds the
+// scale adjoints of the bonds (by one owner bead: p1, b2) and of the
+// contacts (per bead and pair entry).
+template <typename T, typename P>
+__global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    const double3 fb = a.fb[i];
+    double3 z{0.0, 0.0, 0.0};
+    auto img = [&](int j) { return double3{(double)a.ix[j], (double)a.iy[j], (double)a.iz[j]}; };
+    // springs: force on p1 is F(x_p1 - x_p2), on p2 its negation
+    for (int e = a.sp_start[i], e1 = a.sp_start[i + 1]; e < e1; ++e) {
+        const int ent = a.sp_ent[e];
+        const int s = ent >> 1, role = ent & 1;
+        if (s >= a.n_springs || a.sp_on[s] != 1 || a.sp_type[s] < 0 || a.sp_type[s] > 1) continue;
+        const int p1 = a.sp_p1[s], p2 = a.sp_p2[s];
+        const double3 u = sub3(a.fb[p1], a.fb[p2]);
+        const DV3 F = spring_dual((double)a.sp_k[s], a.sp_l0[s], dvec(sub3(img(p1), img(p2)), u));
+        add_d(z, F, role == 0 ? 1.0 : -1.0);
+        if (role == 0) a.g_spring[s] += dot_v(u, F);
+    }
+    // angles: pa on b1, pc on b3, -pa-pc on b2
+    for (int e = a.an_start[i], e1 = a.an_start[i + 1]; e < e1; ++e) {
+        const int ent = a.an_ent[e];
+        const int q = ent >> 2, role = ent & 3;     // role 0: b1, 1: b3, 2: b2
+        if (q >= a.n_angles || a.an_on[q] != 1) continue;
+        const int b1 = a.an_b1[q], b2 = a.an_b2[q], b3 = a.an_b3[q];
+        const double3 x1 = img(b1), x2 = img(b2), x3 = img(b3);
+        const double3 u1 = a.fb[b1], u2 = a.fb[b2], u3 = a.fb[b3];
+        DV3 pa, pc;
+        angle_dual((double)a.an_k[q], (double)a.an_th0[q], dvec(sub3(x1, x2), sub3(u1, u2)),
+                   dvec(sub3(x3, x2), sub3(u3, u2)), pa, pc);
+        if (role == 0) add_d(z, pa, 1.0);
+        else if (role == 1) add_d(z, pc, 1.0);
+        else {
+            add_d(z, pa, -1.0);
+            add_d(z, pc, -1.0);
+            V3<T> fa, fc;                        // the forward's own values
+            angle_force(AngleP<T>{1, a.an_type[q], a.an_th0[q], a.an_k[q]}, a.ix[b1], a.iy[b1], a.iz[b1], a.ix[b2],
+                        a.iy[b2], a.iz[b2], a.ix[b3], a.iy[b3], a.iz[b3], fa, fc);
+            const double3 da = sub3(u1, u2), dc = sub3(u3, u2);
+            a.g_angle[q] += da.x * (double)fa.x + da.y * (double)fa.y + da.z * (double)fa.z + dc.x * (double)fc.x +
+                            dc.y * (double)fc.y + dc.z * (double)fc.z;
+        }
+    }
+    // contacts, from the bead's own side of each listed pair
+    const Box &bf = a.force;
+    const P xi = a.px[i], yi = a.py[i], zi = a.pz[i];
+    const T ri = a.radius[i];
+    for (int e = a.nb_start[i], e1 = e + a.nb_cnt[i]; e < e1; ++e) {
+        const int ent = a.nb_list[e];
+        const int j = ent & kIdxMask, pid = ent >> kPidShift;
+        const PairC pr = a.pairs[pid];
+        P rx = xi - a.px[j], ry = yi - a.py[j], rz = zi - a.pz[j];
+        P len = rx * rx + ry * ry + rz * rz;
+        if (len > (P)bf.hbs) {                   // ResolveContacts' minimum image (kernel.cl:844-870)
+            if (rx * rx > (P)bf.hbs) rx += (rx > (P)0) ? (P)-bf.lx : (P)bf.lx;
+            if (ry * ry > (P)bf.hbs) ry += (ry > (P)0) ? (P)-bf.ly : (P)bf.ly;
+            if (rz * rz > (P)bf.hbs) rz += (rz > (P)0) ? (P)-bf.lz : (P)bf.lz;
+            len = rx * rx + ry * ry + rz * rz;
+        }
+        V3<T> t;
+        if (!contact_force(pr, bf, rx, ry, rz, len, ri, a.radius, j, t)) continue;   // the forward's mask
+        const DV3 F = contact_dual(pr, (double)ri, (double)a.radius[j],
+                                   dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fb, a.fb[j])));
+        add_d(z, F, 1.0);
+        a.g_pair[(size_t)i * a.n_pairs + pid] += dot_v(fb, F);
+    }
+    // walls of a finite box (kernel.cl:252-261): diagonal
+    const Box &bd = a.drift;
+    if (bd.bc == 1) {
+        const double k = -bd.wall_k;
+        if (xi < (P)0 || xi > (P)bd.lx) z.x += k * fb.x;
+        if (yi < (P)0 || yi > (P)bd.ly) z.y += k * fb.y;
+        if (zi < (P)0 || zi > (P)bd.lz) z.z += k * fb.z;
+    }
+    double3 X = a.xb[i];
+    X.x += z.x;
+    X.y += z.y;
+    X.z += z.z;
+    const double lam = a.lam[i], h = a.hlf[i], h2 = a.hlf2[i];
+    const double dt = (double)(T)bd.dt, D = (double)(T)bd.damping;
+    const double3 vb{D * fb.x, D * fb.y, D * fb.z};
+    double3 W = a.wb[i];
+    const double3 fprev{h * W.x + lam * vb.x + h2 * X.x, h * W.y + lam * vb.y + h2 * X.y,
+                        h * W.z + lam * vb.z + h2 * X.z};
+    W.x += vb.x + dt * X.x;
+    W.y += vb.y + dt * X.y;
+    W.z += vb.z + dt * X.z;
+    a.xb[i] = X;
+    a.wb[i] = W;
+    // Fb of the previous step: its own force adjoint, its kick's h Wb and its seed
+    const double seed = (a.group && a.group[i]) ? a.seed_prev : 0.0;
+    a.fb_next[i] = double3{fprev.x + h * W.x + seed, fprev.y + h * W.y, fprev.z + h * W.z};
+}
+
+// Fb of the last step: its seed and h Wb.
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_adjoint_seed(double3 *fb, const double3 *wb, const T *hlf,
+                                                         const int *group, double seed, int n)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const double h = (double)hlf[i];
+    const double3 W = wb[i];
+    fb[i] = double3{h * W.x + ((group && group[i]) ? seed : 0.0), h * W.y, h * W.z};
+}
+
+// Sum of the x-force over the flagged beads, by one block in a fixed order
+// (deterministic).
+template <typename T>
+__global__ void __launch_bounds__(kBlock) k_group_force(const T *fx, const int *group, int n, double *out)
+{
+    __shared__ double part[kBlock];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock)
+        if (group[i]) s += (double)fx[i];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = part[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_sum_columns(const double *g, int n, int np, double *out)
+{
+    __shared__ double part[kBlock];
+    const int t = blockIdx.x;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) s += g[(size_t)i * np + t];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[t] = part[0];
+}
+
+void sum_columns(const double *g, int n, int np, double *out, hipStream_t s)
+{
+    if (np > 0) k_sum_columns<<<np, kBlock, 0, s>>>(g, n, np, out);
+}
+
+template <typename T, typename P>
+void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s)
+{
+    if (a.n > 0) k_adjoint_step<T, P><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+}
+template <typename T>
+void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, double seed, int n, hipStream_t s)
+{
+    if (n > 0) k_adjoint_seed<T><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(fb, wb, hlf, group, seed, n);
+}
+template <typename T>
+void group_force(const T *fx, const int *group, int n, double *out, hipStream_t s)
+{
+    k_group_force<T><<<1, kBlock, 0, s>>>(fx, group, n, out);
+}
+
+template void adjoint_step<double, double>(const AdjArgs<double, double> &, hipStream_t);
+template void adjoint_step<float, double>(const AdjArgs<float, double> &, hipStream_t);
+template void adjoint_seed<double>(double3 *, const double3 *, const double *, const int *, double, int, hipStream_t);
+template void adjoint_seed<float>(double3 *, const double3 *, const float *, const int *, double, int, hipStream_t);
+template void group_force<double>(const double *, const int *, int, double *, hipStream_t);
+template void group_force<float>(const float *, const int *, int, double *, hipStream_t);
+
+}  // namespace fnk

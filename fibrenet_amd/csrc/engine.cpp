@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <list>
 #include <map>
 #include <memory>
@@ -217,6 +218,10 @@ struct fn_engine {
     virtual void copy_probe(long long bytes, double *ms) = 0;
     virtual void load_ensemble(const fn_network *nets, int n) = 0;
     virtual void ensemble_read(int net, int kind, void *out) = 0;
+    virtual void adjoint_forward(const fn_adjoint_spec &spec, const int *group, const int *steps, int n_samples,
+                                 double *samples) = 0;
+    virtual void adjoint_backward(const double *dloss, double *g_spring, double *g_angle, double *g_pair,
+                                  double *x0b, double *w0b) = 0;
 };
 
 namespace {
@@ -247,6 +252,7 @@ class Engine final : public fn_engine {
     // ---- buffers ------------------------------------------------------------------------
     int create_buffer(int kind, int esize, int n, const void *host) override
     {
+        adj_.valid = false;
         const int rs = record_size(kind);
         if (rs < 0) throw Error(FN_ERR_ARG, fmt("unknown buffer kind %d", kind));
         if (esize != rs) throw Error(FN_ERR_MISMATCH, fmt("record size %d, kind %d expects %d", esize, kind, rs));
@@ -271,6 +277,7 @@ class Engine final : public fn_engine {
 
     void write_buffer(int id, const void *host, int n_padded, int esize) override
     {
+        adj_.valid = false;
         const Buf *b = buffer(id);
         if (!b) throw Error(FN_ERR_ARG, fmt("no buffer %d", id));
         if (!host || n_padded <= 0) throw Error(FN_ERR_EMPTY, "source buffer is empty");
@@ -387,6 +394,7 @@ class Engine final : public fn_engine {
 
     void launch(const char *name) override
     {
+        adj_.valid = false;
         KernelRec &kr = kernel(name);
         const int kid = kernel_id(name);
         const int n = active(kid, kr);
@@ -458,11 +466,21 @@ class Engine final : public fn_engine {
     // ---- fast path --------------------------------------------------------------------------
     void run_steps(int n, int guf, bool spr, bool ang) override
     {
+        adj_.valid = false;
+        prep_fast(n, guf, spr, ang);
+        run_chunks(n, guf, spr, ang);
+    }
+
+    // Checks and device set-up of a fast-path run; spr / ang become the bond
+    // passes that run.
+    void prep_fast(int n, int guf, bool &spr, bool &ang)
+    {
         if (n <= 0) throw Error(FN_ERR_ARG, "loop size should be a positive number");
         if (guf <= 0) throw Error(FN_ERR_ARG, "grid_update_freq should be positive");
         if (!ens_boxes_.empty()) {
             prepare_ensemble_run(spr, ang);
-            run_chunks(n, guf, spr && S_ > 0, ang && A_ > 0);
+            spr = spr && S_ > 0;
+            ang = ang && A_ > 0;
             return;
         }
         for (int kid : {K_ADVANCE_POSITION, K_FILL_GRID, K_BUILD_LIST, K_RESOLVE_CONTACTS, K_ADVANCE_VELOCITY}) {
@@ -491,7 +509,6 @@ class Engine final : public fn_engine {
             fnk::fill_int(need_.p, 1, 1, stream_);
             cand_dirty_ = false;
         }
-        run_chunks(n, guf, spr, ang);
     }
 
     // The chunks of a fast-path run, then the checks after it.
@@ -930,13 +947,16 @@ class Engine final : public fn_engine {
         }
         std::vector<int> st(n + 1, 0), slot(2 * (size_t)S_);
         for (int i = 0; i < n; ++i) st[i + 1] = st[i] + cnt[i];
-        std::vector<int> cur(st.begin(), st.end() - 1);
+        std::vector<int> cur(st.begin(), st.end() - 1), ent(2 * (size_t)S_);
         for (int s = 0; s < S_; ++s) {
             slot[2 * (size_t)s] = cur[h_springs_[s].p1]++;
             slot[2 * (size_t)s + 1] = cur[h_springs_[s].p2]++;
+            ent[slot[2 * (size_t)s]] = s << 1;
+            ent[slot[2 * (size_t)s + 1]] = (s << 1) | 1;
         }
         sp_start_.put(st, stream_);
         sp_slot_.put(slot, stream_);
+        sp_ent_.put(ent, stream_);
         sp_term_.ensure(2 * (size_t)S_);
         std::fill(cnt.begin(), cnt.end(), 0);
         for (const fn_angle &a : h_angles_) {
@@ -948,14 +968,16 @@ class Engine final : public fn_engine {
         }
         std::vector<int> as(n + 1, 0), aslot(3 * (size_t)A_);
         for (int i = 0; i < n; ++i) as[i + 1] = as[i] + cnt[i];
-        std::vector<int> acur(as.begin(), as.end() - 1);
+        std::vector<int> acur(as.begin(), as.end() - 1), aent(3 * (size_t)A_);
         for (int q = 0; q < A_; ++q) {
             aslot[3 * (size_t)q] = acur[h_angles_[q].b1]++;
             aslot[3 * (size_t)q + 1] = acur[h_angles_[q].b3]++;
             aslot[3 * (size_t)q + 2] = acur[h_angles_[q].b2]++;
+            for (int r = 0; r < 3; ++r) aent[aslot[3 * (size_t)q + r]] = (q << 2) | r;
         }
         an_start_.put(as, stream_);
         an_slot_.put(aslot, stream_);
+        an_ent_.put(aent, stream_);
         an_term_.ensure(3 * (size_t)A_);
         build_chain();
         HIPCK(hipStreamSynchronize(stream_));
@@ -1421,7 +1443,12 @@ class Engine final : public fn_engine {
         return guf * std::max(1, target / guf);
     }
 
-    void enqueue_chunk(int len, int guf, bool spr, bool ang, bool first, bool last, Events *ev)
+    // Called after each step's bead kernel with the step's index in the chunk,
+    // the position buffer the step read and the force buffer it wrote.
+    using StepHook = std::function<void(int, int, int)>;
+
+    void enqueue_chunk(int len, int guf, bool spr, bool ang, bool first, bool last, Events *ev,
+                       const StepHook *hook = nullptr)
     {
         const fnk::Box bd = kbox(K_ADVANCE_POSITION);
         const fnk::Box bf = kbox(K_FILL_GRID);
@@ -1461,6 +1488,7 @@ class Engine final : public fn_engine {
             mark(sv);
             fnk::Launch<T, P>::bead_step(a, spr, ang, !is_last, is_last, chain, stream_);
             mark(sv);
+            if (hook) (*hook)(k, pc, 1 - fc);
             fc ^= 1;
             if (!is_last) pc ^= 1;
         }
@@ -1486,6 +1514,7 @@ class Engine final : public fn_engine {
     // its own network's cells.
     void load_ensemble(const fn_network *nets, int n) override
     {
+        adj_.valid = false;
         if (n <= 0 || !nets) throw Error(FN_ERR_ARG, "an ensemble needs at least one network");
         if (!bufs_.empty()) throw Error(FN_ERR_ARG, "an ensemble is loaded into an engine without buffers");
         std::vector<fn_bead> beads;
@@ -1654,6 +1683,268 @@ class Engine final : public fn_engine {
         }
     }
 
+    // ---- adjoint ----------------------------------------------------------------------------
+    struct Part {
+        void *p;
+        size_t bytes;
+    };
+    static size_t padded(size_t b) { return (b + 255) / 256 * 256; }
+
+    // The device state a run continues from: the positions and images the
+    // next step reads (buffer pc), velocity_old, the previous force (buffer
+    // fc), bond statuses and break steps, the step counter; with `full` also
+    // the read-back velocity and force_old and what the exports and stats
+    // read (cells, contact list, counters).
+    std::vector<Part> state_parts(int pc, int fc, bool full)
+    {
+        std::vector<Part> v;
+        for (int q = 0; q < 3; ++q) v.push_back({pos_[pc][q].p, sizeof(P) * N_});
+        if (!alias_)
+            for (int q = 0; q < 3; ++q) v.push_back({img_[pc][q].p, sizeof(P) * N_});
+        for (int q = 0; q < 3; ++q) v.push_back({vold_[q].p, sizeof(T) * N_});
+        for (int q = 0; q < 3; ++q) v.push_back({frc_[fc][q].p, sizeof(T) * N_});
+        if (S_) {
+            v.push_back({sp_status_.p, sizeof(int) * S_});
+            v.push_back({sp_break_.p, sizeof(int) * S_});
+        }
+        if (A_) v.push_back({an_status_.p, sizeof(int) * A_});
+        v.push_back({step_.p, sizeof(int)});
+        if (full) {
+            for (int q = 0; q < 3; ++q) v.push_back({vel_[q].p, sizeof(T) * N_});
+            for (int q = 0; q < 3; ++q) v.push_back({frc_[1 - fc][q].p, sizeof(T) * N_});   // read back as force_old
+            v.push_back({key_.p, sizeof(int) * N_});
+            v.push_back({nb_start_.p, sizeof(int) * ((size_t)N_ + 1)});
+            v.push_back({nb_cnt_.p, sizeof(int) * N_});
+            v.push_back({nb_list_.p, sizeof(int) * (size_t)cap_});
+            v.push_back({stats_.p, sizeof(int) * fnk::kStatCount});
+        }
+        return v;
+    }
+    static size_t parts_bytes(const std::vector<Part> &v)
+    {
+        size_t b = 0;
+        for (const Part &x : v) b += padded(x.bytes);
+        return b;
+    }
+    void save_parts(const std::vector<Part> &v, unsigned char *dst)
+    {
+        for (const Part &x : v) {
+            HIPCK(hipMemcpyAsync(dst, x.p, x.bytes, hipMemcpyDeviceToDevice, stream_));
+            dst += padded(x.bytes);
+        }
+    }
+    void load_parts(const std::vector<Part> &v, const unsigned char *src)
+    {
+        for (const Part &x : v) {
+            HIPCK(hipMemcpyAsync(x.p, src, x.bytes, hipMemcpyDeviceToDevice, stream_));
+            src += padded(x.bytes);
+        }
+    }
+
+    // The adjoint uses the one-pass exact rebuild, whose lists depend on the
+    // positions alone, so a recomputed segment lists what the forward listed.
+    struct ExactRebuild {
+        Engine &e;
+        int saved;
+        explicit ExactRebuild(Engine &en) : e(en), saved(en.verlet_) { e.verlet_ = 0; }
+        ~ExactRebuild()
+        {
+            e.verlet_ = saved;
+            e.cand_dirty_ = true;
+        }
+    };
+
+    void adjoint_forward(const fn_adjoint_spec &sp, const int *group, const int *steps, int ns,
+                         double *samples) override
+    {
+        adj_.valid = false;
+        if (!ens_boxes_.empty()) throw Error(FN_ERR_UNSUPPORTED, "the adjoint runs on single-network engines");
+        const int n = sp.n_steps, guf = sp.grid_update_freq;
+        if (n <= 0) throw Error(FN_ERR_ARG, "loop size should be a positive number");
+        if (guf <= 0) throw Error(FN_ERR_ARG, "grid_update_freq should be positive");
+        int every = sp.checkpoint_every;
+        if (every < 0 || (every > 0 && every % guf != 0))
+            throw Error(FN_ERR_ARG, "checkpoint_every must be 0 or a positive multiple of grid_update_freq");
+        if (every == 0) every = guf * std::max(1, (int)std::lround(std::sqrt((double)n) / guf));
+        if (ns < 0 || (ns > 0 && (!steps || !samples || !group)))
+            throw Error(FN_ERR_ARG, "samples need a bead group, their steps and an output array");
+        for (int k = 0; k < ns; ++k)
+            if (steps[k] < 0 || steps[k] >= n || (k > 0 && steps[k] <= steps[k - 1]))
+                throw Error(FN_ERR_ARG, "sample steps must increase strictly within [0, n_steps)");
+        bool spr = sp.run_springs != 0, ang = sp.run_angles != 0;
+        ExactRebuild exact(*this);
+        prep_fast(n, guf, spr, ang);
+        std::vector<int> g(N_, 0);
+        if (group)
+            for (int i = 0; i < N_; ++i) g[i] = group[i] != 0;
+        group_.put(g, stream_);
+        const int nseg = (n + every - 1) / every;
+        const size_t cb = parts_bytes(state_parts(pc_, fc_, false));
+        ck_.ensure(cb * nseg);
+        samp_.ensure(std::max(ns, 1));
+        adj_.ck_pc.assign(nseg, 0);
+        adj_.ck_fc.assign(nseg, 0);
+        if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
+        fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
+        int si = 0;
+        for (int seg = 0; seg < nseg; ++seg) {
+            const int c = seg * every, len = std::min(every, n - c);
+            adj_.ck_pc[seg] = pc_;
+            adj_.ck_fc[seg] = fc_;
+            save_parts(state_parts(pc_, fc_, false), ck_.p + cb * seg);
+            const StepHook hook = [&](int k, int, int fw) {
+                if (si < ns && steps[si] == c + k) {
+                    fnk::group_force(frc_[fw][0].p, group_.p, N_, samp_.p + si, stream_);
+                    ++si;
+                }
+            };
+            const bool first = seg == 0, last = seg == nseg - 1;
+            enqueue_chunk(len, guf, spr, ang, first, last, nullptr, &hook);
+            HIPCK(hipGetLastError());
+            flip(len, first, last);
+        }
+        steps_ += n;
+        rebuilds_ += (n + guf - 1) / guf;
+        HIPCK(hipStreamSynchronize(stream_));
+        check_capacity();
+        const std::vector<Part> fin = state_parts(pc_, fc_, true);
+        fin_.ensure(parts_bytes(fin));
+        save_parts(fin, fin_.p);
+        if (ns) HIPCK(hipMemcpyAsync(samples, samp_.p, sizeof(double) * ns, hipMemcpyDeviceToHost, stream_));
+        HIPCK(hipStreamSynchronize(stream_));
+        adj_.n = n;
+        adj_.guf = guf;
+        adj_.every = every;
+        adj_.nseg = nseg;
+        adj_.spr = spr;
+        adj_.ang = ang;
+        adj_.n_spr = n_spr_;
+        adj_.n_ang = n_ang_;
+        adj_.cb = cb;
+        adj_.cap = cap_;
+        adj_.fin_pc = pc_;
+        adj_.fin_fc = fc_;
+        adj_.steps.assign(steps, steps + ns);
+        adj_.valid = true;
+    }
+
+    void adjoint_backward(const double *dloss, double *gs, double *ga, double *gp, double *x0b,
+                          double *w0b) override
+    {
+        if (!adj_.valid)
+            throw Error(FN_ERR_ARG, "no fn_adjoint_forward run to differentiate (or the engine changed since)");
+        const int n = adj_.n, guf = adj_.guf, every = adj_.every, nseg = adj_.nseg;
+        const bool spr = adj_.spr, ang = adj_.ang;
+        const int ns = (int)adj_.steps.size();
+        if (ns && !dloss) throw Error(FN_ERR_ARG, "dloss is required");
+        if (cap_ != adj_.cap) throw Error(FN_ERR_ARG, "the contact list was resized after the forward run");
+        std::vector<double> seed(n, 0.0);
+        for (int k = 0; k < ns; ++k) seed[adj_.steps[k]] = dloss[k];
+        ExactRebuild exact(*this);
+        n_spr_ = adj_.n_spr;
+        n_ang_ = adj_.n_ang;
+        geometry(settings_of(K_FILL_GRID), settings_of(K_BUILD_LIST));
+        const size_t N = N_;
+        const int np = (int)h_pairs_.size();
+        for (auto *d : {&fb_[0], &fb_[1], &xb_, &wb_}) d->ensure(N);
+        gsp_.ensure(std::max(S_, 1));
+        gan_.ensure(std::max(A_, 1));
+        gpr_.ensure(std::max<size_t>(N * np, 1));
+        gps_.ensure(std::max(np, 1));
+        HIPCK(hipMemsetAsync(xb_.p, 0, sizeof(double3) * N, stream_));
+        HIPCK(hipMemsetAsync(wb_.p, 0, sizeof(double3) * N, stream_));
+        HIPCK(hipMemsetAsync(gsp_.p, 0, sizeof(double) * gsp_.n, stream_));
+        HIPCK(hipMemsetAsync(gan_.p, 0, sizeof(double) * gan_.n, stream_));
+        HIPCK(hipMemsetAsync(gpr_.p, 0, sizeof(double) * gpr_.n, stream_));
+        // per-step records of one segment: positions (and images), bond
+        // statuses; per rebuild: the contact list
+        const int nimg = alias_ ? 3 : 6;
+        const size_t SA = (size_t)S_ + A_;
+        const int nr = (every + guf - 1) / guf;
+        const size_t lrow = 2 * N + 1 + (size_t)cap_;
+        rec_pos_.ensure((size_t)every * nimg * N);
+        rec_st_.ensure(std::max<size_t>((size_t)every * SA, 1));
+        rec_nb_.ensure((size_t)nr * lrow);
+        auto d2d = [&](void *dst, const void *src, size_t bytes) {
+            if (bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_));
+        };
+        fnk::adjoint_seed<T>(fb_[0].p, wb_.p, hlf_.p, group_.p, seed[n - 1], N_, stream_);
+        int cur = 0;
+        for (int seg = nseg - 1; seg >= 0; --seg) {
+            const int c = seg * every, len = std::min(every, n - c);
+            pc_ = adj_.ck_pc[seg];
+            fc_ = adj_.ck_fc[seg];
+            load_parts(state_parts(pc_, fc_, false), ck_.p + adj_.cb * seg);
+            const StepHook hook = [&](int k, int pr, int) {
+                P *dst = rec_pos_.p + (size_t)k * nimg * N;
+                for (int q = 0; q < 3; ++q) d2d(dst + q * N, pos_[pr][q].p, sizeof(P) * N);
+                if (!alias_)
+                    for (int q = 0; q < 3; ++q) d2d(dst + (3 + q) * N, img_[pr][q].p, sizeof(P) * N);
+                int *ds = rec_st_.p + (size_t)k * SA;
+                d2d(ds, sp_status_.p, sizeof(int) * S_);
+                d2d(ds + S_, an_status_.p, sizeof(int) * A_);
+                if (k % guf == 0) {
+                    int *dl = rec_nb_.p + (size_t)(k / guf) * lrow;
+                    d2d(dl, nb_start_.p, sizeof(int) * (N + 1));
+                    d2d(dl + N + 1, nb_cnt_.p, sizeof(int) * N);
+                    d2d(dl + 2 * N + 1, nb_list_.p, sizeof(int) * (size_t)cap_);
+                }
+            };
+            enqueue_chunk(len, guf, spr, ang, seg == 0, seg == nseg - 1, nullptr, &hook);
+            for (int k = len - 1; k >= 0; --k) {
+                const P *rp = rec_pos_.p + (size_t)k * nimg * N;
+                const P *ri = alias_ ? rp : rp + 3 * N;
+                const int *rs = rec_st_.p + (size_t)k * SA;
+                const int *rl = rec_nb_.p + (size_t)(k / guf) * lrow;
+                fnk::AdjArgs<T, P> a{};
+                a.n = N_;
+                a.n_springs = spr ? n_spr_ : 0;
+                a.n_angles = ang ? n_ang_ : 0;
+                a.drift = kbox(K_ADVANCE_POSITION);
+                a.force = kbox(K_RESOLVE_CONTACTS);
+                a.px = rp; a.py = rp + N; a.pz = rp + 2 * N;
+                a.ix = ri; a.iy = ri + N; a.iz = ri + 2 * N;
+                a.lam = lam_.p; a.hlf = hlf_.p; a.hlf2 = hlf2_.p; a.radius = rad_.p;
+                a.sp_start = sp_start_.p; a.sp_ent = sp_ent_.p; a.sp_p1 = sp_p1_.p; a.sp_p2 = sp_p2_.p;
+                a.sp_type = sp_type_.p; a.sp_on = rs; a.sp_l0 = sp_l0_.p; a.sp_k = sp_k_.p;
+                a.an_start = an_start_.p; a.an_ent = an_ent_.p; a.an_b1 = an_b1_.p; a.an_b2 = an_b2_.p;
+                a.an_b3 = an_b3_.p; a.an_type = an_type_.p; a.an_on = rs + S_; a.an_th0 = an_th0_.p; a.an_k = an_k_.p;
+                a.nb_start = rl; a.nb_cnt = rl + N + 1; a.nb_list = rl + 2 * N + 1;
+                a.pairs = pairs_.p;
+                a.n_pairs = np;
+                a.xb = xb_.p; a.wb = wb_.p;
+                a.fb = fb_[cur].p; a.fb_next = fb_[1 - cur].p;
+                a.group = group_.p;
+                a.seed_prev = c + k > 0 ? seed[c + k - 1] : 0.0;
+                a.g_spring = gsp_.p; a.g_angle = gan_.p; a.g_pair = gpr_.p;
+                fnk::adjoint_step(a, stream_);
+                cur ^= 1;
+            }
+            HIPCK(hipGetLastError());
+        }
+        // back to the state the forward run ended in
+        pc_ = adj_.fin_pc;
+        fc_ = adj_.fin_fc;
+        load_parts(state_parts(pc_, fc_, true), fin_.p);
+        fnk::sum_columns(gpr_.p, N_, np, gps_.p, stream_);
+        auto d2h = [&](void *dst, const void *src, size_t bytes) {
+            if (dst && bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
+        };
+        d2h(gs, gsp_.p, sizeof(double) * S_);
+        d2h(ga, gan_.p, sizeof(double) * A_);
+        d2h(gp, gps_.p, sizeof(double) * np);
+        d2h(x0b, xb_.p, sizeof(double3) * N);
+        d2h(w0b, wb_.p, sizeof(double3) * N);
+        HIPCK(hipStreamSynchronize(stream_));
+    }
+
+    struct AdjRun {
+        bool valid = false, spr = false, ang = false;
+        int n = 0, guf = 1, every = 1, nseg = 0, n_spr = 0, n_ang = 0, cap = 0, fin_pc = 0, fin_fc = 0;
+        size_t cb = 0;
+        std::vector<int> ck_pc, ck_fc, steps;
+    };
+
     void run_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
     {
         const auto key = std::make_tuple(len, guf, (int)spr, (int)ang, (int)first, (int)last, pc_, fc_, n_spr_, n_ang_);
@@ -1682,6 +1973,13 @@ class Engine final : public fn_engine {
             if (timing_) used_.push_back(&loose_.back());
         }
         HIPCK(hipGetLastError());
+        flip(len, first, last);
+    }
+
+    // Buffer parity after a chunk: the first drift and every step but the
+    // run's last flip the positions, every step flips the forces.
+    void flip(int len, bool first, bool last)
+    {
         const int flips_p = (first ? 1 : 0) + len - (last ? 1 : 0);
         pc_ ^= (flips_p & 1);
         fc_ ^= (len & 1);
@@ -1765,6 +2063,15 @@ class Engine final : public fn_engine {
     DArr<T> an_th0_, an_k_;
     DArr<fnk::V3<T>> sp_term_, an_term_;   // bond terms of the current step at the incidence slots
     DArr<int> ch_spring_, ch_angle_;       // chain layout: spring (i, i+1), angle centred on i
+    DArr<int> sp_ent_, an_ent_;            // incidence entries (s<<1 | p2; q<<2 | 0 b1, 1 b3, 2 b2) by slot
+    AdjRun adj_;                           // the last adjoint forward run
+    DArr<unsigned char> ck_, fin_;         // its checkpoints (one per segment) and end state
+    DArr<int> group_;                      // sampled beads
+    DArr<double> samp_;                    // samples of the run
+    DArr<double3> fb_[2], xb_, wb_;        // adjoint state
+    DArr<double> gsp_, gan_, gpr_, gps_;   // scale adjoints (contacts per bead and pair entry, then summed)
+    DArr<P> rec_pos_;                      // per step of a segment: positions and images of the force evaluation
+    DArr<int> rec_st_, rec_nb_;            // per step: bond statuses; per rebuild: the contact list
     DArr<fnk::PairC> pairs_;
     DArr<int> key_, ckey_, rank_, cell_count_, cell_start_, sorted_, ncnt_, nb_start_, nb_cnt_, cand_, nb_list_, scan_tmp_,
         stats_, step_, need_;
@@ -1984,6 +2291,21 @@ int fn_batch_read(fn_engine *eng, int network, int kind, void *out)
         if (!out) throw Error(FN_ERR_ARG, "null destination");
         eng->ensemble_read(network, kind, out);
     });
+}
+
+int fn_adjoint_forward(fn_engine *eng, const fn_adjoint_spec *spec, const int *group, const int *sample_steps,
+                       int n_samples, double *samples)
+{
+    return guarded(eng, [&] {
+        if (!spec) throw Error(FN_ERR_ARG, "spec is required");
+        eng->adjoint_forward(*spec, group, sample_steps, n_samples, samples);
+    });
+}
+
+int fn_adjoint_backward(fn_engine *eng, const double *dloss, double *d_spring, double *d_angle, double *d_pair,
+                        double *d_pos0, double *d_vel0)
+{
+    return guarded(eng, [&] { eng->adjoint_backward(dloss, d_spring, d_angle, d_pair, d_pos0, d_vel0); });
 }
 
 int fn_get_timing(fn_engine *eng, fn_timing *out)

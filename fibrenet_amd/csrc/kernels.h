@@ -353,6 +353,46 @@ __device__ __forceinline__ void add_terms(const V3<T> *term, int e0, int e1, V3<
     }
 }
 
+// Force on bead 1 of a listed pair from r = x1 - x_j at its minimum image
+// (len = |r|^2), bead 1's radius r1 and the radii table; false when the pair
+// exerts no force this step.
+template <typename T, typename P>
+__device__ __forceinline__ bool contact_force(const PairC &pr, const Box &b, P rx, P ry, P rz, P len, T r1,
+                                              const T *radius, int j, V3<T> &t)
+{
+    T coef;
+    T r;
+    if (pr.type == 1) {                           // Lennard-Jones, kernel.cl:875-919
+        if (!((double)len < b.cutoff_sq)) return false;
+        r = (T)native_sqrt((double)len);
+        if (!((double)r < pr.c3)) return false;
+        const T r2 = (T)len;
+        const T c1 = (T)pr.c1;
+        const T sigma2 = c1 * c1;
+        const T fr2 = sigma2 / r2;
+        const T fr6 = fr2 * fr2 * fr2;
+        coef = (T)48 * (T)pr.c2 * fr6 * (fr6 - (T)0.5) / r2;
+        coef -= (T)pr.c4;
+    } else if (pr.type == 2) {                    // penalty, kernel.cl:922-957
+        r = (T)native_sqrt((double)len);
+        coef = (r1 + radius[j]) - r;
+        if (!(coef > (T)0)) return false;
+        coef *= (T)pr.c1;
+    } else if (pr.type == 3) {                    // two-step harmonic, kernel.cl:960-999
+        if (!((double)len < pr.c4)) return false;
+        r = (T)native_sqrt((double)len);
+        coef = ((T)pr.c2 - r) * (T)pr.c1;
+        coef += ((T)pr.c6 - r) * (T)pr.c5;
+    } else {
+        return false;
+    }
+    const T inv = (r != (T)0) ? (T)1 / r : (T)0;
+    t.x = coef * ((T)rx * inv);
+    t.y = coef * ((T)ry * inv);
+    t.z = coef * ((T)rz * inv);
+    return true;
+}
+
 // Force on bead 1 (position x1.., radius r1) from the contact-list entry
 // `ent` (kernel.cl:833-1000 seen from either bead of a pair: the reaction on
 // the partner is the exact negation of the owner's term, so each side
@@ -372,37 +412,7 @@ __device__ __forceinline__ bool contact_term(const Args<T, P> &a, const Box &b, 
         if (rz * rz > (P)b.hbs) rz += (rz > (P)0) ? (P)-b.lz : (P)b.lz;
         len = rx * rx + ry * ry + rz * rz;
     }
-    T coef;
-    T r;
-    if (pr.type == 1) {                           // Lennard-Jones, kernel.cl:875-919
-        if (!((double)len < b.cutoff_sq)) return false;
-        r = (T)native_sqrt((double)len);
-        if (!((double)r < pr.c3)) return false;
-        const T r2 = (T)len;
-        const T c1 = (T)pr.c1;
-        const T sigma2 = c1 * c1;
-        const T fr2 = sigma2 / r2;
-        const T fr6 = fr2 * fr2 * fr2;
-        coef = (T)48 * (T)pr.c2 * fr6 * (fr6 - (T)0.5) / r2;
-        coef -= (T)pr.c4;
-    } else if (pr.type == 2) {                    // penalty, kernel.cl:922-957
-        r = (T)native_sqrt((double)len);
-        coef = (r1 + a.radius[j]) - r;
-        if (!(coef > (T)0)) return false;
-        coef *= (T)pr.c1;
-    } else if (pr.type == 3) {                    // two-step harmonic, kernel.cl:960-999
-        if (!((double)len < pr.c4)) return false;
-        r = (T)native_sqrt((double)len);
-        coef = ((T)pr.c2 - r) * (T)pr.c1;
-        coef += ((T)pr.c6 - r) * (T)pr.c5;
-    } else {
-        return false;
-    }
-    const T inv = (r != (T)0) ? (T)1 / r : (T)0;
-    t.x = coef * ((T)rx * inv);
-    t.y = coef * ((T)ry * inv);
-    t.z = coef * ((T)rz * inv);
-    return true;
+    return contact_force(pr, b, rx, ry, rz, len, r1, a.radius, j, t);
 }
 
 // Adds the contact forces of bead i's two-sided list in list order.

@@ -198,6 +198,42 @@ int fn_batch_create(int device, int precision, int n_networks, const fn_network 
  * FN_BUF_ANGLE (bond indices local to the network). */
 int fn_batch_read(fn_engine *eng, int network, int kind, void *out);
 
+/* ---- adjoint (no reference counterpart) ----
+ * Reverse-mode derivative of a fast-path run with respect to the force
+ * scales of every spring, angle and contact pair type and to the initial
+ * positions and velocity_old.  fn_adjoint_forward runs the steps exactly as
+ * fn_run_steps does (same results; the rebuild is the exact one-pass one)
+ * and records, per sample step, the x-force summed over the flagged beads
+ * (a clamp's reaction force) together with a checkpoint of the state every
+ * checkpoint_every steps.  fn_adjoint_backward takes dL/d(sample) and
+ * recomputes each segment from its checkpoint (bitwise the same steps, so
+ * the contact lists and bond statuses of the forward run are reproduced
+ * and held fixed), then runs the adjoint steps through it.  Derivatives
+ * are fp64 in both precisions.  A scale sigma multiplies the force of one
+ * bond (or of every contact of one pair type) and is 1 in the run: for a
+ * spring, dL/dk = dL/dsigma / k.  Single-network engines only; any call that
+ * changes the engine's state in between (runs, writes, launches) makes
+ * fn_adjoint_backward fail until the next fn_adjoint_forward. */
+typedef struct fn_adjoint_spec {
+    int n_steps;            /* as fn_run_steps                                 */
+    int grid_update_freq;
+    int run_springs;
+    int run_angles;
+    int checkpoint_every;   /* a multiple of grid_update_freq; 0: automatic     */
+} fn_adjoint_spec;
+/* group[n_beads]: nonzero for the sampled beads; sample_steps[n_samples]:
+ * strictly increasing step indices in [0, n_steps); samples[n_samples]
+ * receives sum over the group of the force x of those steps. */
+int fn_adjoint_forward(fn_engine *eng, const fn_adjoint_spec *spec, const int *group, const int *sample_steps,
+                       int n_samples, double *samples);
+/* dloss[n_samples] = dL/d(sample).  Outputs (any may be NULL):
+ * d_spring[n_springs], d_angle[n_angles], d_pair[n_pairs] scale derivatives;
+ * d_pos0[3 * n_beads], d_vel0[3 * n_beads] (x, y, z per bead) derivatives
+ * with respect to the positions and velocity_old at the start of the run.
+ * Leaves the engine in the state fn_adjoint_forward ended in. */
+int fn_adjoint_backward(fn_engine *eng, const double *dloss, double *d_spring, double *d_angle, double *d_pair,
+                        double *d_pos0, double *d_vel0);
+
 /* Debug exports for parity checks (no reference counterpart). */
 /* Cell index of every bead at the last rebuild (-1: outside the grid). */
 int fn_export_cells(fn_engine *eng, int *cell_of_bead);
