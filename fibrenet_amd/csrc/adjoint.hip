@@ -26,6 +26,8 @@
 // the overlap r1 + r2 - r are differences of nearly equal numbers), angle
 // values come from the forward's angle_force.  Derivatives are those of
 // the exact expressions at the recorded positions.
+#include <algorithm>
+
 #include "kernels.h"
 #include "engine.h"
 
@@ -162,13 +164,75 @@ __device__ __forceinline__ DV3 dvec(double3 x, double3 u)
 }
 __device__ __forceinline__ double3 sub3(double3 a, double3 b) { return double3{a.x - b.x, a.y - b.y, a.z - b.z}; }
 
-// One adjoint step for every bead (thread per bead).  Reads the positions
-// and images at which the step's forces were evaluated, the bond statuses
-// after the step, the step's contact list and Fb (adjoint of this step's
+// Spring terms of one adjoint step (thread per spring): J Fb of the spring
+// force at both ends' incidence slots (zero for a spring that did not act)
+// and the spring's scale adjoint Fb . F.  The force on p1 is F(x_p1 - x_p2),
+// on p2 its negation.
+template <typename T, typename P>
+__global__ void __launch_bounds__(kBlock) k_adj_springs(AdjArgs<T, P> a)
+{
+    const int s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= a.springs) return;
+    double3 j{0.0, 0.0, 0.0};
+    if (s < a.n_springs && a.sp_on[s] == 1 && (a.sp_type[s] == 0 || a.sp_type[s] == 1)) {
+        const int p1 = a.sp_p1[s], p2 = a.sp_p2[s];
+        const double3 u = sub3(a.fb[p1], a.fb[p2]);
+        const double3 d{(double)(a.ix[p1] - a.ix[p2]), (double)(a.iy[p1] - a.iy[p2]), (double)(a.iz[p1] - a.iz[p2])};
+        const DV3 F = spring_dual((double)a.sp_k[s], a.sp_l0[s], dvec(d, u));
+        j = double3{F.x.d, F.y.d, F.z.d};
+        a.g_spring[s] += dot_v(u, F);
+    }
+    a.sp_jv[a.sp_slot[2 * s]] = j;
+    a.sp_jv[a.sp_slot[2 * s + 1]] = double3{-j.x, -j.y, -j.z};
+}
+
+// Angle terms of one adjoint step (thread per angle): J Fb of pa at the b1
+// slot, of pc at the b3 slot, of -pa-pc at the b2 slot, and the angle's
+// scale adjoint from the forward's own force values.
+template <typename T, typename P>
+__global__ void __launch_bounds__(kBlock) k_adj_angles(AdjArgs<T, P> a)
+{
+    const int q = blockIdx.x * kBlock + threadIdx.x;
+    if (q >= a.angles) return;
+    double3 ja{0.0, 0.0, 0.0}, jc{0.0, 0.0, 0.0};
+    if (q < a.n_angles && a.an_on[q] == 1) {
+        const int b1 = a.an_b1[q], b2 = a.an_b2[q], b3 = a.an_b3[q];
+        const double3 x2{(double)a.ix[b2], (double)a.iy[b2], (double)a.iz[b2]};
+        const double3 ba{(double)a.ix[b1] - x2.x, (double)a.iy[b1] - x2.y, (double)a.iz[b1] - x2.z};
+        const double3 bc{(double)a.ix[b3] - x2.x, (double)a.iy[b3] - x2.y, (double)a.iz[b3] - x2.z};
+        const double3 u1 = a.fb[b1], u2 = a.fb[b2], u3 = a.fb[b3];
+        const double3 da = sub3(u1, u2), dc = sub3(u3, u2);
+        DV3 pa, pc;
+        angle_dual((double)a.an_k[q], (double)a.an_th0[q], dvec(ba, da), dvec(bc, dc), pa, pc);
+        ja = double3{pa.x.d, pa.y.d, pa.z.d};
+        jc = double3{pc.x.d, pc.y.d, pc.z.d};
+        V3<T> fa, fc;                             // the forward's own values
+        angle_force(AngleP<T>{1, a.an_type[q], a.an_th0[q], a.an_k[q]}, a.ix[b1], a.iy[b1], a.iz[b1], a.ix[b2],
+                    a.iy[b2], a.iz[b2], a.ix[b3], a.iy[b3], a.iz[b3], fa, fc);
+        a.g_angle[q] += da.x * (double)fa.x + da.y * (double)fa.y + da.z * (double)fa.z + dc.x * (double)fc.x +
+                        dc.y * (double)fc.y + dc.z * (double)fc.z;
+    }
+    a.an_jv[a.an_slot[3 * q]] = ja;
+    a.an_jv[a.an_slot[3 * q + 1]] = jc;
+    a.an_jv[a.an_slot[3 * q + 2]] = double3{-ja.x - jc.x, -ja.y - jc.y, -ja.z - jc.z};
+}
+
+__device__ __forceinline__ void add_slots(double3 &z, const double3 *jv, int e0, int e1)
+{
+    for (int e = e0; e < e1; ++e) {
+        const double3 t = jv[e];
+        z.x += t.x;
+        z.y += t.y;
+        z.z += t.z;
+    }
+}
+
+// The bead part of one adjoint step (thread per bead).  Reads the positions
+// at which the step's forces were evaluated, the bond terms at the bead's
+// incidence slots, the step's contact list and Fb (adjoint of this step's
 // force, seed and h Wb included); updates X, Wb, writes Fb of the previous
 // step (seed and h Wb of that step included) to fb_next, and adds the
-// scale adjoints of the bonds (by one owner bead: p1, b2) and of the
-// contacts (per bead and pair entry).
+// contacts' scale adjoints (per bead and pair entry).
 template <typename T, typename P>
 __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
 {
@@ -176,42 +240,8 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
     if (i >= a.n) return;
     const double3 fb = a.fb[i];
     double3 z{0.0, 0.0, 0.0};
-    auto img = [&](int j) { return double3{(double)a.ix[j], (double)a.iy[j], (double)a.iz[j]}; };
-    // springs: force on p1 is F(x_p1 - x_p2), on p2 its negation
-    for (int e = a.sp_start[i], e1 = a.sp_start[i + 1]; e < e1; ++e) {
-        const int ent = a.sp_ent[e];
-        const int s = ent >> 1, role = ent & 1;
-        if (s >= a.n_springs || a.sp_on[s] != 1 || a.sp_type[s] < 0 || a.sp_type[s] > 1) continue;
-        const int p1 = a.sp_p1[s], p2 = a.sp_p2[s];
-        const double3 u = sub3(a.fb[p1], a.fb[p2]);
-        const DV3 F = spring_dual((double)a.sp_k[s], a.sp_l0[s], dvec(sub3(img(p1), img(p2)), u));
-        add_d(z, F, role == 0 ? 1.0 : -1.0);
-        if (role == 0) a.g_spring[s] += dot_v(u, F);
-    }
-    // angles: pa on b1, pc on b3, -pa-pc on b2
-    for (int e = a.an_start[i], e1 = a.an_start[i + 1]; e < e1; ++e) {
-        const int ent = a.an_ent[e];
-        const int q = ent >> 2, role = ent & 3;     // role 0: b1, 1: b3, 2: b2
-        if (q >= a.n_angles || a.an_on[q] != 1) continue;
-        const int b1 = a.an_b1[q], b2 = a.an_b2[q], b3 = a.an_b3[q];
-        const double3 x1 = img(b1), x2 = img(b2), x3 = img(b3);
-        const double3 u1 = a.fb[b1], u2 = a.fb[b2], u3 = a.fb[b3];
-        DV3 pa, pc;
-        angle_dual((double)a.an_k[q], (double)a.an_th0[q], dvec(sub3(x1, x2), sub3(u1, u2)),
-                   dvec(sub3(x3, x2), sub3(u3, u2)), pa, pc);
-        if (role == 0) add_d(z, pa, 1.0);
-        else if (role == 1) add_d(z, pc, 1.0);
-        else {
-            add_d(z, pa, -1.0);
-            add_d(z, pc, -1.0);
-            V3<T> fa, fc;                        // the forward's own values
-            angle_force(AngleP<T>{1, a.an_type[q], a.an_th0[q], a.an_k[q]}, a.ix[b1], a.iy[b1], a.iz[b1], a.ix[b2],
-                        a.iy[b2], a.iz[b2], a.ix[b3], a.iy[b3], a.iz[b3], fa, fc);
-            const double3 da = sub3(u1, u2), dc = sub3(u3, u2);
-            a.g_angle[q] += da.x * (double)fa.x + da.y * (double)fa.y + da.z * (double)fa.z + dc.x * (double)fc.x +
-                            dc.y * (double)fc.y + dc.z * (double)fc.z;
-        }
-    }
+    if (a.springs) add_slots(z, a.sp_jv, a.sp_start[i], a.sp_start[i + 1]);
+    if (a.angles) add_slots(z, a.an_jv, a.an_start[i], a.an_start[i + 1]);
     // contacts, from the bead's own side of each listed pair
     const Box &bf = a.force;
     const P xi = a.px[i], yi = a.py[i], zi = a.pz[i];
@@ -275,47 +305,64 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_seed(double3 *fb, const doub
     fb[i] = double3{h * W.x + ((group && group[i]) ? seed : 0.0), h * W.y, h * W.z};
 }
 
-// Sum of the x-force over the flagged beads, by one block in a fixed order
-// (deterministic).
+// Block sum in a fixed order (tree over the block's threads).
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double part[kBlock];
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    return part[0];
+}
+
+// First pass of group_force: block b sums the entries k = b, b + B, ... of
+// its threads' strided share (B = gridDim.x), into part[b].
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_group_force(const T *fx, const int *group, int n, double *out)
+__global__ void __launch_bounds__(kBlock) k_group_partial(const T *fx, const int *idx, int n, double *part)
 {
-    __shared__ double part[kBlock];
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock)
-        if (group[i]) s += (double)fx[i];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = part[0];
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) s += (double)fx[idx[k]];
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kBlock) k_sum_columns(const double *g, int n, int np, double *out)
+// Second pass of both reductions: out[c] = sum over b < nb of part[c * nb + b].
+__global__ void __launch_bounds__(kBlock) k_sum_parts(const double *part, int nb, double *out)
 {
-    __shared__ double part[kBlock];
-    const int t = blockIdx.x;
     double s = 0.0;
-    for (int i = threadIdx.x; i < n; i += kBlock) s += g[(size_t)i * np + t];
-    part[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[t] = part[0];
+    for (int b = threadIdx.x; b < nb; b += kBlock) s += part[(size_t)blockIdx.x * nb + b];
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
-void sum_columns(const double *g, int n, int np, double *out, hipStream_t s)
+// First pass of sum_columns: block (b, c) sums column c over rows b, b + B, ...
+__global__ void __launch_bounds__(kBlock) k_column_partial(const double *g, int n, int np, double *part)
 {
-    if (np > 0) k_sum_columns<<<np, kBlock, 0, s>>>(g, n, np, out);
+    const int c = blockIdx.y;
+    double s = 0.0;
+    for (int r = blockIdx.x * kBlock + threadIdx.x; r < n; r += gridDim.x * kBlock) s += g[(size_t)r * np + c];
+    const double t = block_sum(s);
+    if (threadIdx.x == 0) part[(size_t)c * gridDim.x + blockIdx.x] = t;
+}
+
+static int sum_blocks(int n) { return std::max(1, std::min(kSumBlocks, (n + kBlock - 1) / kBlock)); }
+
+void sum_columns(const double *g, int n, int np, double *part, double *out, hipStream_t s)
+{
+    if (np <= 0) return;
+    const int nb = sum_blocks(n);
+    k_column_partial<<<dim3(nb, np), kBlock, 0, s>>>(g, n, np, part);
+    k_sum_parts<<<np, kBlock, 0, s>>>(part, nb, out);
 }
 
 template <typename T, typename P>
 void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s)
 {
+    if (a.springs > 0) k_adj_springs<T, P><<<(a.springs + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+    if (a.angles > 0) k_adj_angles<T, P><<<(a.angles + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
     if (a.n > 0) k_adjoint_step<T, P><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
 }
 template <typename T>
@@ -324,16 +371,18 @@ void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group
     if (n > 0) k_adjoint_seed<T><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(fb, wb, hlf, group, seed, n);
 }
 template <typename T>
-void group_force(const T *fx, const int *group, int n, double *out, hipStream_t s)
+void group_force(const T *fx, const int *idx, int n, double *part, double *out, hipStream_t s)
 {
-    k_group_force<T><<<1, kBlock, 0, s>>>(fx, group, n, out);
+    const int nb = sum_blocks(n);
+    k_group_partial<T><<<nb, kBlock, 0, s>>>(fx, idx, n, part);
+    k_sum_parts<<<1, kBlock, 0, s>>>(part, nb, out);
 }
 
 template void adjoint_step<double, double>(const AdjArgs<double, double> &, hipStream_t);
 template void adjoint_step<float, double>(const AdjArgs<float, double> &, hipStream_t);
 template void adjoint_seed<double>(double3 *, const double3 *, const double *, const int *, double, int, hipStream_t);
 template void adjoint_seed<float>(double3 *, const double3 *, const float *, const int *, double, int, hipStream_t);
-template void group_force<double>(const double *, const int *, int, double *, hipStream_t);
-template void group_force<float>(const float *, const int *, int, double *, hipStream_t);
+template void group_force<double>(const double *, const int *, int, double *, double *, hipStream_t);
+template void group_force<float>(const float *, const int *, int, double *, double *, hipStream_t);
 
 }  // namespace fnk

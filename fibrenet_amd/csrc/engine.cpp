@@ -947,16 +947,13 @@ class Engine final : public fn_engine {
         }
         std::vector<int> st(n + 1, 0), slot(2 * (size_t)S_);
         for (int i = 0; i < n; ++i) st[i + 1] = st[i] + cnt[i];
-        std::vector<int> cur(st.begin(), st.end() - 1), ent(2 * (size_t)S_);
+        std::vector<int> cur(st.begin(), st.end() - 1);
         for (int s = 0; s < S_; ++s) {
             slot[2 * (size_t)s] = cur[h_springs_[s].p1]++;
             slot[2 * (size_t)s + 1] = cur[h_springs_[s].p2]++;
-            ent[slot[2 * (size_t)s]] = s << 1;
-            ent[slot[2 * (size_t)s + 1]] = (s << 1) | 1;
         }
         sp_start_.put(st, stream_);
         sp_slot_.put(slot, stream_);
-        sp_ent_.put(ent, stream_);
         sp_term_.ensure(2 * (size_t)S_);
         std::fill(cnt.begin(), cnt.end(), 0);
         for (const fn_angle &a : h_angles_) {
@@ -968,16 +965,14 @@ class Engine final : public fn_engine {
         }
         std::vector<int> as(n + 1, 0), aslot(3 * (size_t)A_);
         for (int i = 0; i < n; ++i) as[i + 1] = as[i] + cnt[i];
-        std::vector<int> acur(as.begin(), as.end() - 1), aent(3 * (size_t)A_);
+        std::vector<int> acur(as.begin(), as.end() - 1);
         for (int q = 0; q < A_; ++q) {
             aslot[3 * (size_t)q] = acur[h_angles_[q].b1]++;
             aslot[3 * (size_t)q + 1] = acur[h_angles_[q].b3]++;
             aslot[3 * (size_t)q + 2] = acur[h_angles_[q].b2]++;
-            for (int r = 0; r < 3; ++r) aent[aslot[3 * (size_t)q + r]] = (q << 2) | r;
         }
         an_start_.put(as, stream_);
         an_slot_.put(aslot, stream_);
-        an_ent_.put(aent, stream_);
         an_term_.ensure(3 * (size_t)A_);
         build_chain();
         HIPCK(hipStreamSynchronize(stream_));
@@ -1774,10 +1769,17 @@ class Engine final : public fn_engine {
         bool spr = sp.run_springs != 0, ang = sp.run_angles != 0;
         ExactRebuild exact(*this);
         prep_fast(n, guf, spr, ang);
-        std::vector<int> g(N_, 0);
+        std::vector<int> g(N_, 0), gi;
         if (group)
-            for (int i = 0; i < N_; ++i) g[i] = group[i] != 0;
+            for (int i = 0; i < N_; ++i)
+                if (group[i] != 0) {
+                    g[i] = 1;
+                    gi.push_back(i);
+                }
         group_.put(g, stream_);
+        group_idx_.put(gi, stream_);
+        part_.ensure(fnk::kSumBlocks * std::max<size_t>(h_pairs_.size(), 1));
+        const int ng = (int)gi.size();
         const int nseg = (n + every - 1) / every;
         const size_t cb = parts_bytes(state_parts(pc_, fc_, false));
         ck_.ensure(cb * nseg);
@@ -1794,7 +1796,7 @@ class Engine final : public fn_engine {
             save_parts(state_parts(pc_, fc_, false), ck_.p + cb * seg);
             const StepHook hook = [&](int k, int, int fw) {
                 if (si < ns && steps[si] == c + k) {
-                    fnk::group_force(frc_[fw][0].p, group_.p, N_, samp_.p + si, stream_);
+                    fnk::group_force(frc_[fw][0].p, group_idx_.p, ng, part_.p, samp_.p + si, stream_);
                     ++si;
                 }
             };
@@ -1851,6 +1853,8 @@ class Engine final : public fn_engine {
         gan_.ensure(std::max(A_, 1));
         gpr_.ensure(std::max<size_t>(N * np, 1));
         gps_.ensure(std::max(np, 1));
+        sp_jv_.ensure(std::max<size_t>(2 * (size_t)S_, 1));
+        an_jv_.ensure(std::max<size_t>(3 * (size_t)A_, 1));
         HIPCK(hipMemsetAsync(xb_.p, 0, sizeof(double3) * N, stream_));
         HIPCK(hipMemsetAsync(wb_.p, 0, sizeof(double3) * N, stream_));
         HIPCK(hipMemsetAsync(gsp_.p, 0, sizeof(double) * gsp_.n, stream_));
@@ -1900,14 +1904,18 @@ class Engine final : public fn_engine {
                 a.n = N_;
                 a.n_springs = spr ? n_spr_ : 0;
                 a.n_angles = ang ? n_ang_ : 0;
+                a.springs = spr ? S_ : 0;
+                a.angles = ang ? A_ : 0;
+                a.sp_jv = sp_jv_.p;
+                a.an_jv = an_jv_.p;
                 a.drift = kbox(K_ADVANCE_POSITION);
                 a.force = kbox(K_RESOLVE_CONTACTS);
                 a.px = rp; a.py = rp + N; a.pz = rp + 2 * N;
                 a.ix = ri; a.iy = ri + N; a.iz = ri + 2 * N;
                 a.lam = lam_.p; a.hlf = hlf_.p; a.hlf2 = hlf2_.p; a.radius = rad_.p;
-                a.sp_start = sp_start_.p; a.sp_ent = sp_ent_.p; a.sp_p1 = sp_p1_.p; a.sp_p2 = sp_p2_.p;
+                a.sp_start = sp_start_.p; a.sp_slot = sp_slot_.p; a.sp_p1 = sp_p1_.p; a.sp_p2 = sp_p2_.p;
                 a.sp_type = sp_type_.p; a.sp_on = rs; a.sp_l0 = sp_l0_.p; a.sp_k = sp_k_.p;
-                a.an_start = an_start_.p; a.an_ent = an_ent_.p; a.an_b1 = an_b1_.p; a.an_b2 = an_b2_.p;
+                a.an_start = an_start_.p; a.an_slot = an_slot_.p; a.an_b1 = an_b1_.p; a.an_b2 = an_b2_.p;
                 a.an_b3 = an_b3_.p; a.an_type = an_type_.p; a.an_on = rs + S_; a.an_th0 = an_th0_.p; a.an_k = an_k_.p;
                 a.nb_start = rl; a.nb_cnt = rl + N + 1; a.nb_list = rl + 2 * N + 1;
                 a.pairs = pairs_.p;
@@ -1926,7 +1934,7 @@ class Engine final : public fn_engine {
         pc_ = adj_.fin_pc;
         fc_ = adj_.fin_fc;
         load_parts(state_parts(pc_, fc_, true), fin_.p);
-        fnk::sum_columns(gpr_.p, N_, np, gps_.p, stream_);
+        fnk::sum_columns(gpr_.p, N_, np, part_.p, gps_.p, stream_);
         auto d2h = [&](void *dst, const void *src, size_t bytes) {
             if (dst && bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
         };
@@ -2063,10 +2071,11 @@ class Engine final : public fn_engine {
     DArr<T> an_th0_, an_k_;
     DArr<fnk::V3<T>> sp_term_, an_term_;   // bond terms of the current step at the incidence slots
     DArr<int> ch_spring_, ch_angle_;       // chain layout: spring (i, i+1), angle centred on i
-    DArr<int> sp_ent_, an_ent_;            // incidence entries (s<<1 | p2; q<<2 | 0 b1, 1 b3, 2 b2) by slot
     AdjRun adj_;                           // the last adjoint forward run
     DArr<unsigned char> ck_, fin_;         // its checkpoints (one per segment) and end state
-    DArr<int> group_;                      // sampled beads
+    DArr<int> group_, group_idx_;          // sampled beads: flags, ascending indices
+    DArr<double> part_;                    // partial sums of the deterministic reductions
+    DArr<double3> sp_jv_, an_jv_;          // bond terms of an adjoint step at the incidence slots
     DArr<double> samp_;                    // samples of the run
     DArr<double3> fb_[2], xb_, wb_;        // adjoint state
     DArr<double> gsp_, gan_, gpr_, gps_;   // scale adjoints (contacts per bead and pair entry, then summed)

@@ -99,17 +99,19 @@ void note_build(const int *total, int *stats, const int *gate, hipStream_t s);
 template <typename T, typename P>
 struct AdjArgs {
     int n, n_springs, n_angles;   // bonds at or past n_springs / n_angles did not run
+    int springs, angles;          // all bonds (incidence slots 2 * springs, 3 * angles)
     Box drift, force;             // integrator and walls; contacts
     const P *px, *py, *pz, *ix, *iy, *iz;
     const T *lam, *hlf, *hlf2, *radius;
-    const int *sp_start, *sp_ent, *sp_p1, *sp_p2, *sp_type, *sp_on;
+    const int *sp_start, *sp_slot, *sp_p1, *sp_p2, *sp_type, *sp_on;
     const double *sp_l0;
     const T *sp_k;
-    const int *an_start, *an_ent, *an_b1, *an_b2, *an_b3, *an_type, *an_on;
+    const int *an_start, *an_slot, *an_b1, *an_b2, *an_b3, *an_type, *an_on;
     const T *an_th0, *an_k;
     const int *nb_start, *nb_cnt, *nb_list;
     const PairC *pairs;
     int n_pairs;
+    double3 *sp_jv, *an_jv;     // per incidence slot: the bond's J Fb on that bead
     double3 *xb, *wb;           // adjoints of positions and velocity_old
     const double3 *fb;          // adjoint of this step's force (seed and kick term included)
     double3 *fb_next;           // the same for the previous step
@@ -117,15 +119,20 @@ struct AdjArgs {
     double seed_prev;           // dL / d(sample) of the previous step (0: not sampled)
     double *g_spring, *g_angle, *g_pair;   // adjoints of the force scales
 };
+// One adjoint step: the bond terms (when a.springs / a.angles), then the
+// bead update.
 template <typename T, typename P>
 void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s);
 template <typename T>
 void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, double seed, int n, hipStream_t s);
-// out[t] = sum over the n rows of g[row * np + t] (one block per column, fixed order).
-void sum_columns(const double *g, int n, int np, double *out, hipStream_t s);
-// Sum over the flagged beads of the x-force, into *out (one block, fixed order).
+// out[t] = sum over the n rows of g[row * np + t]; part holds kSumBlocks * np
+// doubles.  Deterministic (fixed partition and order).
+constexpr int kSumBlocks = 256;
+void sum_columns(const double *g, int n, int np, double *part, double *out, hipStream_t s);
+// *out = sum of fx[idx[k]] over k < n; part holds kSumBlocks doubles.
+// Deterministic (fixed partition and order).
 template <typename T>
-void group_force(const T *fx, const int *group, int n, double *out, hipStream_t s);
+void group_force(const T *fx, const int *idx, int n, double *part, double *out, hipStream_t s);
 
 void fill_int(int *p, int v, int n, hipStream_t s);
 void fill_gated(int *p, int v, int n, const int *gate, hipStream_t s);

@@ -175,6 +175,6 @@ def test_torch_function(need_gpu):
     want = [g["spring"].sum(), g["angle"].sum(), g["pair"].sum()]
     assert np.allclose([float(v) for v in grads], want, rtol=1e-12, atol=0.0)
     step = 1e-3 / max(abs(float(v)) for v in grads)
-    x2 = [float(a) - step * float(b) for a, b in zip(x, grads)]
+    x2 = [float(a.detach()) - step * float(b) for a, b in zip(x, grads)]
     loss2 = float(((torch.as_tensor(run.forward(x2)) - target) ** 2).sum())
-    assert loss2 < float(loss)
+    assert loss2 < float(loss.detach())
