@@ -227,70 +227,130 @@ __device__ __forceinline__ void add_slots(double3 &z, const double3 *jv, int e0,
     }
 }
 
+// Pair-table entries the bead kernel keeps in LDS, and pair entries whose
+// scale adjoints it sums per block in LDS (more go per bead to global memory).
+constexpr int kAdjPairCap = 32;
+constexpr int kAdjPairsLds = 8;
+
 // The bead part of one adjoint step (thread per bead).  Reads the positions
 // at which the step's forces were evaluated, the bond terms at the bead's
 // incidence slots, the step's contact list and Fb (adjoint of this step's
 // force, seed and h Wb included); updates X, Wb, writes Fb of the previous
 // step (seed and h Wb of that step included) to fb_next, and adds the
-// contacts' scale adjoints (per bead and pair entry).
-template <typename T, typename P>
+// contacts' scale adjoints: LDSG, per block row (g_pair[block * n_pairs +
+// entry], summed over the block in a fixed order); otherwise per bead row.
+template <typename T, typename P, bool LDSG>
 __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
 {
+    __shared__ PairC prs[kAdjPairCap];
+    __shared__ double gl[LDSG ? kAdjPairsLds * kBlock : 1];
+    const int np = a.n_pairs;
+    for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
+    if (LDSG)
+        for (int k = 0; k < np; ++k) gl[k * kBlock + threadIdx.x] = 0.0;
+    __syncthreads();
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= a.n) return;
-    const double3 fb = a.fb[i];
-    double3 z{0.0, 0.0, 0.0};
-    if (a.springs) add_slots(z, a.sp_jv, a.sp_start[i], a.sp_start[i + 1]);
-    if (a.angles) add_slots(z, a.an_jv, a.an_start[i], a.an_start[i + 1]);
-    // contacts, from the bead's own side of each listed pair
-    const Box &bf = a.force;
-    const P xi = a.px[i], yi = a.py[i], zi = a.pz[i];
-    const T ri = a.radius[i];
-    for (int e = a.nb_start[i], e1 = e + a.nb_cnt[i]; e < e1; ++e) {
-        const int ent = a.nb_list[e];
-        const int j = ent & kIdxMask, pid = ent >> kPidShift;
-        const PairC pr = a.pairs[pid];
-        P rx = xi - a.px[j], ry = yi - a.py[j], rz = zi - a.pz[j];
-        P len = rx * rx + ry * ry + rz * rz;
-        if (len > (P)bf.hbs) {                   // ResolveContacts' minimum image (kernel.cl:844-870)
-            if (rx * rx > (P)bf.hbs) rx += (rx > (P)0) ? (P)-bf.lx : (P)bf.lx;
-            if (ry * ry > (P)bf.hbs) ry += (ry > (P)0) ? (P)-bf.ly : (P)bf.ly;
-            if (rz * rz > (P)bf.hbs) rz += (rz > (P)0) ? (P)-bf.lz : (P)bf.lz;
-            len = rx * rx + ry * ry + rz * rz;
+    if (i < a.n) {
+        const double3 fb = a.fb[i];
+        double3 z{0.0, 0.0, 0.0};
+        if (a.springs) add_slots(z, a.sp_jv, a.sp_start[i], a.sp_start[i + 1]);
+        if (a.angles) add_slots(z, a.an_jv, a.an_start[i], a.an_start[i + 1]);
+        // contacts, from the bead's own side of each listed pair
+        const Box &bf = a.force;
+        const P xi = a.px[i], yi = a.py[i], zi = a.pz[i];
+        const T ri = a.radius[i];
+        for (int e = a.nb_start[i], e1 = e + a.nb_cnt[i]; e < e1; ++e) {
+            const int ent = a.nb_list[e];
+            const int j = ent & kIdxMask, pid = ent >> kPidShift;
+            const PairC &pr = prs[pid];
+            P rx = xi - a.px[j], ry = yi - a.py[j], rz = zi - a.pz[j];
+            P len = rx * rx + ry * ry + rz * rz;
+            if (len > (P)bf.hbs) {               // ResolveContacts' minimum image (kernel.cl:844-870)
+                if (rx * rx > (P)bf.hbs) rx += (rx > (P)0) ? (P)-bf.lx : (P)bf.lx;
+                if (ry * ry > (P)bf.hbs) ry += (ry > (P)0) ? (P)-bf.ly : (P)bf.ly;
+                if (rz * rz > (P)bf.hbs) rz += (rz > (P)0) ? (P)-bf.lz : (P)bf.lz;
+                len = rx * rx + ry * ry + rz * rz;
+            }
+            V3<T> t;
+            if (!contact_force(pr, bf, rx, ry, rz, len, ri, a.radius, j, t)) continue;   // the forward's mask
+            const double rj = pr.type == 2 ? (double)a.radius[j] : 0.0;
+            const DV3 F = contact_dual(pr, (double)ri, rj,
+                                       dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fb, a.fb[j])));
+            add_d(z, F, 1.0);
+            if (LDSG) gl[pid * kBlock + threadIdx.x] += dot_v(fb, F);
+            else a.g_pair[(size_t)i * np + pid] += dot_v(fb, F);
         }
-        V3<T> t;
-        if (!contact_force(pr, bf, rx, ry, rz, len, ri, a.radius, j, t)) continue;   // the forward's mask
-        const DV3 F = contact_dual(pr, (double)ri, (double)a.radius[j],
-                                   dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fb, a.fb[j])));
-        add_d(z, F, 1.0);
-        a.g_pair[(size_t)i * a.n_pairs + pid] += dot_v(fb, F);
+        // walls of a finite box (kernel.cl:252-261): diagonal
+        const Box &bd = a.drift;
+        if (bd.bc == 1) {
+            const double k = -bd.wall_k;
+            if (xi < (P)0 || xi > (P)bd.lx) z.x += k * fb.x;
+            if (yi < (P)0 || yi > (P)bd.ly) z.y += k * fb.y;
+            if (zi < (P)0 || zi > (P)bd.lz) z.z += k * fb.z;
+        }
+        double3 X = a.xb[i];
+        X.x += z.x;
+        X.y += z.y;
+        X.z += z.z;
+        const double lam = a.lam[i], h = a.hlf[i], h2 = a.hlf2[i];
+        const double dt = (double)(T)bd.dt, D = (double)(T)bd.damping;
+        const double3 vb{D * fb.x, D * fb.y, D * fb.z};
+        double3 W = a.wb[i];
+        const double3 fprev{h * W.x + lam * vb.x + h2 * X.x, h * W.y + lam * vb.y + h2 * X.y,
+                            h * W.z + lam * vb.z + h2 * X.z};
+        W.x += vb.x + dt * X.x;
+        W.y += vb.y + dt * X.y;
+        W.z += vb.z + dt * X.z;
+        a.xb[i] = X;
+        a.wb[i] = W;
+        // Fb of the previous step: its own force adjoint, its kick's h Wb and its seed
+        const double seed = (a.group && a.group[i]) ? a.seed_prev : 0.0;
+        a.fb_next[i] = double3{fprev.x + h * W.x + seed, fprev.y + h * W.y, fprev.z + h * W.z};
     }
-    // walls of a finite box (kernel.cl:252-261): diagonal
-    const Box &bd = a.drift;
-    if (bd.bc == 1) {
-        const double k = -bd.wall_k;
-        if (xi < (P)0 || xi > (P)bd.lx) z.x += k * fb.x;
-        if (yi < (P)0 || yi > (P)bd.ly) z.y += k * fb.y;
-        if (zi < (P)0 || zi > (P)bd.lz) z.z += k * fb.z;
+    if (LDSG) {
+        for (int k = 0; k < np; ++k)
+            for (int w = kBlock / 2; w > 0; w >>= 1) {
+                __syncthreads();
+                if ((int)threadIdx.x < w) gl[k * kBlock + threadIdx.x] += gl[k * kBlock + threadIdx.x + w];
+            }
+        __syncthreads();
+        if ((int)threadIdx.x < np) a.g_pair[(size_t)blockIdx.x * np + threadIdx.x] += gl[threadIdx.x * kBlock];
     }
-    double3 X = a.xb[i];
-    X.x += z.x;
-    X.y += z.y;
-    X.z += z.z;
-    const double lam = a.lam[i], h = a.hlf[i], h2 = a.hlf2[i];
-    const double dt = (double)(T)bd.dt, D = (double)(T)bd.damping;
-    const double3 vb{D * fb.x, D * fb.y, D * fb.z};
-    double3 W = a.wb[i];
-    const double3 fprev{h * W.x + lam * vb.x + h2 * X.x, h * W.y + lam * vb.y + h2 * X.y,
-                        h * W.z + lam * vb.z + h2 * X.z};
-    W.x += vb.x + dt * X.x;
-    W.y += vb.y + dt * X.y;
-    W.z += vb.z + dt * X.z;
-    a.xb[i] = X;
-    a.wb[i] = W;
-    // Fb of the previous step: its own force adjoint, its kick's h Wb and its seed
-    const double seed = (a.group && a.group[i]) ? a.seed_prev : 0.0;
-    a.fb_next[i] = double3{fprev.x + h * W.x + seed, fprev.y + h * W.y, fprev.z + h * W.z};
+}
+
+int adjoint_pair_rows(int n, int np) { return np <= kAdjPairsLds ? (n + kBlock - 1) / kBlock : n; }
+
+// Positions (and images, when ix is not null) and bond statuses of one step
+// into its record: rec_pos = x, y, z[, ix, iy, iz] (n each), rec_st = spring
+// then angle statuses.
+__global__ void __launch_bounds__(kBlock) k_record_step(const double *px, const double *py, const double *pz,
+                                                        const double *ix, const double *iy, const double *iz,
+                                                        const int *sp_st, int ns, const int *an_st, int na, int n,
+                                                        double *rec_pos, int *rec_st)
+{
+    const int k = blockIdx.x * kBlock + threadIdx.x;
+    if (k < n) {
+        rec_pos[k] = px[k];
+        rec_pos[(size_t)n + k] = py[k];
+        rec_pos[2 * (size_t)n + k] = pz[k];
+        if (ix) {
+            rec_pos[3 * (size_t)n + k] = ix[k];
+            rec_pos[4 * (size_t)n + k] = iy[k];
+            rec_pos[5 * (size_t)n + k] = iz[k];
+        }
+    }
+    if (k < ns) rec_st[k] = sp_st[k];
+    if (k < na) rec_st[ns + k] = an_st[k];
+}
+
+void record_step(const double *const p[3], const double *const im[3], const int *sp_st, int ns, const int *an_st,
+                 int na, int n, double *rec_pos, int *rec_st, hipStream_t s)
+{
+    const int m = std::max(n, std::max(ns, na));
+    if (m > 0)
+        k_record_step<<<(m + kBlock - 1) / kBlock, kBlock, 0, s>>>(p[0], p[1], p[2], im ? im[0] : nullptr,
+                                                                  im ? im[1] : nullptr, im ? im[2] : nullptr, sp_st,
+                                                                  ns, an_st, na, n, rec_pos, rec_st);
 }
 
 // Fb of the last step: its seed and h Wb.
@@ -363,7 +423,9 @@ void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s)
 {
     if (a.springs > 0) k_adj_springs<T, P><<<(a.springs + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
     if (a.angles > 0) k_adj_angles<T, P><<<(a.angles + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
-    if (a.n > 0) k_adjoint_step<T, P><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+    if (a.n <= 0) return;
+    if (a.n_pairs <= kAdjPairsLds) k_adjoint_step<T, P, true><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+    else k_adjoint_step<T, P, false><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
 }
 template <typename T>
 void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, double seed, int n, hipStream_t s)

@@ -1851,7 +1851,8 @@ class Engine final : public fn_engine {
         for (auto *d : {&fb_[0], &fb_[1], &xb_, &wb_}) d->ensure(N);
         gsp_.ensure(std::max(S_, 1));
         gan_.ensure(std::max(A_, 1));
-        gpr_.ensure(std::max<size_t>(N * np, 1));
+        const int prow = fnk::adjoint_pair_rows(N_, np);
+        gpr_.ensure(std::max<size_t>((size_t)prow * np, 1));
         gps_.ensure(std::max(np, 1));
         sp_jv_.ensure(std::max<size_t>(2 * (size_t)S_, 1));
         an_jv_.ensure(std::max<size_t>(3 * (size_t)A_, 1));
@@ -1880,13 +1881,10 @@ class Engine final : public fn_engine {
             fc_ = adj_.ck_fc[seg];
             load_parts(state_parts(pc_, fc_, false), ck_.p + adj_.cb * seg);
             const StepHook hook = [&](int k, int pr, int) {
-                P *dst = rec_pos_.p + (size_t)k * nimg * N;
-                for (int q = 0; q < 3; ++q) d2d(dst + q * N, pos_[pr][q].p, sizeof(P) * N);
-                if (!alias_)
-                    for (int q = 0; q < 3; ++q) d2d(dst + (3 + q) * N, img_[pr][q].p, sizeof(P) * N);
-                int *ds = rec_st_.p + (size_t)k * SA;
-                d2d(ds, sp_status_.p, sizeof(int) * S_);
-                d2d(ds + S_, an_status_.p, sizeof(int) * A_);
+                const double *p[3] = {pos_[pr][0].p, pos_[pr][1].p, pos_[pr][2].p};
+                const double *im[3] = {img_[pr][0].p, img_[pr][1].p, img_[pr][2].p};
+                fnk::record_step(p, alias_ ? nullptr : im, sp_status_.p, S_, an_status_.p, A_, N_,
+                                 rec_pos_.p + (size_t)k * nimg * N, rec_st_.p + (size_t)k * SA, stream_);
                 if (k % guf == 0) {
                     int *dl = rec_nb_.p + (size_t)(k / guf) * lrow;
                     d2d(dl, nb_start_.p, sizeof(int) * (N + 1));
@@ -1934,7 +1932,7 @@ class Engine final : public fn_engine {
         pc_ = adj_.fin_pc;
         fc_ = adj_.fin_fc;
         load_parts(state_parts(pc_, fc_, true), fin_.p);
-        fnk::sum_columns(gpr_.p, N_, np, part_.p, gps_.p, stream_);
+        fnk::sum_columns(gpr_.p, prow, np, part_.p, gps_.p, stream_);
         auto d2h = [&](void *dst, const void *src, size_t bytes) {
             if (dst && bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
         };

@@ -117,7 +117,8 @@ struct AdjArgs {
     double3 *fb_next;           // the same for the previous step
     const int *group;           // beads whose x-force is sampled
     double seed_prev;           // dL / d(sample) of the previous step (0: not sampled)
-    double *g_spring, *g_angle, *g_pair;   // adjoints of the force scales
+    double *g_spring, *g_angle;   // adjoints of the force scales of the bonds
+    double *g_pair;               // and of the contact pair entries, adjoint_pair_rows rows
 };
 // One adjoint step: the bond terms (when a.springs / a.angles), then the
 // bead update.
@@ -125,6 +126,13 @@ template <typename T, typename P>
 void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s);
 template <typename T>
 void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, double seed, int n, hipStream_t s);
+// Rows of the contact scale adjoints g_pair (row * n_pairs + entry) that the
+// adjoint step accumulates for n beads.
+int adjoint_pair_rows(int n, int np);
+// One step's positions (images when im is not null) and bond statuses into
+// its record (see k_record_step).
+void record_step(const double *const p[3], const double *const im[3], const int *sp_st, int ns, const int *an_st,
+                 int na, int n, double *rec_pos, int *rec_st, hipStream_t s);
 // out[t] = sum over the n rows of g[row * np + t]; part holds kSumBlocks * np
 // doubles.  Deterministic (fixed partition and order).
 constexpr int kSumBlocks = 256;
