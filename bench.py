@@ -10,6 +10,10 @@ steps.  A step is the spring-term, angle-term and bead kernels over the whole
 network (plus the grid rebuild every 10th step); inputs are resident in HBM
 when the timed region starts.
 
+The settings that cannot change a result (chain bead kernel, staged search,
+kept contact candidates, graph chunk length) are chosen by the autotuner,
+run on a second engine with the same network from the same warmed-up state.
+
 Prints one JSON line: bead-steps/s of the timed steps, the roofline of the
 dominant kernel (the step kernel with the largest duration, measured with
 HIP event nodes in the captured graphs of the timed region) and a CPU
@@ -170,6 +174,8 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the adjoint cost measurement on the same network (0: none)")
     ap.add_argument("--checkpoint-every", type=int, default=50)
+    ap.add_argument("--tune-steps", type=int, default=3000,
+                    help="steps of the autotuner, run on a second engine from the warmed-up state (0: none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-bytes", type=float, default=0,
                     help="also run one device copy of this many bytes (counter calibration)")
@@ -216,6 +222,20 @@ def main():
     if args.warmup > 0:
         eng.run_steps(args.warmup, net.grid_update_freq, spr, ang)
     eng.finish()
+    tune = None
+    if args.tune_steps > 0:
+        # The autotuner on a second engine holding the same network, from the
+        # same warmed-up state, so that the timed region below starts where it
+        # would without tuning; its result-neutral choices are then applied.
+        tuner = Engine(args.precision)
+        bind_standard(tuner, net)
+        if args.warmup > 0:
+            tuner.run_steps(args.warmup, net.grid_update_freq, spr, ang)
+        trials, chosen = tuner.autotune(args.tune_steps, net.grid_update_freq, spr, ang)
+        tuner.close()
+        for k, v in chosen.items():
+            eng.set_option(k, v)
+        tune = {"steps": args.tune_steps, "chosen": chosen, "trials": trials}
     print("bench: %s, warmup %d done, timing %d steps" % (desc, args.warmup, args.steps), file=sys.stderr,
           flush=True)
 
@@ -314,7 +334,7 @@ def main():
                   "step_algorithmic_GBps_incl_rebuild": (b_step + b_rebuild / net.grid_update_freq)
                   * args.steps / wall / 1e9, "counts": counts, "stats": st, "generate_s": gen_s,
                   "gpu_over_cpu": (value / cpu["value"]) if cpu else None, "copy_bytes": copy,
-                  "ensemble": ens, "adjoint": adj},
+                  "ensemble": ens, "adjoint": adj, "autotune": tune},
     }
     print(json.dumps(out))
 

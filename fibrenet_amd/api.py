@@ -49,6 +49,7 @@ SIGNATURES = [
     ("fn_copy_probe", _I, [_VP, _c.c_longlong, _VP]),
     ("fn_batch_create", _I, [_I, _I, _I, _VP, _VP]),
     ("fn_batch_read", _I, [_VP, _I, _I, _VP]),
+    ("fn_autotune", _I, [_VP, _I, _I, _I, _I, _VP, _I, _c.POINTER(_I)]),
     ("fn_adjoint_forward", _I, [_VP, _VP, _VP, _VP, _I, _VP]),
     ("fn_adjoint_backward", _I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
     ("fn_model_load", _I, [_CP, _c.POINTER(_VP)]),
@@ -90,6 +91,14 @@ class FnNetwork(ctypes.Structure):
     """fn_network: one network of a batched ensemble (host records + settings)."""
     _fields_ = [("beads", _VP), ("info", _VP), ("n_beads", _I), ("springs", _VP), ("n_springs", _I),
                 ("angles", _VP), ("n_angles", _I), ("pairs", _VP), ("n_pairs", _I), ("settings", _VP)]
+
+
+class FnTuneTrial(ctypes.Structure):
+    """fn_tune_trial: one timed chunk of the autotuner."""
+    _fields_ = [("knob", _I), ("value", _I), ("ms_per_step", _c.c_double)]
+
+
+TUNE_KNOBS = ("chunk", "chain", "staged", "verlet")
 
 
 class FnAdjointSpec(ctypes.Structure):
@@ -291,6 +300,19 @@ class Engine:
         ms = ctypes.c_double()
         self._chk(lib().fn_copy_probe(self.h, int(nbytes), ctypes.byref(ms)))
         return ms.value
+
+    def autotune(self, n, grid_update_freq=8, springs=True, angles=True):
+        """Runs n steps as run_steps does while timing the result-neutral
+        settings (fn_autotune); returns the timed trials and the settings
+        chosen (which stay set)."""
+        arr = (FnTuneTrial * 64)()
+        cnt = ctypes.c_int()
+        self._chk(lib().fn_autotune(self.h, int(n), int(grid_update_freq), int(bool(springs)), int(bool(angles)),
+                                    arr, 64, ctypes.byref(cnt)))
+        trials = [{"knob": TUNE_KNOBS[t.knob], "value": t.value, "ms_per_step": t.ms_per_step}
+                  for t in arr[:min(cnt.value, 64)]]
+        chosen = {k: self.get_option(k) for k in TUNE_KNOBS}
+        return trials, chosen
 
     # -- adjoint ------------------------------------------------------------------------------
     def adjoint_forward(self, n, group, sample_steps, grid_update_freq=8, springs=True, angles=True,

@@ -218,6 +218,7 @@ struct fn_engine {
     virtual void copy_probe(long long bytes, double *ms) = 0;
     virtual void load_ensemble(const fn_network *nets, int n) = 0;
     virtual void ensemble_read(int net, int kind, void *out) = 0;
+    virtual void autotune(int n, int guf, bool spr, bool ang, std::vector<fn_tune_trial> &trials) = 0;
     virtual void adjoint_forward(const fn_adjoint_spec &spec, const int *group, const int *steps, int n_samples,
                                  double *samples) = 0;
     virtual void adjoint_backward(const double *dloss, double *g_spring, double *g_angle, double *g_pair,
@@ -1678,6 +1679,94 @@ class Engine final : public fn_engine {
         }
     }
 
+    // ---- autotuner --------------------------------------------------------------------------
+    // Runs n steps exactly as run_steps does while trying the settings that
+    // leave every result bitwise the same (chunk length, chain kernel,
+    // staged search, kept candidates), one knob at a time: each value runs
+    // one chunk (graph capture, warm caches) and one timed chunk, and the
+    // fastest value per step is kept before the next knob.  Steps left over
+    // run with the chosen settings, which stay set.
+    void autotune(int n, int guf, bool spr, bool ang, std::vector<fn_tune_trial> &trials) override
+    {
+        adj_.valid = false;
+        trials.clear();
+        prep_fast(n, guf, spr, ang);
+        if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
+        fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
+        const bool ens = !ens_boxes_.empty();
+        std::vector<std::pair<int, std::vector<int>>> knobs;
+        std::vector<int> chunks;
+        for (int c : {32, 64, 128, 256}) {
+            const int l = guf * std::max(1, c / guf);
+            if (std::find(chunks.begin(), chunks.end(), l) == chunks.end()) chunks.push_back(l);
+        }
+        if (chain_ok_ && n_spr_ == S_ && n_ang_ == A_) knobs.push_back({FN_TUNE_CHAIN, {1, 0}});
+        if (!ens) {
+            knobs.push_back({FN_TUNE_STAGED, {0, 1}});
+            knobs.push_back({FN_TUNE_VERLET, {verlet_, 0, 1}});
+        }
+        knobs.push_back({FN_TUNE_CHUNK, chunks});       // last: its trials are the longest
+        static const char *names[] = {"chunk", "chain", "staged", "verlet"};
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        HIPCK(hipEventCreate(&e0));
+        HIPCK(hipEventCreate(&e1));
+        int done = 0, rb = 0;
+        auto chunk = [&](bool timed) {
+            const int len = std::min(chunk_len(guf), n - done);
+            if (timed) HIPCK(hipEventRecord(e0, stream_));
+            run_chunk(len, guf, spr, ang, done == 0, done + len == n);
+            rb += (len + guf - 1) / guf;
+            done += len;
+            float ms = 0.0f;
+            if (timed) {
+                HIPCK(hipEventRecord(e1, stream_));
+                HIPCK(hipEventSynchronize(e1));
+                HIPCK(hipEventElapsedTime(&ms, e0, e1));
+            }
+            return (double)ms / len;
+        };
+        auto apply = [&](int knob, int v) {
+            set_option(names[knob], v);
+            if (!ens) geometry(settings_of(K_FILL_GRID), settings_of(K_BUILD_LIST));
+        };
+        try {
+            for (auto &kv : knobs) {
+                const int knob = kv.first;
+                int best_v = get_option(names[knob]);
+                double best = 1e300;
+                std::vector<int> tried;
+                for (int v : kv.second) {
+                    if (std::find(tried.begin(), tried.end(), v) != tried.end()) continue;
+                    tried.push_back(v);
+                    if (n - done < 2 * guf) break;
+                    apply(knob, v);
+                    chunk(false);
+                    if (done >= n) break;
+                    const double t = chunk(true);
+                    trials.push_back(fn_tune_trial{knob, v, t});
+                    log(fmt("autotune: %s = %d: %.4f ms per step", names[knob], v, t));
+                    if (t < best) {
+                        best = t;
+                        best_v = v;
+                    }
+                }
+                if (get_option(names[knob]) != best_v) apply(knob, best_v);
+            }
+            while (done < n) chunk(false);
+        } catch (...) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            throw;
+        }
+        HIPCK(hipEventDestroy(e0));
+        HIPCK(hipEventDestroy(e1));
+        steps_ += n;
+        rebuilds_ += rb;
+        HIPCK(hipStreamSynchronize(stream_));
+        collect_timing();
+        check_capacity();
+    }
+
     // ---- adjoint ----------------------------------------------------------------------------
     struct Part {
         void *p;
@@ -2297,6 +2386,18 @@ int fn_batch_read(fn_engine *eng, int network, int kind, void *out)
     return guarded(eng, [&] {
         if (!out) throw Error(FN_ERR_ARG, "null destination");
         eng->ensemble_read(network, kind, out);
+    });
+}
+
+int fn_autotune(fn_engine *eng, int n_steps, int grid_update_freq, int run_springs, int run_angles,
+                fn_tune_trial *trials, int capacity, int *n_trials)
+{
+    return guarded(eng, [&] {
+        std::vector<fn_tune_trial> t;
+        eng->autotune(n_steps, grid_update_freq, run_springs != 0, run_angles != 0, t);
+        if (n_trials) *n_trials = (int)t.size();
+        if (trials)
+            for (int k = 0; k < (int)t.size() && k < capacity; ++k) trials[k] = t[k];
     });
 }
 

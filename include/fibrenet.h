@@ -198,6 +198,26 @@ int fn_batch_create(int device, int precision, int n_networks, const fn_network 
  * FN_BUF_ANGLE (bond indices local to the network). */
 int fn_batch_read(fn_engine *eng, int network, int kind, void *out);
 
+/* ---- autotuner (no reference counterpart) ----
+ * Runs n_steps steps exactly as fn_run_steps does (bitwise the same
+ * results) while timing the settings that cannot change a result, one knob
+ * at a time: the chain bead kernel, the LDS-staged search, kept contact
+ * candidates and the chunk length of the captured graphs (options "chain",
+ * "staged", "verlet", "chunk"; in that order).  Each value runs one chunk to
+ * capture its graphs and one timed chunk; the fastest value of a knob is
+ * kept before the next knob is tried, and the chosen settings stay set
+ * (fn_get_option reads them).  A run too short for every trial tries what
+ * fits.  trials (capacity entries, may be NULL) receives each timing;
+ * *n_trials their number. */
+enum { FN_TUNE_CHUNK = 0, FN_TUNE_CHAIN = 1, FN_TUNE_STAGED = 2, FN_TUNE_VERLET = 3 };
+typedef struct fn_tune_trial {
+    int knob;               /* FN_TUNE_*                                       */
+    int value;              /* the option's value during the timed chunk       */
+    double ms_per_step;     /* device time of the timed chunk per step         */
+} fn_tune_trial;
+int fn_autotune(fn_engine *eng, int n_steps, int grid_update_freq, int run_springs, int run_angles,
+                fn_tune_trial *trials, int capacity, int *n_trials);
+
 /* ---- adjoint (no reference counterpart) ----
  * Reverse-mode derivative of a fast-path run with respect to the force
  * scales of every spring, angle and contact pair type and to the initial

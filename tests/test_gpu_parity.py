@@ -281,3 +281,40 @@ def test_ensemble_rejects_bad_networks(need_gpu):
     with pytest.raises(FibreNetError):
         Ensemble([net, bad], "f64")
 
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("workload", ["tensile_lj_breaking", "periodic_hs"])
+def test_graphs_equal_direct_launches(need_gpu, workload, precision):
+    """Captured graphs of any chunk length give the direct launches' results."""
+    net = WORKLOADS[workload]()
+    spr, ang = len(net.springs) > 0, len(net.angles) > 0
+    out = []
+    for graphs, chunk in ((0, 128), (1, 8), (1, 64), (1, 128), (1, 256)):
+        eng, ids = engine_for(net, precision)
+        eng.set_option("graphs", graphs)
+        eng.set_option("chunk", chunk)
+        for n in (37, 300):
+            eng.run_steps(n, net.grid_update_freq, spr, ang)
+        out.append((read_beads(eng, ids).tobytes(), eng.export_breaks(len(net.springs)).tobytes(),
+                    eng.export_pairs().tobytes()))
+    assert all(o == out[0] for o in out[1:])
+
+
+@pytest.mark.parametrize("workload", ["tensile_lj_breaking", "periodic_hs"])
+def test_autotune_equals_run_steps(need_gpu, workload):
+    """The autotuner's run is a plain run bit for bit, whatever it tries."""
+    net = WORKLOADS[workload]()
+    spr, ang = len(net.springs) > 0, len(net.angles) > 0
+    a, ia = engine_for(net, "f32")
+    trials, chosen = a.autotune(6000, net.grid_update_freq, spr, ang)   # room for every trial
+    a.run_steps(100, net.grid_update_freq, spr, ang)        # continues with the chosen settings
+    b, ib = engine_for(net, "f32")
+    b.run_steps(6000, net.grid_update_freq, spr, ang)
+    b.run_steps(100, net.grid_update_freq, spr, ang)
+    assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
+    assert np.array_equal(a.export_breaks(len(net.springs)), b.export_breaks(len(net.springs)))
+    assert np.array_equal(a.export_pairs(), b.export_pairs())
+    knobs = {t["knob"] for t in trials}
+    assert {"chunk", "staged", "verlet"} <= knobs and all(t["ms_per_step"] > 0 for t in trials)
+    assert a.stats()["steps"] == 6100
