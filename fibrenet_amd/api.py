@@ -11,7 +11,8 @@ import numpy as np
 from . import records as R
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libfibrenet_amd.so")
+# The in-tree build; FIBRENET_AMD_LIB names another build of the same library.
+LIB_PATH = os.environ.get("FIBRENET_AMD_LIB") or os.path.join(_HERE, "libfibrenet_amd.so")
 
 FN_OK = 0
 PREC = {"f64": 0, "f32": 1}

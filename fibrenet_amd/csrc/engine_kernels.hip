@@ -226,21 +226,22 @@ template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool
 __global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
 {
     __shared__ PairC prs[kPairCap];
-    __shared__ V3<T> cterm[kBlock / 64][kWaveChunk];
+    __shared__ WaveContactLds<T> cl[kBlock / 64];
     const int i = xcd_block() * kBlock + threadIdx.x;
     for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = a.pairs[k];
     __syncthreads();
     const bool live = i < a.n;
     const int ic = live ? i : a.n - 1;      // lanes past the end follow along for the wave-wide contacts
     const int e0 = a.nb_start[ic], cnt = live ? a.nb_cnt[ic] : 0;
+    WaveContactLds<T> &w = cl[threadIdx.x >> 6];
+    const ContactAhead ca = contacts_ahead(a, e0, cnt, w);
     const T r1 = a.radius[ic];
     const Box &bx = drift_box<ENS>(a, ic);
     BeadState<T, P> b;
     V3<T> f = bead_begin(a, bx, ic, b);
     if (SPR) add_terms(a.sp_term, a.sp_start[ic], a.sp_start[ic + 1], f);
     if (ANG) add_terms(a.an_term, a.an_start[ic], a.an_start[ic + 1], f);
-    wave_contacts<T, P, ENS>(a, e0, cnt, r1, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6],
-                             ENS ? a.net[ic] : 0);
+    contacts_finish<T, P, ENS>(a, ca, e0, cnt, r1, b.px, b.py, b.pz, f, prs, w, ENS ? a.net[ic] : 0);
     if (!live) return;
     if (NEXT) bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, a.ix[i], a.iy[i], a.iz[i]);
     else bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, (P)0, (P)0, (P)0);
@@ -264,7 +265,7 @@ template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool
 __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
 {
     __shared__ PairC prs[kPairCap];
-    __shared__ V3<T> cterm[kBlock / 64][kWaveChunk];
+    __shared__ WaveContactLds<T> cl[kBlock / 64];
     for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = a.pairs[k];
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -292,6 +293,9 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
     AngleP<T> anp{};
     if (sr >= 0) spp = load_spring(a, sr);
     if (q >= 0) anp = load_angle(a, q);
+    // and the first list entries of the wave's contacts
+    WaveContactLds<T> &w = cl[threadIdx.x >> 6];
+    const ContactAhead ca = contacts_ahead(a, e0, cnt, w);
 
     // images of beads i + 1 and i - 1 (meaningless in the outermost lanes,
     // whose results no owned lane uses)
@@ -363,8 +367,7 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
             f.z += pan.z;
         }
     }
-    wave_contacts<T, P, ENS>(a, e0, cnt, r1, b.px, b.py, b.pz, f, prs, cterm[threadIdx.x >> 6],
-                             ENS ? a.net[ic] : 0);
+    contacts_finish<T, P, ENS>(a, ca, e0, cnt, r1, b.px, b.py, b.pz, f, prs, w, ENS ? a.net[ic] : 0);
     if (!own) return;
     bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, x, y, z);
 }
@@ -521,13 +524,12 @@ __device__ __forceinline__ double list_len2(double rx, double ry, double rz, con
 // cheap first rejection).  Slot t's record is src[t - shift] (global memory
 // or a staged copy in LDS).
 template <bool EXACT>
-__device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBead &me, int ti, double ri, double si,
-                                           const SortedBead *src, int shift, const Sorted &so, const int *sorted,
-                                           const PairC *pairs, const double *radius, const double *skin,
-                                           const Box &b, double r2, int *row, int K, int &k)
+__device__ __forceinline__ void scan_range_from(int lo, int hi, SortedBead nxt, int s, const SortedBead &me, int ti,
+                                                double ri, double si, const SortedBead *src, int shift,
+                                                const Sorted &so, const int *sorted, const PairC *pairs,
+                                                const double *radius, const double *skin, const Box &b, double r2,
+                                                int *row, int K, int &k)
 {
-    if (lo >= hi) return;
-    SortedBead nxt = src[lo - shift];
     for (int t = lo; t < hi; ++t) {
         const SortedBead o = nxt;                  // the next record is in flight while this one is tested
         if (t + 1 < hi) nxt = src[t + 1 - shift];
@@ -554,6 +556,18 @@ __device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBe
         if (k < K) row[k] = sorted[t] | (pid << kPidShift);
         ++k;
     }
+}
+
+// The same over slots [lo, hi), loading the first record itself.
+template <bool EXACT>
+__device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBead &me, int ti, double ri, double si,
+                                           const SortedBead *src, int shift, const Sorted &so, const int *sorted,
+                                           const PairC *pairs, const double *radius, const double *skin,
+                                           const Box &b, double r2, int *row, int K, int &k)
+{
+    if (lo >= hi) return;
+    scan_range_from<EXACT>(lo, hi, src[lo - shift], s, me, ti, ri, si, src, shift, so, sorted, pairs, radius, skin,
+                           b, r2, row, K, k);
 }
 
 // Cells visited along one axis around cell c: c-w..c+w, wrapped (periodic) or
@@ -606,6 +620,26 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
         const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, st.w[1], st.all[1], per},
             az{cz, b.cz, st.w[2], st.all[2], per};
         const bool row_run = !st.all[0] && cx - st.w[0] >= 0 && cx + st.w[0] < b.cx;
+        if (EXACT && code >= 0 && row_run && st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !st.all[1] &&
+            !st.all[2]) {
+            // the reference's 3x3x3 stencil with x-rows inside the grid: the
+            // nine row ranges in one round trip, then the rows
+            int lo[9], hi[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                const int z = az.at(q / 3 - 1), y = ay.at(q % 3 - 1);
+                lo[q] = hi[q] = 0;
+                if (y >= 0 && z >= 0) {
+                    const int rb = b.cell_base + y * b.cx + z * b.cxy + cx;
+                    lo[q] = cell_start[rb - 1];
+                    hi[q] = cell_start[rb + 2];
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 9; ++q)
+                scan_range<EXACT>(lo[q], hi[q], s, me, ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b, r2,
+                                  row, K, k);
+        } else
         for (int dz = az.first(); dz <= az.last() && !(EXACT && code < 0); ++dz) {
             const int z = az.at(dz);
             if (z < 0) continue;

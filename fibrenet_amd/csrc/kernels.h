@@ -432,22 +432,55 @@ __device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T
     }
 }
 
-// Contact-list entries evaluated per pass by one wave (LDS: kWaveChunk terms).
+// Contact-list entries evaluated per pass by one wave, and the passes of 64.
 constexpr int kWaveChunk = 256;
+constexpr int kChunkRounds = kWaveChunk / 64;
 
-// The same sums as gather_contacts, evaluated by the whole wave: the lists
-// of the wave's beads are laid end to end and each lane evaluates every
-// 64th entry, so a wave costs (its total entries / 64) rounds instead of
-// its longest list; the terms go through LDS and each bead adds its own in
-// list order.  An entry without force stores -0.0, which leaves any sum
-// unchanged, so the result equals gather_contacts bit for bit.  Every lane
-// of the wave must call this; a live lane's bead is at x1.. with radius r1
-// and list entries [e0, e0 + cnt) (cnt 0 for other lanes).  `term` is this
-// wave's LDS area of kWaveChunk entries.  ENS: the lane's bead belongs to
-// network `netw` (its Box and pair table); otherwise a.force and `table`.
-template <typename T, typename P, bool ENS>
-__device__ __forceinline__ void wave_contacts(const Args<T, P> &a, int e0, int cnt, T r1, P x1, P y1, P z1, V3<T> &f,
-                                              const PairC *table, V3<T> *term, int netw = 0)
+// A wave's LDS for its contacts: the terms of one pass and the lane that
+// owns each entry of it.
+template <typename T>
+struct WaveContactLds {
+    V3<T> term[kWaveChunk];
+    unsigned char own[kWaveChunk];
+};
+
+__device__ __forceinline__ void wave_sync_lds()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The first pass's list entries of a wave, loaded ahead of their use.
+struct ContactAhead {
+    int off, total;            // this lane's first entry in the wave's entries; their number
+    int ent[kChunkRounds];     // entry k = r * 64 + lane of the first pass (0 past the end)
+};
+
+// Entry k of the pass starting at `base`: its owner lane (from the LDS map)
+// and its position in the contact list.
+__device__ __forceinline__ int owner_entry(int k, bool ok, int e0, int off, const unsigned char *own, int &L)
+{
+    L = ok ? own[k] : 0;
+    return __shfl(e0, L, 64) + (k - __shfl(off, L, 64));
+}
+
+// Marks in `own` the entries of [base, base + kWaveChunk) that this lane owns.
+__device__ __forceinline__ void mark_owner(int off, int cnt, int base, unsigned char *own)
+{
+    const int lane = threadIdx.x & 63;
+    const int k0 = off > base ? off : base;
+    const int k1 = (off + cnt < base + kWaveChunk) ? off + cnt : base + kWaveChunk;
+    for (int k = k0; k < k1; ++k) own[k - base] = (unsigned char)lane;
+}
+
+// Starts a wave's contacts (every lane of the wave calls it; lane L's bead
+// has list entries [e0, e0 + cnt), cnt 0 for lanes without a bead): the
+// counts are scanned over the wave, the first pass's owner map goes to LDS
+// and its list entries are loaded, so that their latency passes while the
+// caller does other work.
+template <typename T, typename P>
+__device__ __forceinline__ ContactAhead contacts_ahead(const Args<T, P> &a, int e0, int cnt, WaveContactLds<T> &w)
 {
     const int lane = threadIdx.x & 63;
     int incl = cnt;                                   // inclusive scan of the counts over the wave
@@ -456,47 +489,125 @@ __device__ __forceinline__ void wave_contacts(const Args<T, P> &a, int e0, int c
         const int y = __shfl_up(incl, d, 64);
         if (lane >= d) incl += y;
     }
-    const int off = incl - cnt;
-    const int total = __shfl(incl, 63, 64);
-    for (int base = 0; base < total; base += kWaveChunk) {
-#pragma unroll 2
-        for (int r = 0; r < kWaveChunk; r += 64) {
-            const int k = base + r + lane;
-            int L = 0;                                // lane whose entries hold k: largest L with off_L <= k
+    ContactAhead c;
+    c.off = incl - cnt;
+    c.total = __shfl(incl, 63, 64);
+    mark_owner(c.off, cnt, 0, w.own);
+    wave_sync_lds();
+    const int n0 = c.total < kWaveChunk ? c.total : kWaveChunk;
 #pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const int cand = L + step;
-                const int oc = __shfl(off, cand & 63, 64);
-                if (cand < 64 && oc <= k) L = cand;
-            }
-            const int eL = __shfl(e0, L, 64) + (k - __shfl(off, L, 64));
-            const P xL = __shfl(x1, L, 64), yL = __shfl(y1, L, 64), zL = __shfl(z1, L, 64);
-            const T rL = __shfl(r1, L, 64);
-            V3<T> t{(T)-0.0, (T)-0.0, (T)-0.0};
-            if (ENS) {
-                const Box &bk = a.boxes[__shfl(netw, L, 64)];
-                if (k < total && !contact_term(a, bk, xL, yL, zL, rL, a.nb_list[eL], a.pairs + bk.pair_base, t))
-                    t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
-            } else {
-                if (k < total && !contact_term(a, a.force, xL, yL, zL, rL, a.nb_list[eL], table, t))
-                    t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
-            }
-            term[r + lane] = t;
+    for (int r = 0; r < kChunkRounds; ++r) {
+        const int k = r * 64 + lane;
+        int L;
+        const int e = owner_entry(k, k < n0, e0, c.off, w.own, L);
+        c.ent[r] = k < n0 ? a.nb_list[e] : 0;
+    }
+    return c;
+}
+
+// The contact term of entry `ent` for bead 1 at x1.. with radius r1 and the
+// partner at xj..: ResolveContacts' minimum image (kernel.cl:844-870), then
+// contact_force.  The same operations as contact_term.
+template <typename T, typename P>
+__device__ __forceinline__ bool contact_term_at(const Args<T, P> &a, const Box &b, P x1, P y1, P z1, T r1, int ent,
+                                                P xj, P yj, P zj, const PairC *table, V3<T> &t)
+{
+    const int j = ent & kIdxMask;
+    const PairC pr = table[ent >> kPidShift];
+    P rx = x1 - xj, ry = y1 - yj, rz = z1 - zj;
+    P len = rx * rx + ry * ry + rz * rz;
+    if (len > (P)b.hbs) {
+        if (rx * rx > (P)b.hbs) rx += (rx > (P)0) ? (P)-b.lx : (P)b.lx;
+        if (ry * ry > (P)b.hbs) ry += (ry > (P)0) ? (P)-b.ly : (P)b.ly;
+        if (rz * rz > (P)b.hbs) rz += (rz > (P)0) ? (P)-b.lz : (P)b.lz;
+        len = rx * rx + ry * ry + rz * rz;
+    }
+    return contact_force(pr, b, rx, ry, rz, len, r1, a.radius, j, t);
+}
+
+// One pass of a wave's contacts: entries [base, base + n) with their list
+// entries ent[r] at k = base + r * 64 + lane and owners in w.own; the terms
+// go to w.term (an entry without force stores -0.0), and each lane adds its
+// own entries in list order.  Every partner position of the pass is loaded
+// before the terms are evaluated.
+template <typename T, typename P, bool ENS>
+__device__ __forceinline__ void contacts_pass(const Args<T, P> &a, const int (&ent)[kChunkRounds], int base, int n,
+                                              int off, int cnt, T r1, P x1, P y1, P z1, V3<T> &f,
+                                              const PairC *table, WaveContactLds<T> &w, int netw)
+{
+    const int lane = threadIdx.x & 63;
+    P xj[kChunkRounds], yj[kChunkRounds], zj[kChunkRounds];
+#pragma unroll
+    for (int r = 0; r < kChunkRounds; ++r) {
+        xj[r] = yj[r] = zj[r] = (P)0;
+        if (r * 64 + lane < n) {
+            const int j = ent[r] & kIdxMask;
+            xj[r] = a.px[j];
+            yj[r] = a.py[j];
+            zj[r] = a.pz[j];
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int k0 = off > base ? off : base;
-        const int k1 = (off + cnt < base + kWaveChunk) ? off + cnt : base + kWaveChunk;
-        for (int k = k0; k < k1; ++k) {
-            const V3<T> t = term[k - base];
-            f.x += t.x;
-            f.y += t.y;
-            f.z += t.z;
+    }
+#pragma unroll
+    for (int r = 0; r < kChunkRounds; ++r) {
+        if (r * 64 >= n) break;                       // wave-uniform
+        const int k = r * 64 + lane;
+        const int L = k < n ? w.own[k] : 0;
+        const P xL = __shfl(x1, L, 64), yL = __shfl(y1, L, 64), zL = __shfl(z1, L, 64);
+        const T rL = __shfl(r1, L, 64);
+        V3<T> t{(T)-0.0, (T)-0.0, (T)-0.0};
+        if (ENS) {
+            const Box &bk = a.boxes[__shfl(netw, L, 64)];
+            if (k < n && !contact_term_at(a, bk, xL, yL, zL, rL, ent[r], xj[r], yj[r], zj[r],
+                                          a.pairs + bk.pair_base, t))
+                t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
+        } else {
+            if (k < n && !contact_term_at(a, a.force, xL, yL, zL, rL, ent[r], xj[r], yj[r], zj[r], table, t))
+                t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        w.term[k] = t;
+    }
+    wave_sync_lds();
+    const int k0 = off > base ? off : base;
+    const int k1 = (off + cnt < base + n) ? off + cnt : base + n;
+    for (int k = k0; k < k1; ++k) {
+        const V3<T> t = w.term[k - base];
+        f.x += t.x;
+        f.y += t.y;
+        f.z += t.z;
+    }
+    wave_sync_lds();
+}
+
+// Finishes a wave's contacts started by contacts_ahead (every lane calls
+// it): the first pass with the entries loaded ahead, then further passes of
+// kWaveChunk entries for a wave with more.  The wave's lists are laid end to
+// end and each lane evaluates every 64th entry, so a wave costs (its total
+// entries / 64) rounds instead of its longest list; each bead then adds its
+// own terms in list order, which equals the per-bead loop (gather_contacts)
+// bit for bit.  ENS: the lane's bead belongs to network `netw` (its Box and
+// pair table); otherwise a.force and `table`.
+template <typename T, typename P, bool ENS>
+__device__ __forceinline__ void contacts_finish(const Args<T, P> &a, const ContactAhead &c, int e0, int cnt, T r1,
+                                                P x1, P y1, P z1, V3<T> &f, const PairC *table,
+                                                WaveContactLds<T> &w, int netw = 0)
+{
+    const int lane = threadIdx.x & 63;
+    if (c.total == 0) return;                         // wave-uniform
+    contacts_pass<T, P, ENS>(a, c.ent, 0, c.total < kWaveChunk ? c.total : kWaveChunk, c.off, cnt, r1, x1, y1, z1,
+                             f, table, w, netw);
+    for (int base = kWaveChunk; base < c.total; base += kWaveChunk) {
+        const int n = (c.total - base) < kWaveChunk ? c.total - base : kWaveChunk;
+        mark_owner(c.off, cnt, base, w.own);
+        wave_sync_lds();
+        int ent[kChunkRounds];
+#pragma unroll
+        for (int r = 0; r < kChunkRounds; ++r) {
+            const int k = r * 64 + lane;
+            int L;
+            const int e = owner_entry(k, k < n, e0, c.off - base, w.own, L);
+            ent[r] = k < n ? a.nb_list[e] : 0;
+        }
+        contacts_pass<T, P, ENS>(a, ent, base, n, c.off, cnt, r1, x1, y1, z1, f, table, w, netw);
     }
 }
 
