@@ -149,6 +149,123 @@ def adjoint_bench(eng, net, steps, plain_ms, every):
             "d_contact_scale": [float(x) for x in g["pair"]]}
 
 
+def tensile_fit_bench(args):
+    """The differentiated tensile fit: 5,000 beads (100 bent fibres of 50
+    beads, finite box 100x100x20), HR springs, HR angles, penalty
+    contacts, clamp x < 5 and pull x > 95 at +0.01, 2,000 steps at dt 1e-3,
+    the clamp's reaction force sampled at 20 evenly spaced strains; loss =
+    squared error against the forward model's samples at hidden stiffnesses
+    (spring 120, angle 8, contact 900 against 100, 10, 1000).  Measures the
+    adjoint's cost against the plain forward, checks its gradient against
+    central differences (h = 1e-3) of the fp64 engine and, as the CPU leg,
+    against the fp64 dual-number reference on the host, then fits the three
+    stiffnesses by gradient steps."""
+    import numpy as np
+    import torch
+    from fibrenet_amd import Engine, bind_standard, generate as G
+    from fibrenet_amd.autograd import TensileRun, tensile_samples
+
+    net = G.tensile_fit()
+    n, guf, every = net.run_steps, net.grid_update_freq, 100
+    group = (np.asarray(net.info["type"]) == 1).astype(np.int32)
+    steps = (np.arange(1, 21) * (n // 20) - 1).astype(np.int32)
+    hidden = [1.2, 0.8, 0.9]
+    base = [1.0, 1.0, 1.0]
+
+    # plain forward: the captured-graph path over the same steps
+    eng = Engine(args.precision)
+    ids = bind_standard(eng, net)
+    eng.run_steps(n, guf, True, True)
+    eng.write_buffer(ids["bead"], net.beads)
+    eng.write_buffer(ids["spring"], net.springs)
+    eng.write_buffer(ids["angle"], net.angles)
+    eng.finish()
+    t = time.perf_counter()
+    eng.run_steps(n, guf, True, True)
+    eng.finish()
+    plain_ms = (time.perf_counter() - t) * 1e3 / n
+    eng.close()
+
+    run = TensileRun(net, args.precision, group, steps, n, guf, every)
+    target = run.forward(hidden)
+    run.forward(base)
+    run.backward(np.ones(len(steps)))                      # warm
+    t = time.perf_counter()
+    s0 = run.forward(base)
+    t1 = time.perf_counter()
+    g = run.backward(2.0 * (s0 - target))
+    t2 = time.perf_counter()
+    fwd_ms, bwd_ms = (t1 - t) * 1e3 / n, (t2 - t1) * 1e3 / n
+    grad = np.array([g["spring"].sum(), g["angle"].sum(), g["pair"].sum()])
+    loss0 = float(np.sum((s0 - target) ** 2))
+
+    # central differences of the fp64 engine's loss, and its own adjoint
+    run64 = TensileRun(net, "f64", group, steps, n, guf, every)
+    target64 = run64.forward(hidden)
+    s64 = run64.forward(base)
+    g64 = run64.backward(2.0 * (s64 - target64))
+    grad64 = np.array([g64["spring"].sum(), g64["angle"].sum(), g64["pair"].sum()])
+    h = 1e-3
+    fd = []
+    for k in range(3):
+        sp, sm = list(base), list(base)
+        sp[k], sm[k] = 1 + h, 1 - h
+        lp = np.sum((run64.forward(sp) - target64) ** 2)
+        lm = np.sum((run64.forward(sm) - target64) ** 2)
+        fd.append((lp - lm) / (2 * h))
+    fd = np.array(fd)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        # the CPU leg: the fp64 dual-number reference on the host, one run
+        # with the three scale derivatives carried along
+        from oracle.dual_reference import DualRun
+        N = len(net.beads)
+        t = time.perf_counter()
+        d = DualRun(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, np.zeros((N, 3)),
+                    np.zeros((N, 3)))
+        vals, ders = d.run(n, guf, group, steps)
+        cpu_s = time.perf_counter() - t
+        gref = ders[:3] @ (2.0 * (vals - target64))
+        rel = lambda a, b: (np.abs(a - b) / np.maximum(np.abs(b), 1e-300)).tolist()
+        cpu = {"value": N * n / cpu_s, "unit": "bead-steps/s (fp64 forward-mode, 5 directions)", "cores": 1,
+               "kind": "port", "sample": "the whole workload: %d beads, %d steps, %.1f s" % (N, n, cpu_s),
+               "gradient": gref.tolist(), "rel_err_f64_adjoint": rel(grad64, gref),
+               "rel_err_%s_adjoint" % args.precision: rel(grad, gref),
+               "samples_rel_err": float(np.max(np.abs(vals - s64)) / np.max(np.abs(vals)))}
+
+    # fit the three stiffnesses by gradient steps on their logarithms
+    x = [torch.tensor(0.0, dtype=torch.float64, requires_grad=True) for _ in range(3)]
+    opt = torch.optim.Adam(x, lr=0.05)
+    tgt = torch.as_tensor(target)
+    fit = []
+    for it in range(args.fit_iterations):
+        opt.zero_grad()
+        scales = [torch.exp(v) for v in x]
+        loss = ((tensile_samples(run, *scales) - tgt) ** 2).sum()
+        loss.backward()
+        opt.step()
+        fit.append({"iteration": it, "loss": float(loss.detach()),
+                    "stiffness": [100.0 * float(torch.exp(x[0].detach())), 10.0 * float(torch.exp(x[1].detach())),
+                                  1000.0 * float(torch.exp(x[2].detach()))]})
+    out = {"metric": "adjoint/forward cost ratio", "value": bwd_ms / plain_ms, "unit": "x", "n_gpus": 1,
+           "steps": n, "warmup": n, "ms_per_step": bwd_ms, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "%s state, f64 gradients" % args.precision,
+           "data": "synthetic (generated bent-fibre network, seed %d)" % net.meta["seed"],
+           "config": {"workload": "tensile_fit: 100 bent fibres x 50 beads (%d beads), finite box 100x100x20, "
+                                  "HR springs k 100, HR angles k 10, penalty contacts k 1000, clamp x<5, "
+                                  "pull x>95 at +0.01, dt 1e-3, 2000 steps, checkpoint every 100, 20 samples"
+                                  % len(net.beads), "beads": len(net.beads)},
+           "cpu_baseline": cpu,
+           "extra": {"plain_forward_ms_per_step": plain_ms, "checkpointed_forward_ms_per_step": fwd_ms,
+                     "backward_ms_per_step": bwd_ms, "total_over_forward": (fwd_ms + bwd_ms) / plain_ms,
+                     "loss": loss0, "gradient": grad.tolist(), "gradient_f64": grad64.tolist(),
+                     "central_differences_f64": fd.tolist(),
+                     "fd_rel_err_f64": (np.abs(fd - grad64) / np.abs(grad64)).tolist(),
+                     "hidden_stiffness": [120.0, 8.0, 900.0], "fit": fit}}
+    print(json.dumps(out))
+
+
 def cpu_baseline(net, steps):
     from oracle.oracle import Oracle
     orc = Oracle(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, net.cell_nh)
@@ -164,7 +281,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--workload", default="tensile_lj", choices=["tensile_lj", "periodic_hs", "ensemble"])
+    ap.add_argument("--workload", default="tensile_lj",
+                    choices=["tensile_lj", "periodic_hs", "ensemble", "tensile_fit"])
+    ap.add_argument("--fit-iterations", type=int, default=30, help="tensile_fit: gradient steps of the fit")
     ap.add_argument("--networks", type=int, default=512, help="ensemble: number of networks")
     ap.add_argument("--loop-steps", type=int, default=40, help="ensemble: steps of the looped plain engine")
     ap.add_argument("--ensemble-networks", type=int, default=128,
@@ -188,6 +307,10 @@ def main():
         sys.exit(2)
 
     from fibrenet_amd import Engine, bind_standard, generate as G
+
+    if args.workload == "tensile_fit":
+        tensile_fit_bench(args)
+        return
 
     if args.workload == "ensemble":
         e = ensemble_bench(args.networks, args.steps, args.warmup, args.loop_steps, args.precision)

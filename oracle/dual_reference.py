@@ -110,8 +110,9 @@ def darccos(a):
 
 
 def scatter_add(target, rows, val):
-    np.add.at(target.v, rows, val.v)
-    np.add.at(target.d, (slice(None), rows), val.d)
+    n = len(target.v)
+    target.v = target.v + np.bincount(rows, weights=val.v, minlength=n)
+    target.d = target.d + np.stack([np.bincount(rows, weights=val.d[m], minlength=n) for m in range(len(val.d))])
 
 
 def _cross(a, b):
@@ -186,7 +187,7 @@ class DualRun:
         hashed = np.all((cell >= 0) & (cell < n), axis=1)
         idx = np.nonzero(hashed)[0]
         ci = cell[idx]
-        i, j = np.triu_indices(len(idx), 1)
+        i, j = self._candidates(px[idx])
         dc = np.abs(ci[i] - ci[j])
         if int(s["bc_type"]) == 0:
             near = np.all((dc <= 1) | (dc == n - 1), axis=1)
@@ -218,6 +219,24 @@ class DualRun:
         self.list_i = np.concatenate([i, j])
         self.list_j = np.concatenate([j, i])
         self.list_t = np.concatenate([t, t])
+
+    def _candidates(self, p):
+        """Index pairs (i < j) of p that may be listed: every pair for a few
+        beads, else every pair within the largest listing reach (a k-d tree
+        query; the listing rule is applied after)."""
+        if len(p) <= 2000:
+            return np.triu_indices(len(p), 1)
+        from scipy.spatial import cKDTree
+        s = self.s
+        skin = np.asarray(self.info["skin_radius"], np.float64)
+        reach = max(float(np.sqrt(s["cutoff_sq"])), float(self.rad.max() + skin.max()))
+        if int(s["bc_type"]) == 0:
+            L = np.array([s["box_size"][c] for c in "xyz"], np.float64)
+            pairs = cKDTree(np.mod(p, L), boxsize=L).query_pairs(reach * (1 + 1e-9), output_type="ndarray")
+        else:
+            pairs = cKDTree(p).query_pairs(reach * (1 + 1e-9), output_type="ndarray")
+        pairs = np.sort(pairs, axis=1)
+        return pairs[:, 0], pairs[:, 1]
 
     # -- one step ----------------------------------------------------------------------------------
     def step(self):

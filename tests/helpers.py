@@ -84,54 +84,14 @@ def field_errors(beads, ref):
 
 
 def adjoint_net(seed=21, n_fibres=40, n_beads=12, box=(24.0, 24.0, 8.0), jitter=0.15, pull_v=0.5, hr_fl=False):
-    """A small differentiable tensile case: bent fibres (straight chains with
-    a random transverse jitter, so every angle is well away from straight)
-    in a finite box with walls, damping and gravity; HR springs (or HR_FL
-    with failure), HR angles, penalty contacts between three bead types;
-    type 1 clamped (mass 0) at x < 3, type 2 pulled at +x (mass 0)."""
-    net = G.Network()
-    s = G.make_settings(box, 1, 2.0, 1.5, 2e-3, 0.5, wall_k=100.0, damping=0.5, gravity=(0.0, 0.0, -0.2), types=3)
-    xyz, _ = G.place_fibres(seed, n_fibres, n_beads, (box[0], box[1], box[2]), finite=True, d_min=0.9)
-    rs = np.random.Generator(np.random.PCG64(seed))
-    pos = xyz.reshape(-1, 3) + rs.uniform(-jitter, jitter, size=(n_fibres * n_beads, 3))
-    L = np.array(box)
-    pos = np.clip(pos, 0.05, L - 0.05)
-    N = len(pos)
-    net.beads = np.zeros(N, R.BEAD)
-    R.set_vec(net.beads, "position", pos)
-    R.set_vec(net.beads, "image_position", pos)
-    lo = pos[:, 0] < 3.0
-    hi = pos[:, 0] > box[0] - 3.0
-    typ = np.where(lo, 1, np.where(hi, 2, 0)).astype(np.int32)
-    net.info = np.zeros(N, R.BEAD_INFO)
-    net.info["type"] = typ
-    net.info["tag"] = np.repeat(np.arange(n_fibres), n_beads)
-    net.info["loc_id"] = np.tile(np.arange(n_beads), n_fibres)
-    net.info["radius"] = 0.5
-    net.info["radius_sq"] = 0.25
-    net.info["skin_radius"] = 1.0
-    net.info["skin_radius_sq"] = 1.0
-    G.mass_coeffs(net.info, s, np.where(typ > 0, 0.0, 1.0))
-    v = np.zeros((N, 3))
-    v[hi, 0] = pull_v
-    R.set_vec(net.beads, "velocity", v)
-    R.set_vec(net.beads, "velocity_old", v)
-    if hr_fl:
-        sigma_f = rs.uniform(20.0, 60.0, size=n_fibres * (n_beads - 1))
-        net.springs, net.angles = G._chain_bonds(n_fibres, n_beads, 1, 100.0, 1.0, e_mod=1000.0, sigma_f=sigma_f,
-                                                 angle_k=10.0)
-    else:
-        net.springs, net.angles = G._chain_bonds(n_fibres, n_beads, 0, 100.0, 1.0, angle_k=10.0)
-    T = 3
-    net.pairs = np.zeros(T * (T + 1) // 2, R.PAIR)
-    for a in range(T):
-        for b in range(a, T):
-            net.pairs[G.pair_index(a, b, T)] = G.hs_pair(a, b, 1000.0)
-    net.settings = s
-    net.cell_nh = G.grid_neighbours(1, int(s["cells_x"]), int(s["cells_y"]), int(s["cells_z"]))
-    net.grid_update_freq = 10
-    net.run_steps = 300
-    return net
+    """A small differentiable tensile case (generate.tensile_fit at 480
+    beads): bent fibres in a finite box with walls, damping and gravity;
+    HR springs (or HR_FL with failure stress in [20, 60]), HR angles,
+    penalty contacts between three bead types; type 1 clamped at x < 3,
+    type 2 pulled at +x."""
+    return G.tensile_fit(n_fibres=n_fibres, n_beads=n_beads, box=box, seed=seed, dt=2e-3, steps=300, jitter=jitter,
+                         clamp_x=3.0, pull_v=pull_v, hr_fl=hr_fl, sigma_f=(20.0, 60.0), damping=0.5,
+                         gravity=(0.0, 0.0, -0.2), d_min=0.9)
 
 
 def dual_run(net, u_pos=None, u_vel=None, round_lengths=True, scales=(1.0, 1.0, 1.0)):

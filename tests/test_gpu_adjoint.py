@@ -178,3 +178,23 @@ def test_torch_function(need_gpu):
     x2 = [float(a.detach()) - step * float(b) for a, b in zip(x, grads)]
     loss2 = float(((torch.as_tensor(run.forward(x2)) - target) ** 2).sum())
     assert loss2 < float(loss.detach())
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-7), ("f32", 1e-3)])
+def test_tensile_fit_network_gradients(need_gpu, prec, tol):
+    """The differentiated tensile fit's 5,000-bead network (300 of its 2,000
+    steps): the loss gradient with respect to the three stiffness scales
+    against the fp64 dual-number reference."""
+    from fibrenet_amd import generate as G
+    net = G.tensile_fit()
+    n = 300
+    steps = np.arange(14, n, 15)
+    group = _group(net)
+    run = TensileRun(net, prec, group, steps, n, net.grid_update_freq, checkpoint_every=100)
+    target = run.forward([1.2, 0.8, 0.9])
+    s = run.forward([1.0, 1.0, 1.0])
+    g = run.backward(2.0 * (s - target))
+    got = np.array([g["spring"].sum(), g["angle"].sum(), g["pair"].sum()])
+    vals, ders = dual_run(net).run(n, net.grid_update_freq, group, steps)
+    ref = ders[:3] @ (2.0 * (vals - target))
+    assert np.all(np.abs(got - ref) <= tol * np.abs(ref)), (got, ref)
