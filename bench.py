@@ -266,6 +266,103 @@ def tensile_fit_bench(args):
     print(json.dumps(out))
 
 
+def batched_fit_bench(args):
+    """The batched adjoint: 128 tensile networks of uneven size (fibre counts
+    uniform in [50, 300], lengths in [30, 60] beads, HR_FL springs with
+    failure stress uniform in [50, 150]), 1,000 steps each, the clamp's
+    reaction force sampled at 20 strains per network; per-network loss =
+    squared error against the forward model's samples at a hidden spring
+    stiffness per network (log-uniform in [50, 200]).  Measures the batched
+    adjoint's cost against the batched forward, compares the fp32 gradient
+    with the fp64 engine's, then fits every network's spring stiffness by
+    gradient steps (Adam on its logarithm)."""
+    import numpy as np
+    from fibrenet_amd import generate as G
+    from fibrenet_amd.autograd import BatchedTensileRun
+
+    nets = G.tensile_batch(args.fit_networks)
+    nn = len(nets)
+    n, guf, every = nets[0].run_steps, nets[0].grid_update_freq, 100
+    steps = (np.arange(1, 21) * (n // 20) - 1).astype(np.int32)
+    k_hidden = np.array([net.meta["hidden_spring_k"] for net in nets])
+    base = np.ones((nn, 3))
+    hidden = base.copy()
+    hidden[:, 0] = k_hidden / 100.0
+    beads = sum(len(net.beads) for net in nets)
+
+    run = BatchedTensileRun(nets, args.precision, steps, n, guf, every)
+    target = run.forward(hidden)
+    # the plain batched forward (captured graphs) from the same records
+    run.ens.reload(run._scaled(base))
+    run.ens.run_steps(n, guf, True, True)
+    run.ens.reload(run._scaled(base))
+    run.ens.finish()
+    t = time.perf_counter()
+    run.ens.run_steps(n, guf, True, True)
+    run.ens.finish()
+    plain_ms = (time.perf_counter() - t) * 1e3 / n
+    s0 = run.forward(base)
+    run.ens.finish()
+    t = time.perf_counter()
+    s0 = run.ens.adjoint_forward(n, run.group, steps, guf, True, True, every)   # records already in place
+    t1 = time.perf_counter()
+    g0 = run.backward(2.0 * (s0 - target))
+    t2 = time.perf_counter()
+    fwd_ms, bwd_ms = (t1 - t) * 1e3 / n, (t2 - t1) * 1e3 / n
+
+    # the fp64 engine's gradient at the same point (its own fp64 targets)
+    run64 = BatchedTensileRun(nets, "f64", steps, n, guf, every)
+    target64 = run64.forward(hidden)
+    s64 = run64.forward(base)
+    g64 = run64.backward(2.0 * (s64 - target64))
+    run64.ens.close()
+    rel = np.abs(g0[:, 0] - g64[:, 0]) / np.maximum(np.abs(g64[:, 0]), 1e-300)
+
+    # fit: Adam on x = log(spring scale), one parameter per network
+    x = np.zeros(nn)
+    m = np.zeros(nn)
+    v = np.zeros(nn)
+    lr, b1, b2 = 0.05, 0.9, 0.999
+    fit = []
+    t = time.perf_counter()
+    for it in range(args.fit_iterations):
+        sc = base.copy()
+        sc[:, 0] = np.exp(x)
+        s = run.forward(sc)
+        r = s - target
+        loss = np.sum(r * r, axis=1)
+        g = run.backward(2.0 * r)[:, 0]                    # dL_k / d log(spring scale of k)
+        m = b1 * m + (1 - b1) * g
+        v = b2 * v + (1 - b2) * g * g
+        x -= lr * (m / (1 - b1 ** (it + 1))) / (np.sqrt(v / (1 - b2 ** (it + 1))) + 1e-12)
+        if it % 10 == 0 or it == args.fit_iterations - 1:
+            err = np.abs(100.0 * np.exp(x) - k_hidden) / k_hidden
+            fit.append({"iteration": it, "loss": float(loss.sum()), "median_rel_err": float(np.median(err)),
+                        "max_rel_err": float(err.max())})
+            print("batched fit: iteration %d, loss %.4g, median stiffness error %.3g"
+                  % (it, loss.sum(), np.median(err)), file=sys.stderr, flush=True)
+    fit_s = time.perf_counter() - t
+    k_fit = 100.0 * np.exp(x)
+    err = np.abs(k_fit - k_hidden) / k_hidden
+    out = {"metric": "adjoint/forward cost ratio (batched)", "value": bwd_ms / plain_ms, "unit": "x", "n_gpus": 1,
+           "steps": n, "warmup": n, "ms_per_step": bwd_ms, "higher_is_better": False, "scaling": "weak",
+           "vs_baseline": None, "dtype": "%s state, f64 gradients" % args.precision,
+           "data": "synthetic (generated bent-fibre networks)",
+           "config": {"workload": "batched_fit: %d tensile networks (%d beads), fibres U[50,300] x beads U[30,60], "
+                                  "HR_FL springs, penalty contacts, clamp/pull 0.01, dt 1e-3, 1000 steps, "
+                                  "checkpoint every 100, 20 samples per network, hidden spring k log-uniform "
+                                  "[50,200]" % (nn, beads), "networks": nn, "beads": beads},
+           "cpu_baseline": None,
+           "extra": {"plain_forward_ms_per_step": plain_ms, "checkpointed_forward_ms_per_step": fwd_ms,
+                     "backward_ms_per_step": bwd_ms, "total_over_forward": (fwd_ms + bwd_ms) / plain_ms,
+                     "gradient_f32_vs_f64_rel_err": {"median": float(np.median(rel)), "max": float(rel.max())},
+                     "fit_iterations": args.fit_iterations, "fit_seconds": fit_s, "fit": fit,
+                     "stiffness_rel_err": {"median": float(np.median(err)), "p90": float(np.quantile(err, 0.9)),
+                                           "max": float(err.max()), "within_1pct": int(np.sum(err < 0.01)),
+                                           "within_5pct": int(np.sum(err < 0.05))}}}
+    print(json.dumps(out))
+
+
 def cpu_baseline(net, steps):
     from oracle.oracle import Oracle
     orc = Oracle(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, net.cell_nh)
@@ -282,8 +379,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--workload", default="tensile_lj",
-                    choices=["tensile_lj", "periodic_hs", "ensemble", "tensile_fit"])
-    ap.add_argument("--fit-iterations", type=int, default=30, help="tensile_fit: gradient steps of the fit")
+                    choices=["tensile_lj", "periodic_hs", "ensemble", "tensile_fit", "batched_fit"])
+    ap.add_argument("--fit-iterations", type=int, default=None,
+                    help="tensile_fit / batched_fit: gradient steps of the fit (30 / 200)")
+    ap.add_argument("--fit-networks", type=int, default=128, help="batched_fit: networks")
     ap.add_argument("--networks", type=int, default=512, help="ensemble: number of networks")
     ap.add_argument("--loop-steps", type=int, default=40, help="ensemble: steps of the looped plain engine")
     ap.add_argument("--ensemble-networks", type=int, default=128,
@@ -309,7 +408,12 @@ def main():
     from fibrenet_amd import Engine, bind_standard, generate as G
 
     if args.workload == "tensile_fit":
+        args.fit_iterations = 30 if args.fit_iterations is None else args.fit_iterations
         tensile_fit_bench(args)
+        return
+    if args.workload == "batched_fit":
+        args.fit_iterations = 200 if args.fit_iterations is None else args.fit_iterations
+        batched_fit_bench(args)
         return
 
     if args.workload == "ensemble":

@@ -50,6 +50,7 @@ SIGNATURES = [
     ("fn_copy_probe", _I, [_VP, _c.c_longlong, _VP]),
     ("fn_batch_create", _I, [_I, _I, _I, _VP, _VP]),
     ("fn_batch_read", _I, [_VP, _I, _I, _VP]),
+    ("fn_batch_load", _I, [_VP, _I, _VP]),
     ("fn_autotune", _I, [_VP, _I, _I, _I, _I, _VP, _I, _c.POINTER(_I)]),
     ("fn_adjoint_forward", _I, [_VP, _VP, _VP, _VP, _I, _VP]),
     ("fn_adjoint_backward", _I, [_VP, _VP, _VP, _VP, _VP, _VP, _VP]),
@@ -187,6 +188,9 @@ class Engine:
         self.h = ctypes.c_void_p(handle)
         self.precision = precision
 
+    n_networks = 1          # an Ensemble holds several
+    batched = False
+
     def close(self):
         if self._owned and self.h:
             lib().fn_destroy(self.h)
@@ -323,18 +327,20 @@ class Engine:
         `sample_steps` (strictly increasing, in [0, n))."""
         g = np.ascontiguousarray(group, np.int32)
         st = np.ascontiguousarray(sample_steps, np.int32)
-        out = np.zeros(max(len(st), 1), np.float64)
+        nn = self.n_networks
+        out = np.zeros(max(len(st) * nn, 1), np.float64)
         spec = FnAdjointSpec(int(n), int(grid_update_freq), int(bool(springs)), int(bool(angles)),
                              int(checkpoint_every))
         self._chk(lib().fn_adjoint_forward(self.h, ctypes.byref(spec), _ptr(g), _ptr(st), len(st), _ptr(out)))
         self._adj_beads = len(g)
-        return out[:len(st)]
+        out = out[:len(st) * nn]
+        return out.reshape(nn, len(st)) if self.batched else out
 
     def adjoint_backward(self, dloss, n_springs, n_angles, n_pairs):
         """Derivatives of a loss with dL/d(sample) = dloss through the last
         adjoint_forward run: per-bond and per-pair-type force-scale
         derivatives and those of the initial positions and velocity_old."""
-        d = np.ascontiguousarray(dloss, np.float64)
+        d = np.ascontiguousarray(dloss, np.float64).ravel()
         n = getattr(self, "_adj_beads", 0)
         out = {"spring": np.zeros(max(n_springs, 1)), "angle": np.zeros(max(n_angles, 1)),
                "pair": np.zeros(max(n_pairs, 1)), "position": np.zeros((n, 3)), "velocity_old": np.zeros((n, 3))}
@@ -484,10 +490,12 @@ class Ensemble(Engine):
     _KINDS = {"bead": (R.KIND["bead"], R.BEAD), "spring": (R.KIND["spring"], R.SPRING),
               "angle": (R.KIND["angle"], R.ANGLE)}
 
-    def __init__(self, nets, precision="f32", device=0):
-        self._keep = []
+    batched = True
+
+    @staticmethod
+    def _networks(nets):
+        keep, counts = [], []
         arr = (FnNetwork * len(nets))()
-        self.counts = []
         for k, net in enumerate(nets):
             b = np.ascontiguousarray(net.beads, R.BEAD)
             q = np.ascontiguousarray(net.info, R.BEAD_INFO)
@@ -495,11 +503,16 @@ class Ensemble(Engine):
             an = np.ascontiguousarray(net.angles, R.ANGLE)
             pr = np.ascontiguousarray(net.pairs, R.PAIR)
             st = np.array(net.settings, R.SETTINGS).reshape(())
-            self._keep += [b, q, sp, an, pr, st]
+            keep += [b, q, sp, an, pr, st]
             ptr = lambda a: a.ctypes.data if a.size else None
             arr[k] = FnNetwork(ptr(b), ptr(q), len(b), ptr(sp), len(sp), ptr(an), len(an), ptr(pr), len(pr),
                                st.ctypes.data)
-            self.counts.append({"bead": len(b), "spring": len(sp), "angle": len(an)})
+            counts.append({"bead": len(b), "spring": len(sp), "angle": len(an), "pair": len(pr)})
+        return arr, keep, counts
+
+    def __init__(self, nets, precision="f32", device=0):
+        arr, self._keep, self.counts = self._networks(nets)
+        self.n_networks = len(nets)
         h = ctypes.c_void_p()
         L = lib()
         rc = L.fn_batch_create(int(device), PREC[precision], len(nets), arr, ctypes.byref(h))
@@ -510,6 +523,16 @@ class Ensemble(Engine):
             raise FibreNetError(rc, msg)
         super().__init__(precision, device, handle=h.value)
         self._owned = True
+
+    def reload(self, nets):
+        """New records for every network, of the loaded sizes (fn_batch_load)."""
+        arr, keep, _ = self._networks(nets)
+        self._chk(lib().fn_batch_load(self.h, len(nets), arr))
+        del keep
+
+    def offsets(self, kind):
+        """First global index of each network's records of `kind`, and the total."""
+        return np.concatenate([[0], np.cumsum([c[kind] for c in self.counts])]).astype(np.int64)
 
     def read(self, k, kind="bead"):
         code, rec = self._KINDS[kind]

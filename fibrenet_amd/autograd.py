@@ -89,3 +89,51 @@ def tensile_samples(run, spring_scale, angle_scale, contact_scale):
     """Clamp reaction-force samples of `run` as a differentiable function of
     the three scales (0-d float64 tensors)."""
     return _Tensile.apply(run, spring_scale, angle_scale, contact_scale)
+
+
+class BatchedTensileRun:
+    """Independent networks in one batched ensemble, each with its own three
+    force scales; each forward reloads every network's initial records with
+    its scales (fn_batch_load) and runs the adjoint forward for all of them
+    at once; the backward gives every network's scale derivatives."""
+
+    def __init__(self, nets, precision="f32", sample_steps=(), n_steps=None, grid_update_freq=None,
+                 checkpoint_every=0):
+        self.nets = nets
+        self.ens = api.Ensemble(nets, precision)
+        self.group = np.concatenate([np.asarray(n.info["type"]) == 1 for n in nets]).astype(np.int32)
+        self.steps = np.asarray(sample_steps, np.int32)
+        self.n = int(n_steps if n_steps is not None else nets[0].run_steps)
+        self.guf = int(grid_update_freq if grid_update_freq is not None else nets[0].grid_update_freq)
+        self.every = int(checkpoint_every)
+        self.off = {k: self.ens.offsets(k) for k in ("spring", "angle", "pair")}
+
+    def _scaled(self, scales):
+        out = []
+        for net, (ss, sa, sc) in zip(self.nets, np.asarray(scales, np.float64).reshape(len(self.nets), 3)):
+            m = type("Scaled", (), {})()
+            m.beads, m.info, m.settings = net.beads, net.info, net.settings
+            m.springs = net.springs.copy()
+            m.springs["stiffness"] = m.springs["stiffness"] * ss
+            m.angles = net.angles.copy()
+            m.angles["c1"] = m.angles["c1"] * sa
+            m.pairs = scaled_pairs(net.pairs, sc)
+            out.append(m)
+        return out
+
+    def forward(self, scales):
+        """Samples (networks x sample steps) at per-network scales (networks x 3)."""
+        self.ens.reload(self._scaled(scales))
+        return self.ens.adjoint_forward(self.n, self.group, self.steps, self.guf, True, True, self.every)
+
+    def backward(self, dloss):
+        """Per-network derivatives of the loss with respect to each scale at
+        sigma = 1 of the scaled run (networks x 3)."""
+        nn = len(self.nets)
+        g = self.ens.adjoint_backward(np.asarray(dloss, np.float64).reshape(nn, -1), int(self.off["spring"][-1]),
+                                      int(self.off["angle"][-1]), int(self.off["pair"][-1]))
+        out = np.zeros((nn, 3))
+        for k, key in enumerate(("spring", "angle", "pair")):
+            o = self.off[key]
+            out[:, k] = np.add.reduceat(np.concatenate([g[key], [0.0]]), o[:-1])[:nn] * (np.diff(o) > 0)
+        return out

@@ -239,13 +239,14 @@ constexpr int kAdjPairsLds = 8;
 // step (seed and h Wb of that step included) to fb_next, and adds the
 // contacts' scale adjoints: LDSG, per block row (g_pair[block * n_pairs +
 // entry], summed over the block in a fixed order); otherwise per bead row.
-template <typename T, typename P, bool LDSG>
+template <typename T, typename P, bool LDSG, bool ENS>
 __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
 {
-    __shared__ PairC prs[kAdjPairCap];
+    __shared__ PairC prs[ENS ? 1 : kAdjPairCap];
     __shared__ double gl[LDSG ? kAdjPairsLds * kBlock : 1];
     const int np = a.n_pairs;
-    for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
+    if (!ENS)
+        for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
     if (LDSG)
         for (int k = 0; k < np; ++k) gl[k * kBlock + threadIdx.x] = 0.0;
     __syncthreads();
@@ -256,13 +257,14 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
         if (a.springs) add_slots(z, a.sp_jv, a.sp_start[i], a.sp_start[i + 1]);
         if (a.angles) add_slots(z, a.an_jv, a.an_start[i], a.an_start[i + 1]);
         // contacts, from the bead's own side of each listed pair
-        const Box &bf = a.force;
+        const Box &bf = ENS ? a.boxes[a.net[i]] : a.force;
+        const PairC *table = ENS ? a.pairs + bf.pair_base : prs;
         const P xi = a.px[i], yi = a.py[i], zi = a.pz[i];
         const T ri = a.radius[i];
         for (int e = a.nb_start[i], e1 = e + a.nb_cnt[i]; e < e1; ++e) {
             const int ent = a.nb_list[e];
             const int j = ent & kIdxMask, pid = ent >> kPidShift;
-            const PairC &pr = prs[pid];
+            const PairC pr = table[pid];
             P rx = xi - a.px[j], ry = yi - a.py[j], rz = zi - a.pz[j];
             P len = rx * rx + ry * ry + rz * rz;
             if (len > (P)bf.hbs) {               // ResolveContacts' minimum image (kernel.cl:844-870)
@@ -281,7 +283,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
             else a.g_pair[(size_t)i * np + pid] += dot_v(fb, F);
         }
         // walls of a finite box (kernel.cl:252-261): diagonal
-        const Box &bd = a.drift;
+        const Box &bd = ENS ? bf : a.drift;
         if (bd.bc == 1) {
             const double k = -bd.wall_k;
             if (xi < (P)0 || xi > (P)bd.lx) z.x += k * fb.x;
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
         a.xb[i] = X;
         a.wb[i] = W;
         // Fb of the previous step: its own force adjoint, its kick's h Wb and its seed
-        const double seed = (a.group && a.group[i]) ? a.seed_prev : 0.0;
+        const double seed = (a.seed_prev && a.group[i]) ? a.seed_prev[ENS ? a.net[i] : 0] : 0.0;
         a.fb_next[i] = double3{fprev.x + h * W.x + seed, fprev.y + h * W.y, fprev.z + h * W.z};
     }
     if (LDSG) {
@@ -318,7 +320,10 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
     }
 }
 
-int adjoint_pair_rows(int n, int np) { return np <= kAdjPairsLds ? (n + kBlock - 1) / kBlock : n; }
+int adjoint_pair_rows(int n, int np, bool batched)
+{
+    return (!batched && np <= kAdjPairsLds) ? (n + kBlock - 1) / kBlock : n;
+}
 
 // Positions (and images, when ix is not null) and bond statuses of one step
 // into its record: rec_pos = x, y, z[, ix, iy, iz] (n each), rec_st = spring
@@ -356,13 +361,14 @@ void record_step(const double *const p[3], const double *const im[3], const int 
 // Fb of the last step: its seed and h Wb.
 template <typename T>
 __global__ void __launch_bounds__(kBlock) k_adjoint_seed(double3 *fb, const double3 *wb, const T *hlf,
-                                                         const int *group, double seed, int n)
+                                                         const int *group, const double *seed, const int *net, int n)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const double h = (double)hlf[i];
     const double3 W = wb[i];
-    fb[i] = double3{h * W.x + ((group && group[i]) ? seed : 0.0), h * W.y, h * W.z};
+    const double sd = (seed && group[i]) ? seed[net ? net[i] : 0] : 0.0;
+    fb[i] = double3{h * W.x + sd, h * W.y, h * W.z};
 }
 
 // Block sum in a fixed order (tree over the block's threads).
@@ -381,12 +387,13 @@ __device__ __forceinline__ double block_sum(double v)
 // First pass of group_force: block b sums the entries k = b, b + B, ... of
 // its threads' strided share (B = gridDim.x), into part[b].
 template <typename T>
-__global__ void __launch_bounds__(kBlock) k_group_partial(const T *fx, const int *idx, int n, double *part)
+__global__ void __launch_bounds__(kBlock) k_group_partial(const T *fx, const int *idx, const int *off, double *part)
 {
+    const int net = blockIdx.y, k0 = off[net], k1 = off[net + 1];
     double s = 0.0;
-    for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) s += (double)fx[idx[k]];
+    for (int k = k0 + blockIdx.x * kBlock + threadIdx.x; k < k1; k += gridDim.x * kBlock) s += (double)fx[idx[k]];
     const double t = block_sum(s);
-    if (threadIdx.x == 0) part[blockIdx.x] = t;
+    if (threadIdx.x == 0) part[(size_t)net * gridDim.x + blockIdx.x] = t;
 }
 
 // Second pass of both reductions: out[c] = sum over b < nb of part[c * nb + b].
@@ -410,6 +417,24 @@ __global__ void __launch_bounds__(kBlock) k_column_partial(const double *g, int 
 
 static int sum_blocks(int n) { return std::max(1, std::min(kSumBlocks, (n + kBlock - 1) / kBlock)); }
 
+// Block (k, t) of sum_network_columns.
+__global__ void __launch_bounds__(kBlock) k_network_columns(const double *g, int cols, const int *bead_off,
+                                                            const int *pair_base, double *out)
+{
+    const int k = blockIdx.x, t = blockIdx.y;
+    if (t >= pair_base[k + 1] - pair_base[k]) return;          // block-uniform
+    double s = 0.0;
+    for (int i = bead_off[k] + threadIdx.x; i < bead_off[k + 1]; i += kBlock) s += g[(size_t)i * cols + t];
+    const double v = block_sum(s);
+    if (threadIdx.x == 0) out[pair_base[k] + t] = v;
+}
+
+void sum_network_columns(const double *g, int cols, const int *bead_off, const int *pair_base, int n_net,
+                         double *out, hipStream_t s)
+{
+    if (n_net > 0 && cols > 0) k_network_columns<<<dim3(n_net, cols), kBlock, 0, s>>>(g, cols, bead_off, pair_base, out);
+}
+
 void sum_columns(const double *g, int n, int np, double *part, double *out, hipStream_t s)
 {
     if (np <= 0) return;
@@ -424,27 +449,35 @@ void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s)
     if (a.springs > 0) k_adj_springs<T, P><<<(a.springs + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
     if (a.angles > 0) k_adj_angles<T, P><<<(a.angles + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
     if (a.n <= 0) return;
-    if (a.n_pairs <= kAdjPairsLds) k_adjoint_step<T, P, true><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
-    else k_adjoint_step<T, P, false><<<(a.n + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
+    const int nb = (a.n + kBlock - 1) / kBlock;
+    if (a.net) k_adjoint_step<T, P, false, true><<<nb, kBlock, 0, s>>>(a);
+    else if (a.n_pairs <= kAdjPairsLds) k_adjoint_step<T, P, true, false><<<nb, kBlock, 0, s>>>(a);
+    else k_adjoint_step<T, P, false, false><<<nb, kBlock, 0, s>>>(a);
 }
 template <typename T>
-void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, double seed, int n, hipStream_t s)
+void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, const double *seed,
+                  const int *net, int n, hipStream_t s)
 {
-    if (n > 0) k_adjoint_seed<T><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(fb, wb, hlf, group, seed, n);
+    if (n > 0) k_adjoint_seed<T><<<(n + kBlock - 1) / kBlock, kBlock, 0, s>>>(fb, wb, hlf, group, seed, net, n);
 }
 template <typename T>
-void group_force(const T *fx, const int *idx, int n, double *part, double *out, hipStream_t s)
+void group_force(const T *fx, const int *idx, const int *off, int n_net, int max_len, double *part, double *out,
+                 hipStream_t s)
 {
-    const int nb = sum_blocks(n);
-    k_group_partial<T><<<nb, kBlock, 0, s>>>(fx, idx, n, part);
-    k_sum_parts<<<1, kBlock, 0, s>>>(part, nb, out);
+    if (n_net <= 0) return;
+    const int nb = sum_blocks(max_len);
+    k_group_partial<T><<<dim3(nb, n_net), kBlock, 0, s>>>(fx, idx, off, part);
+    k_sum_parts<<<n_net, kBlock, 0, s>>>(part, nb, out);
 }
 
 template void adjoint_step<double, double>(const AdjArgs<double, double> &, hipStream_t);
 template void adjoint_step<float, double>(const AdjArgs<float, double> &, hipStream_t);
-template void adjoint_seed<double>(double3 *, const double3 *, const double *, const int *, double, int, hipStream_t);
-template void adjoint_seed<float>(double3 *, const double3 *, const float *, const int *, double, int, hipStream_t);
-template void group_force<double>(const double *, const int *, int, double *, double *, hipStream_t);
-template void group_force<float>(const float *, const int *, int, double *, double *, hipStream_t);
+template void adjoint_seed<double>(double3 *, const double3 *, const double *, const int *, const double *,
+                                   const int *, int, hipStream_t);
+template void adjoint_seed<float>(double3 *, const double3 *, const float *, const int *, const double *, const int *,
+                                  int, hipStream_t);
+template void group_force<double>(const double *, const int *, const int *, int, int, double *, double *,
+                                  hipStream_t);
+template void group_force<float>(const float *, const int *, const int *, int, int, double *, double *, hipStream_t);
 
 }  // namespace fnk

@@ -217,6 +217,7 @@ struct fn_engine {
     virtual void timing(fn_timing *out) = 0;
     virtual void copy_probe(long long bytes, double *ms) = 0;
     virtual void load_ensemble(const fn_network *nets, int n) = 0;
+    virtual void reload_ensemble(const fn_network *nets, int n) = 0;
     virtual void ensemble_read(int net, int kind, void *out) = 0;
     virtual void autotune(int n, int guf, bool spr, bool ang, std::vector<fn_tune_trial> &trials) = 0;
     virtual void adjoint_forward(const fn_adjoint_spec &spec, const int *group, const int *steps, int n_samples,
@@ -1508,11 +1509,9 @@ class Engine final : public fn_engine {
     // cells are numbered after the previous networks' and whose pair entries
     // start at pair_base.  Contacts never cross networks: a bead only searches
     // its own network's cells.
-    void load_ensemble(const fn_network *nets, int n) override
-    {
-        adj_.valid = false;
-        if (n <= 0 || !nets) throw Error(FN_ERR_ARG, "an ensemble needs at least one network");
-        if (!bufs_.empty()) throw Error(FN_ERR_ARG, "an ensemble is loaded into an engine without buffers");
+    // The networks' records concatenated (bond indices shifted) with their
+    // Boxes, after the checks of fn_batch_create.
+    struct Packed {
         std::vector<fn_bead> beads;
         std::vector<fn_bead_info> info;
         std::vector<fn_spring> sp;
@@ -1524,6 +1523,25 @@ class Engine final : public fn_engine {
         std::vector<int> b0{0}, s0{0}, a0{0};
         long long cells = 0;
         bool finite = true;
+    };
+
+    Packed pack_ensemble(const fn_network *nets, int n)
+    {
+        if (n <= 0 || !nets) throw Error(FN_ERR_ARG, "an ensemble needs at least one network");
+        Packed pk;
+        auto &beads = pk.beads;
+        auto &info = pk.info;
+        auto &sp = pk.sp;
+        auto &an = pk.an;
+        auto &pr = pk.pr;
+        auto &boxes = pk.boxes;
+        auto &net = pk.net;
+        auto &sets = pk.sets;
+        auto &b0 = pk.b0;
+        auto &s0 = pk.s0;
+        auto &a0 = pk.a0;
+        long long &cells = pk.cells;
+        bool &finite = pk.finite;
         for (int k = 0; k < n; ++k) {
             const fn_network &w = nets[k];
             if (w.n_beads <= 0 || !w.beads || !w.info || !w.settings || w.n_pairs <= 0 || !w.pairs)
@@ -1578,24 +1596,57 @@ class Engine final : public fn_engine {
             s0.push_back((int)sp.size());
             a0.push_back((int)an.size());
         }
-        create_buffer(FN_BUF_BEAD, sizeof(fn_bead), (int)beads.size(), beads.data());
-        create_buffer(FN_BUF_BEAD_INFO, sizeof(fn_bead_info), (int)info.size(), info.data());
-        if (!sp.empty()) create_buffer(FN_BUF_SPRING, sizeof(fn_spring), (int)sp.size(), sp.data());
-        if (!an.empty()) create_buffer(FN_BUF_ANGLE, sizeof(fn_angle), (int)an.size(), an.data());
-        create_buffer(FN_BUF_PAIR, sizeof(fn_pair), (int)pr.size(), pr.data());
-        ens_boxes_ = boxes;
-        ens_sets_ = sets;
-        ens_b0_ = b0;
-        ens_s0_ = s0;
-        ens_a0_ = a0;
-        ens_finite_ = finite;
-        boxes_.put(boxes, stream_);
-        net_.put(net, stream_);
-        C_ = (int)cells;
+        return pk;
+    }
+
+    void load_ensemble(const fn_network *nets, int n) override
+    {
+        adj_.valid = false;
+        if (!bufs_.empty()) throw Error(FN_ERR_ARG, "an ensemble is loaded into an engine without buffers");
+        Packed pk = pack_ensemble(nets, n);
+        create_buffer(FN_BUF_BEAD, sizeof(fn_bead), (int)pk.beads.size(), pk.beads.data());
+        create_buffer(FN_BUF_BEAD_INFO, sizeof(fn_bead_info), (int)pk.info.size(), pk.info.data());
+        if (!pk.sp.empty()) create_buffer(FN_BUF_SPRING, sizeof(fn_spring), (int)pk.sp.size(), pk.sp.data());
+        if (!pk.an.empty()) create_buffer(FN_BUF_ANGLE, sizeof(fn_angle), (int)pk.an.size(), pk.an.data());
+        create_buffer(FN_BUF_PAIR, sizeof(fn_pair), (int)pk.pr.size(), pk.pr.data());
+        adopt_ensemble(pk);
         cell_count_.alloc((size_t)C_ + 1);
         cell_start_.alloc((size_t)C_ + 1);
         HIPCK(hipStreamSynchronize(stream_));
         clear_graphs();
+    }
+
+    // New records for every network of a loaded ensemble, of the same sizes
+    // (beads, bonds, pair entries, grid cells); runs continue from them.
+    void reload_ensemble(const fn_network *nets, int n) override
+    {
+        adj_.valid = false;
+        if (ens_boxes_.empty()) throw Error(FN_ERR_ARG, "no ensemble is loaded");
+        Packed pk = pack_ensemble(nets, n);
+        if (n != (int)ens_boxes_.size() || pk.b0 != ens_b0_ || pk.s0 != ens_s0_ || pk.a0 != ens_a0_ ||
+            pk.pr.size() != h_pairs_.size() || pk.cells != C_)
+            throw Error(FN_ERR_MISMATCH, "the networks differ in size from the loaded ones");
+        upload(FN_BUF_BEAD, (int)pk.beads.size(), pk.beads.data());
+        upload(FN_BUF_BEAD_INFO, (int)pk.info.size(), pk.info.data());
+        if (!pk.sp.empty()) upload(FN_BUF_SPRING, (int)pk.sp.size(), pk.sp.data());
+        if (!pk.an.empty()) upload(FN_BUF_ANGLE, (int)pk.an.size(), pk.an.data());
+        upload(FN_BUF_PAIR, (int)pk.pr.size(), pk.pr.data());
+        adopt_ensemble(pk);
+        HIPCK(hipStreamSynchronize(stream_));
+        clear_graphs();
+    }
+
+    void adopt_ensemble(const Packed &pk)
+    {
+        ens_boxes_ = pk.boxes;
+        ens_sets_ = pk.sets;
+        ens_b0_ = pk.b0;
+        ens_s0_ = pk.s0;
+        ens_a0_ = pk.a0;
+        ens_finite_ = pk.finite;
+        boxes_.put(pk.boxes, stream_);
+        net_.put(pk.net, stream_);
+        C_ = (int)pk.cells;
     }
 
     void prepare_ensemble_run(bool spr, bool ang)
@@ -1842,7 +1893,6 @@ class Engine final : public fn_engine {
                          double *samples) override
     {
         adj_.valid = false;
-        if (!ens_boxes_.empty()) throw Error(FN_ERR_UNSUPPORTED, "the adjoint runs on single-network engines");
         const int n = sp.n_steps, guf = sp.grid_update_freq;
         if (n <= 0) throw Error(FN_ERR_ARG, "loop size should be a positive number");
         if (guf <= 0) throw Error(FN_ERR_ARG, "grid_update_freq should be positive");
@@ -1867,12 +1917,21 @@ class Engine final : public fn_engine {
                 }
         group_.put(g, stream_);
         group_idx_.put(gi, stream_);
-        part_.ensure(fnk::kSumBlocks * std::max<size_t>(h_pairs_.size(), 1));
-        const int ng = (int)gi.size();
+        // the group's entries of each network (networks hold consecutive beads)
+        const int nn = n_networks();
+        std::vector<int> goff(nn + 1, 0);
+        int max_len = 0;
+        for (int k = 0; k < nn; ++k) {
+            const int b1 = ens_boxes_.empty() ? N_ : ens_b0_[k + 1];
+            goff[k + 1] = (int)(std::lower_bound(gi.begin(), gi.end(), b1) - gi.begin());
+            max_len = std::max(max_len, goff[k + 1] - goff[k]);
+        }
+        group_off_.put(goff, stream_);
+        part_.ensure(fnk::kSumBlocks * (size_t)std::max<size_t>(std::max<size_t>(h_pairs_.size(), nn), 1));
         const int nseg = (n + every - 1) / every;
         const size_t cb = parts_bytes(state_parts(pc_, fc_, false));
         ck_.ensure(cb * nseg);
-        samp_.ensure(std::max(ns, 1));
+        samp_.ensure(std::max((size_t)ns * nn, (size_t)1));
         adj_.ck_pc.assign(nseg, 0);
         adj_.ck_fc.assign(nseg, 0);
         if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
@@ -1885,7 +1944,8 @@ class Engine final : public fn_engine {
             save_parts(state_parts(pc_, fc_, false), ck_.p + cb * seg);
             const StepHook hook = [&](int k, int, int fw) {
                 if (si < ns && steps[si] == c + k) {
-                    fnk::group_force(frc_[fw][0].p, group_idx_.p, ng, part_.p, samp_.p + si, stream_);
+                    fnk::group_force(frc_[fw][0].p, group_idx_.p, group_off_.p, nn, max_len, part_.p,
+                                     samp_.p + (size_t)si * nn, stream_);
                     ++si;
                 }
             };
@@ -1901,8 +1961,11 @@ class Engine final : public fn_engine {
         const std::vector<Part> fin = state_parts(pc_, fc_, true);
         fin_.ensure(parts_bytes(fin));
         save_parts(fin, fin_.p);
-        if (ns) HIPCK(hipMemcpyAsync(samples, samp_.p, sizeof(double) * ns, hipMemcpyDeviceToHost, stream_));
+        std::vector<double> sv;
+        samp_.get(sv, (size_t)ns * nn, stream_);
         HIPCK(hipStreamSynchronize(stream_));
+        for (int k = 0; k < nn; ++k)          // samples[network][sample]
+            for (int q = 0; q < ns; ++q) samples[(size_t)k * ns + q] = sv[(size_t)q * nn + k];
         adj_.n = n;
         adj_.guf = guf;
         adj_.every = every;
@@ -1916,8 +1979,11 @@ class Engine final : public fn_engine {
         adj_.fin_pc = pc_;
         adj_.fin_fc = fc_;
         adj_.steps.assign(steps, steps + ns);
+        adj_.n_net = nn;
         adj_.valid = true;
     }
+
+    int n_networks() const { return ens_boxes_.empty() ? 1 : (int)ens_boxes_.size(); }
 
     void adjoint_backward(const double *dloss, double *gs, double *ga, double *gp, double *x0b,
                           double *w0b) override
@@ -1929,19 +1995,45 @@ class Engine final : public fn_engine {
         const int ns = (int)adj_.steps.size();
         if (ns && !dloss) throw Error(FN_ERR_ARG, "dloss is required");
         if (cap_ != adj_.cap) throw Error(FN_ERR_ARG, "the contact list was resized after the forward run");
-        std::vector<double> seed(n, 0.0);
-        for (int k = 0; k < ns; ++k) seed[adj_.steps[k]] = dloss[k];
+        const int nn = adj_.n_net;
+        const bool ens = !ens_boxes_.empty();
+        // seeds: dL/d(sample) of network k at sample step q, as one row of nn
+        // values per sampled step (dloss is [network][sample])
+        std::vector<int> row_of(n, -1);
+        std::vector<double> seedm((size_t)std::max(ns, 1) * nn, 0.0);
+        for (int q = 0; q < ns; ++q) {
+            row_of[adj_.steps[q]] = q;
+            for (int k = 0; k < nn; ++k) seedm[(size_t)q * nn + k] = dloss[(size_t)k * ns + q];
+        }
+        seed_.put(seedm, stream_);
+        auto seed_at = [&](int step) -> const double * {
+            return (step >= 0 && row_of[step] >= 0) ? seed_.p + (size_t)row_of[step] * nn : nullptr;
+        };
         ExactRebuild exact(*this);
-        n_spr_ = adj_.n_spr;
-        n_ang_ = adj_.n_ang;
-        geometry(settings_of(K_FILL_GRID), settings_of(K_BUILD_LIST));
+        if (ens) {
+            prepare_ensemble_run(spr, ang);
+        } else {
+            n_spr_ = adj_.n_spr;
+            n_ang_ = adj_.n_ang;
+            geometry(settings_of(K_FILL_GRID), settings_of(K_BUILD_LIST));
+        }
         const size_t N = N_;
         const int np = (int)h_pairs_.size();
+        // contact-scale columns: the pair entries (single network), or the
+        // largest pair table of a network (ensemble, entries local to it)
+        int cols = np;
+        if (ens) {
+            cols = 0;
+            for (int k = 0; k < nn; ++k) {
+                const int e1 = k + 1 < nn ? ens_boxes_[k + 1].pair_base : np;
+                cols = std::max(cols, e1 - ens_boxes_[k].pair_base);
+            }
+        }
         for (auto *d : {&fb_[0], &fb_[1], &xb_, &wb_}) d->ensure(N);
         gsp_.ensure(std::max(S_, 1));
         gan_.ensure(std::max(A_, 1));
-        const int prow = fnk::adjoint_pair_rows(N_, np);
-        gpr_.ensure(std::max<size_t>((size_t)prow * np, 1));
+        const int prow = fnk::adjoint_pair_rows(N_, cols, ens);
+        gpr_.ensure(std::max<size_t>((size_t)prow * cols, 1));
         gps_.ensure(std::max(np, 1));
         sp_jv_.ensure(std::max<size_t>(2 * (size_t)S_, 1));
         an_jv_.ensure(std::max<size_t>(3 * (size_t)A_, 1));
@@ -1962,7 +2054,7 @@ class Engine final : public fn_engine {
         auto d2d = [&](void *dst, const void *src, size_t bytes) {
             if (bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_));
         };
-        fnk::adjoint_seed<T>(fb_[0].p, wb_.p, hlf_.p, group_.p, seed[n - 1], N_, stream_);
+        fnk::adjoint_seed<T>(fb_[0].p, wb_.p, hlf_.p, group_.p, seed_at(n - 1), ens_net(), N_, stream_);
         int cur = 0;
         for (int seg = nseg - 1; seg >= 0; --seg) {
             const int c = seg * every, len = std::min(every, n - c);
@@ -1997,6 +2089,8 @@ class Engine final : public fn_engine {
                 a.an_jv = an_jv_.p;
                 a.drift = kbox(K_ADVANCE_POSITION);
                 a.force = kbox(K_RESOLVE_CONTACTS);
+                a.net = ens_net();
+                a.boxes = boxes_.p;
                 a.px = rp; a.py = rp + N; a.pz = rp + 2 * N;
                 a.ix = ri; a.iy = ri + N; a.iz = ri + 2 * N;
                 a.lam = lam_.p; a.hlf = hlf_.p; a.hlf2 = hlf2_.p; a.radius = rad_.p;
@@ -2006,11 +2100,11 @@ class Engine final : public fn_engine {
                 a.an_b3 = an_b3_.p; a.an_type = an_type_.p; a.an_on = rs + S_; a.an_th0 = an_th0_.p; a.an_k = an_k_.p;
                 a.nb_start = rl; a.nb_cnt = rl + N + 1; a.nb_list = rl + 2 * N + 1;
                 a.pairs = pairs_.p;
-                a.n_pairs = np;
+                a.n_pairs = cols;
                 a.xb = xb_.p; a.wb = wb_.p;
                 a.fb = fb_[cur].p; a.fb_next = fb_[1 - cur].p;
                 a.group = group_.p;
-                a.seed_prev = c + k > 0 ? seed[c + k - 1] : 0.0;
+                a.seed_prev = seed_at(c + k - 1);
                 a.g_spring = gsp_.p; a.g_angle = gan_.p; a.g_pair = gpr_.p;
                 fnk::adjoint_step(a, stream_);
                 cur ^= 1;
@@ -2021,7 +2115,15 @@ class Engine final : public fn_engine {
         pc_ = adj_.fin_pc;
         fc_ = adj_.fin_fc;
         load_parts(state_parts(pc_, fc_, true), fin_.p);
-        fnk::sum_columns(gpr_.p, prow, np, part_.p, gps_.p, stream_);
+        if (ens) {
+            std::vector<int> boff(ens_b0_.begin(), ens_b0_.end()), pbase(nn + 1, np);
+            for (int k = 0; k < nn; ++k) pbase[k] = ens_boxes_[k].pair_base;
+            net_off_.put(boff, stream_);
+            pair_off_.put(pbase, stream_);
+            fnk::sum_network_columns(gpr_.p, cols, net_off_.p, pair_off_.p, nn, gps_.p, stream_);
+        } else {
+            fnk::sum_columns(gpr_.p, prow, np, part_.p, gps_.p, stream_);
+        }
         auto d2h = [&](void *dst, const void *src, size_t bytes) {
             if (dst && bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream_));
         };
@@ -2035,7 +2137,7 @@ class Engine final : public fn_engine {
 
     struct AdjRun {
         bool valid = false, spr = false, ang = false;
-        int n = 0, guf = 1, every = 1, nseg = 0, n_spr = 0, n_ang = 0, cap = 0, fin_pc = 0, fin_fc = 0;
+        int n = 0, guf = 1, every = 1, nseg = 0, n_spr = 0, n_ang = 0, cap = 0, fin_pc = 0, fin_fc = 0, n_net = 1;
         size_t cb = 0;
         std::vector<int> ck_pc, ck_fc, steps;
     };
@@ -2161,6 +2263,8 @@ class Engine final : public fn_engine {
     AdjRun adj_;                           // the last adjoint forward run
     DArr<unsigned char> ck_, fin_;         // its checkpoints (one per segment) and end state
     DArr<int> group_, group_idx_;          // sampled beads: flags, ascending indices
+    DArr<int> group_off_, net_off_, pair_off_;   // per network: group entries, beads, pair entries
+    DArr<double> seed_;                    // dL/d(sample) rows of the sampled steps
     DArr<double> part_;                    // partial sums of the deterministic reductions
     DArr<double3> sp_jv_, an_jv_;          // bond terms of an adjoint step at the incidence slots
     DArr<double> samp_;                    // samples of the run
@@ -2379,6 +2483,11 @@ int fn_batch_create(int device, int precision, int n_networks, const fn_network 
     if (rc != FN_OK) return rc;
     const int lr = guarded(*out, [&] { (*out)->load_ensemble(nets, n_networks); });
     return lr;
+}
+
+int fn_batch_load(fn_engine *eng, int n_networks, const fn_network *nets)
+{
+    return guarded(eng, [&] { eng->reload_ensemble(nets, n_networks); });
 }
 
 int fn_batch_read(fn_engine *eng, int network, int kind, void *out)

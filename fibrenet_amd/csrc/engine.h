@@ -100,7 +100,9 @@ template <typename T, typename P>
 struct AdjArgs {
     int n, n_springs, n_angles;   // bonds at or past n_springs / n_angles did not run
     int springs, angles;          // all bonds (incidence slots 2 * springs, 3 * angles)
-    Box drift, force;             // integrator and walls; contacts
+    Box drift, force;             // integrator and walls; contacts (single network)
+    const int *net;               // batched ensemble: each bead's network, whose Box is boxes[net[i]]
+    const Box *boxes;
     const P *px, *py, *pz, *ix, *iy, *iz;
     const T *lam, *hlf, *hlf2, *radius;
     const int *sp_start, *sp_slot, *sp_p1, *sp_p2, *sp_type, *sp_on;
@@ -116,19 +118,26 @@ struct AdjArgs {
     const double3 *fb;          // adjoint of this step's force (seed and kick term included)
     double3 *fb_next;           // the same for the previous step
     const int *group;           // beads whose x-force is sampled
-    double seed_prev;           // dL / d(sample) of the previous step (0: not sampled)
+    const double *seed_prev;    // dL / d(sample) of the previous step per network (null: not sampled)
     double *g_spring, *g_angle;   // adjoints of the force scales of the bonds
-    double *g_pair;               // and of the contact pair entries, adjoint_pair_rows rows
+    double *g_pair;               // and of the contact pair entries: adjoint_pair_rows rows of n_pairs
+                                  // (batched: one row per bead, entries local to its network)
 };
 // One adjoint step: the bond terms (when a.springs / a.angles), then the
 // bead update.
 template <typename T, typename P>
 void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s);
+// Fb of the last step: seed[net[i]] (seed[0] without net) at the flagged beads.
 template <typename T>
-void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, double seed, int n, hipStream_t s);
+void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, const double *seed,
+                  const int *net, int n, hipStream_t s);
 // Rows of the contact scale adjoints g_pair (row * n_pairs + entry) that the
 // adjoint step accumulates for n beads.
-int adjoint_pair_rows(int n, int np);
+int adjoint_pair_rows(int n, int np, bool batched);
+// Batched ensemble: out[pair_base[k] + t] = sum of g[i * cols + t] over the
+// beads i in [bead_off[k], bead_off[k + 1]), t < pair_base[k + 1] - pair_base[k].
+void sum_network_columns(const double *g, int cols, const int *bead_off, const int *pair_base, int n_net,
+                         double *out, hipStream_t s);
 // One step's positions (images when im is not null) and bond statuses into
 // its record (see k_record_step).
 void record_step(const double *const p[3], const double *const im[3], const int *sp_st, int ns, const int *an_st,
@@ -137,10 +146,12 @@ void record_step(const double *const p[3], const double *const im[3], const int 
 // doubles.  Deterministic (fixed partition and order).
 constexpr int kSumBlocks = 256;
 void sum_columns(const double *g, int n, int np, double *part, double *out, hipStream_t s);
-// *out = sum of fx[idx[k]] over k < n; part holds kSumBlocks doubles.
-// Deterministic (fixed partition and order).
+// out[k] = sum of fx[idx[j]] over j in [off[k], off[k + 1]) for k < n_net
+// (off on the device, max_len the longest range); part holds
+// kSumBlocks * n_net doubles.  Deterministic (fixed partition and order).
 template <typename T>
-void group_force(const T *fx, const int *idx, int n, double *part, double *out, hipStream_t s);
+void group_force(const T *fx, const int *idx, const int *off, int n_net, int max_len, double *part, double *out,
+                 hipStream_t s);
 
 void fill_int(int *p, int v, int n, hipStream_t s);
 void fill_gated(int *p, int v, int n, const int *gate, hipStream_t s);

@@ -336,6 +336,29 @@ def tensile_fit(n_fibres=100, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B3,
     return net
 
 
+def tensile_batch(n_networks=128, seed=0xF1B4, fibres=(50, 300), lengths=(30, 60), hidden=(50.0, 200.0),
+                  density=0.025, height=20.0, jitter=0.05, dt=1e-3, steps=1000):
+    """Independent tensile networks of uneven size for a batched fit: per
+    network a fibre count and a fibre length uniform in the given ranges, a
+    finite box of the given height whose square base holds the beads at
+    `density`, HR_FL springs (failure stress uniform in [50, 150]), HR
+    angles, penalty contacts, clamp and pull as in tensile_fit.  Each
+    network's meta carries its hidden spring stiffness, log-uniform in
+    `hidden`.  Network i uses the seed seed + 3000 + i."""
+    rs = np.random.Generator(np.random.PCG64(seed))
+    nets = []
+    for i in range(n_networks):
+        nf = int(rs.integers(fibres[0], fibres[1] + 1))
+        nb = int(rs.integers(lengths[0], lengths[1] + 1))
+        side = float(max(2.0 * nb, np.sqrt(nf * nb / (height * density))))
+        k = float(np.exp(rs.uniform(np.log(hidden[0]), np.log(hidden[1]))))
+        net = tensile_fit(n_fibres=nf, n_beads=nb, box=(side, side, height), seed=seed + 3000 + i, dt=dt,
+                          steps=steps, jitter=jitter, hr_fl=True)
+        net.meta.update(network=i, hidden_spring_k=k, side=side)
+        nets.append(net)
+    return nets
+
+
 def _g(x):
     return "%.17g" % x
 

@@ -197,6 +197,10 @@ int fn_batch_create(int device, int precision, int n_networks, const fn_network 
 /* Records of network `network`: kind FN_BUF_BEAD, FN_BUF_SPRING or
  * FN_BUF_ANGLE (bond indices local to the network). */
 int fn_batch_read(fn_engine *eng, int network, int kind, void *out);
+/* New records for every network of the ensemble, of the sizes loaded by
+ * fn_batch_create (beads, bonds, pair entries, grid cells per network);
+ * runs continue from them (a fit updating parameters between runs). */
+int fn_batch_load(fn_engine *eng, int n_networks, const fn_network *nets);
 
 /* ---- autotuner (no reference counterpart) ----
  * Runs n_steps steps exactly as fn_run_steps does (bitwise the same
@@ -231,9 +235,13 @@ int fn_autotune(fn_engine *eng, int n_steps, int grid_update_freq, int run_sprin
  * and held fixed), then runs the adjoint steps through it.  Derivatives
  * are fp64 in both precisions.  A scale sigma multiplies the force of one
  * bond (or of every contact of one pair type) and is 1 in the run: for a
- * spring, dL/dk = dL/dsigma / k.  Single-network engines only; any call that
- * changes the engine's state in between (runs, writes, launches) makes
- * fn_adjoint_backward fail until the next fn_adjoint_forward. */
+ * spring, dL/dk = dL/dsigma / k.  On a batched ensemble every network is
+ * sampled on its own: samples and dloss hold n_networks x n_samples values
+ * (network-major), d_pair is indexed like the concatenated pair tables, and
+ * forward and backward are each one launch per kernel for all networks.
+ * Any call that changes the engine's state in between (runs, writes,
+ * launches) makes fn_adjoint_backward fail until the next
+ * fn_adjoint_forward. */
 typedef struct fn_adjoint_spec {
     int n_steps;            /* as fn_run_steps                                 */
     int grid_update_freq;
@@ -243,10 +251,11 @@ typedef struct fn_adjoint_spec {
 } fn_adjoint_spec;
 /* group[n_beads]: nonzero for the sampled beads; sample_steps[n_samples]:
  * strictly increasing step indices in [0, n_steps); samples[n_samples]
- * receives sum over the group of the force x of those steps. */
+ * (times n_networks on an ensemble) receives the sum over the group (of
+ * each network) of the force x of those steps. */
 int fn_adjoint_forward(fn_engine *eng, const fn_adjoint_spec *spec, const int *group, const int *sample_steps,
                        int n_samples, double *samples);
-/* dloss[n_samples] = dL/d(sample).  Outputs (any may be NULL):
+/* dloss[n_samples] (times n_networks) = dL/d(sample).  Outputs (any may be NULL):
  * d_spring[n_springs], d_angle[n_angles], d_pair[n_pairs] scale derivatives;
  * d_pos0[3 * n_beads], d_vel0[3 * n_beads] (x, y, z per bead) derivatives
  * with respect to the positions and velocity_old at the start of the run.

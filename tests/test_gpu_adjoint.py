@@ -198,3 +198,55 @@ def test_tensile_fit_network_gradients(need_gpu, prec, tol):
     vals, ders = dual_run(net).run(n, net.grid_update_freq, group, steps)
     ref = ders[:3] @ (2.0 * (vals - target))
     assert np.all(np.abs(got - ref) <= tol * np.abs(ref)), (got, ref)
+
+
+def _batch_members():
+    """Three small tensile networks of different sizes and loads."""
+    return [adjoint_net(seed=31, n_fibres=30, n_beads=10, hr_fl=True),
+            adjoint_net(seed=32, n_fibres=44, n_beads=12, box=(26.0, 22.0, 8.0)),
+            adjoint_net(seed=33, n_fibres=25, n_beads=14, pull_v=0.8, hr_fl=True)]
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batched_adjoint_equals_single_networks(need_gpu, prec):
+    """The ensemble's adjoint gives every network what its own engine gives:
+    samples and the per-bead derivatives bit for bit (the same kernels bead
+    by bead), the per-bond and per-pair-entry scale derivatives to rounding
+    (their sums are grouped differently)."""
+    from fibrenet_amd import Ensemble
+    nets = _batch_members()
+    rs = np.random.default_rng(3)
+    w = rs.normal(size=(len(nets), len(SAMPLES)))
+    ens = Ensemble(nets, prec)
+    group = np.concatenate([_group(n) for n in nets])
+    s = ens.adjoint_forward(N_STEPS, group, SAMPLES, grid_update_freq=GUF, checkpoint_every=50)
+    off = {k: ens.offsets(k) for k in ("bead", "spring", "angle", "pair")}
+    g = ens.adjoint_backward(w, int(off["spring"][-1]), int(off["angle"][-1]), int(off["pair"][-1]))
+    for k, net in enumerate(nets):
+        eng, ids = engine_for(net, prec)
+        sk = eng.adjoint_forward(N_STEPS, _group(net), SAMPLES, grid_update_freq=GUF, checkpoint_every=50)
+        gk = eng.adjoint_backward(w[k], len(net.springs), len(net.angles), len(net.pairs))
+        b0, b1 = off["bead"][k], off["bead"][k + 1]
+        assert np.allclose(s[k], sk, rtol=1e-12, atol=1e-12 * np.max(np.abs(sk)))
+        assert g["position"][b0:b1].tobytes() == gk["position"].tobytes()
+        assert g["velocity_old"][b0:b1].tobytes() == gk["velocity_old"].tobytes()
+        for key in ("spring", "angle", "pair"):
+            o0, o1 = off[key][k], off[key][k + 1]
+            scale = max(np.max(np.abs(gk[key])), 1e-300)
+            assert np.allclose(g[key][o0:o1], gk[key], rtol=1e-10, atol=1e-12 * scale), key
+
+
+def test_batch_reload_restarts_every_network(need_gpu):
+    """fn_batch_load puts new records in place: a run from them equals a run
+    of a freshly created ensemble bit for bit."""
+    from fibrenet_amd import Ensemble
+    nets = _batch_members()
+    a = Ensemble(nets, "f32")
+    a.run_steps(120, GUF, True, True)
+    a.reload(nets)
+    a.run_steps(120, GUF, True, True)
+    b = Ensemble(nets, "f32")
+    b.run_steps(120, GUF, True, True)
+    for k in range(len(nets)):
+        assert a.read(k, "bead").tobytes() == b.read(k, "bead").tobytes()
+        assert a.read(k, "spring").tobytes() == b.read(k, "spring").tobytes()
