@@ -279,7 +279,7 @@ def tensile_lj(n_fibres=20000, n_beads=100, box=(1000.0, 1000.0, 50.0), seed=0xF
 def tensile_fit(n_fibres=100, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B3, dt=1e-3, steps=2000, jitter=0.1,
                 clamp_x=5.0, pull_v=0.01, hr_fl=False, spring_k=100.0, angle_k=10.0, contact_k=1000.0, e_mod=1000.0,
                 sigma_f=(50.0, 150.0), damping=None, gravity=(0.0, 0.0, 0.0), unit=2.0, grid_update_freq=10,
-                d_min=0.0):
+                d_min=0.0, contact="hs", periodic=False):
     """A differentiable tensile case: bent fibres (straight chains with a
     random transverse jitter, so that angles and springs start away from
     rest and every stiffness shapes the response) in a finite box with
@@ -287,18 +287,27 @@ def tensile_fit(n_fibres=100, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B3,
     springs, HR angles, penalty contacts between three bead types; type 1
     clamped (mass 0) at x < clamp_x, type 2 pulled at +pull_v (mass 0) at
     x > box - clamp_x.  Fibres are placed as in periodic_hs (d_min = 0:
-    overlaps allowed) unless d_min asks for rejection sampling."""
+    overlaps allowed) unless d_min asks for rejection sampling.  contact
+    "lj": Lennard-Jones contacts (eps 1, sigma 1, cutoff 2.5) instead of
+    penalty ones; periodic: a periodic box (images unwrapped, positions
+    wrapped; the clamp and pull groups keep their masses and velocities)."""
     net = Network()
-    s = make_settings(box, 1, unit, 1.5, dt, 0.5, wall_k=100.0, damping=damping, gravity=gravity, types=3)
-    xyz, tries = place_fibres(seed, n_fibres, n_beads, (box[0], box[1], box[2]), finite=True, d_min=d_min)
+    cut = 2.5 if contact == "lj" else 1.5
+    s = make_settings(box, 0 if periodic else 1, max(unit, cut), cut, dt, 0.5, wall_k=100.0, damping=damping,
+                      gravity=gravity, types=3)
+    xyz, tries = place_fibres(seed, n_fibres, n_beads, (box[0], box[1], box[2]), finite=not periodic, d_min=d_min)
     rs = np.random.Generator(np.random.PCG64(seed))
-    pos = xyz.reshape(-1, 3) + rs.uniform(-jitter, jitter, size=(n_fibres * n_beads, 3))
+    img = xyz.reshape(-1, 3) + rs.uniform(-jitter, jitter, size=(n_fibres * n_beads, 3))
     L = np.array(box)
-    pos = np.clip(pos, 0.05, L - 0.05)
+    if periodic:
+        pos = np.mod(img, L)
+        pos = np.where(pos >= L, 0.0, pos)
+    else:
+        img = pos = np.clip(img, 0.05, L - 0.05)
     N = len(pos)
     net.beads = np.zeros(N, R.BEAD)
     R.set_vec(net.beads, "position", pos)
-    R.set_vec(net.beads, "image_position", pos)
+    R.set_vec(net.beads, "image_position", img)
     lo = pos[:, 0] < clamp_x
     hi = pos[:, 0] > box[0] - clamp_x
     typ = np.where(lo, 1, np.where(hi, 2, 0)).astype(np.int32)
@@ -325,9 +334,10 @@ def tensile_fit(n_fibres=100, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B3,
     net.pairs = np.zeros(T * (T + 1) // 2, R.PAIR)
     for a in range(T):
         for b in range(a, T):
-            net.pairs[pair_index(a, b, T)] = hs_pair(a, b, contact_k)
+            net.pairs[pair_index(a, b, T)] = (lj_pair(a, b, 1.0, 1.0, 2.5) if contact == "lj"
+                                              else hs_pair(a, b, contact_k))
     net.settings = s
-    net.cell_nh = grid_neighbours(1, int(s["cells_x"]), int(s["cells_y"]), int(s["cells_z"]))
+    net.cell_nh = grid_neighbours(0 if periodic else 1, int(s["cells_x"]), int(s["cells_y"]), int(s["cells_z"]))
     net.grid_update_freq = grid_update_freq
     net.run_steps = steps
     net.type_names = ["C1", "C2", "C3"]

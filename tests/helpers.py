@@ -83,15 +83,23 @@ def field_errors(beads, ref):
     return {f: rel_err(R.vec(beads, f), R.vec(ref, f)) for f in FIELDS}
 
 
-def adjoint_net(seed=21, n_fibres=40, n_beads=12, box=(24.0, 24.0, 8.0), jitter=0.15, pull_v=0.5, hr_fl=False):
+def adjoint_net(seed=21, n_fibres=40, n_beads=12, box=(24.0, 24.0, 8.0), jitter=0.15, pull_v=0.5, hr_fl=False,
+                contact="hs", periodic=False):
     """A small differentiable tensile case (generate.tensile_fit at 480
     beads): bent fibres in a finite box with walls, damping and gravity;
     HR springs (or HR_FL with failure stress in [20, 60]), HR angles,
-    penalty contacts between three bead types; type 1 clamped at x < 3,
-    type 2 pulled at +x."""
+    penalty (or Lennard-Jones) contacts between three bead types; type 1
+    clamped at x < 3, type 2 pulled at +x; or the same in a periodic box."""
     return G.tensile_fit(n_fibres=n_fibres, n_beads=n_beads, box=box, seed=seed, dt=2e-3, steps=300, jitter=jitter,
                          clamp_x=3.0, pull_v=pull_v, hr_fl=hr_fl, sigma_f=(20.0, 60.0), damping=0.5,
-                         gravity=(0.0, 0.0, -0.2), d_min=0.9)
+                         gravity=(0.0, 0.0, -0.2), d_min=0.9, contact=contact, periodic=periodic)
+
+
+# Adjoint test networks by kind: penalty contacts in a finite box (with or
+# without failures), Lennard-Jones contacts, a periodic box.
+ADJ_NETS = {"hs": lambda: adjoint_net(), "hs_fl": lambda: adjoint_net(hr_fl=True),
+            "lj": lambda: adjoint_net(seed=41, contact="lj"),
+            "periodic": lambda: adjoint_net(seed=41, box=(20.0, 20.0, 8.0), periodic=True)}
 
 
 def dual_run(net, u_pos=None, u_vel=None, round_lengths=True, scales=(1.0, 1.0, 1.0)):

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from fibrenet_amd import records as R
-from helpers import adjoint_net, dual_run, oracle_for, rel_err
+from helpers import ADJ_NETS, adjoint_net, dual_run, oracle_for, rel_err
 
 STEPS = 150
 GUF = 10
@@ -20,9 +20,10 @@ def _samples(net):
     return group, np.arange(9, STEPS, 10)
 
 
-@pytest.mark.parametrize("hr_fl", [False, True])
-def test_values_follow_the_oracle(hr_fl):
-    net = adjoint_net(hr_fl=hr_fl)
+@pytest.mark.parametrize("kind", sorted(ADJ_NETS))
+def test_values_follow_the_oracle(kind):
+    net = ADJ_NETS[kind]()
+    hr_fl = kind == "hs_fl"
     group, steps = _samples(net)
     d = dual_run(net)
     vals, _ = d.run(300, GUF, group, np.arange(14, 300, 15))

@@ -14,7 +14,7 @@ import torch
 
 from fibrenet_amd import records as R
 from fibrenet_amd.autograd import TensileRun, tensile_samples
-from helpers import adjoint_net, dual_run, engine_for, read_beads
+from helpers import ADJ_NETS, adjoint_net, dual_run, engine_for, read_beads
 
 pytestmark = pytest.mark.gpu
 
@@ -111,12 +111,14 @@ def test_backward_needs_a_current_forward(need_gpu):
         eng.adjoint_forward(50, _group(net), [10], grid_update_freq=GUF, checkpoint_every=15)
 
 
-# (precision, HR_FL springs, tolerance): fp64 differentiates the same
-# trajectory as the reference up to rounding; fp32 differentiates its own
-# trajectory, whose fp32 forces move it away from the fp64 one.
-@pytest.mark.parametrize("prec,hr_fl,tol", [("f64", False, 1e-7), ("f64", True, 1e-7), ("f32", False, 1e-3)])
-def test_matches_dual_number_reference(need_gpu, prec, hr_fl, tol):
-    net = adjoint_net(hr_fl=hr_fl)
+# (precision, network, tolerance): fp64 differentiates the same trajectory
+# as the reference up to rounding; fp32 differentiates its own trajectory,
+# whose fp32 forces move it away from the fp64 one.
+@pytest.mark.parametrize("prec,kind,tol", [("f64", "hs", 1e-7), ("f64", "hs_fl", 1e-7), ("f32", "hs", 1e-3),
+                                           ("f64", "lj", 1e-7), ("f32", "lj", 1e-3), ("f64", "periodic", 1e-7),
+                                           ("f32", "periodic", 1e-3)])
+def test_matches_dual_number_reference(need_gpu, prec, kind, tol):
+    net = ADJ_NETS[kind]()
     w = _weights()
     _, _, s, g = _adjoint(net, prec, w)
     up, uv = _dirs(net)
