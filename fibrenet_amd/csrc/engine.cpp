@@ -664,6 +664,10 @@ class Engine final : public fn_engine {
             prepare();
             return chain_ok_ ? 1 : 0;
         }
+        if (name == "chain_sets") {
+            prepare();
+            return chain_ok_ && cmp_ok_ ? 1 : 0;
+        }
         if (name == "graphs") return graphs_on_ ? 1 : 0;
         throw Error(FN_ERR_ARG, "unknown option " + name);
     }
@@ -1021,6 +1025,57 @@ class Engine final : public fn_engine {
             ch_spring_.put(cs, stream_);
             ch_angle_.put(ca, stream_);
         }
+        build_chain_sets(cs, ca);
+    }
+
+    // The chain layout's bond parameters by bead: a code per bead into the
+    // distinct (type, l0, E, k) spring sets and (type, theta0, k) angle sets,
+    // the failure stress as a float where every one is exact in float.
+    void build_chain_sets(const std::vector<int> &cs, const std::vector<int> &ca)
+    {
+        cmp_ok_ = false;
+        if (!chain_ok_) return;
+        const int n = N_;
+        std::map<std::tuple<int, double, double, double>, int> sk;
+        std::map<std::tuple<int, double, double>, int> ak;
+        std::vector<fnk::SpringSet> ss;
+        std::vector<fnk::AngleSet> as;
+        std::vector<unsigned short> scode(n, fnk::kNoSet), acode(n, fnk::kNoSet);
+        std::vector<float> c2(n, 0.0f);
+        for (int i = 0; i < n; ++i) {
+            if (cs[i] >= 0) {
+                const fn_spring &q = h_springs_[cs[i]];
+                const auto key = std::make_tuple(q.type, q.rest_length, q.c1, (double)(T)q.stiffness);
+                auto it = sk.find(key);
+                if (it == sk.end()) {
+                    it = sk.emplace(key, (int)ss.size()).first;
+                    ss.push_back(fnk::SpringSet{q.type, q.rest_length, q.c1, (double)(T)q.stiffness});
+                }
+                if ((double)(float)q.c2 != q.c2 || it->second >= fnk::kNoSet) return;
+                scode[i] = (unsigned short)it->second;
+                c2[i] = (float)q.c2;
+            }
+            if (ca[i] >= 0) {
+                const fn_angle &g = h_angles_[ca[i]];
+                const auto key = std::make_tuple(g.type, (double)(T)g.angle, (double)(T)g.c1);
+                auto it = ak.find(key);
+                if (it == ak.end()) {
+                    it = ak.emplace(key, (int)as.size()).first;
+                    as.push_back(fnk::AngleSet{g.type, (double)(T)g.angle, (double)(T)g.c1});
+                }
+                if (it->second >= fnk::kNoSet) return;
+                acode[i] = (unsigned short)it->second;
+            }
+        }
+        cs_set_.put(scode, stream_);
+        ca_set_.put(acode, stream_);
+        cs_c2_.put(c2, stream_);
+        sp_sets_.put(ss.empty() ? std::vector<fnk::SpringSet>(1) : ss, stream_);
+        an_sets_.put(as.empty() ? std::vector<fnk::AngleSet>(1) : as, stream_);
+        cs_st_.ensure(n);
+        ca_st_.ensure(n);
+        HIPCK(hipStreamSynchronize(stream_));
+        cmp_ok_ = true;
     }
 
     void prepare()
@@ -1084,6 +1139,11 @@ class Engine final : public fn_engine {
         a.sp_term = sp_term_.p; a.an_term = an_term_.p;
         a.sp_slot = sp_slot_.p; a.an_slot = an_slot_.p;
         a.ch_spring = ch_spring_.p; a.ch_angle = ch_angle_.p;
+        if (chain_ok_ && cmp_ok_) {
+            a.cs_set = cs_set_.p; a.ca_set = ca_set_.p; a.cs_c2 = cs_c2_.p;
+            a.cs_st = cs_st_.p; a.ca_st = ca_st_.p;
+            a.sp_sets = sp_sets_.p; a.an_sets = an_sets_.p;
+        }
         a.net = ens_net();
         a.boxes = boxes_.p;
         a.step = step_.p;
@@ -1453,6 +1513,8 @@ class Engine final : public fn_engine {
         const fnk::Box bc = kbox(K_RESOLVE_CONTACTS);
         int pc = pc_, fc = fc_;
         const bool chain = use_chain();
+        if (chain && cmp_ok_)        // the chain kernel's status copies, whatever ran before
+            fnk::chain_status(ch_spring_.p, ch_angle_.p, sp_status_.p, an_status_.p, cs_st_.p, ca_st_.p, N_, stream_);
         if (first) {
             fnk::Args<T, P> a = args_base();
             a.drift = bd;
@@ -2260,6 +2322,12 @@ class Engine final : public fn_engine {
     DArr<T> an_th0_, an_k_;
     DArr<fnk::V3<T>> sp_term_, an_term_;   // bond terms of the current step at the incidence slots
     DArr<int> ch_spring_, ch_angle_;       // chain layout: spring (i, i+1), angle centred on i
+    bool cmp_ok_ = false;                  // the chain layout's bond parameters by bead are valid
+    DArr<unsigned short> cs_set_, ca_set_; // per bead: parameter set of spring (i, i+1), of the angle at i
+    DArr<float> cs_c2_;                    // per bead: failure stress of spring (i, i+1)
+    DArr<int> cs_st_, ca_st_;              // per bead: the two statuses (copies)
+    DArr<fnk::SpringSet> sp_sets_;
+    DArr<fnk::AngleSet> an_sets_;
     AdjRun adj_;                           // the last adjoint forward run
     DArr<unsigned char> ck_, fin_;         // its checkpoints (one per segment) and end state
     DArr<int> group_, group_idx_;          // sampled beads: flags, ascending indices

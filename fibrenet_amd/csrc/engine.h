@@ -153,6 +153,9 @@ template <typename T>
 void group_force(const T *fx, const int *idx, const int *off, int n_net, int max_len, double *part, double *out,
                  hipStream_t s);
 
+// cs_st[i] = sp_status[ch_spring[i]], ca_st[i] = an_status[ch_angle[i]] (0 without a bond).
+void chain_status(const int *ch_spring, const int *ch_angle, const int *sp_status, const int *an_status, int *cs_st,
+                  int *ca_st, int n, hipStream_t s);
 void fill_int(int *p, int v, int n, hipStream_t s);
 void fill_gated(int *p, int v, int n, const int *gate, hipStream_t s);
 void set_int(int *p, int v, const int *gate, hipStream_t s);

@@ -286,13 +286,37 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
     V3<T> f = bead_begin(a, bx, ic, b);
     const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
     const T r1 = a.radius[ic];
-    const int sr = (in && (SPR || ANG)) ? a.ch_spring[i] : -1;
-    const int q = (in && ANG) ? a.ch_angle[i] : -1;
-    // then the bond parameters, in one round trip
+    // the chain bonds of bead i: spring (i, i+1) and the angle centred on i,
+    // their parameters by bead (parameter sets) or through the bond indices
+    bool hs = false, hq = false;
+    int sr = -1, q = -1;
     SpringP<T> spp{};
     AngleP<T> anp{};
-    if (sr >= 0) spp = load_spring(a, sr);
-    if (q >= 0) anp = load_angle(a, q);
+    if (a.cs_set) {
+        if (in && (SPR || ANG)) {
+            const unsigned code = a.cs_set[i];
+            if (code != kNoSet) {
+                const SpringSet ss = a.sp_sets[code];
+                hs = true;
+                spp = SpringP<T>{ss.ty, a.cs_st[i], ss.l0, ss.c1, (double)a.cs_c2[i], (T)ss.k};
+            }
+        }
+        if (in && ANG) {
+            const unsigned code = a.ca_set[i];
+            if (code != kNoSet) {
+                const AngleSet as = a.an_sets[code];
+                hq = true;
+                anp = AngleP<T>{a.ca_st[i], as.ty, (T)as.th0, (T)as.k};
+            }
+        }
+    } else {
+        sr = (in && (SPR || ANG)) ? a.ch_spring[i] : -1;
+        q = (in && ANG) ? a.ch_angle[i] : -1;
+        hs = sr >= 0;
+        hq = q >= 0;
+        if (hs) spp = load_spring(a, sr);
+        if (hq) anp = load_angle(a, q);
+    }
     // and the first list entries of the wave's contacts
     WaveContactLds<T> &w = cl[threadIdx.x >> 6];
     const ContactAhead ca = contacts_ahead(a, e0, cnt, w);
@@ -305,7 +329,7 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
     // spring (i, i+1): its term and its status after this step's spring pass
     V3<T> fr{(T)0, (T)0, (T)0};
     int str = 1;
-    if (sr >= 0) {
+    if (hs) {
         str = spp.st;
         if (SPR) {
             bool broke;
@@ -313,55 +337,60 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
             if (broke) {
                 str = 0;
                 if (own) {
-                    a.sp_status[sr] = 0;
-                    a.sp_break[sr] = *a.step + a.step_off;
+                    const int s = a.cs_set ? a.ch_spring[i] : sr;
+                    a.sp_status[s] = 0;
+                    a.sp_break[s] = *a.step + a.step_off;
+                    if (a.cs_set) a.cs_st[i] = 0;
                 }
             }
         }
     }
-    const int sl = __shfl_up(sr, 1, 64);
+    const int sl = __shfl_up(hs ? 1 : 0, 1, 64);
     const int stl = __shfl_up(str, 1, 64);
     const V3<T> fl{__shfl_up(fr.x, 1, 64), __shfl_up(fr.y, 1, 64), __shfl_up(fr.z, 1, 64)};
 
     // angle (i-1, i, i+1) with this step's HR_FL rule (its springs are the
     // two chain springs of bead i)
     V3<T> pa{(T)0, (T)0, (T)0}, pc{(T)0, (T)0, (T)0};
-    if (q >= 0) {
+    if (hq) {
         int st = anp.st;
         if (st == 1 && anp.ty == 1 && (stl == 0 || str == 0)) {
             st = 0;
-            if (own) a.an_status[q] = 0;
+            if (own) {
+                a.an_status[a.cs_set ? a.ch_angle[i] : q] = 0;
+                if (a.cs_set) a.ca_st[i] = 0;
+            }
         }
         if (st == 1) angle_force(anp, xp, yp, zp, x, y, z, xn, yn, zn, pa, pc);
     }
-    const int ql = __shfl_up(q, 1, 64), qn = __shfl_down(q, 1, 64);
+    const int hql = __shfl_up(hq ? 1 : 0, 1, 64), hqn = __shfl_down(hq ? 1 : 0, 1, 64);
     const V3<T> pcl{__shfl_up(pc.x, 1, 64), __shfl_up(pc.y, 1, 64), __shfl_up(pc.z, 1, 64)};
     const V3<T> pan{__shfl_down(pa.x, 1, 64), __shfl_down(pa.y, 1, 64), __shfl_down(pa.z, 1, 64)};
 
     if (SPR) {
-        if (sl >= 0) {          // bead i is p2 of spring (i-1, i)
+        if (sl) {               // bead i is p2 of spring (i-1, i)
             f.x += fl.x * (T)-1;
             f.y += fl.y * (T)-1;
             f.z += fl.z * (T)-1;
         }
-        if (sr >= 0) {          // p1 of spring (i, i+1)
+        if (hs) {               // p1 of spring (i, i+1)
             f.x += fr.x;
             f.y += fr.y;
             f.z += fr.z;
         }
     }
     if (ANG) {
-        if (ql >= 0) {          // b3 of angle (i-2, i-1, i)
+        if (hql) {              // b3 of angle (i-2, i-1, i)
             f.x += pcl.x;
             f.y += pcl.y;
             f.z += pcl.z;
         }
-        if (q >= 0) {           // b2 of angle (i-1, i, i+1)
+        if (hq) {               // b2 of angle (i-1, i, i+1)
             f.x += -pa.x - pc.x;
             f.y += -pa.y - pc.y;
             f.z += -pa.z - pc.z;
         }
-        if (qn >= 0) {          // b1 of angle (i, i+1, i+2)
+        if (hqn) {              // b1 of angle (i, i+1, i+2)
             f.x += pan.x;
             f.y += pan.y;
             f.z += pan.z;
@@ -373,6 +402,18 @@ __global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
 }
 
 __global__ void k_add_int(int *p, int v) { *p += v; }
+
+// The chain layout's status copies from the bond-ordered statuses.
+__global__ void __launch_bounds__(kBlock) k_chain_status(const int *ch_spring, const int *ch_angle,
+                                                         const int *sp_status, const int *an_status, int *cs_st,
+                                                         int *ca_st, int n)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const int s = ch_spring[i], q = ch_angle[i];
+    cs_st[i] = s >= 0 ? sp_status[s] : 0;
+    ca_st[i] = q >= 0 ? an_status[q] : 0;
+}
 
 __global__ void __launch_bounds__(kBlock) k_fill_int(int *p, int v, int n)
 {
@@ -1138,6 +1179,11 @@ void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next,
 template struct Launch<double, double>;
 template struct Launch<float, double>;
 
+void chain_status(const int *ch_spring, const int *ch_angle, const int *sp_status, const int *an_status, int *cs_st,
+                  int *ca_st, int n, hipStream_t s)
+{
+    if (n > 0) k_chain_status<<<blocks(n), kBlock, 0, s>>>(ch_spring, ch_angle, sp_status, an_status, cs_st, ca_st, n);
+}
 void fill_int(int *p, int v, int n, hipStream_t s)
 {
     if (n > 0) k_fill_int<<<blocks(n), kBlock, 0, s>>>(p, v, n);

@@ -59,6 +59,19 @@ struct PairC {
 template <typename T>
 struct V3 { T x, y, z; };
 
+// A distinct parameter set of the chain layout's springs (type, rest length,
+// E, stiffness as the engine's T widened to fp64) and angles (type, rest
+// angle and stiffness as T widened); kNoSet marks a bead without that bond.
+struct SpringSet {
+    int ty;
+    double l0, c1, k;
+};
+struct AngleSet {
+    int ty;
+    double th0, k;
+};
+constexpr unsigned short kNoSet = 0xFFFF;
+
 template <typename T, typename P>
 struct Args {
     int n;                                  // beads
@@ -95,6 +108,16 @@ struct Args {
     // spring (p1 i, p2 i+1), ch_angle[i] the angle (b1 i-1, b2 i, b3 i+1),
     // -1 where there is none.
     const int *ch_spring, *ch_angle;
+    // The chain layout's bond parameters by bead, when the bonds have few
+    // distinct parameter sets (null otherwise): the set of spring (i, i+1)
+    // and of the angle centred on i, the spring's failure stress (exact as a
+    // float), and copies of both statuses (refreshed from sp_status /
+    // an_status at the start of every chunk, written with them on a change).
+    const unsigned short *cs_set, *ca_set;
+    const float *cs_c2;
+    int *cs_st, *ca_st;
+    const SpringSet *sp_sets;
+    const AngleSet *an_sets;
     // contacts: bead i's partners are nb_list[nb_start[i] .. nb_start[i] + nb_cnt[i])
     const int *nb_start, *nb_cnt, *nb_list;
     const PairC *pairs;

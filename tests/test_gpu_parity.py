@@ -99,6 +99,48 @@ def test_chain_kernel_equals_slot_path(need_gpu, workload, precision):
     assert np.array_equal(a.export_breaks(len(net.springs)), b.export_breaks(len(net.springs)))
 
 
+def _inexact_failure_stress():
+    net = small_tensile(weak=True)
+    net.springs["c2"] = net.springs["c2"] + 0.1          # not exact in float
+    return net
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("case", ["sets", "indexed"])
+def test_chain_parameter_sets_and_indexed_loads_equal_slot_path(need_gpu, case, precision):
+    """The chain kernel reads bond parameters through per-bead set codes when
+    every failure stress is exact in float, else through the bond indices;
+    both give the slot path's bytes."""
+    net = small_tensile(weak=True) if case == "sets" else _inexact_failure_stress()
+    a, ia = engine_for(net, precision)
+    b, ib = engine_for(net, precision)
+    assert a.get_option("chain_sets") == (1 if case == "sets" else 0)
+    b.set_option("chain", 0)
+    for n in (5, 200):
+        run_engine(a, net, n, fast=True)
+        run_engine(b, net, n, fast=True)
+    assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
+    sa = a.read_buffer(ia["spring"], R.SPRING)
+    assert sa.tobytes() == b.read_buffer(ib["spring"], R.SPRING).tobytes()
+    assert np.sum(sa["status"] == 0) > 0                  # springs broke on the way
+
+
+def test_chain_status_copies_follow_breaks_on_other_paths(need_gpu):
+    """Springs broken while the slot path runs are broken for the chain kernel
+    that runs next: alternating paths gives the bytes of the chain path alone."""
+    net = small_tensile(weak=True)
+    a, ia = engine_for(net, "f64")
+    b, ib = engine_for(net, "f64")
+    for k in range(6):
+        a.set_option("chain", k % 2)
+        run_engine(a, net, 40, fast=True)
+        run_engine(b, net, 40, fast=True)
+    assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
+    sa = a.read_buffer(ia["spring"], R.SPRING)
+    assert sa.tobytes() == b.read_buffer(ib["spring"], R.SPRING).tobytes()
+    assert np.sum(sa["status"] == 0) > 0
+
+
 # Candidate-list settings: kept with the default skin (30% of the reach), kept
 # with a tiny skin (rebuilt at most rebuilds), rebuilt every time, and the
 # automatic choice.
