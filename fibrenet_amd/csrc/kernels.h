@@ -145,6 +145,13 @@ __device__ __forceinline__ int pair_index(int t1, int t2, int n)
 // native_sqrt(double) as clang-based OpenCL resolves it: the float overload.
 __device__ __forceinline__ float native_sqrt(double x) { return __builtin_sqrtf((float)x); }
 
+// The hardware's fp32 square root and reciprocal (v_sqrt_f32, v_rcp_f32, 1 ulp)
+// for the fp32 build's contact and angle arithmetic: a few instructions where
+// the correctly rounded forms take fifteen.  The fp64 build and the spring
+// break decision keep the correctly rounded operations.
+__device__ __forceinline__ float hw_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__device__ __forceinline__ float hw_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
 __device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double tacos(double x) { return acos(x); }
@@ -324,10 +331,10 @@ __device__ __forceinline__ void angle_force(const AngleP<T> &g, P x1, P y1, P z1
         const T c2 = cx * cx + cy * cy + cz * cz;
         const T la2 = fx * fx + fy * fy + fz * fz, lc2 = gx * gx + gy * gy + gz * gz;
         if (c2 > (T)1e-10 * la2 * lc2) {
-            const T cn = tsqrt(c2), l_ba = tsqrt(la2), l_bc = tsqrt(lc2);
+            const float cn = hw_sqrt((float)c2);
             const T ang = atan2f(cn, fx * gx + fy * gy + fz * gz);
-            const T mag = (T)-2 * g.k * (ang - g.th0);
-            const T sa = mag / (l_ba * l_ba * cn), sc = mag / (l_bc * l_bc * cn);
+            const T q = (T)-2 * g.k * (ang - g.th0) * hw_rcp(cn);
+            const T sa = q * hw_rcp((float)la2), sc = q * hw_rcp((float)lc2);
             pa_out.x = (fy * cz - fz * cy) * sa;
             pa_out.y = (fz * cx - fx * cz) * sa;
             pa_out.z = (fx * cy - fy * cx) * sa;
@@ -385,6 +392,44 @@ __device__ __forceinline__ bool contact_force(const PairC &pr, const Box &b, P r
 {
     T coef;
     T r;
+    if constexpr (sizeof(T) == sizeof(float)) {
+        // fp32 build: the same terms through the hardware square root and
+        // reciprocal, one reciprocal of r^2 shared by the Lennard-Jones ratios
+        // and 1/r = r / r^2
+        float inv;
+        if (pr.type == 1) {
+            if (!((double)len < b.cutoff_sq)) return false;
+            const float r2 = (float)len;
+            r = hw_sqrt(r2);
+            if (!((double)r < pr.c3)) return false;
+            const float rr = hw_rcp(r2);
+            const float c1 = (float)pr.c1;
+            const float fr2 = c1 * c1 * rr;
+            const float fr6 = fr2 * fr2 * fr2;
+            coef = 48.0f * (float)pr.c2 * fr6 * (fr6 - 0.5f) * rr;
+            coef -= (float)pr.c4;
+            inv = r * rr;
+        } else if (pr.type == 2) {
+            r = hw_sqrt((float)len);
+            coef = (r1 + radius[j]) - r;
+            if (!(coef > 0.0f)) return false;
+            coef *= (float)pr.c1;
+            inv = hw_rcp(r);
+        } else if (pr.type == 3) {
+            if (!((double)len < pr.c4)) return false;
+            r = hw_sqrt((float)len);
+            coef = ((float)pr.c2 - r) * (float)pr.c1;
+            coef += ((float)pr.c6 - r) * (float)pr.c5;
+            inv = hw_rcp(r);
+        } else {
+            return false;
+        }
+        if (r == 0.0f) inv = 0.0f;
+        t.x = coef * ((float)rx * inv);
+        t.y = coef * ((float)ry * inv);
+        t.z = coef * ((float)rz * inv);
+        return true;
+    }
     if (pr.type == 1) {                           // Lennard-Jones, kernel.cl:875-919
         if (!((double)len < b.cutoff_sq)) return false;
         r = (T)native_sqrt((double)len);
