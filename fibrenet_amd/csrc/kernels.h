@@ -301,6 +301,30 @@ __device__ __forceinline__ void angle_dirs(R bax, R bay, R baz, R bcx, R bcy, R 
 // segment the force (|F| from the clamped 0.999999) points along the fp64
 // cross product of the bond vectors, which is rounding noise that only the
 // fp64 evaluation reproduces.
+// acos(d) for the cosines of a straight or folded triple, |d| >= 0.999998
+// (including the clamped +-0.999999): with s = sqrt((1 - |d|) / 2),
+// acos(|d|) = 2 asin(s) = 2 s (1 + s^2/6 + 3 s^4/40 + ...), and the first
+// omitted term is below 5e-20 relative for s^2 <= 1e-6.  Other cosines (and
+// NaN) take the library acos.
+__device__ __forceinline__ double acos_near_unit(double d)
+{
+    const double ad = fabs(d);
+    if (!(ad >= 0.999998)) return acos(d);
+    const double s = __builtin_sqrt((1.0 - ad) * 0.5);
+    const double s2 = s * s;
+    const double a = 2.0 * s * (1.0 + s2 * (1.0 / 6.0 + s2 * 0.075));
+    return d < 0.0 ? (3.141592653589793 - a) + 1.2246467991473532e-16 : a;
+}
+
+// fp32 factor f / |v| for a vector with |v|^2 = n2 given in fp64 (0 for a
+// zero vector, as the reference's normalisation leaves it).
+__device__ __forceinline__ float unit_scale(double n2, float f)
+{
+    const float n = (float)n2;
+    if (n >= 1.17549435e-38f) return f * __builtin_amdgcn_rsqf(n);
+    return n2 > 0.0 ? (float)((double)f / __builtin_sqrt(n2)) : 0.0f;
+}
+
 // Parameters of one angle, loaded together.
 template <typename T>
 struct AngleP {
@@ -321,7 +345,7 @@ __device__ __forceinline__ void angle_force(const AngleP<T> &g, P x1, P y1, P z1
 {
     const double bax = (double)(x1 - tx), bay = (double)(y1 - ty), baz = (double)(z1 - tz);
     const double bcx = (double)(x3 - tx), bcy = (double)(y3 - ty), bcz = (double)(z3 - tz);
-    if (sizeof(T) < sizeof(double)) {
+    if constexpr (sizeof(T) < sizeof(double)) {
         // fp32 build away from the straight and the folded triple: the angle as
         // atan2(|ba x bc|, ba . bc), well conditioned at any angle, and the
         // perpendicular directions from the same cross product
@@ -343,6 +367,25 @@ __device__ __forceinline__ void angle_force(const AngleP<T> &g, P x1, P y1, P z1
             pc_out.z = (gy * cx - gx * cy) * sc;
             return;
         }
+        // the straight or the folded triple: the fp64 cosine and its clamp as
+        // below, acos near +-1 by its series, the perpendicular directions
+        // from the fp64 cross products (their rounding noise on an exactly
+        // straight segment is the direction) normalised in fp32
+        const double l_ba = tsqrt(bax * bax + bay * bay + baz * baz);
+        const double l_bc = tsqrt(bcx * bcx + bcy * bcy + bcz * bcz);
+        double dotp = (bax * bcx + bay * bcy + baz * bcz) / (l_ba * l_bc);
+        if (dotp < -1.0) dotp = -0.999999;
+        if (dotp > 1.0) dotp = 0.999999;
+        const T mag = (T)(-2.0 * (double)g.k * (acos_near_unit(dotp) - (double)g.th0));
+        const double cbx = -bcx, cby = -bcy, cbz = -bcz;
+        const double dx = bay * bcz - baz * bcy, dy = baz * bcx - bax * bcz, dz = bax * bcy - bay * bcx;
+        const double pax = bay * dz - baz * dy, pay = baz * dx - bax * dz, paz = bax * dy - bay * dx;
+        const double pcx = cby * dz - cbz * dy, pcy = cbz * dx - cbx * dz, pcz = cbx * dy - cby * dx;
+        const T sa = unit_scale(pax * pax + pay * pay + paz * paz, mag * hw_rcp((float)l_ba));
+        const T sc = unit_scale(pcx * pcx + pcy * pcy + pcz * pcz, mag * hw_rcp((float)l_bc));
+        pa_out = V3<T>{(T)pax * sa, (T)pay * sa, (T)paz * sa};
+        pc_out = V3<T>{(T)pcx * sc, (T)pcy * sc, (T)pcz * sc};
+        return;
     }
     const double l_ba = tsqrt(bax * bax + bay * bay + baz * baz);
     const double l_bc = tsqrt(bcx * bcx + bcy * bcy + bcz * bcz);
