@@ -638,6 +638,7 @@ class Engine final : public fn_engine {
         if (name == "chunk") chunk_ = value;
         else if (name == "chain") chain_on_ = value != 0;
         else if (name == "staged") staged_ = value != 0;
+        else if (name == "flat_search") flat_ = value != 0;
         else if (name == "verlet") {
             if (value < 0 || value > 2) throw Error(FN_ERR_ARG, "verlet is 0 (off), 1 (on) or 2 (auto)");
             verlet_ = value;
@@ -658,6 +659,7 @@ class Engine final : public fn_engine {
         if (name == "chunk") return chunk_;
         if (name == "chain") return chain_on_ ? 1 : 0;
         if (name == "staged") return staged_ ? 1 : 0;
+        if (name == "flat_search") return flat_ ? 1 : 0;
         if (name == "verlet") return verlet_;
         if (name == "skin") return skin_permille_;
         if (name == "chain_layout") {
@@ -1399,7 +1401,7 @@ class Engine final : public fn_engine {
             fnk::fill_gated(ncnt_.p, 0, N_, gate, stream_);
             fnk::search(sorted_.p, cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b, ens_net(),
                         boxes_.p, final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, final_list,
-                        ncnt_.p, scratch_.p, K_, stats_.p, N_, staged_, gate, stream_);
+                        ncnt_.p, scratch_.p, K_, stats_.p, N_, staged_, flat_, gate, stream_);
             fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, gate, stream_);
             if (sync) {
                 int tot = 0, over = 0;
@@ -2303,6 +2305,7 @@ class Engine final : public fn_engine {
     bool verlet_trial_ = false;                      // auto: candidates currently kept
     int verlet_runs_ = 0;                            // auto: runs since the last trial
     bool staged_ = false;                            // neighbour search reads its block's neighbourhood from LDS
+    bool flat_ = true;                               // 3x3x3 neighbour search flattened over each wave
     int skin_permille_ = 300;                        // skin as a fraction of the listing reach
     bool keep_cand_ = false, cand_dirty_ = true;
     double cand_r2_ = -1.0, lim2_ = 0.0;

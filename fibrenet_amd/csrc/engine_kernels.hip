@@ -715,6 +715,174 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
     }
 }
 
+// Slot ranges of stencil row q (dz = q / 3 - 1, dy = q % 3 - 1) around cell
+// (cx, cy, cz) for the 3x3x3 stencil with no axis visited whole: the cells
+// cx - 1 .. cx + 1 of that row as one slot range, or two where a periodic
+// row wraps (a finite box clips the row and skips rows outside the grid).
+__device__ __forceinline__ void stencil_row(const int *cell_start, const Box &b, int cx, int cy, int cz, int q,
+                                            int &lo0, int &hi0, int &lo1, int &hi1)
+{
+    lo0 = hi0 = lo1 = hi1 = 0;
+    const bool per = b.bc == 0;
+    int z = cz + q / 3 - 1, y = cy + q % 3 - 1;
+    if (per) {
+        z = z < 0 ? z + b.cz : (z >= b.cz ? z - b.cz : z);
+        y = y < 0 ? y + b.cy : (y >= b.cy ? y - b.cy : y);
+    } else if (z < 0 || z >= b.cz || y < 0 || y >= b.cy) {
+        return;
+    }
+    const int rb = b.cell_base + y * b.cx + z * b.cxy;
+    if (!per || (cx >= 1 && cx + 1 < b.cx)) {
+        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < b.cx ? cx + 1 : b.cx - 1;
+        lo0 = cell_start[rb + x0];
+        hi0 = cell_start[rb + x1 + 1];
+    } else if (cx == 0) {                      // cells cx - 1 = b.cx - 1, then 0 and 1
+        lo0 = cell_start[rb + b.cx - 1];
+        hi0 = cell_start[rb + b.cx];
+        lo1 = cell_start[rb];
+        hi1 = cell_start[rb + 2];
+    } else {                                   // cx = b.cx - 1: cells b.cx - 2, b.cx - 1, then 0
+        lo0 = cell_start[rb + b.cx - 2];
+        hi0 = cell_start[rb + b.cx];
+        lo1 = cell_start[rb];
+        hi1 = cell_start[rb + 1];
+    }
+}
+
+// Candidates of a wave staged per pass, and the owner records.
+constexpr int kFlatChunk = 768;
+struct FlatOwner {
+    double x, y, z;
+    int tag, loc, type, slot, bead, pad;
+};
+struct WaveSearchLds {
+    int cand[kFlatChunk];      // candidate slot | owner lane << kPidShift
+    int cnt[64];               // entries each lane's bead has collected
+    FlatOwner own[64];
+};
+
+// The same search as k_search for the 3x3x3 stencil, flattened over the
+// wave: the wave's beads lay their candidate slots end to end in LDS, every
+// lane tests every 64th of them against its owner (so a wave costs its total
+// candidates / 64 rounds, and the loads of a round are independent and
+// mostly consecutive), and an accepted entry takes the next place in its
+// owner's row through an LDS counter.  The rows hold the same entries as
+// k_search's, in another order, which the compaction's per-bead sort
+// removes.
+template <bool EXACT>
+__global__ void __launch_bounds__(kBlock) k_search_flat(const int *sorted, const int *cell_start, int n_cells,
+                                                        Sorted so, const PairC *pairs_all, const double *radius,
+                                                        const double *skin, Box b, double r2, int *cnt, int *scratch,
+                                                        int K, int *stats, const int *gate)
+{
+    if (gated_off(gate)) return;
+    __shared__ WaveSearchLds wl[kBlock / 64];
+    WaveSearchLds &w = wl[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int s = xcd_block() * kBlock + threadIdx.x;
+    const PairC *pairs = pairs_all + b.pair_base;
+    const bool live = s < cell_start[n_cells];
+    int i = 0, code = -1, occ = 0, cx = 0, cy = 0, cz = 0;
+    bool elig = false;
+    if (live) {
+        i = sorted[s];
+        code = so.cell[s];
+        const int cg = slot_cell(code);
+        if (cell_start[cg] == s) occ = cell_start[cg + 1] - s;
+        const int c = cg - b.cell_base;
+        cz = c / b.cxy;
+        cy = (c - cz * b.cxy) / b.cx;
+        cx = c - cz * b.cxy - cy * b.cx;
+        elig = !(EXACT && code < 0);
+        const SortedBead me = so.bead[s];
+        w.own[lane] = FlatOwner{me.x, me.y, me.z, me.tag, me.loc, so.type[s], s, i, 0};
+    }
+    w.cnt[lane] = 0;
+    int nb = 0;
+    if (elig) {
+        for (int q = 0; q < 9; ++q) {
+            int lo0, hi0, lo1, hi1;
+            stencil_row(cell_start, b, cx, cy, cz, q, lo0, hi0, lo1, hi1);
+            nb += (hi0 - lo0) + (hi1 - lo1);
+        }
+    }
+    int incl = nb;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const int off = incl - nb;
+    const int total = __shfl(incl, 63, 64);
+    for (int base = 0; base < total; base += kFlatChunk) {          // wave-uniform
+        if (elig && off < base + kFlatChunk && off + nb > base) {
+            int pos = off;
+            for (int q = 0; q < 9; ++q) {
+                int lo[2], hi[2];
+                stencil_row(cell_start, b, cx, cy, cz, q, lo[0], hi[0], lo[1], hi[1]);
+                for (int h = 0; h < 2; ++h) {
+                    const int len = hi[h] - lo[h];
+                    const int a0 = pos > base ? pos : base;
+                    const int a1 = (pos + len < base + kFlatChunk) ? pos + len : base + kFlatChunk;
+                    for (int k = a0; k < a1; ++k)
+                        w.cand[k - base] = (int)((unsigned)(lo[h] + (k - pos)) | ((unsigned)lane << kPidShift));
+                    pos += len;
+                }
+            }
+        }
+        wave_sync_lds();
+        const int n = (total - base) < kFlatChunk ? total - base : kFlatChunk;
+        for (int r0 = 0; r0 < n; r0 += 64) {
+            const int k = r0 + lane;
+            if (k < n) {
+                const int e = w.cand[k];
+                const int t = e & kIdxMask, L = (int)((unsigned)e >> kPidShift);
+                const SortedBead o = so.bead[t];
+                const int tt = so.type[t], jt = sorted[t];
+                const int ct = EXACT ? so.cell[t] : 0;
+                const FlatOwner me = w.own[L];
+                const double len = list_len2(me.x - o.x, me.y - o.y, me.z - o.z, b);
+                bool take = len < r2 && t != me.slot && !(me.tag == o.tag && abs(me.loc - o.loc) <= 1);
+                int pid = 0;
+                if (take) {
+                    pid = pair_index(me.type, tt, b.types);
+                    const int pt = pairs[pid].type;
+                    take = pt >= 1 && pt <= 3;
+                    if (EXACT && take) {
+                        take = ct >= 0;               // FillGrid does not hash a bead outside the grid
+                        if (take && (pt == 1 || pt == 3)) take = len < b.cutoff_sq;
+                        if (take && pt == 2) {
+                            const double ri = radius[me.bead], si = skin[me.bead];
+                            const double rj = radius[jt], sj = skin[jt];
+                            const double min_skin = (si > sj) ? sj : si;
+                            const double max_r = (ri > rj) ? ri : rj;
+                            const double th = max_r + min_skin;
+                            take = len < th * th;
+                        }
+                    }
+                }
+                if (take) {
+                    const int p = atomicAdd(&w.cnt[L], 1);
+                    if (p < K) scratch[(size_t)me.slot * K + p] = jt | (pid << kPidShift);
+                }
+            }
+        }
+        wave_sync_lds();
+    }
+    const int kc = w.cnt[lane];
+    if (live) cnt[i] = kc;
+    const int mx = wave_max(kc);
+    const int over = wave_sum(kc > K ? 1 : 0);
+    const int mocc = wave_max(occ);
+    const int big = wave_sum(occ > 64 ? 1 : 0);
+    if (lane == 0) {
+        raise_max(&stats[EXACT ? kStatMaxList : kStatMaxCand], mx);
+        if (over > 0) atomicAdd(&stats[kStatListOver], over);
+        raise_max(&stats[kStatMaxCell], mocc);
+        if (big > 0) atomicAdd(&stats[kStatCellsOver], big);
+    }
+}
+
 // The same search with the block's neighbourhood staged in LDS.  A block of
 // kStageBlock consecutive slots covers cells [c_lo, c_hi]; for every (dy, dz)
 // of the stencil the cells its beads can reach in that row offset form one
@@ -1222,9 +1390,13 @@ template <bool EXACT>
 static void search_t(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
                      const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
                      const Stencil &st, double r2, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
-                     const int *gate, hipStream_t s)
+                     bool flat, const int *gate, hipStream_t s)
 {
-    if (net)
+    const bool stencil3 = st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !st.all[0] && !st.all[1] && !st.all[2];
+    if (!net && flat && !staged && stencil3)
+        k_search_flat<EXACT><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, r2,
+                                                          cnt, scratch, K, stats, gate);
+    else if (net)
         k_search<EXACT, true><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b,
                                                            net, boxes, st, r2, cnt, scratch, K, stats, gate);
     else if (staged && st.w[1] <= 2 && st.w[2] <= 2)
@@ -1237,14 +1409,14 @@ static void search_t(const int *sorted, const int *cell_start, int n_cells, cons
 void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
             const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
             const Stencil &st, double r2, bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
-            const int *gate, hipStream_t s)
+            bool flat, const int *gate, hipStream_t s)
 {
     if (exact)
         search_t<true>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K,
-                       stats, n, staged, gate, s);
+                       stats, n, staged, flat, gate, s);
     else
         search_t<false>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K,
-                        stats, n, staged, gate, s);
+                        stats, n, staged, flat, gate, s);
 }
 void compact(const int *cnt, const int *cand_start, const int *slot_of, const int *scratch, int K, int n, int *cand,
              int cap, int *cnt_out, const int *gate, hipStream_t s)

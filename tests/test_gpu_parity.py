@@ -141,6 +141,27 @@ def test_chain_status_copies_follow_breaks_on_other_paths(need_gpu):
     assert np.sum(sa["status"] == 0) > 0
 
 
+@pytest.mark.parametrize("verlet", ["on", "off"])
+@pytest.mark.parametrize("workload", sorted(WORKLOADS))
+def test_flat_search_equals_per_bead_search(need_gpu, workload, verlet):
+    """The wave-flattened neighbour search lists the same pairs as the
+    per-bead search, for the exact rule and for kept candidates, in finite
+    and periodic boxes (rows that wrap)."""
+    net = WORKLOADS[workload]()
+    a, ia = engine_for(net, "f64")
+    b, ib = engine_for(net, "f64")
+    assert a.get_option("flat_search") == 1
+    b.set_option("flat_search", 0)
+    for e in (a, b):
+        e.set_option("verlet", 1 if verlet == "on" else 0)
+    for n in (1, 60, 100):
+        run_engine(a, net, n, fast=True)
+        run_engine(b, net, n, fast=True)
+        assert np.array_equal(a.export_pairs(), b.export_pairs())
+    assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
+    assert a.stats()["max_list_length"] == b.stats()["max_list_length"]
+
+
 # Candidate-list settings: kept with the default skin (30% of the reach), kept
 # with a tiny skin (rebuilt at most rebuilds), rebuilt every time, and the
 # automatic choice.
