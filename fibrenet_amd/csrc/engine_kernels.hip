@@ -261,8 +261,14 @@ constexpr int kChainOwned = 64 - 2 * kHalo;
 // values for inactive bonds and their summation order are those of the
 // per-kernel path, so both paths agree bit for bit.  Only owned lanes write
 // bond statuses and bead state.
+// Waves per SIMD the chain kernel is compiled for: five for the fp32 build
+// (95 registers, no spills), the compiler's own choice otherwise (the fp64
+// and ensemble forms would spill).
+template <typename T, bool ENS>
+constexpr int kChainWaves = (sizeof(T) == sizeof(float) && !ENS) ? 5 : 1;
+
 template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS>
-__global__ void __launch_bounds__(kBlock) k_bead_step_chain(Args<T, P> a)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChainWaves<T, ENS>, 8))) k_bead_step_chain(Args<T, P> a)
 {
     __shared__ PairC prs[kPairCap];
     __shared__ WaveContactLds<T> cl[kBlock / 64];
