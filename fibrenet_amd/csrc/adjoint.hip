@@ -320,6 +320,147 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
     }
 }
 
+// A wave's LDS for its contacts in k_adjoint_step_flat: the J Fb terms of
+// one pass, the lane that owns each entry, and the wave's scale adjoints.
+struct AdjWaveLds {
+    double3 term[kWaveChunk];
+    unsigned char own[kWaveChunk];
+    double g[kAdjPairsLds];
+};
+
+// k_adjoint_step for a plain network with at most kAdjPairsLds pair-table
+// entries, its contacts evaluated by the whole wave as in the forward's
+// contacts_finish: the wave's lists laid end to end, each lane evaluating
+// every 64th entry (the forward's mask, then the dual force) and leaving its
+// J Fb term in LDS, each bead then adding its own terms in list order.  The
+// scale adjoints Fb . F accumulate per evaluating lane and pair entry, are
+// summed over the wave by a fixed shuffle tree and over the block's waves in
+// order, into one row per block as in the LDSG form.
+template <typename T, typename P>
+__global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
+{
+    __shared__ PairC prs[kAdjPairCap];
+    __shared__ AdjWaveLds wl[kBlock / 64];
+    const int np = a.n_pairs;
+    for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
+    __syncthreads();
+    AdjWaveLds &w = wl[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const bool live = i < a.n;
+    const int ic = live ? i : a.n - 1;
+    const double3 fb = a.fb[ic];
+    double3 z{0.0, 0.0, 0.0};
+    if (live && a.springs) add_slots(z, a.sp_jv, a.sp_start[i], a.sp_start[i + 1]);
+    if (live && a.angles) add_slots(z, a.an_jv, a.an_start[i], a.an_start[i + 1]);
+    const Box &bf = a.force;
+    const P xi = a.px[ic], yi = a.py[ic], zi = a.pz[ic];
+    const T ri = a.radius[ic];
+    const int e0 = a.nb_start[ic], cnt = live ? a.nb_cnt[ic] : 0;
+    int incl = cnt;                                   // inclusive scan of the counts over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const int off = incl - cnt;
+    const int total = __shfl(incl, 63, 64);
+    double g[kAdjPairsLds];
+#pragma unroll
+    for (int p = 0; p < kAdjPairsLds; ++p) g[p] = 0.0;
+    for (int base = 0; base < total; base += kWaveChunk) {           // wave-uniform
+        const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
+        mark_owner(off, cnt, base, w.own);
+        wave_sync_lds();
+        for (int r = 0; r * 64 < n; ++r) {
+            const int k = r * 64 + lane;
+            int L;
+            const int e = owner_entry(k, k < n, e0, off - base, w.own, L);
+            const P xL = __shfl(xi, L, 64), yL = __shfl(yi, L, 64), zL = __shfl(zi, L, 64);
+            const T rL = __shfl(ri, L, 64);
+            const double3 fL{__shfl(fb.x, L, 64), __shfl(fb.y, L, 64), __shfl(fb.z, L, 64)};
+            double3 t{-0.0, -0.0, -0.0};
+            if (k < n) {
+                const int ent = a.nb_list[e];
+                const int j = ent & kIdxMask, pid = ent >> kPidShift;
+                const PairC pr = prs[pid];
+                P rx = xL - a.px[j], ry = yL - a.py[j], rz = zL - a.pz[j];
+                P len = rx * rx + ry * ry + rz * rz;
+                if (len > (P)bf.hbs) {           // ResolveContacts' minimum image (kernel.cl:844-870)
+                    if (rx * rx > (P)bf.hbs) rx += (rx > (P)0) ? (P)-bf.lx : (P)bf.lx;
+                    if (ry * ry > (P)bf.hbs) ry += (ry > (P)0) ? (P)-bf.ly : (P)bf.ly;
+                    if (rz * rz > (P)bf.hbs) rz += (rz > (P)0) ? (P)-bf.lz : (P)bf.lz;
+                    len = rx * rx + ry * ry + rz * rz;
+                }
+                V3<T> tf;
+                if (contact_force(pr, bf, rx, ry, rz, len, rL, a.radius, j, tf)) {   // the forward's mask
+                    const double rj = pr.type == 2 ? (double)a.radius[j] : 0.0;
+                    const DV3 F = contact_dual(pr, (double)rL, rj,
+                                               dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fL, a.fb[j])));
+                    t = double3{F.x.d, F.y.d, F.z.d};
+                    const double gv = dot_v(fL, F);
+#pragma unroll
+                    for (int p = 0; p < kAdjPairsLds; ++p)
+                        if (p == pid) g[p] += gv;
+                }
+            }
+            if (k < n) w.term[k] = t;
+        }
+        wave_sync_lds();
+        const int k0 = off > base ? off : base;
+        const int k1 = (off + cnt < base + n) ? off + cnt : base + n;
+        for (int k = k0; k < k1; ++k) {
+            const double3 t = w.term[k - base];
+            z.x += t.x;
+            z.y += t.y;
+            z.z += t.z;
+        }
+        wave_sync_lds();
+    }
+    if (live) {
+        // walls of a finite box (kernel.cl:252-261): diagonal
+        const Box &bd = a.drift;
+        if (bd.bc == 1) {
+            const double k = -bd.wall_k;
+            if (xi < (P)0 || xi > (P)bd.lx) z.x += k * fb.x;
+            if (yi < (P)0 || yi > (P)bd.ly) z.y += k * fb.y;
+            if (zi < (P)0 || zi > (P)bd.lz) z.z += k * fb.z;
+        }
+        double3 X = a.xb[i];
+        X.x += z.x;
+        X.y += z.y;
+        X.z += z.z;
+        const double lam = a.lam[i], h = a.hlf[i], h2 = a.hlf2[i];
+        const double dt = (double)(T)bd.dt, D = (double)(T)bd.damping;
+        const double3 vb{D * fb.x, D * fb.y, D * fb.z};
+        double3 W = a.wb[i];
+        const double3 fprev{h * W.x + lam * vb.x + h2 * X.x, h * W.y + lam * vb.y + h2 * X.y,
+                            h * W.z + lam * vb.z + h2 * X.z};
+        W.x += vb.x + dt * X.x;
+        W.y += vb.y + dt * X.y;
+        W.z += vb.z + dt * X.z;
+        a.xb[i] = X;
+        a.wb[i] = W;
+        // Fb of the previous step: its own force adjoint, its kick's h Wb and its seed
+        const double seed = (a.seed_prev && a.group[i]) ? a.seed_prev[0] : 0.0;
+        a.fb_next[i] = double3{fprev.x + h * W.x + seed, fprev.y + h * W.y, fprev.z + h * W.z};
+    }
+    // scale adjoints: the wave's sum by a fixed tree, then the block's waves in order
+#pragma unroll
+    for (int p = 0; p < kAdjPairsLds; ++p) {
+        double v = g[p];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) w.g[p] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < np) {
+        double sum = 0.0;
+        for (int q = 0; q < kBlock / 64; ++q) sum += wl[q].g[threadIdx.x];
+        a.g_pair[(size_t)blockIdx.x * np + threadIdx.x] += sum;
+    }
+}
+
 int adjoint_pair_rows(int n, int np, bool batched)
 {
     return (!batched && np <= kAdjPairsLds) ? (n + kBlock - 1) / kBlock : n;
@@ -451,7 +592,7 @@ void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s)
     if (a.n <= 0) return;
     const int nb = (a.n + kBlock - 1) / kBlock;
     if (a.net) k_adjoint_step<T, P, false, true><<<nb, kBlock, 0, s>>>(a);
-    else if (a.n_pairs <= kAdjPairsLds) k_adjoint_step<T, P, true, false><<<nb, kBlock, 0, s>>>(a);
+    else if (a.n_pairs <= kAdjPairsLds) k_adjoint_step_flat<T, P><<<nb, kBlock, 0, s>>>(a);
     else k_adjoint_step<T, P, false, false><<<nb, kBlock, 0, s>>>(a);
 }
 template <typename T>
