@@ -1415,7 +1415,7 @@ class Engine final : public fn_engine {
                 if (tot > cap_) grow_list((long long)tot * 2);
                 last_total_ = tot;
             }
-            fnk::compact(ncnt_.p, nb_start_.p, so_slot_.p, scratch_.p, K_, N_, final_list ? nb_list_.p : cand_.p, cap_,
+            fnk::compact(ncnt_.p, nb_start_.p, scratch_.p, K_, N_, final_list ? nb_list_.p : cand_.p, cap_,
                          final_list ? nb_cnt_.p : nullptr, gate, stream_);
             fnk::note_build(nb_start_.p + N_, stats_.p, gate, stream_);
             return;

@@ -71,7 +71,7 @@ void hash(const double *px, const double *py, const double *pz, int n, const Box
 void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
            const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
            const Sorted &so, const int *gate, hipStream_t s);
-// Partners over the stencil into scratch rows of K entries per slot: the
+// Partners over the stencil into scratch rows of K entries per bead: the
 // candidates within sqrt(r2), or (exact) the pairs BuildContactList lists;
 // staged: the neighbourhood of each block read through LDS; flat: the 3x3x3
 // stencil's candidates tested by whole waves (k_search_flat).  Then
@@ -81,7 +81,7 @@ void search(const int *sorted, const int *cell_start, int n_cells, const Sorted 
             const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
             const Stencil &st, double r2, bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
             bool flat, const int *gate, hipStream_t s);
-void compact(const int *cnt, const int *cand_start, const int *slot_of, const int *scratch, int K, int n, int *cand,
+void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
              int cap, int *cnt_out, const int *gate, hipStream_t s);
 // *need = 1 when a bead moved more than sqrt(lim2) since the snapshot.
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,

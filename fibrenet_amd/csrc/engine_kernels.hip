@@ -485,17 +485,32 @@ __global__ void __launch_bounds__(kBlock) k_hash(const double *px, const double 
 {
     if (gated_off(gate)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const Box &b = net_box<ENS>(bs, boxes, net, i);
-    const int cx = (int)floor(px[i] / b.sx);
-    const int cy = (int)floor(py[i] / b.sy);
-    const int cz = (int)floor(pz[i] / b.sz);
-    const bool in = cx >= 0 && cx < b.cx && cy >= 0 && cy < b.cy && cz >= 0 && cz < b.cz;
-    key[i] = in ? b.cell_base + cx + cy * b.cx + cz * b.cxy : -1;
-    const int qx = min(max(cx, 0), b.cx - 1), qy = min(max(cy, 0), b.cy - 1), qz = min(max(cz, 0), b.cz - 1);
-    const int c = b.cell_base + qx + qy * b.cx + qz * b.cxy;
-    ckey[i] = c;
-    rank[i] = atomicAdd(&count[c], 1);
+    const int lane = threadIdx.x & 63;
+    int c = -1;                                   // -1: no bead on this lane
+    if (i < n) {
+        const Box &b = net_box<ENS>(bs, boxes, net, i);
+        const int cx = (int)floor(px[i] / b.sx);
+        const int cy = (int)floor(py[i] / b.sy);
+        const int cz = (int)floor(pz[i] / b.sz);
+        const bool in = cx >= 0 && cx < b.cx && cy >= 0 && cy < b.cy && cz >= 0 && cz < b.cz;
+        key[i] = in ? b.cell_base + cx + cy * b.cx + cz * b.cxy : -1;
+        const int qx = min(max(cx, 0), b.cx - 1), qy = min(max(cy, 0), b.cy - 1), qz = min(max(cz, 0), b.cz - 1);
+        c = b.cell_base + qx + qy * b.cx + qz * b.cxy;
+        ckey[i] = c;
+    }
+    // consecutive beads (along a fibre) often share a cell: one atomic per
+    // run of equal cells among adjacent lanes, ranks within the run in lane order
+    const int prev = __shfl_up(c, 1, 64);
+    const bool start = lane == 0 || prev != c;
+    const unsigned long long starts = __ballot(start);
+    const unsigned long long below = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
+    const int lead = 63 - __builtin_clzll(below);                 // the run's first lane
+    const unsigned long long above = lane == 63 ? 0ull : (starts >> (lane + 1));
+    const int len = above ? __builtin_ctzll(above) + 1 : 64 - lane;  // run length, read at its first lane
+    int r0 = 0;
+    if (start && c >= 0) r0 = atomicAdd(&count[c], len);
+    const int base = __shfl(r0, lead, 64);
+    if (i < n) rank[i] = base + (lane - lead);
 }
 
 // Second pass of the counting sort: each bead goes to slot cell_start[cell] +
@@ -638,7 +653,7 @@ struct AxisRange {
 
 // Candidate search of the bead in slot s over the stencil's cells; the cells
 // of one x-row are one contiguous slot range unless the row wraps or is
-// clipped.  Up to K entries go to the slot's scratch row; cnt[bead] = count.
+// clipped.  Up to K entries go to the bead's scratch row; cnt[bead] = count.
 template <bool EXACT, bool ENS>
 __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int *cell_start, int n_cells, Sorted so,
                                                    const PairC *pairs_all, const double *radius, const double *skin,
@@ -662,7 +677,7 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
         const SortedBead me = so.bead[s];
         const int ti = so.type[s];
         const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
-        int *row = scratch + (size_t)s * K;
+        int *row = scratch + (size_t)i * K;
         const int per = b.bc == 0 ? 1 : 0;
         const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, st.w[1], st.all[1], per},
             az{cz, b.cz, st.w[2], st.all[2], per};
@@ -869,7 +884,7 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *sorted, const
                 }
                 if (take) {
                     const int p = atomicAdd(&w.cnt[L], 1);
-                    if (p < K) scratch[(size_t)me.slot * K + p] = jt | (pid << kPidShift);
+                    if (p < K) scratch[(size_t)me.bead * K + p] = jt | (pid << kPidShift);
                 }
             }
         }
@@ -963,7 +978,7 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
         const SortedBead me = so.bead[s];
         const int ti = so.type[s];
         const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
-        int *row = scratch + (size_t)s * K;
+        int *row = scratch + (size_t)i * K;
         const int per = b.bc == 0 ? 1 : 0;
         const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, wy, st.all[1], per},
             az{cz, b.cz, wz, st.all[2], per};
@@ -1013,14 +1028,13 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
     }
 }
 
-// Scratch rows (by slot) -> the bead-indexed candidate CSR, each bead's
+// Scratch rows (one per bead) -> the candidate CSR, each bead's
 // entries in ascending order.  The order of beads inside a cell follows the
 // atomic counting sort and is arbitrary; ordering the entries makes the
 // lists, and so the contact-force summation order, deterministic.
 // With cnt_out (the final list): also the list length of every bead.
-__global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *cand_start, const int *slot_of,
-                                                    const int *scratch, int K, int n, int *cand, int cap, int *cnt_out,
-                                                    const int *gate)
+__global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *cand_start, const int *scratch, int K,
+                                                    int n, int *cand, int cap, int *cnt_out, const int *gate)
 {
     if (gated_off(gate)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1030,7 +1044,7 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *c
     if (cnt_out) cnt_out[i] = m;
     if (m == 0) return;
     const int base = cand_start[i];
-    const int *row = scratch + (size_t)slot_of[i] * K;
+    const int *row = scratch + (size_t)i * K;
     int *out = cand + base;
     if (m <= 8) {
         int v[8];
@@ -1424,10 +1438,10 @@ void search(const int *sorted, const int *cell_start, int n_cells, const Sorted 
         search_t<false>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K,
                         stats, n, staged, flat, gate, s);
 }
-void compact(const int *cnt, const int *cand_start, const int *slot_of, const int *scratch, int K, int n, int *cand,
+void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
              int cap, int *cnt_out, const int *gate, hipStream_t s)
 {
-    k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, slot_of, scratch, K, n, cand, cap, cnt_out, gate);
+    k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, scratch, K, n, cand, cap, cnt_out, gate);
 }
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
                 const double *rz, int n, const Box &b, double lim2, int *need, hipStream_t s)
