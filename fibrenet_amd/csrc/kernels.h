@@ -346,15 +346,28 @@ __device__ __forceinline__ void angle_force(const AngleP<T> &g, P x1, P y1, P z1
     const double bax = (double)(x1 - tx), bay = (double)(y1 - ty), baz = (double)(z1 - tz);
     const double bcx = (double)(x3 - tx), bcy = (double)(y3 - ty), bcz = (double)(z3 - tz);
     if constexpr (sizeof(T) < sizeof(double)) {
-        // fp32 build away from the straight and the folded triple: the angle as
-        // atan2(|ba x bc|, ba . bc), well conditioned at any angle, and the
-        // perpendicular directions from the same cross product
-        // (|ba x (ba x bc)| = |ba| |ba x bc|).
+        // fp32 build: the angle as atan2(|ba x bc|, ba . bc), well conditioned
+        // at any angle, and the perpendicular directions from the same cross
+        // product (|ba x (ba x bc)| = |ba| |ba x bc|) ...
         const T fx = (T)bax, fy = (T)bay, fz = (T)baz, gx = (T)bcx, gy = (T)bcy, gz = (T)bcz;
         const T cx = fy * gz - fz * gy, cy = fz * gx - fx * gz, cz = fx * gy - fy * gx;
         const T c2 = cx * cx + cy * cy + cz * cz;
         const T la2 = fx * fx + fy * fy + fz * fz, lc2 = gx * gx + gy * gy + gz * gz;
-        if (c2 > (T)1e-10 * la2 * lc2) {
+        bool clamped = false, neg = false;
+        if (!(c2 > (T)1e-10 * la2 * lc2)) {
+            // ... except on a straight or folded triple whose fp64 cosine the
+            // reference clamps (rounded beyond -1 or 1, kernel.cl:311-312)
+            const double l_ba = tsqrt(bax * bax + bay * bay + baz * baz);
+            const double l_bc = tsqrt(bcx * bcx + bcy * bcy + bcz * bcz);
+            const double dotp = (bax * bcx + bay * bcy + baz * bcz) / (l_ba * l_bc);
+            clamped = dotp < -1.0 || dotp > 1.0;
+            neg = dotp < 0.0;
+        }
+        if (!clamped) {
+            if (!(c2 > (T)1e-30)) {               // no fp32 direction: no force
+                pa_out = pc_out = V3<T>{(T)0, (T)0, (T)0};
+                return;
+            }
             const float cn = hw_sqrt((float)c2);
             const T ang = atan2f(cn, fx * gx + fy * gy + fz * gz);
             const T q = (T)-2 * g.k * (ang - g.th0) * hw_rcp(cn);
@@ -367,16 +380,13 @@ __device__ __forceinline__ void angle_force(const AngleP<T> &g, P x1, P y1, P z1
             pc_out.z = (gy * cx - gx * cy) * sc;
             return;
         }
-        // the straight or the folded triple: the fp64 cosine and its clamp as
-        // below, acos near +-1 by its series, the perpendicular directions
-        // from the fp64 cross products (their rounding noise on an exactly
-        // straight segment is the direction) normalised in fp32
+        // the clamped cosine: the angle acos(-+0.999999) (its series), the
+        // perpendicular directions from the fp64 cross products (their
+        // rounding noise on an exactly straight segment is the direction)
+        // normalised in fp32
         const double l_ba = tsqrt(bax * bax + bay * bay + baz * baz);
         const double l_bc = tsqrt(bcx * bcx + bcy * bcy + bcz * bcz);
-        double dotp = (bax * bcx + bay * bcy + baz * bcz) / (l_ba * l_bc);
-        if (dotp < -1.0) dotp = -0.999999;
-        if (dotp > 1.0) dotp = 0.999999;
-        const T mag = (T)(-2.0 * (double)g.k * (acos_near_unit(dotp) - (double)g.th0));
+        const T mag = (T)(-2.0 * (double)g.k * (acos_near_unit(neg ? -0.999999 : 0.999999) - (double)g.th0));
         const double cbx = -bcx, cby = -bcy, cbz = -bcz;
         const double dx = bay * bcz - baz * bcy, dy = baz * bcx - bax * bcz, dz = bax * bcy - bay * bcx;
         const double pax = bay * dz - baz * dy, pay = baz * dx - bax * dz, paz = bax * dy - bay * dx;
