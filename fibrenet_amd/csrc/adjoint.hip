@@ -461,9 +461,220 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
     }
 }
 
-int adjoint_pair_rows(int n, int np, bool batched)
+// The adjoint step of a single network in the chain layout with at most
+// kAdjPairsLds pair entries, in one launch: each lane evaluates the spring
+// to its right and the angle centred on it as k_adj_springs / k_adj_angles
+// do, the terms reach the neighbouring lanes by shuffles, and each owned
+// bead adds them in its incidence-slot order (spring (i-1, i), spring
+// (i, i+1); angles at i-1, i, i+1) with the slot path's values, so the bead
+// adjoints equal k_adj_springs + k_adj_angles + k_adjoint_step_flat bit for
+// bit.  Contacts, walls and the update as in k_adjoint_step_flat; a wave
+// owns kChainOwned beads with kHalo lanes of neighbours on each side.
+template <typename T, typename P>
+__global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
 {
-    return (!batched && np <= kAdjPairsLds) ? (n + kBlock - 1) / kBlock : n;
+    __shared__ PairC prs[kAdjPairCap];
+    __shared__ AdjWaveLds wl[kBlock / 64];
+    const int np = a.n_pairs;
+    for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
+    __syncthreads();
+    AdjWaveLds &w = wl[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int i = (blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kChainOwned + lane - kHalo;
+    const bool in = i >= 0 && i < a.n;
+    const bool own = in && lane >= kHalo && lane < 64 - kHalo;
+    const int ic = in ? i : (i < 0 ? 0 : a.n - 1);
+    const double3 fb = a.fb[ic];
+    const P xi = a.ix[ic], yi = a.iy[ic], zi = a.iz[ic];
+    const int s = (in && a.springs) ? a.ch_spring[i] : -1;
+    const int q = (in && a.angles) ? a.ch_angle[i] : -1;
+    // neighbours i + 1 and i - 1 (meaningless in the outermost lanes, whose
+    // results no owned lane uses)
+    const P xn = __shfl_down(xi, 1, 64), yn = __shfl_down(yi, 1, 64), zn = __shfl_down(zi, 1, 64);
+    const P xp = __shfl_up(xi, 1, 64), yp = __shfl_up(yi, 1, 64), zp = __shfl_up(zi, 1, 64);
+    const double3 fbn{__shfl_down(fb.x, 1, 64), __shfl_down(fb.y, 1, 64), __shfl_down(fb.z, 1, 64)};
+    const double3 fbp{__shfl_up(fb.x, 1, 64), __shfl_up(fb.y, 1, 64), __shfl_up(fb.z, 1, 64)};
+    // spring (i, i+1): J Fb on p1 = i (p2 receives its negation)
+    double3 jr{0.0, 0.0, 0.0};
+    if (s >= 0 && s < a.n_springs && a.sp_on[s] == 1 && (a.sp_type[s] == 0 || a.sp_type[s] == 1)) {
+        const double3 u = sub3(fb, fbn);
+        const double3 d{(double)(xi - xn), (double)(yi - yn), (double)(zi - zn)};
+        const DV3 F = spring_dual((double)a.sp_k[s], a.sp_l0[s], dvec(d, u));
+        jr = double3{F.x.d, F.y.d, F.z.d};
+        if (own) a.g_spring[s] += dot_v(u, F);
+    }
+    // angle (i-1, i, i+1): J Fb on b1 (ja), b3 (jc) and b2 (-ja - jc)
+    double3 ja{0.0, 0.0, 0.0}, jc{0.0, 0.0, 0.0};
+    if (q >= 0 && q < a.n_angles && a.an_on[q] == 1) {
+        const double3 x2{(double)xi, (double)yi, (double)zi};
+        const double3 ba{(double)xp - x2.x, (double)yp - x2.y, (double)zp - x2.z};
+        const double3 bc{(double)xn - x2.x, (double)yn - x2.y, (double)zn - x2.z};
+        const double3 da = sub3(fbp, fb), dc = sub3(fbn, fb);
+        DV3 pa, pc;
+        angle_dual((double)a.an_k[q], (double)a.an_th0[q], dvec(ba, da), dvec(bc, dc), pa, pc);
+        ja = double3{pa.x.d, pa.y.d, pa.z.d};
+        jc = double3{pc.x.d, pc.y.d, pc.z.d};
+        if (own) {
+            V3<T> fa, fc;                             // the forward's own values
+            angle_force(AngleP<T>{1, a.an_type[q], a.an_th0[q], a.an_k[q]}, xp, yp, zp, xi, yi, zi, xn, yn, zn, fa,
+                        fc);
+            a.g_angle[q] += da.x * (double)fa.x + da.y * (double)fa.y + da.z * (double)fa.z +
+                            dc.x * (double)fc.x + dc.y * (double)fc.y + dc.z * (double)fc.z;
+        }
+    }
+    const int hsl = __shfl_up(s >= 0 ? 1 : 0, 1, 64);
+    const int hql = __shfl_up(q >= 0 ? 1 : 0, 1, 64), hqn = __shfl_down(q >= 0 ? 1 : 0, 1, 64);
+    const double3 jl{__shfl_up(jr.x, 1, 64), __shfl_up(jr.y, 1, 64), __shfl_up(jr.z, 1, 64)};
+    const double3 jcl{__shfl_up(jc.x, 1, 64), __shfl_up(jc.y, 1, 64), __shfl_up(jc.z, 1, 64)};
+    const double3 jan{__shfl_down(ja.x, 1, 64), __shfl_down(ja.y, 1, 64), __shfl_down(ja.z, 1, 64)};
+    double3 z{0.0, 0.0, 0.0};
+    if (a.springs) {
+        if (hsl) {              // bead i is p2 of spring (i-1, i)
+            z.x += -jl.x;
+            z.y += -jl.y;
+            z.z += -jl.z;
+        }
+        if (s >= 0) {           // p1 of spring (i, i+1)
+            z.x += jr.x;
+            z.y += jr.y;
+            z.z += jr.z;
+        }
+    }
+    if (a.angles) {
+        if (hql) {              // b3 of the angle at i-1
+            z.x += jcl.x;
+            z.y += jcl.y;
+            z.z += jcl.z;
+        }
+        if (q >= 0) {           // b2 of the angle at i
+            z.x += -ja.x - jc.x;
+            z.y += -ja.y - jc.y;
+            z.z += -ja.z - jc.z;
+        }
+        if (hqn) {              // b1 of the angle at i+1
+            z.x += jan.x;
+            z.y += jan.y;
+            z.z += jan.z;
+        }
+    }
+    // contacts, flattened over the wave as in k_adjoint_step_flat
+    const Box &bf = a.force;
+    const P pxi = a.px[ic], pyi = a.py[ic], pzi = a.pz[ic];
+    const T ri = a.radius[ic];
+    const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    const int off = incl - cnt;
+    const int total = __shfl(incl, 63, 64);
+    double g[kAdjPairsLds];
+#pragma unroll
+    for (int p = 0; p < kAdjPairsLds; ++p) g[p] = 0.0;
+    for (int base = 0; base < total; base += kWaveChunk) {           // wave-uniform
+        const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
+        mark_owner(off, cnt, base, w.own);
+        wave_sync_lds();
+        for (int r = 0; r * 64 < n; ++r) {
+            const int k = r * 64 + lane;
+            int L;
+            const int e = owner_entry(k, k < n, e0, off - base, w.own, L);
+            const P xL = __shfl(pxi, L, 64), yL = __shfl(pyi, L, 64), zL = __shfl(pzi, L, 64);
+            const T rL = __shfl(ri, L, 64);
+            const double3 fL{__shfl(fb.x, L, 64), __shfl(fb.y, L, 64), __shfl(fb.z, L, 64)};
+            double3 t{-0.0, -0.0, -0.0};
+            if (k < n) {
+                const int ent = a.nb_list[e];
+                const int j = ent & kIdxMask, pid = ent >> kPidShift;
+                const PairC pr = prs[pid];
+                P rx = xL - a.px[j], ry = yL - a.py[j], rz = zL - a.pz[j];
+                P len = rx * rx + ry * ry + rz * rz;
+                if (len > (P)bf.hbs) {           // ResolveContacts' minimum image (kernel.cl:844-870)
+                    if (rx * rx > (P)bf.hbs) rx += (rx > (P)0) ? (P)-bf.lx : (P)bf.lx;
+                    if (ry * ry > (P)bf.hbs) ry += (ry > (P)0) ? (P)-bf.ly : (P)bf.ly;
+                    if (rz * rz > (P)bf.hbs) rz += (rz > (P)0) ? (P)-bf.lz : (P)bf.lz;
+                    len = rx * rx + ry * ry + rz * rz;
+                }
+                V3<T> tf;
+                if (contact_force(pr, bf, rx, ry, rz, len, rL, a.radius, j, tf)) {   // the forward's mask
+                    const double rj = pr.type == 2 ? (double)a.radius[j] : 0.0;
+                    const DV3 F = contact_dual(pr, (double)rL, rj,
+                                               dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fL, a.fb[j])));
+                    t = double3{F.x.d, F.y.d, F.z.d};
+                    const double gv = dot_v(fL, F);
+#pragma unroll
+                    for (int p = 0; p < kAdjPairsLds; ++p)
+                        if (p == pid) g[p] += gv;
+                }
+            }
+            if (k < n) w.term[k] = t;
+        }
+        wave_sync_lds();
+        const int k0 = off > base ? off : base;
+        const int k1 = (off + cnt < base + n) ? off + cnt : base + n;
+        for (int k = k0; k < k1; ++k) {
+            const double3 t = w.term[k - base];
+            z.x += t.x;
+            z.y += t.y;
+            z.z += t.z;
+        }
+        wave_sync_lds();
+    }
+    if (own) {
+        // walls of a finite box (kernel.cl:252-261): diagonal
+        const Box &bd = a.drift;
+        if (bd.bc == 1) {
+            const double k = -bd.wall_k;
+            if (pxi < (P)0 || pxi > (P)bd.lx) z.x += k * fb.x;
+            if (pyi < (P)0 || pyi > (P)bd.ly) z.y += k * fb.y;
+            if (pzi < (P)0 || pzi > (P)bd.lz) z.z += k * fb.z;
+        }
+        double3 X = a.xb[i];
+        X.x += z.x;
+        X.y += z.y;
+        X.z += z.z;
+        const double lam = a.lam[i], h = a.hlf[i], h2 = a.hlf2[i];
+        const double dt = (double)(T)bd.dt, D = (double)(T)bd.damping;
+        const double3 vb{D * fb.x, D * fb.y, D * fb.z};
+        double3 W = a.wb[i];
+        const double3 fprev{h * W.x + lam * vb.x + h2 * X.x, h * W.y + lam * vb.y + h2 * X.y,
+                            h * W.z + lam * vb.z + h2 * X.z};
+        W.x += vb.x + dt * X.x;
+        W.y += vb.y + dt * X.y;
+        W.z += vb.z + dt * X.z;
+        a.xb[i] = X;
+        a.wb[i] = W;
+        const double seed = (a.seed_prev && a.group[i]) ? a.seed_prev[0] : 0.0;
+        a.fb_next[i] = double3{fprev.x + h * W.x + seed, fprev.y + h * W.y, fprev.z + h * W.z};
+    }
+#pragma unroll
+    for (int p = 0; p < kAdjPairsLds; ++p) {
+        double v = g[p];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        if (lane == 0) w.g[p] = v;
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < np) {
+        double sum = 0.0;
+        for (int qq = 0; qq < kBlock / 64; ++qq) sum += wl[qq].g[threadIdx.x];
+        a.g_pair[(size_t)blockIdx.x * np + threadIdx.x] += sum;
+    }
+}
+
+// Blocks of the chain-layout adjoint kernel for n beads.
+static inline int chain_adj_blocks(int n)
+{
+    constexpr int per_block = kChainOwned * (kBlock / 64);
+    return (n + per_block - 1) / per_block;
+}
+
+int adjoint_pair_rows(int n, int np, bool batched, bool chain)
+{
+    if (batched || np > kAdjPairsLds) return n;
+    return chain ? chain_adj_blocks(n) : (n + kBlock - 1) / kBlock;
 }
 
 // Positions (and images, when ix is not null) and bond statuses of one step
@@ -587,6 +798,10 @@ void sum_columns(const double *g, int n, int np, double *part, double *out, hipS
 template <typename T, typename P>
 void adjoint_step(const AdjArgs<T, P> &a, hipStream_t s)
 {
+    if (a.ch_spring && !a.net && a.n_pairs <= kAdjPairsLds) {
+        if (a.n > 0) k_adjoint_step_chain<T, P><<<chain_adj_blocks(a.n), kBlock, 0, s>>>(a);
+        return;
+    }
     if (a.springs > 0) k_adj_springs<T, P><<<(a.springs + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
     if (a.angles > 0) k_adj_angles<T, P><<<(a.angles + kBlock - 1) / kBlock, kBlock, 0, s>>>(a);
     if (a.n <= 0) return;

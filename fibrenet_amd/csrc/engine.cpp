@@ -2096,7 +2096,8 @@ class Engine final : public fn_engine {
         for (auto *d : {&fb_[0], &fb_[1], &xb_, &wb_}) d->ensure(N);
         gsp_.ensure(std::max(S_, 1));
         gan_.ensure(std::max(A_, 1));
-        const int prow = fnk::adjoint_pair_rows(N_, cols, ens);
+        const bool chain_adj = chain_ok_ && !ens;
+        const int prow = fnk::adjoint_pair_rows(N_, cols, ens, chain_adj);
         gpr_.ensure(std::max<size_t>((size_t)prow * cols, 1));
         gps_.ensure(std::max(np, 1));
         sp_jv_.ensure(std::max<size_t>(2 * (size_t)S_, 1));
@@ -2170,6 +2171,10 @@ class Engine final : public fn_engine {
                 a.group = group_.p;
                 a.seed_prev = seed_at(c + k - 1);
                 a.g_spring = gsp_.p; a.g_angle = gan_.p; a.g_pair = gpr_.p;
+                if (chain_adj) {
+                    a.ch_spring = ch_spring_.p;
+                    a.ch_angle = ch_angle_.p;
+                }
                 fnk::adjoint_step(a, stream_);
                 cur ^= 1;
             }

@@ -123,6 +123,7 @@ struct AdjArgs {
     double *g_spring, *g_angle;   // adjoints of the force scales of the bonds
     double *g_pair;               // and of the contact pair entries: adjoint_pair_rows rows of n_pairs
                                   // (batched: one row per bead, entries local to its network)
+    const int *ch_spring, *ch_angle;   // chain layout (single network): spring (i, i+1), angle at i; or null
 };
 // One adjoint step: the bond terms (when a.springs / a.angles), then the
 // bead update.
@@ -133,8 +134,8 @@ template <typename T>
 void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group, const double *seed,
                   const int *net, int n, hipStream_t s);
 // Rows of the contact scale adjoints g_pair (row * n_pairs + entry) that the
-// adjoint step accumulates for n beads.
-int adjoint_pair_rows(int n, int np, bool batched);
+// adjoint step accumulates for n beads (chain: bonds in the chain layout).
+int adjoint_pair_rows(int n, int np, bool batched, bool chain);
 // Batched ensemble: out[pair_base[k] + t] = sum of g[i * cols + t] over the
 // beads i in [bead_off[k], bead_off[k + 1]), t < pair_base[k + 1] - pair_base[k].
 void sum_network_columns(const double *g, int cols, const int *bead_off, const int *pair_base, int n_net,

@@ -250,8 +250,6 @@ __global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
 // Chain layout: a wave covers 64 consecutive beads and owns the middle 60;
 // two halo lanes on each side supply the images, spring terms and angle
 // terms that the owned beads need from their chain neighbours.
-constexpr int kHalo = 2;
-constexpr int kChainOwned = 64 - 2 * kHalo;
 
 // The bead kernel for networks whose bonds all follow the chain layout
 // (spring (i, i+1), angle (i-1, i, i+1), numbered along the chain).  Each

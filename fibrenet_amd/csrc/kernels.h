@@ -131,6 +131,11 @@ struct Args {
     int step_off;
 };
 
+// Chain layout: a wave owns kChainOwned consecutive beads, with kHalo lanes
+// on either side holding their neighbours.
+constexpr int kHalo = 2;
+constexpr int kChainOwned = 64 - 2 * kHalo;
+
 // Bits of a contact-list entry: partner index | pair-table index << kPidShift.
 constexpr int kPidShift = 26;
 constexpr int kIdxMask = (1 << kPidShift) - 1;
