@@ -97,9 +97,33 @@ def adjoint_net(seed=21, n_fibres=40, n_beads=12, box=(24.0, 24.0, 8.0), jitter=
 
 # Adjoint test networks by kind: penalty contacts in a finite box (with or
 # without failures), Lennard-Jones contacts, a periodic box.
+def adjoint_xlink(seed=23, n_links=40):
+    """adjoint_net plus HR springs between beads of different fibres (both
+    index orders) and its angles listed in reverse: bonds outside the chain
+    layout, so the adjoint sums bond terms through incidence slots."""
+    net = adjoint_net(seed=seed)
+    img = R.vec(net.beads, "image_position")
+    fib = np.asarray(net.info["tag"])
+    d = np.linalg.norm(img[:, None, :] - img[None, :, :], axis=2)
+    i, j = np.nonzero((d < 1.6) & (fib[:, None] != fib[None, :]))
+    keep = i < j
+    i, j = i[keep][:n_links], j[keep][:n_links]
+    extra = np.zeros(len(i), R.SPRING)
+    extra["type"] = 0
+    extra["status"] = 1
+    extra["p1"] = np.where(np.arange(len(i)) % 2 == 0, i, j)
+    extra["p2"] = np.where(np.arange(len(i)) % 2 == 0, j, i)
+    extra["rest_length"] = 0.9 * d[i, j]
+    extra["stiffness"] = 20.0
+    net.springs = np.concatenate([net.springs, extra])
+    net.angles = net.angles[::-1].copy()
+    return net
+
+
 ADJ_NETS = {"hs": lambda: adjoint_net(), "hs_fl": lambda: adjoint_net(hr_fl=True),
             "lj": lambda: adjoint_net(seed=41, contact="lj"),
-            "periodic": lambda: adjoint_net(seed=41, box=(20.0, 20.0, 8.0), periodic=True)}
+            "periodic": lambda: adjoint_net(seed=41, box=(20.0, 20.0, 8.0), periodic=True),
+            "xlink": lambda: adjoint_xlink()}
 
 
 def dual_run(net, u_pos=None, u_vel=None, round_lengths=True, scales=(1.0, 1.0, 1.0)):

@@ -116,11 +116,14 @@ def test_backward_needs_a_current_forward(need_gpu):
 # whose fp32 forces move it away from the fp64 one.
 @pytest.mark.parametrize("prec,kind,tol", [("f64", "hs", 1e-7), ("f64", "hs_fl", 1e-7), ("f32", "hs", 1e-3),
                                            ("f64", "lj", 1e-7), ("f32", "lj", 1e-3), ("f64", "periodic", 1e-7),
-                                           ("f32", "periodic", 1e-3)])
+                                           ("f32", "periodic", 1e-3), ("f64", "xlink", 1e-7),
+                                           ("f32", "xlink", 1e-3)])
 def test_matches_dual_number_reference(need_gpu, prec, kind, tol):
     net = ADJ_NETS[kind]()
     w = _weights()
-    _, _, s, g = _adjoint(net, prec, w)
+    eng, _, s, g = _adjoint(net, prec, w)
+    # every network but xlink is in the chain layout (one-launch adjoint step)
+    assert eng.get_option("chain_layout") == (0 if kind == "xlink" else 1)
     up, uv = _dirs(net)
     vals, ders = dual_run(net, up, uv).run(N_STEPS, GUF, _group(net), SAMPLES)
     ref = ders @ w
@@ -212,9 +215,11 @@ def _batch_members():
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_batched_adjoint_equals_single_networks(need_gpu, prec):
     """The ensemble's adjoint gives every network what its own engine gives:
-    samples and the per-bead derivatives bit for bit (the same kernels bead
-    by bead), the per-bond and per-pair-entry scale derivatives to rounding
-    (their sums are grouped differently)."""
+    samples and the per-bead derivatives bit for bit (the same terms in the
+    same order bead by bead: incidence slots in the ensemble, the chain
+    kernel's in-lane bonds for a single network), the per-bond and
+    per-pair-entry scale derivatives to rounding (their sums are grouped
+    differently)."""
     from fibrenet_amd import Ensemble
     nets = _batch_members()
     rs = np.random.default_rng(3)
