@@ -563,6 +563,25 @@ __device__ __forceinline__ void raise_max(int *p, int v)
     if (v > *(volatile int *)p) atomicMax(p, v);
 }
 
+// The search rows of n beads in one scratch array of n * K entries: the
+// first kRowHead entries of every bead packed densely (32 B per bead, read
+// coalesced by the compaction; most lists are that short), the rest in a
+// tail of K - kRowHead entries per bead after them.
+constexpr int kRowHead = 8;
+__device__ __forceinline__ size_t row_at(int i, int p, int n, int K)
+{
+    return p < kRowHead ? (size_t)i * kRowHead + p
+                        : (size_t)n * kRowHead + (size_t)i * (K - kRowHead) + (size_t)(p - kRowHead);
+}
+struct SearchRow {
+    int *scratch;
+    int i, n, K;
+    __device__ void put(int p, int v) const
+    {
+        if (p < K) scratch[row_at(i, p, n, K)] = v;
+    }
+};
+
 // BuildContactList's distance (kernel.cl:600-628): minimum image per
 // component, only in a periodic box and only beyond halfBoxSq.
 __device__ __forceinline__ double list_len2(double rx, double ry, double rz, const Box &b)
@@ -588,7 +607,7 @@ __device__ __forceinline__ void scan_range_from(int lo, int hi, SortedBead nxt, 
                                                 double ri, double si, const SortedBead *src, int shift,
                                                 const Sorted &so, const int *sorted, const PairC *pairs,
                                                 const double *radius, const double *skin, const Box &b, double r2,
-                                                int *row, int K, int &k)
+                                                const SearchRow &row, int &k)
 {
     for (int t = lo; t < hi; ++t) {
         const SortedBead o = nxt;                  // the next record is in flight while this one is tested
@@ -613,7 +632,7 @@ __device__ __forceinline__ void scan_range_from(int lo, int hi, SortedBead nxt, 
             }
             if (!take) continue;
         }
-        if (k < K) row[k] = sorted[t] | (pid << kPidShift);
+        row.put(k, sorted[t] | (pid << kPidShift));
         ++k;
     }
 }
@@ -623,11 +642,11 @@ template <bool EXACT>
 __device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBead &me, int ti, double ri, double si,
                                            const SortedBead *src, int shift, const Sorted &so, const int *sorted,
                                            const PairC *pairs, const double *radius, const double *skin,
-                                           const Box &b, double r2, int *row, int K, int &k)
+                                           const Box &b, double r2, const SearchRow &row, int &k)
 {
     if (lo >= hi) return;
     scan_range_from<EXACT>(lo, hi, src[lo - shift], s, me, ti, ri, si, src, shift, so, sorted, pairs, radius, skin,
-                           b, r2, row, K, k);
+                           b, r2, row, k);
 }
 
 // Cells visited along one axis around cell c: c-w..c+w, wrapped (periodic) or
@@ -675,7 +694,7 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
         const SortedBead me = so.bead[s];
         const int ti = so.type[s];
         const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
-        int *row = scratch + (size_t)i * K;
+        const SearchRow row{scratch, i, cell_start[n_cells], K};
         const int per = b.bc == 0 ? 1 : 0;
         const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, st.w[1], st.all[1], per},
             az{cz, b.cz, st.w[2], st.all[2], per};
@@ -698,7 +717,7 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
 #pragma unroll
             for (int q = 0; q < 9; ++q)
                 scan_range<EXACT>(lo[q], hi[q], s, me, ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b, r2,
-                                  row, K, k);
+                                  row, k);
         } else
         for (int dz = az.first(); dz <= az.last() && !(EXACT && code < 0); ++dz) {
             const int z = az.at(dz);
@@ -709,13 +728,13 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
                 const int rbase = b.cell_base + y * b.cx + z * b.cxy;
                 if (row_run) {
                     scan_range<EXACT>(cell_start[rbase + cx - st.w[0]], cell_start[rbase + cx + st.w[0] + 1], s, me,
-                                      ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b, r2, row, K, k);
+                                      ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b, r2, row, k);
                 } else {
                     for (int dx = ax.first(); dx <= ax.last(); ++dx) {
                         const int x = ax.at(dx);
                         if (x < 0) continue;
                         scan_range<EXACT>(cell_start[rbase + x], cell_start[rbase + x + 1], s, me, ti, ri, si, so.bead,
-                                          0, so, sorted, pairs, radius, skin, b, r2, row, K, k);
+                                          0, so, sorted, pairs, radius, skin, b, r2, row, k);
                     }
                 }
             }
@@ -800,7 +819,8 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *sorted, const
     const int lane = threadIdx.x & 63;
     const int s = xcd_block() * kBlock + threadIdx.x;
     const PairC *pairs = pairs_all + b.pair_base;
-    const bool live = s < cell_start[n_cells];
+    const int n_slots = cell_start[n_cells];
+    const bool live = s < n_slots;
     int i = 0, code = -1, occ = 0, cx = 0, cy = 0, cz = 0;
     bool elig = false;
     if (live) {
@@ -882,7 +902,7 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *sorted, const
                 }
                 if (take) {
                     const int p = atomicAdd(&w.cnt[L], 1);
-                    if (p < K) scratch[(size_t)me.bead * K + p] = jt | (pid << kPidShift);
+                    if (p < K) scratch[row_at(me.bead, p, n_slots, K)] = jt | (pid << kPidShift);
                 }
             }
         }
@@ -976,7 +996,7 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
         const SortedBead me = so.bead[s];
         const int ti = so.type[s];
         const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
-        int *row = scratch + (size_t)i * K;
+        const SearchRow row{scratch, i, cell_start[n_cells], K};
         const int per = b.bc == 0 ? 1 : 0;
         const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, wy, st.all[1], per},
             az{cz, b.cz, wz, st.all[2], per};
@@ -996,10 +1016,10 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
                     if (lo >= hi) return;
                     if (lo >= slo && hi <= shi)
                         scan_range<EXACT>(lo, hi, s, me, ti, ri, si, rec + sbase, slo, so, sorted, pairs, radius, skin,
-                                          b, r2, row, K, k);
+                                          b, r2, row, k);
                     else
                         scan_range<EXACT>(lo, hi, s, me, ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b,
-                                          r2, row, K, k);
+                                          r2, row, k);
                 };
                 if (row_run) {
                     scan(cell_start[rbase + cx - st.w[0]], cell_start[rbase + cx + st.w[0] + 1]);
@@ -1042,12 +1062,15 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *c
     if (cnt_out) cnt_out[i] = m;
     if (m == 0) return;
     const int base = cand_start[i];
-    const int *row = scratch + (size_t)i * K;
     int *out = cand + base;
-    if (m <= 8) {
+    if (m <= kRowHead) {
+        // the head: two aligned 16-byte loads (entries past m are masked)
+        const int4 h0 = *(const int4 *)(scratch + (size_t)i * kRowHead);
+        const int4 h1 = *(const int4 *)(scratch + (size_t)i * kRowHead + 4);
+        const int hv[8] = {h0.x, h0.y, h0.z, h0.w, h1.x, h1.y, h1.z, h1.w};
         int v[8];
 #pragma unroll
-        for (int k = 0; k < 8; ++k) v[k] = k < m ? row[k] : 0x7fffffff;
+        for (int k = 0; k < 8; ++k) v[k] = k < m ? hv[k] : 0x7fffffff;
 #pragma unroll
         for (int p = 0; p < 8; ++p)          // odd-even transposition sort of 8 registers
 #pragma unroll
@@ -1063,7 +1086,7 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *c
         return;
     }
     for (int k = 0; k < m; ++k) {
-        const int v = row[k];
+        const int v = scratch[row_at(i, k, n, K)];
         int j = k - 1;
         while (j >= 0 && out[j] > v) {
             out[j + 1] = out[j];
