@@ -126,25 +126,63 @@ def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8):
             "grid_update_freq": guf, "generate_s": gen_s}
 
 
-def adjoint_bench(eng, net, steps, plain_ms, every):
-    """Adjoint cost on the benchmark network, from the engine's current
-    state: the checkpointing forward over `steps` steps sampling the clamp's
-    reaction force 20 times, then the backward sweep (segment recompute,
-    per-step records, adjoint kernel) with dL/d(sample) = 1.  Ratios are
-    against the plain forward's ms per step."""
+def bead_state(eng, ids):
+    """Regime of the network after the timed steps: non-finite beads and the
+    largest speed among the finite ones."""
     import numpy as np
+    from fibrenet_amd import records as R
+    b = eng.read_buffer(ids["bead"], R.BEAD)
+    v = R.vec(b, "velocity_old")
+    ok = np.isfinite(R.vec(b, "position")).all(1) & np.isfinite(v).all(1)
+    return {"nonfinite_beads": int((~ok).sum()), "max_speed": float(np.linalg.norm(v[ok], axis=1).max())}
+
+
+def adjoint_bench(net, precision, options, warmup, steps, every):
+    """Adjoint cost on the benchmark network, on an engine of its own started
+    from the warmed-up state (`warmup` steps), where the dynamics are still
+    regular: the checkpointing forward over `steps` steps sampling the
+    clamp's reaction force 20 times, then the backward sweep (segment
+    recompute, per-step records, adjoint kernel) with dL/d(sample) = 1.  The
+    plain forward is timed on the same engine over the next `steps` steps
+    (graph-captured, after a capture run), and the ratios are against it.
+
+    The randomly placed fibres of this workload cross one another, and with
+    Lennard-Jones contacts the overlapping beads are launched: beads that
+    leave the finite box are not hashed (kernel.cl:494-499), lose their
+    contacts and can come back overlapping other beads.  From about step 900
+    the largest speed grows from ~60 to 1e6 and beyond, in the fp64 build
+    as in the fp32 one, so later states carry non-finite derivatives."""
+    import numpy as np
+    from fibrenet_amd import Engine, bind_standard
     group = (np.asarray(net.info["type"]) == 1).astype(np.int32)
     samples = np.unique(np.linspace(steps // 20 - 1, steps - 1, 20).astype(np.int32))
-    eng.finish()
-    t0 = time.perf_counter()
-    eng.adjoint_forward(steps, group, samples, net.grid_update_freq, True, True, every)
-    t1 = time.perf_counter()
-    g = eng.adjoint_backward(np.ones(len(samples)), len(net.springs), len(net.angles), len(net.pairs))
-    t2 = time.perf_counter()
-    fwd, bwd = (t1 - t0) * 1e3 / steps, (t2 - t1) * 1e3 / steps
-    return {"steps": steps, "checkpoint_every": every or "auto", "samples": len(samples),
-            "forward_ms_per_step": fwd, "backward_ms_per_step": bwd,
-            "backward_over_forward": bwd / plain_ms, "total_over_forward": (fwd + bwd) / plain_ms,
+    guf = net.grid_update_freq
+    eng = Engine(precision)
+    try:
+        bind_standard(eng, net)
+        for k, v in options.items():
+            eng.set_option(k, v)
+        if warmup > 0:
+            eng.run_steps(warmup, guf, True, True)
+        eng.finish()
+        t0 = time.perf_counter()
+        eng.adjoint_forward(steps, group, samples, guf, True, True, every)
+        t1 = time.perf_counter()
+        g = eng.adjoint_backward(np.ones(len(samples)), len(net.springs), len(net.angles), len(net.pairs))
+        t2 = time.perf_counter()
+        eng.run_steps(steps, guf, True, True)          # graph capture
+        eng.finish()
+        t3 = time.perf_counter()
+        eng.run_steps(steps, guf, True, True)
+        t4 = time.perf_counter()
+    finally:
+        eng.close()
+    fwd, bwd, plain = (t1 - t0) * 1e3 / steps, (t2 - t1) * 1e3 / steps, (t4 - t3) * 1e3 / steps
+    finite = all(bool(np.isfinite(g[k]).all()) for k in ("spring", "angle", "pair", "position", "velocity_old"))
+    return {"steps": steps, "start_step": warmup, "checkpoint_every": every or "auto", "samples": len(samples),
+            "plain_forward_ms_per_step": plain, "forward_ms_per_step": fwd, "backward_ms_per_step": bwd,
+            "backward_over_forward": bwd / plain, "total_over_forward": (fwd + bwd) / plain,
+            "gradients_finite": finite,
             "d_spring_scale_sum": float(g["spring"].sum()), "d_angle_scale_sum": float(g["angle"].sum()),
             "d_contact_scale": [float(x) for x in g["pair"]]}
 
@@ -447,7 +485,7 @@ def main():
     gen_s = time.perf_counter() - t0
 
     eng = Engine(args.precision)
-    bind_standard(eng, net)
+    ids = bind_standard(eng, net)
     spr, ang = len(net.springs) > 0, len(net.angles) > 0
     if args.warmup > 0:
         eng.run_steps(args.warmup, net.grid_update_freq, spr, ang)
@@ -481,6 +519,7 @@ def main():
     wall = time.perf_counter() - t
     tim = eng.timing()
     st = eng.stats()
+    state = bead_state(eng, ids)
     eng.set_timing(False)
     # The same again without timing (no event nodes, no synchronisation per chunk).
     eng.run_steps(args.steps, net.grid_update_freq, spr, ang)   # capture
@@ -520,7 +559,8 @@ def main():
     adj = None
     if args.adjoint_steps > 0:
         try:
-            adj = adjoint_bench(eng, net, args.adjoint_steps, wall * 1e3 / args.steps, args.checkpoint_every)
+            opts = {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet")}
+            adj = adjoint_bench(net, args.precision, opts, args.warmup, args.adjoint_steps, args.checkpoint_every)
         except Exception as exc:          # the plain measurement stands on its own
             adj = {"error": repr(exc)}
 
@@ -562,7 +602,8 @@ def main():
                   "untimed_ms_per_step": wall_graph * 1e3 / args.steps,
                   "rebuild_ms": rebuild_ms, "rebuild_algorithmic_bytes": b_rebuild,
                   "step_algorithmic_GBps_incl_rebuild": (b_step + b_rebuild / net.grid_update_freq)
-                  * args.steps / wall / 1e9, "counts": counts, "stats": st, "generate_s": gen_s,
+                  * args.steps / wall / 1e9, "counts": counts, "stats": st, "state_after_timed": state,
+                  "generate_s": gen_s,
                   "gpu_over_cpu": (value / cpu["value"]) if cpu else None, "copy_bytes": copy,
                   "ensemble": ens, "adjoint": adj, "autotune": tune},
     }

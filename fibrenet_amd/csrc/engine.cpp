@@ -2096,7 +2096,7 @@ class Engine final : public fn_engine {
         for (auto *d : {&fb_[0], &fb_[1], &xb_, &wb_}) d->ensure(N);
         gsp_.ensure(std::max(S_, 1));
         gan_.ensure(std::max(A_, 1));
-        const bool chain_adj = chain_ok_ && !ens;
+        const bool chain_adj = chain_ok_ && chain_on_ && !ens;
         const int prow = fnk::adjoint_pair_rows(N_, cols, ens, chain_adj);
         gpr_.ensure(std::max<size_t>((size_t)prow * cols, 1));
         gps_.ensure(std::max(np, 1));

@@ -133,6 +133,30 @@ def test_matches_dual_number_reference(need_gpu, prec, kind, tol):
     assert np.all(err <= tol), (got, ref, err)
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_chain_adjoint_equals_slot_path(need_gpu, prec):
+    """The one-launch chain-layout adjoint step (option chain = 1) and the
+    bond kernels with incidence slots (chain = 0) give the same bead and bond
+    adjoints bit for bit, on a network whose HR_FL springs break; the contact
+    scale adjoints are summed over different block rows, so they agree to
+    rounding."""
+    net = adjoint_net(hr_fl=True)
+    w = _weights()
+    out = []
+    for chain in (1, 0):
+        eng, _ = engine_for(net, prec)
+        eng.set_option("chain", chain)
+        assert eng.get_option("chain_layout") == 1
+        eng.adjoint_forward(N_STEPS, _group(net), SAMPLES, grid_update_freq=GUF, checkpoint_every=50)
+        assert np.sum(eng.export_breaks(len(net.springs)) >= 0) > 0
+        out.append(eng.adjoint_backward(w, len(net.springs), len(net.angles), len(net.pairs)))
+    for k in ("position", "velocity_old", "spring", "angle"):
+        assert np.all(np.isfinite(out[0][k])), k
+        assert out[0][k].tobytes() == out[1][k].tobytes(), k
+    np.testing.assert_allclose(out[0]["pair"], out[1]["pair"], rtol=1e-12,
+                               atol=1e-12 * np.abs(out[1]["pair"]).max())
+
+
 def test_matches_central_differences(need_gpu):
     """h = 1e-3 relative on each force scale, on the fp64 engine.  The
     forward rounds spring and contact lengths to float (native_sqrt, as the
