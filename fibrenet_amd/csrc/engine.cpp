@@ -1184,7 +1184,6 @@ class Engine final : public fn_engine {
         key_.ensure(N_);
         ckey_.ensure(N_);
         rank_.ensure(N_);
-        sorted_.ensure(N_);
         need_.ensure(1);
         for (auto &r : ref_) r.ensure(N_);
         if (!nb_list_.p) grow_list(std::max<long long>(32LL * N_, 1 << 20));
@@ -1291,12 +1290,12 @@ class Engine final : public fn_engine {
                   stream_);
         fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, gate, stream_);
         fnk::place(key_.p, ckey_.p, rank_.p, cell_start_.p, pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, type_.p, tag_.p,
-                   loc_.p, N_, sorted_.p, sorted_view(), gate, stream_);
+                   loc_.p, N_, sorted_view(), gate, stream_);
     }
 
     fnk::Sorted sorted_view()
     {
-        return fnk::Sorted{so_bead_.p, so_type_.p, so_cell_.p, so_slot_.p};
+        return fnk::Sorted{so_bead_.p};
     }
 
     // Largest squared listing radius over the pair types in use: cutoff^2 for
@@ -1382,7 +1381,7 @@ class Engine final : public fn_engine {
 
     void ensure_scratch()
     {
-        so_bead_.ensure(N_); so_type_.ensure(N_); so_cell_.ensure(N_); so_slot_.ensure(N_);
+        so_bead_.ensure(N_);
         if (scratch_.n < (size_t)N_ * K_) {
             clear_graphs();
             scratch_.alloc((size_t)N_ * K_);
@@ -1399,7 +1398,7 @@ class Engine final : public fn_engine {
         for (;;) {
             if (sync) fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
             fnk::fill_gated(ncnt_.p, 0, N_, gate, stream_);
-            fnk::search(sorted_.p, cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b, ens_net(),
+            fnk::search(cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b, ens_net(),
                         boxes_.p, final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, final_list,
                         ncnt_.p, scratch_.p, K_, stats_.p, N_, staged_, flat_, gate, stream_);
             fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, gate, stream_);
@@ -2349,7 +2348,7 @@ class Engine final : public fn_engine {
     DArr<P> rec_pos_;                      // per step of a segment: positions and images of the force evaluation
     DArr<int> rec_st_, rec_nb_;            // per step: bond statuses; per rebuild: the contact list
     DArr<fnk::PairC> pairs_;
-    DArr<int> key_, ckey_, rank_, cell_count_, cell_start_, sorted_, ncnt_, nb_start_, nb_cnt_, cand_, nb_list_, scan_tmp_,
+    DArr<int> key_, ckey_, rank_, cell_count_, cell_start_, ncnt_, nb_start_, nb_cnt_, cand_, nb_list_, scan_tmp_,
         stats_, step_, need_;
     DArr<double> ref_[3];   // positions at the last candidate build
     // ensemble: Box and settings of every network, first bead / spring / angle
@@ -2363,7 +2362,7 @@ class Engine final : public fn_engine {
     DArr<fnk::Box> boxes_;
     DArr<int> net_;
     DArr<fnk::SortedBead> so_bead_;
-    DArr<int> so_type_, so_cell_, so_slot_, scratch_;
+    DArr<int> scratch_;
     double reach2_ = 0.0;
 
     std::vector<fn_bead_info> h_info_;

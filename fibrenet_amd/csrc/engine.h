@@ -69,7 +69,7 @@ struct Launch {
 void hash(const double *px, const double *py, const double *pz, int n, const Box &b, const int *net,
           const Box *boxes, int *key, int *ckey, int *rank, int *count, const int *gate, hipStream_t s);
 void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
-           const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
+           const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n,
            const Sorted &so, const int *gate, hipStream_t s);
 // Partners over the stencil into scratch rows of K entries per bead: the
 // candidates within sqrt(r2), or (exact) the pairs BuildContactList lists;
@@ -77,10 +77,10 @@ void place(const int *key, const int *ckey, const int *rank, const int *cell_sta
 // stencil's candidates tested by whole waves (k_search_flat).  Then
 // compaction into a bead-indexed CSR sorted per bead (with cnt_out: the final
 // list, its lengths and the listed total).
-void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
-            const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
-            const Stencil &st, double r2, bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
-            bool flat, const int *gate, hipStream_t s);
+void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs, const double *radius,
+            const double *skin, const Box &b, const int *net, const Box *boxes, const Stencil &st, double r2,
+            bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat, const int *gate,
+            hipStream_t s);
 void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
              int cap, int *cnt_out, const int *gate, hipStream_t s);
 // *need = 1 when a bead moved more than sqrt(lim2) since the snapshot.

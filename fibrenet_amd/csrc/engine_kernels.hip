@@ -512,13 +512,13 @@ __global__ void __launch_bounds__(kBlock) k_hash(const double *px, const double 
 }
 
 // Second pass of the counting sort: each bead goes to slot cell_start[cell] +
-// rank with its data (fp64 position, tag, loc, type), read in bead order;
-// slot_of is the inverse permutation.  The slot's cell is stored as c for a
-// bead inside the grid and as ~c for one placed by its clamped position.
+// rank as one record (fp64 position, tag, loc, type, cell, bead index), read
+// in bead order.  The slot's cell is stored as c for a bead inside the grid
+// and as ~c for one placed by its clamped position.
 __global__ void __launch_bounds__(kBlock) k_place(const int *key, const int *ckey, const int *rank,
                                                   const int *cell_start, const double *px, const double *py,
                                                   const double *pz, const int *type, const int *tag, const int *loc,
-                                                  int n, int *sorted, Sorted so, const int *gate)
+                                                  int n, Sorted so, const int *gate)
 {
     if (gated_off(gate)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -531,11 +531,11 @@ __global__ void __launch_bounds__(kBlock) k_place(const int *key, const int *cke
     b.z = pz[i];
     b.tag = tag[i];
     b.loc = loc[i];
+    b.type = type[i];
+    b.cell = key[i] >= 0 ? c : ~c;
+    b.bead = i;
+    b.pad = 0;
     so.bead[s] = b;
-    so.type[s] = type[i];
-    so.cell[s] = key[i] >= 0 ? c : ~c;
-    sorted[s] = i;
-    so.slot_of[i] = s;
 }
 
 // Cell of a sorted slot (see k_place) and whether its bead is inside the grid.
@@ -603,11 +603,10 @@ __device__ __forceinline__ double list_len2(double rx, double ry, double rz, con
 // cheap first rejection).  Slot t's record is src[t - shift] (global memory
 // or a staged copy in LDS).
 template <bool EXACT>
-__device__ __forceinline__ void scan_range_from(int lo, int hi, SortedBead nxt, int s, const SortedBead &me, int ti,
+__device__ __forceinline__ void scan_range_from(int lo, int hi, SortedBead nxt, int s, const SortedBead &me,
                                                 double ri, double si, const SortedBead *src, int shift,
-                                                const Sorted &so, const int *sorted, const PairC *pairs,
-                                                const double *radius, const double *skin, const Box &b, double r2,
-                                                const SearchRow &row, int &k)
+                                                const PairC *pairs, const double *radius, const double *skin,
+                                                const Box &b, double r2, const SearchRow &row, int &k)
 {
     for (int t = lo; t < hi; ++t) {
         const SortedBead o = nxt;                  // the next record is in flight while this one is tested
@@ -615,15 +614,15 @@ __device__ __forceinline__ void scan_range_from(int lo, int hi, SortedBead nxt, 
         const double len = list_len2(me.x - o.x, me.y - o.y, me.z - o.z, b);
         if (!(len < r2) || t == s) continue;
         if (me.tag == o.tag && abs(me.loc - o.loc) <= 1) continue;
-        const int pid = pair_index(ti, so.type[t], b.types);
+        const int pid = pair_index(me.type, o.type, b.types);
         const int pt = pairs[pid].type;
         if (pt < 1 || pt > 3) continue;
         if (EXACT) {
-            if (so.cell[t] < 0) continue;   // FillGrid does not hash a bead outside the grid
+            if (o.cell < 0) continue;   // FillGrid does not hash a bead outside the grid
             bool take = false;
             if (pt == 1 || pt == 3) take = len < b.cutoff_sq;
             if (pt == 2) {
-                const int j = sorted[t];
+                const int j = o.bead;
                 const double rj = radius[j], sj = skin[j];
                 const double min_skin = (si > sj) ? sj : si;
                 const double max_r = (ri > rj) ? ri : rj;
@@ -632,21 +631,20 @@ __device__ __forceinline__ void scan_range_from(int lo, int hi, SortedBead nxt, 
             }
             if (!take) continue;
         }
-        row.put(k, sorted[t] | (pid << kPidShift));
+        row.put(k, o.bead | (pid << kPidShift));
         ++k;
     }
 }
 
 // The same over slots [lo, hi), loading the first record itself.
 template <bool EXACT>
-__device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBead &me, int ti, double ri, double si,
-                                           const SortedBead *src, int shift, const Sorted &so, const int *sorted,
-                                           const PairC *pairs, const double *radius, const double *skin,
-                                           const Box &b, double r2, const SearchRow &row, int &k)
+__device__ __forceinline__ void scan_range(int lo, int hi, int s, const SortedBead &me, double ri, double si,
+                                           const SortedBead *src, int shift, const PairC *pairs,
+                                           const double *radius, const double *skin, const Box &b, double r2,
+                                           const SearchRow &row, int &k)
 {
     if (lo >= hi) return;
-    scan_range_from<EXACT>(lo, hi, src[lo - shift], s, me, ti, ri, si, src, shift, so, sorted, pairs, radius, skin,
-                           b, r2, row, k);
+    scan_range_from<EXACT>(lo, hi, src[lo - shift], s, me, ri, si, src, shift, pairs, radius, skin, b, r2, row, k);
 }
 
 // Cells visited along one axis around cell c: c-w..c+w, wrapped (periodic) or
@@ -672,7 +670,7 @@ struct AxisRange {
 // of one x-row are one contiguous slot range unless the row wraps or is
 // clipped.  Up to K entries go to the bead's scratch row; cnt[bead] = count.
 template <bool EXACT, bool ENS>
-__global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int *cell_start, int n_cells, Sorted so,
+__global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_cells, Sorted so,
                                                    const PairC *pairs_all, const double *radius, const double *skin,
                                                    Box bs, const int *net, const Box *boxes, Stencil st, double r2,
                                                    int *cnt, int *scratch, int K, int *stats, const int *gate)
@@ -681,18 +679,17 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
     const int s = xcd_block() * kBlock + threadIdx.x;   // neighbouring slot ranges share one XCD's L2
     int k = 0, occ = 0;
     if (s < cell_start[n_cells]) {
-        const int i = sorted[s];
+        const SortedBead me = so.bead[s];
+        const int i = me.bead;
         const Box &b = net_box<ENS>(bs, boxes, net, i);
         const PairC *pairs = pairs_all + b.pair_base;
-        const int code = so.cell[s];
+        const int code = me.cell;
         const int cg = slot_cell(code);
         if (cell_start[cg] == s) occ = cell_start[cg + 1] - s;   // the cell's first slot reports its occupancy
         const int c = cg - b.cell_base;                          // cell within the bead's network
         const int cz = c / b.cxy;
         const int cy = (c - cz * b.cxy) / b.cx;
         const int cx = c - cz * b.cxy - cy * b.cx;
-        const SortedBead me = so.bead[s];
-        const int ti = so.type[s];
         const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
         const SearchRow row{scratch, i, cell_start[n_cells], K};
         const int per = b.bc == 0 ? 1 : 0;
@@ -716,8 +713,7 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
             }
 #pragma unroll
             for (int q = 0; q < 9; ++q)
-                scan_range<EXACT>(lo[q], hi[q], s, me, ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b, r2,
-                                  row, k);
+                scan_range<EXACT>(lo[q], hi[q], s, me, ri, si, so.bead, 0, pairs, radius, skin, b, r2, row, k);
         } else
         for (int dz = az.first(); dz <= az.last() && !(EXACT && code < 0); ++dz) {
             const int z = az.at(dz);
@@ -728,13 +724,13 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *sorted, const int 
                 const int rbase = b.cell_base + y * b.cx + z * b.cxy;
                 if (row_run) {
                     scan_range<EXACT>(cell_start[rbase + cx - st.w[0]], cell_start[rbase + cx + st.w[0] + 1], s, me,
-                                      ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b, r2, row, k);
+                                      ri, si, so.bead, 0, pairs, radius, skin, b, r2, row, k);
                 } else {
                     for (int dx = ax.first(); dx <= ax.last(); ++dx) {
                         const int x = ax.at(dx);
                         if (x < 0) continue;
-                        scan_range<EXACT>(cell_start[rbase + x], cell_start[rbase + x + 1], s, me, ti, ri, si, so.bead,
-                                          0, so, sorted, pairs, radius, skin, b, r2, row, k);
+                        scan_range<EXACT>(cell_start[rbase + x], cell_start[rbase + x + 1], s, me, ri, si, so.bead,
+                                          0, pairs, radius, skin, b, r2, row, k);
                     }
                 }
             }
@@ -787,8 +783,10 @@ __device__ __forceinline__ void stencil_row(const int *cell_start, const Box &b,
     }
 }
 
-// Candidates of a wave staged per pass, and the owner records.
+// Candidates of a wave staged per pass, and the owner records; pair-table
+// entries whose types the flat search keeps in LDS.
 constexpr int kFlatChunk = 768;
+constexpr int kFlatPairCap = 64;
 struct FlatOwner {
     double x, y, z;
     int tag, loc, type, slot, bead, pad;
@@ -808,24 +806,32 @@ struct WaveSearchLds {
 // k_search's, in another order, which the compaction's per-bead sort
 // removes.
 template <bool EXACT>
-__global__ void __launch_bounds__(kBlock) k_search_flat(const int *sorted, const int *cell_start, int n_cells,
-                                                        Sorted so, const PairC *pairs_all, const double *radius,
+__global__ void __launch_bounds__(kBlock) k_search_flat(const int *cell_start, int n_cells, Sorted so,
+                                                        const PairC *pairs_all, const double *radius,
                                                         const double *skin, Box b, double r2, int *cnt, int *scratch,
                                                         int K, int *stats, const int *gate)
 {
     if (gated_off(gate)) return;
     __shared__ WaveSearchLds wl[kBlock / 64];
+    __shared__ int ptab[kFlatPairCap];
     WaveSearchLds &w = wl[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const int s = xcd_block() * kBlock + threadIdx.x;
     const PairC *pairs = pairs_all + b.pair_base;
+    // pair types of the table in LDS (the table of a handful of bead types)
+    const int np = b.types * (b.types + 1) / 2;
+    const bool lds_pairs = np <= kFlatPairCap;
+    if (lds_pairs)
+        for (int k = threadIdx.x; k < np; k += kBlock) ptab[k] = pairs[k].type;
+    __syncthreads();
     const int n_slots = cell_start[n_cells];
     const bool live = s < n_slots;
     int i = 0, code = -1, occ = 0, cx = 0, cy = 0, cz = 0;
     bool elig = false;
     if (live) {
-        i = sorted[s];
-        code = so.cell[s];
+        const SortedBead me = so.bead[s];
+        i = me.bead;
+        code = me.cell;
         const int cg = slot_cell(code);
         if (cell_start[cg] == s) occ = cell_start[cg + 1] - s;
         const int c = cg - b.cell_base;
@@ -833,8 +839,7 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *sorted, const
         cy = (c - cz * b.cxy) / b.cx;
         cx = c - cz * b.cxy - cy * b.cx;
         elig = !(EXACT && code < 0);
-        const SortedBead me = so.bead[s];
-        w.own[lane] = FlatOwner{me.x, me.y, me.z, me.tag, me.loc, so.type[s], s, i, 0};
+        w.own[lane] = FlatOwner{me.x, me.y, me.z, me.tag, me.loc, me.type, s, i, 0};
     }
     w.cnt[lane] = 0;
     int nb = 0;
@@ -877,15 +882,15 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *sorted, const
                 const int e = w.cand[k];
                 const int t = e & kIdxMask, L = (int)((unsigned)e >> kPidShift);
                 const SortedBead o = so.bead[t];
-                const int tt = so.type[t], jt = sorted[t];
-                const int ct = EXACT ? so.cell[t] : 0;
+                const int jt = o.bead;
+                const int ct = EXACT ? o.cell : 0;
                 const FlatOwner me = w.own[L];
                 const double len = list_len2(me.x - o.x, me.y - o.y, me.z - o.z, b);
                 bool take = len < r2 && t != me.slot && !(me.tag == o.tag && abs(me.loc - o.loc) <= 1);
                 int pid = 0;
                 if (take) {
-                    pid = pair_index(me.type, tt, b.types);
-                    const int pt = pairs[pid].type;
+                    pid = pair_index(me.type, o.type, b.types);
+                    const int pt = lds_pairs ? ptab[pid] : pairs[pid].type;
                     take = pt >= 1 && pt <= 3;
                     if (EXACT && take) {
                         take = ct >= 0;               // FillGrid does not hash a bead outside the grid
@@ -931,11 +936,11 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *sorted, const
 // range lies inside its staged interval, and from global memory otherwise
 // (rows that wrap around a periodic box, intervals that did not fit).
 constexpr int kStageBlock = 128;
-constexpr int kStageCap = 1248;       // records (32 B each) staged per block
+constexpr int kStageCap = 832;        // records (48 B each) staged per block
 constexpr int kStageRows = 25;        // (2w+1)^2 for w <= 2
 
 template <bool EXACT>
-__global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted, const int *cell_start, int n_cells,
+__global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *cell_start, int n_cells,
                                                                Sorted so, const PairC *pairs, const double *radius,
                                                                const double *skin, Box b, Stencil st, double r2,
                                                                int *cnt, int *scratch, int K, int *stats,
@@ -951,7 +956,7 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
     const int wy = st.w[1], wz = st.w[2], ny = 2 * wy + 1, nq = ny * (2 * wz + 1);
     if ((int)threadIdx.x < nq) {          // one thread per row offset: the interval's slots
         const int q = threadIdx.x;
-        const int c_lo = slot_cell(so.cell[s0]), c_hi = slot_cell(so.cell[s_end - 1]);
+        const int c_lo = slot_cell(so.bead[s0].cell), c_hi = slot_cell(so.bead[s_end - 1].cell);
         const int dy = q % ny - wy, dz = q / ny - wz;
         const long long off = (long long)dy * b.cx + (long long)dz * b.cxy;
         const long long a = max(0LL, (long long)c_lo - st.w[0] + off);
@@ -986,15 +991,14 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
     const int s = s0 + threadIdx.x;
     int k = 0, occ = 0;
     if (s < s_end) {
-        const int i = sorted[s];
-        const int code = so.cell[s];
+        const SortedBead me = so.bead[s];
+        const int i = me.bead;
+        const int code = me.cell;
         const int c = slot_cell(code);
         if (cell_start[c] == s) occ = cell_start[c + 1] - s;
         const int cz = c / b.cxy;
         const int cy = (c - cz * b.cxy) / b.cx;
         const int cx = c - cz * b.cxy - cy * b.cx;
-        const SortedBead me = so.bead[s];
-        const int ti = so.type[s];
         const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
         const SearchRow row{scratch, i, cell_start[n_cells], K};
         const int per = b.bc == 0 ? 1 : 0;
@@ -1015,11 +1019,10 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
                 auto scan = [&](int lo, int hi) {
                     if (lo >= hi) return;
                     if (lo >= slo && hi <= shi)
-                        scan_range<EXACT>(lo, hi, s, me, ti, ri, si, rec + sbase, slo, so, sorted, pairs, radius, skin,
-                                          b, r2, row, k);
+                        scan_range<EXACT>(lo, hi, s, me, ri, si, rec + sbase, slo, pairs, radius, skin, b, r2, row,
+                                          k);
                     else
-                        scan_range<EXACT>(lo, hi, s, me, ti, ri, si, so.bead, 0, so, sorted, pairs, radius, skin, b,
-                                          r2, row, k);
+                        scan_range<EXACT>(lo, hi, s, me, ri, si, so.bead, 0, pairs, radius, skin, b, r2, row, k);
                 };
                 if (row_run) {
                     scan(cell_start[rbase + cx - st.w[0]], cell_start[rbase + cx + st.w[0] + 1]);
@@ -1051,18 +1054,15 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *sorted
 // atomic counting sort and is arbitrary; ordering the entries makes the
 // lists, and so the contact-force summation order, deterministic.
 // With cnt_out (the final list): also the list length of every bead.
-__global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *cand_start, const int *scratch, int K,
-                                                    int n, int *cand, int cap, int *cnt_out, const int *gate)
+// A wave's beads are consecutive, so their lists form one range of the CSR:
+// when it fits kCompactStage entries, each bead sorts its row into the
+// wave's LDS and the wave writes the range with consecutive lanes on
+// consecutive entries; otherwise each bead writes its own list.
+constexpr int kCompactStage = 1024;
+
+// Sorts bead i's row of m entries (m > 0) into out[0..m).
+__device__ __forceinline__ void sort_row(const int *scratch, int i, int m, int n, int K, int *out)
 {
-    if (gated_off(gate)) return;
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    int m = min(cnt[i], K);
-    if (cand_start[i] + m > cap) m = 0;   // capacity exceeded: the host reports the run invalid
-    if (cnt_out) cnt_out[i] = m;
-    if (m == 0) return;
-    const int base = cand_start[i];
-    int *out = cand + base;
     if (m <= kRowHead) {
         // the head: two aligned 16-byte loads (entries past m are masked)
         const int4 h0 = *(const int4 *)(scratch + (size_t)i * kRowHead);
@@ -1094,6 +1094,31 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *c
         }
         out[j + 1] = v;
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *cand_start, const int *scratch, int K,
+                                                    int n, int *cand, int cap, int *cnt_out, const int *gate)
+{
+    if (gated_off(gate)) return;
+    __shared__ int stage[kBlock / 64][kCompactStage];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    if (i - lane >= n) return;                         // wave-uniform: no bead in this wave
+    const bool live = i < n;
+    int m = live ? min(cnt[i], K) : 0;
+    const int base = live ? cand_start[i] : 0;
+    if (base + m > cap) m = 0;   // capacity exceeded: the host reports the run invalid
+    if (live && cnt_out) cnt_out[i] = m;
+    const int wbase = __shfl(base, 0, 64);             // lane 0 holds a bead
+    const int span = wave_max(m > 0 ? base + m : wbase) - wbase;   // every written entry lies below cap
+    if (span > kCompactStage) {                        // wave-uniform
+        if (m > 0) sort_row(scratch, i, m, n, K, cand + base);
+        return;
+    }
+    int *w = stage[threadIdx.x >> 6];
+    if (m > 0) sort_row(scratch, i, m, n, K, w + (base - wbase));
+    wave_sync_lds();
+    for (int k = lane; k < span; k += 64) cand[wbase + k] = w[k];
 }
 
 // Sets *need when some bead has moved more than sqrt(lim2) from its position
@@ -1421,43 +1446,42 @@ void hash(const double *px, const double *py, const double *pz, int n, const Box
     else k_hash<false><<<blocks(n), kBlock, 0, s>>>(px, py, pz, n, b, nullptr, nullptr, key, ckey, rank, count, gate);
 }
 void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
-           const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n, int *sorted,
+           const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n,
            const Sorted &so, const int *gate, hipStream_t s)
 {
-    k_place<<<blocks(n), kBlock, 0, s>>>(key, ckey, rank, cell_start, px, py, pz, type, tag, loc, n, sorted, so,
-                                         gate);
+    k_place<<<blocks(n), kBlock, 0, s>>>(key, ckey, rank, cell_start, px, py, pz, type, tag, loc, n, so, gate);
 }
 template <bool EXACT>
-static void search_t(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
-                     const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
-                     const Stencil &st, double r2, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
-                     bool flat, const int *gate, hipStream_t s)
+static void search_t(const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs, const double *radius,
+                     const double *skin, const Box &b, const int *net, const Box *boxes, const Stencil &st, double r2,
+                     int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat, const int *gate,
+                     hipStream_t s)
 {
     const bool stencil3 = st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !st.all[0] && !st.all[1] && !st.all[2];
     if (!net && flat && !staged && stencil3)
-        k_search_flat<EXACT><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, r2,
-                                                          cnt, scratch, K, stats, gate);
+        k_search_flat<EXACT><<<blocks(n), kBlock, 0, s>>>(cell_start, n_cells, so, pairs, radius, skin, b, r2, cnt,
+                                                          scratch, K, stats, gate);
     else if (net)
-        k_search<EXACT, true><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b,
-                                                           net, boxes, st, r2, cnt, scratch, K, stats, gate);
+        k_search<EXACT, true><<<blocks(n), kBlock, 0, s>>>(cell_start, n_cells, so, pairs, radius, skin, b, net,
+                                                           boxes, st, r2, cnt, scratch, K, stats, gate);
     else if (staged && st.w[1] <= 2 && st.w[2] <= 2)
         k_search_staged<EXACT><<<(n + kStageBlock - 1) / kStageBlock, kStageBlock, 0, s>>>(
-            sorted, cell_start, n_cells, so, pairs, radius, skin, b, st, r2, cnt, scratch, K, stats, gate);
+            cell_start, n_cells, so, pairs, radius, skin, b, st, r2, cnt, scratch, K, stats, gate);
     else
-        k_search<EXACT, false><<<blocks(n), kBlock, 0, s>>>(sorted, cell_start, n_cells, so, pairs, radius, skin, b,
+        k_search<EXACT, false><<<blocks(n), kBlock, 0, s>>>(cell_start, n_cells, so, pairs, radius, skin, b,
                                                             nullptr, nullptr, st, r2, cnt, scratch, K, stats, gate);
 }
-void search(const int *sorted, const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs,
-            const double *radius, const double *skin, const Box &b, const int *net, const Box *boxes,
-            const Stencil &st, double r2, bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged,
-            bool flat, const int *gate, hipStream_t s)
+void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs, const double *radius,
+            const double *skin, const Box &b, const int *net, const Box *boxes, const Stencil &st, double r2,
+            bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat, const int *gate,
+            hipStream_t s)
 {
     if (exact)
-        search_t<true>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K,
-                       stats, n, staged, flat, gate, s);
+        search_t<true>(cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K, stats,
+                       n, staged, flat, gate, s);
     else
-        search_t<false>(sorted, cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K,
-                        stats, n, staged, flat, gate, s);
+        search_t<false>(cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K, stats,
+                        n, staged, flat, gate, s);
 }
 void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
              int cap, int *cnt_out, const int *gate, hipStream_t s)

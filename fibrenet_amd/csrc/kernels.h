@@ -34,20 +34,20 @@ struct Box {
     double wall_k, hbs, cutoff_sq, damping, gx, gy, gz, dt;
 };
 
-// One bead in cell order: what the neighbour search reads per candidate, in
-// two 16-byte loads.
+// One bead in cell order: everything the neighbour search reads per
+// candidate, in three 16-byte loads (and written by the placement as one
+// 48-byte record): position, tag, loc, type, the slot's cell (c, or ~c for a
+// bead outside the grid, placed by its clamped position) and the bead index.
 struct alignas(16) SortedBead {
     double x, y, z;
-    int tag, loc;
+    int tag, loc, type, cell, bead, pad;
 };
+static_assert(sizeof(SortedBead) == 48, "three 16-byte loads per sorted record");
 
-// Bead data copied into cell order at a rebuild (the neighbour search reads
-// these contiguously), the cell of each slot (c, or ~c for a bead outside
-// the grid, placed by its clamped position) and the inverse permutation
-// bead -> slot.
+// Bead records copied into cell order at a rebuild (the neighbour search
+// reads them contiguously).
 struct Sorted {
     SortedBead *bead;
-    int *type, *cell, *slot_of;
 };
 
 // One pair-table entry (Pair.h:5-16): comparisons use the fp64 values.
