@@ -225,10 +225,10 @@ __device__ __forceinline__ void bead_end(const Args<T, P> &a, const Box &bx, int
 template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS>
 __global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
 {
-    __shared__ PairC prs[kPairCap];
+    __shared__ PairTab<T> prs[kPairCap];
     __shared__ WaveContactLds<T> cl[kBlock / 64];
     const int i = xcd_block() * kBlock + threadIdx.x;
-    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = a.pairs[k];
+    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = to_tab<T>(a.pairs[k]);
     __syncthreads();
     const bool live = i < a.n;
     const int ic = live ? i : a.n - 1;      // lanes past the end follow along for the wave-wide contacts
@@ -268,9 +268,9 @@ constexpr int kChainWaves = (sizeof(T) == sizeof(float) && !ENS) ? 5 : 1;
 template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChainWaves<T, ENS>, 8))) k_bead_step_chain(Args<T, P> a)
 {
-    __shared__ PairC prs[kPairCap];
+    __shared__ PairTab<T> prs[kPairCap];
     __shared__ WaveContactLds<T> cl[kBlock / 64];
-    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = a.pairs[k];
+    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = to_tab<T>(a.pairs[k]);
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int i = (xcd_block() * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kChainOwned + lane - kHalo;

@@ -56,6 +56,63 @@ struct PairC {
     double c1, c2, c3, c4, c5, c6;
 };
 
+// The same entry in the form the fp32 contact terms use: the float
+// conversions and products that every contact would otherwise repeat, and c3
+// rounded up to a float, so that r < c3f holds exactly when (double)r < c3
+// for a float r.  The two-step harmonic's bound stays fp64 (it is compared
+// with the fp64 squared distance).
+struct alignas(16) PairF {
+    int type;
+    float lj_s2, lj_e48, c3f;   // ((float)c1)^2, 48 (float)c2, the least float >= c3
+    float c4f, c1f, c2f, c5f;
+    float c6f, pad;
+    double c4;
+};
+
+// The least float not below v (v rounded to nearest, then one float up when
+// that fell below v).
+__device__ __forceinline__ float float_at_least(double v)
+{
+    float f = (float)v;
+    if ((double)f < v) {
+        const int bits = __float_as_int(f);
+        f = f == 0.0f ? __int_as_float(1) : __int_as_float(f > 0.0f ? bits + 1 : bits - 1);
+    }
+    return f;
+}
+
+__device__ __forceinline__ PairF pair_f(const PairC &p)
+{
+    PairF q;
+    q.type = p.type;
+    q.c1f = (float)p.c1;
+    q.c2f = (float)p.c2;
+    q.c4f = (float)p.c4;
+    q.c5f = (float)p.c5;
+    q.c6f = (float)p.c6;
+    q.lj_s2 = q.c1f * q.c1f;
+    q.lj_e48 = 48.0f * q.c2f;
+    q.c3f = float_at_least(p.c3);
+    q.pad = 0.0f;
+    q.c4 = p.c4;
+    return q;
+}
+
+// The pair-table entry a build's bead kernels keep in LDS: PairF for the
+// fp32 build, PairC for the fp64 one.
+template <typename T>
+struct PairTabOf { using type = PairC; };
+template <>
+struct PairTabOf<float> { using type = PairF; };
+template <typename T>
+using PairTab = typename PairTabOf<T>::type;
+template <typename T>
+__device__ __forceinline__ PairTab<T> to_tab(const PairC &p)
+{
+    if constexpr (sizeof(T) == sizeof(float)) return pair_f(p);
+    else return p;
+}
+
 template <typename T>
 struct V3 { T x, y, z; };
 
@@ -441,6 +498,49 @@ __device__ __forceinline__ void add_terms(const V3<T> *term, int e0, int e1, V3<
     }
 }
 
+// fp32 build: force on bead 1 of a listed pair from r = x1 - x_j at its
+// minimum image (len = |r|^2), bead 1's radius r1 and the radii table; false
+// when the pair exerts no force this step.  The terms of contact_force below
+// through the hardware square root and reciprocal, one reciprocal of r^2
+// shared by the Lennard-Jones ratios and 1/r = r / r^2.
+template <typename P>
+__device__ __forceinline__ bool contact_force_f(const PairF &pr, const Box &b, P rx, P ry, P rz, P len, float r1,
+                                                const float *radius, int j, V3<float> &t)
+{
+    float coef, r, inv;
+    if (pr.type == 1) {                           // Lennard-Jones, kernel.cl:875-919
+        if (!((double)len < b.cutoff_sq)) return false;
+        const float r2 = (float)len;
+        r = hw_sqrt(r2);
+        if (!(r < pr.c3f)) return false;
+        const float rr = hw_rcp(r2);
+        const float fr2 = pr.lj_s2 * rr;
+        const float fr6 = fr2 * fr2 * fr2;
+        coef = pr.lj_e48 * fr6 * (fr6 - 0.5f) * rr;
+        coef -= pr.c4f;
+        inv = r * rr;
+    } else if (pr.type == 2) {                    // penalty, kernel.cl:922-957
+        r = hw_sqrt((float)len);
+        coef = (r1 + radius[j]) - r;
+        if (!(coef > 0.0f)) return false;
+        coef *= pr.c1f;
+        inv = hw_rcp(r);
+    } else if (pr.type == 3) {                    // two-step harmonic, kernel.cl:960-999
+        if (!((double)len < pr.c4)) return false;
+        r = hw_sqrt((float)len);
+        coef = (pr.c2f - r) * pr.c1f;
+        coef += (pr.c6f - r) * pr.c5f;
+        inv = hw_rcp(r);
+    } else {
+        return false;
+    }
+    if (r == 0.0f) inv = 0.0f;
+    t.x = coef * ((float)rx * inv);
+    t.y = coef * ((float)ry * inv);
+    t.z = coef * ((float)rz * inv);
+    return true;
+}
+
 // Force on bead 1 of a listed pair from r = x1 - x_j at its minimum image
 // (len = |r|^2), bead 1's radius r1 and the radii table; false when the pair
 // exerts no force this step.
@@ -448,75 +548,47 @@ template <typename T, typename P>
 __device__ __forceinline__ bool contact_force(const PairC &pr, const Box &b, P rx, P ry, P rz, P len, T r1,
                                               const T *radius, int j, V3<T> &t)
 {
-    T coef;
-    T r;
     if constexpr (sizeof(T) == sizeof(float)) {
-        // fp32 build: the same terms through the hardware square root and
-        // reciprocal, one reciprocal of r^2 shared by the Lennard-Jones ratios
-        // and 1/r = r / r^2
-        float inv;
-        if (pr.type == 1) {
+        return contact_force_f(pair_f(pr), b, rx, ry, rz, len, r1, radius, j, t);
+    } else {
+        T coef;
+        T r;
+        if (pr.type == 1) {                       // Lennard-Jones, kernel.cl:875-919
             if (!((double)len < b.cutoff_sq)) return false;
-            const float r2 = (float)len;
-            r = hw_sqrt(r2);
+            r = (T)native_sqrt((double)len);
             if (!((double)r < pr.c3)) return false;
-            const float rr = hw_rcp(r2);
-            const float c1 = (float)pr.c1;
-            const float fr2 = c1 * c1 * rr;
-            const float fr6 = fr2 * fr2 * fr2;
-            coef = 48.0f * (float)pr.c2 * fr6 * (fr6 - 0.5f) * rr;
-            coef -= (float)pr.c4;
-            inv = r * rr;
-        } else if (pr.type == 2) {
-            r = hw_sqrt((float)len);
+            const T r2 = (T)len;
+            const T c1 = (T)pr.c1;
+            const T sigma2 = c1 * c1;
+            const T fr2 = sigma2 / r2;
+            const T fr6 = fr2 * fr2 * fr2;
+            coef = (T)48 * (T)pr.c2 * fr6 * (fr6 - (T)0.5) / r2;
+            coef -= (T)pr.c4;
+        } else if (pr.type == 2) {                // penalty, kernel.cl:922-957
+            r = (T)native_sqrt((double)len);
             coef = (r1 + radius[j]) - r;
-            if (!(coef > 0.0f)) return false;
-            coef *= (float)pr.c1;
-            inv = hw_rcp(r);
-        } else if (pr.type == 3) {
+            if (!(coef > (T)0)) return false;
+            coef *= (T)pr.c1;
+        } else if (pr.type == 3) {                // two-step harmonic, kernel.cl:960-999
             if (!((double)len < pr.c4)) return false;
-            r = hw_sqrt((float)len);
-            coef = ((float)pr.c2 - r) * (float)pr.c1;
-            coef += ((float)pr.c6 - r) * (float)pr.c5;
-            inv = hw_rcp(r);
+            r = (T)native_sqrt((double)len);
+            coef = ((T)pr.c2 - r) * (T)pr.c1;
+            coef += ((T)pr.c6 - r) * (T)pr.c5;
         } else {
             return false;
         }
-        if (r == 0.0f) inv = 0.0f;
-        t.x = coef * ((float)rx * inv);
-        t.y = coef * ((float)ry * inv);
-        t.z = coef * ((float)rz * inv);
+        const T inv = (r != (T)0) ? (T)1 / r : (T)0;
+        t.x = coef * ((T)rx * inv);
+        t.y = coef * ((T)ry * inv);
+        t.z = coef * ((T)rz * inv);
         return true;
     }
-    if (pr.type == 1) {                           // Lennard-Jones, kernel.cl:875-919
-        if (!((double)len < b.cutoff_sq)) return false;
-        r = (T)native_sqrt((double)len);
-        if (!((double)r < pr.c3)) return false;
-        const T r2 = (T)len;
-        const T c1 = (T)pr.c1;
-        const T sigma2 = c1 * c1;
-        const T fr2 = sigma2 / r2;
-        const T fr6 = fr2 * fr2 * fr2;
-        coef = (T)48 * (T)pr.c2 * fr6 * (fr6 - (T)0.5) / r2;
-        coef -= (T)pr.c4;
-    } else if (pr.type == 2) {                    // penalty, kernel.cl:922-957
-        r = (T)native_sqrt((double)len);
-        coef = (r1 + radius[j]) - r;
-        if (!(coef > (T)0)) return false;
-        coef *= (T)pr.c1;
-    } else if (pr.type == 3) {                    // two-step harmonic, kernel.cl:960-999
-        if (!((double)len < pr.c4)) return false;
-        r = (T)native_sqrt((double)len);
-        coef = ((T)pr.c2 - r) * (T)pr.c1;
-        coef += ((T)pr.c6 - r) * (T)pr.c5;
-    } else {
-        return false;
-    }
-    const T inv = (r != (T)0) ? (T)1 / r : (T)0;
-    t.x = coef * ((T)rx * inv);
-    t.y = coef * ((T)ry * inv);
-    t.z = coef * ((T)rz * inv);
-    return true;
+}
+template <typename P>
+__device__ __forceinline__ bool contact_force(const PairF &pr, const Box &b, P rx, P ry, P rz, P len, float r1,
+                                              const float *radius, int j, V3<float> &t)
+{
+    return contact_force_f(pr, b, rx, ry, rz, len, r1, radius, j, t);
 }
 
 // Force on bead 1 (position x1.., radius r1) from the contact-list entry
@@ -562,12 +634,22 @@ __device__ __forceinline__ void gather_contacts(const Args<T, P> &a, int i, V3<T
 constexpr int kWaveChunk = 256;
 constexpr int kChunkRounds = kWaveChunk / 64;
 
-// A wave's LDS for its contacts: the terms of one pass and the lane that
-// owns each entry of it.
+// The bead of one lane as the wave's contact passes read it: image-free
+// position, radius and network.
+template <typename T>
+struct alignas(16) ContactOwner {
+    double x, y, z;
+    T r;
+    int net;
+};
+
+// A wave's LDS for its contacts: the terms of one pass, the lane that owns
+// each entry of it, and every lane's bead.
 template <typename T>
 struct WaveContactLds {
     V3<T> term[kWaveChunk];
     unsigned char own[kWaveChunk];
+    ContactOwner<T> ow[64];
 };
 
 __device__ __forceinline__ void wave_sync_lds()
@@ -634,12 +716,12 @@ __device__ __forceinline__ ContactAhead contacts_ahead(const Args<T, P> &a, int 
 // The contact term of entry `ent` for bead 1 at x1.. with radius r1 and the
 // partner at xj..: ResolveContacts' minimum image (kernel.cl:844-870), then
 // contact_force.  The same operations as contact_term.
-template <typename T, typename P>
+template <typename T, typename P, typename E>
 __device__ __forceinline__ bool contact_term_at(const Args<T, P> &a, const Box &b, P x1, P y1, P z1, T r1, int ent,
-                                                P xj, P yj, P zj, const PairC *table, V3<T> &t)
+                                                P xj, P yj, P zj, const E *table, V3<T> &t)
 {
     const int j = ent & kIdxMask;
-    const PairC pr = table[ent >> kPidShift];
+    const E pr = table[ent >> kPidShift];
     P rx = x1 - xj, ry = y1 - yj, rz = z1 - zj;
     P len = rx * rx + ry * ry + rz * rz;
     if (len > (P)b.hbs) {
@@ -652,14 +734,13 @@ __device__ __forceinline__ bool contact_term_at(const Args<T, P> &a, const Box &
 }
 
 // One pass of a wave's contacts: entries [base, base + n) with their list
-// entries ent[r] at k = base + r * 64 + lane and owners in w.own; the terms
-// go to w.term (an entry without force stores -0.0), and each lane adds its
-// own entries in list order.  Every partner position of the pass is loaded
-// before the terms are evaluated.
-template <typename T, typename P, bool ENS>
+// entries ent[r] at k = base + r * 64 + lane, owner lanes in w.own and the
+// owners' beads in w.ow; the terms go to w.term (an entry without force
+// stores -0.0), and each lane adds its own entries in list order.  Every
+// partner position of the pass is loaded before the terms are evaluated.
+template <typename T, typename P, bool ENS, typename E>
 __device__ __forceinline__ void contacts_pass(const Args<T, P> &a, const int (&ent)[kChunkRounds], int base, int n,
-                                              int off, int cnt, T r1, P x1, P y1, P z1, V3<T> &f,
-                                              const PairC *table, WaveContactLds<T> &w, int netw)
+                                              int off, int cnt, V3<T> &f, const E *table, WaveContactLds<T> &w)
 {
     const int lane = threadIdx.x & 63;
     P xj[kChunkRounds], yj[kChunkRounds], zj[kChunkRounds];
@@ -678,16 +759,16 @@ __device__ __forceinline__ void contacts_pass(const Args<T, P> &a, const int (&e
         if (r * 64 >= n) break;                       // wave-uniform
         const int k = r * 64 + lane;
         const int L = k < n ? w.own[k] : 0;
-        const P xL = __shfl(x1, L, 64), yL = __shfl(y1, L, 64), zL = __shfl(z1, L, 64);
-        const T rL = __shfl(r1, L, 64);
+        const ContactOwner<T> o = w.ow[L];
         V3<T> t{(T)-0.0, (T)-0.0, (T)-0.0};
         if (ENS) {
-            const Box &bk = a.boxes[__shfl(netw, L, 64)];
-            if (k < n && !contact_term_at(a, bk, xL, yL, zL, rL, ent[r], xj[r], yj[r], zj[r],
+            const Box &bk = a.boxes[o.net];
+            if (k < n && !contact_term_at(a, bk, (P)o.x, (P)o.y, (P)o.z, o.r, ent[r], xj[r], yj[r], zj[r],
                                           a.pairs + bk.pair_base, t))
                 t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
         } else {
-            if (k < n && !contact_term_at(a, a.force, xL, yL, zL, rL, ent[r], xj[r], yj[r], zj[r], table, t))
+            if (k < n && !contact_term_at(a, a.force, (P)o.x, (P)o.y, (P)o.z, o.r, ent[r], xj[r], yj[r], zj[r], table,
+                                          t))
                 t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
         }
         w.term[k] = t;
@@ -712,15 +793,17 @@ __device__ __forceinline__ void contacts_pass(const Args<T, P> &a, const int (&e
 // own terms in list order, which equals the per-bead loop (gather_contacts)
 // bit for bit.  ENS: the lane's bead belongs to network `netw` (its Box and
 // pair table); otherwise a.force and `table`.
-template <typename T, typename P, bool ENS>
+template <typename T, typename P, bool ENS, typename E>
 __device__ __forceinline__ void contacts_finish(const Args<T, P> &a, const ContactAhead &c, int e0, int cnt, T r1,
-                                                P x1, P y1, P z1, V3<T> &f, const PairC *table,
-                                                WaveContactLds<T> &w, int netw = 0)
+                                                P x1, P y1, P z1, V3<T> &f, const E *table, WaveContactLds<T> &w,
+                                                int netw = 0)
 {
+    static_assert(sizeof(P) == sizeof(double), "positions are fp64 in both builds");
     const int lane = threadIdx.x & 63;
     if (c.total == 0) return;                         // wave-uniform
-    contacts_pass<T, P, ENS>(a, c.ent, 0, c.total < kWaveChunk ? c.total : kWaveChunk, c.off, cnt, r1, x1, y1, z1,
-                             f, table, w, netw);
+    w.ow[lane] = ContactOwner<T>{(double)x1, (double)y1, (double)z1, r1, netw};
+    wave_sync_lds();
+    contacts_pass<T, P, ENS>(a, c.ent, 0, c.total < kWaveChunk ? c.total : kWaveChunk, c.off, cnt, f, table, w);
     for (int base = kWaveChunk; base < c.total; base += kWaveChunk) {
         const int n = (c.total - base) < kWaveChunk ? c.total - base : kWaveChunk;
         mark_owner(c.off, cnt, base, w.own);
@@ -733,7 +816,7 @@ __device__ __forceinline__ void contacts_finish(const Args<T, P> &a, const Conta
             const int e = owner_entry(k, k < n, e0, c.off - base, w.own, L);
             ent[r] = k < n ? a.nb_list[e] : 0;
         }
-        contacts_pass<T, P, ENS>(a, ent, base, n, c.off, cnt, r1, x1, y1, z1, f, table, w, netw);
+        contacts_pass<T, P, ENS>(a, ent, base, n, c.off, cnt, f, table, w);
     }
 }
 
