@@ -166,7 +166,7 @@ void add_int(int *p, int v, hipStream_t s);
 void copy_probe(const void *src, void *dst, size_t n16, hipStream_t s);
 int scan_blocks_needed(int n);
 // out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total.  tmp holds
-// scan_blocks_needed(n) + 1 ints.
+// scan_blocks_needed(n) ints; in and out are 16-byte aligned.
 void exclusive_scan(const int *in, int n, int *out, int *tmp, const int *gate, hipStream_t s);
 
 }  // namespace fnk

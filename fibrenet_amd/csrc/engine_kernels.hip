@@ -7,6 +7,8 @@
 // step, so that the bead state makes one pass over HBM per step.  Each launch
 // is a short chain of independent loads: the bond kernels read their beads
 // through the bond arrays, the bead kernel its incidence lists.
+#include <cstdint>
+#include <stdexcept>
 #include <utility>
 
 #include "kernels.h"
@@ -1276,34 +1278,34 @@ __device__ __forceinline__ int block_excl_scan(int v, int *lds, int &total)
     return off + x - v;
 }
 
+// A block's tile of kScanTile elements is read as kScanItems / 4 rounds of
+// kBlock 16-byte chunks, consecutive threads on consecutive chunks: in round
+// j thread t holds elements tile + 4 (j kBlock + t) .. + 3 (in and out are
+// device allocations, so 16-byte aligned).
+constexpr int kScanRounds = kScanItems / 4;
+__device__ __forceinline__ int4 scan_load4(const int *in, int idx, int n)
+{
+    if (idx + 3 < n) return *(const int4 *)(in + idx);
+    int4 v{0, 0, 0, 0};
+    if (idx < n) v.x = in[idx];
+    if (idx + 1 < n) v.y = in[idx + 1];
+    if (idx + 2 < n) v.z = in[idx + 2];
+    return v;
+}
+
 __global__ void __launch_bounds__(kBlock) k_scan_reduce(const int *in, int n, int *bsum, const int *gate)
 {
     if (gated_off(gate)) return;
     __shared__ int lds[kBlock / 64];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
     int s = 0;
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) s += (base + k < n) ? in[base + k] : 0;
+    for (int j = 0; j < kScanRounds; ++j) {
+        const int4 v = scan_load4(in, blockIdx.x * kScanTile + (j * kBlock + (int)threadIdx.x) * 4, n);
+        s += v.x + v.y + v.z + v.w;
+    }
     int total;
     block_excl_scan(s, lds, total);
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
-}
-
-// Exclusive scan of the block sums by one block; bsum[nb] receives the total.
-__global__ void __launch_bounds__(kBlock) k_scan_blocks(int *bsum, int nb, const int *gate)
-{
-    if (gated_off(gate)) return;
-    __shared__ int lds[kBlock / 64];
-    int carry = 0;
-    for (int b0 = 0; b0 < nb; b0 += kBlock) {
-        const int k = b0 + threadIdx.x;
-        const int v = k < nb ? bsum[k] : 0;
-        int total;
-        const int ex = block_excl_scan(v, lds, total);
-        if (k < nb) bsum[k] = carry + ex;
-        carry += total;
-    }
-    if (threadIdx.x == 0) bsum[nb] = carry;
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_apply(const int *in, int n, const int *bsum, int nb, int *out,
@@ -1311,22 +1313,36 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(const int *in, int n, con
 {
     if (gated_off(gate)) return;
     __shared__ int lds[kBlock / 64];
-    const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
-    int v[kScanItems];
-    int s = 0;
+    int4 v[kScanRounds];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        v[k] = (base + k < n) ? in[base + k] : 0;
-        s += v[k];
+    for (int j = 0; j < kScanRounds; ++j)
+        v[j] = scan_load4(in, blockIdx.x * kScanTile + (j * kBlock + (int)threadIdx.x) * 4, n);
+    // this block's offset: the sum of the block totals before it (and, for
+    // block 0, of all of them: the scan's total)
+    const int upto = blockIdx.x == 0 ? nb : (int)blockIdx.x;
+    int part = 0;
+    for (int k = threadIdx.x; k < upto; k += kBlock) part += bsum[k];
+    int carry;
+    block_excl_scan(part, lds, carry);
+    if (blockIdx.x == 0) {
+        if (threadIdx.x == 0) out[n] = carry;
+        carry = 0;
     }
-    int total;
-    int run = block_excl_scan(s, lds, total) + bsum[blockIdx.x];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        if (base + k < n) out[base + k] = run;
-        run += v[k];
+    for (int j = 0; j < kScanRounds; ++j) {
+        const int idx = blockIdx.x * kScanTile + (j * kBlock + (int)threadIdx.x) * 4;
+        int total;
+        const int e = carry + block_excl_scan(v[j].x + v[j].y + v[j].z + v[j].w, lds, total);
+        const int4 o{e, e + v[j].x, e + v[j].x + v[j].y, e + v[j].x + v[j].y + v[j].z};
+        if (idx + 3 < n) {
+            *(int4 *)(out + idx) = o;
+        } else {
+            if (idx < n) out[idx] = o.x;
+            if (idx + 1 < n) out[idx + 1] = o.y;
+            if (idx + 2 < n) out[idx + 2] = o.z;
+        }
+        carry += total;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = bsum[nb];
 }
 
 // ---- explicit instantiation through launchers ----------------------------------------
@@ -1506,9 +1522,9 @@ void note_build(const int *total, int *stats, const int *gate, hipStream_t s)
 int scan_blocks_needed(int n) { return (n + kScanTile - 1) / kScanTile; }
 void exclusive_scan(const int *in, int n, int *out, int *tmp, const int *gate, hipStream_t s)
 {
+    if (((uintptr_t)in | (uintptr_t)out) & 15) throw std::invalid_argument("exclusive_scan: unaligned buffers");
     const int nb = scan_blocks_needed(n > 0 ? n : 1);
     k_scan_reduce<<<nb, kBlock, 0, s>>>(in, n, tmp, gate);
-    k_scan_blocks<<<1, kBlock, 0, s>>>(tmp, nb, gate);
     k_scan_apply<<<nb, kBlock, 0, s>>>(in, n, tmp, nb, out, gate);
 }
 
