@@ -61,8 +61,9 @@ def algorithmic_bytes(net, pairs, tb, aliased, chain):
       per spring two incidence entries and its term (8 B + 3 tb), per angle
       three entries and its terms (12 B + 6 tb), per listed pair two list
       entries (8 B).
-    rebuild: per bead position read, key, slot and the 32 B packed record
-      written (56 B), per listed pair two entries (8 B)."""
+    rebuild: per bead its position read (24 B) and its position, tag and
+      loc written in cell order (32 B), per listed pair two list entries
+      (8 B)."""
     import numpy as np
     n = len(net.beads)
     s_hr = int(np.sum(net.springs["type"] == 0))
