@@ -134,7 +134,8 @@ int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_spri
  *   "chunk"  n    steps captured per hipGraph (rounded to grid_update_freq)
  *   "chain"  1/0  bead kernel that evaluates chain bonds (spring (i, i+1),
  *                 angle (i-1, i, i+1), numbered along the chain) in-wave,
- *                 used when every bond follows that layout
+ *                 used when every bond follows that layout; also the
+ *                 one-launch adjoint step of fn_adjoint_backward
  *   "chain_layout" (read only) 1 when every bond follows the chain layout
  *   "verlet" 0/1/2 keep the contact candidates (pairs within reach + skin)
  *                 between rebuilds while every bead stays within skin/2 of
