@@ -2134,7 +2134,7 @@ class Engine final : public fn_engine {
                     int *dl = rec_nb_.p + (size_t)(k / guf) * lrow;
                     d2d(dl, nb_start_.p, sizeof(int) * (N + 1));
                     d2d(dl + N + 1, nb_cnt_.p, sizeof(int) * N);
-                    d2d(dl + 2 * N + 1, nb_list_.p, sizeof(int) * (size_t)cap_);
+                    fnk::copy_bounded(nb_list_.p, dl + 2 * N + 1, nb_start_.p + N, cap_, stream_);
                 }
             };
             enqueue_chunk(len, guf, spr, ang, seg == 0, seg == nseg - 1, nullptr, &hook);

@@ -164,6 +164,8 @@ void set_int(int *p, int v, const int *gate, hipStream_t s);
 void add_int(int *p, int v, hipStream_t s);
 // Streaming copy of n16 16-byte words (the reference point for traffic counters).
 void copy_probe(const void *src, void *dst, size_t n16, hipStream_t s);
+// dst[0..m) = src[0..m) with m = min(*count, cap) read on the device.
+void copy_bounded(const int *src, int *dst, const int *count, int cap, hipStream_t s);
 int scan_blocks_needed(int n);
 // out[0..n] = exclusive prefix sums of in[0..n-1], out[n] = total.  tmp holds
 // scan_blocks_needed(n) ints; in and out are 16-byte aligned.

@@ -7,6 +7,7 @@
 // step, so that the bead state makes one pass over HBM per step.  Each launch
 // is a short chain of independent loads: the bond kernels read their beads
 // through the bond arrays, the bead kernel its incidence lists.
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <utility>
@@ -408,6 +409,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kCh
 }
 
 __global__ void k_add_int(int *p, int v) { *p += v; }
+
+// Copies src[0 .. min(*count, cap)) to dst: the used part of a list whose
+// length is known on the device only.
+__global__ void __launch_bounds__(kBlock) k_copy_bounded(const int *src, int *dst, const int *count, int cap)
+{
+    const int m = min(*count, cap);
+    for (int k = blockIdx.x * kBlock + threadIdx.x; k < m; k += gridDim.x * kBlock) dst[k] = src[k];
+}
 
 // The chain layout's status copies from the bond-ordered statuses.
 __global__ void __launch_bounds__(kBlock) k_chain_status(const int *ch_spring, const int *ch_angle,
@@ -1518,6 +1527,10 @@ void filter(const FilterArgs &f, hipStream_t s) { k_filter<<<blocks(f.n), kBlock
 void note_build(const int *total, int *stats, const int *gate, hipStream_t s)
 {
     k_note_build<<<1, 1, 0, s>>>(total, stats, gate);
+}
+void copy_bounded(const int *src, int *dst, const int *count, int cap, hipStream_t s)
+{
+    if (cap > 0) k_copy_bounded<<<std::min(blocks(cap), 4096), kBlock, 0, s>>>(src, dst, count, cap);
 }
 int scan_blocks_needed(int n) { return (n + kScanTile - 1) / kScanTile; }
 void exclusive_scan(const int *in, int n, int *out, int *tmp, const int *gate, hipStream_t s)
