@@ -43,10 +43,13 @@ __device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual{a.v - b.
 __device__ __forceinline__ Dual operator-(Dual a) { return Dual{-a.v, -a.d}; }
 __device__ __forceinline__ Dual operator*(Dual a, Dual b) { return Dual{a.v * b.v, a.v * b.d + a.d * b.v}; }
 __device__ __forceinline__ Dual operator*(double s, Dual a) { return Dual{s * a.v, s * a.d}; }
+// a / b through one fp64 reciprocal of b (the quotient within two ulps;
+// every adjoint path shares it).
 __device__ __forceinline__ Dual operator/(Dual a, Dual b)
 {
-    const double q = a.v / b.v;
-    return Dual{q, (a.d - q * b.d) / b.v};
+    const double r = 1.0 / b.v;
+    const double q = a.v * r;
+    return Dual{q, (a.d - q * b.d) * r};
 }
 __device__ __forceinline__ Dual dsqrt(Dual a)
 {
@@ -136,9 +139,10 @@ __device__ __forceinline__ DV3 contact_dual(const PairC &pr, double r1, double r
     const Dual rr = flen(len2);
     Dual coef;
     if (pr.type == 1) {
-        const Dual fr2 = cst(pr.c1 * pr.c1) / len2;
+        const Dual il2 = cst(1.0) / len2;      // one reciprocal for both ratios
+        const Dual fr2 = (pr.c1 * pr.c1) * il2;
         const Dual fr6 = fr2 * fr2 * fr2;
-        coef = (48.0 * pr.c2) * fr6 * (fr6 - cst(0.5)) / len2 - cst(pr.c4);
+        coef = (48.0 * pr.c2) * fr6 * (fr6 - cst(0.5)) * il2 - cst(pr.c4);
     } else if (pr.type == 2) {
         coef = pr.c1 * (cst(r1 + r2) - rr);
     } else {
