@@ -324,12 +324,23 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step(AdjArgs<T, P> a)
     }
 }
 
-// A wave's LDS for its contacts in k_adjoint_step_flat: the J Fb terms of
-// one pass, the lane that owns each entry, and the wave's scale adjoints.
+// A lane's bead as the contact passes read it: position and force adjoint.
+struct AdjOwner {
+    double x, y, z, fx, fy, fz;
+};
+
+// A wave's LDS for its contacts in k_adjoint_step_flat / _chain: the J Fb
+// terms of one pass, the lane that owns each entry, the wave's scale
+// adjoints, and every lane's bead (position, force adjoint, radius).  Four
+// blocks of four waves fit a CU's LDS in the fp32 build, as many as the
+// registers allow.
+template <typename T>
 struct AdjWaveLds {
     double3 term[kWaveChunk];
     unsigned char own[kWaveChunk];
     double g[kAdjPairsLds];
+    AdjOwner ow[64];
+    T orad[64];
 };
 
 // k_adjoint_step for a plain network with at most kAdjPairsLds pair-table
@@ -344,11 +355,11 @@ template <typename T, typename P>
 __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
 {
     __shared__ PairC prs[kAdjPairCap];
-    __shared__ AdjWaveLds wl[kBlock / 64];
+    __shared__ AdjWaveLds<T> wl[kBlock / 64];
     const int np = a.n_pairs;
     for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
     __syncthreads();
-    AdjWaveLds &w = wl[threadIdx.x >> 6];
+    AdjWaveLds<T> &w = wl[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < a.n;
@@ -372,6 +383,8 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
     double g[kAdjPairsLds];
 #pragma unroll
     for (int p = 0; p < kAdjPairsLds; ++p) g[p] = 0.0;
+    w.ow[lane] = AdjOwner{(double)xi, (double)yi, (double)zi, fb.x, fb.y, fb.z};
+    w.orad[lane] = ri;
     for (int base = 0; base < total; base += kWaveChunk) {           // wave-uniform
         const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
         mark_owner(off, cnt, base, w.own);
@@ -380,9 +393,10 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
             const int k = r * 64 + lane;
             int L;
             const int e = owner_entry(k, k < n, e0, off - base, w.own, L);
-            const P xL = __shfl(xi, L, 64), yL = __shfl(yi, L, 64), zL = __shfl(zi, L, 64);
-            const T rL = __shfl(ri, L, 64);
-            const double3 fL{__shfl(fb.x, L, 64), __shfl(fb.y, L, 64), __shfl(fb.z, L, 64)};
+            const AdjOwner o = w.ow[L];
+            const P xL = (P)o.x, yL = (P)o.y, zL = (P)o.z;
+            const T rL = w.orad[L];
+            const double3 fL{o.fx, o.fy, o.fz};
             double3 t{-0.0, -0.0, -0.0};
             if (k < n) {
                 const int ent = a.nb_list[e];
@@ -478,11 +492,11 @@ template <typename T, typename P>
 __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
 {
     __shared__ PairC prs[kAdjPairCap];
-    __shared__ AdjWaveLds wl[kBlock / 64];
+    __shared__ AdjWaveLds<T> wl[kBlock / 64];
     const int np = a.n_pairs;
     for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
     __syncthreads();
-    AdjWaveLds &w = wl[threadIdx.x >> 6];
+    AdjWaveLds<T> &w = wl[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const int i = (blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kChainOwned + lane - kHalo;
     const bool in = i >= 0 && i < a.n;
@@ -577,6 +591,8 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     double g[kAdjPairsLds];
 #pragma unroll
     for (int p = 0; p < kAdjPairsLds; ++p) g[p] = 0.0;
+    w.ow[lane] = AdjOwner{(double)pxi, (double)pyi, (double)pzi, fb.x, fb.y, fb.z};
+    w.orad[lane] = ri;
     for (int base = 0; base < total; base += kWaveChunk) {           // wave-uniform
         const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
         mark_owner(off, cnt, base, w.own);
@@ -585,9 +601,10 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
             const int k = r * 64 + lane;
             int L;
             const int e = owner_entry(k, k < n, e0, off - base, w.own, L);
-            const P xL = __shfl(pxi, L, 64), yL = __shfl(pyi, L, 64), zL = __shfl(pzi, L, 64);
-            const T rL = __shfl(ri, L, 64);
-            const double3 fL{__shfl(fb.x, L, 64), __shfl(fb.y, L, 64), __shfl(fb.z, L, 64)};
+            const AdjOwner o = w.ow[L];
+            const P xL = (P)o.x, yL = (P)o.y, zL = (P)o.z;
+            const T rL = w.orad[L];
+            const double3 fL{o.fx, o.fy, o.fz};
             double3 t{-0.0, -0.0, -0.0};
             if (k < n) {
                 const int ent = a.nb_list[e];
