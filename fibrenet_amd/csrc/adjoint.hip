@@ -508,10 +508,10 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     const int q = (in && a.angles) ? a.ch_angle[i] : -1;
     // neighbours i + 1 and i - 1 (meaningless in the outermost lanes, whose
     // results no owned lane uses)
-    const P xn = __shfl_down(xi, 1, 64), yn = __shfl_down(yi, 1, 64), zn = __shfl_down(zi, 1, 64);
-    const P xp = __shfl_up(xi, 1, 64), yp = __shfl_up(yi, 1, 64), zp = __shfl_up(zi, 1, 64);
-    const double3 fbn{__shfl_down(fb.x, 1, 64), __shfl_down(fb.y, 1, 64), __shfl_down(fb.z, 1, 64)};
-    const double3 fbp{__shfl_up(fb.x, 1, 64), __shfl_up(fb.y, 1, 64), __shfl_up(fb.z, 1, 64)};
+    const P xn = lane_next(xi), yn = lane_next(yi), zn = lane_next(zi);
+    const P xp = lane_prev(xi), yp = lane_prev(yi), zp = lane_prev(zi);
+    const double3 fbn{lane_next(fb.x), lane_next(fb.y), lane_next(fb.z)};
+    const double3 fbp{lane_prev(fb.x), lane_prev(fb.y), lane_prev(fb.z)};
     // spring (i, i+1): J Fb on p1 = i (p2 receives its negation)
     double3 jr{0.0, 0.0, 0.0};
     if (s >= 0 && s < a.n_springs && a.sp_on[s] == 1 && (a.sp_type[s] == 0 || a.sp_type[s] == 1)) {
@@ -540,11 +540,11 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
                             dc.x * (double)fc.x + dc.y * (double)fc.y + dc.z * (double)fc.z;
         }
     }
-    const int hsl = __shfl_up(s >= 0 ? 1 : 0, 1, 64);
-    const int hql = __shfl_up(q >= 0 ? 1 : 0, 1, 64), hqn = __shfl_down(q >= 0 ? 1 : 0, 1, 64);
-    const double3 jl{__shfl_up(jr.x, 1, 64), __shfl_up(jr.y, 1, 64), __shfl_up(jr.z, 1, 64)};
-    const double3 jcl{__shfl_up(jc.x, 1, 64), __shfl_up(jc.y, 1, 64), __shfl_up(jc.z, 1, 64)};
-    const double3 jan{__shfl_down(ja.x, 1, 64), __shfl_down(ja.y, 1, 64), __shfl_down(ja.z, 1, 64)};
+    const int hsl = lane_prev(s >= 0 ? 1 : 0);
+    const int hql = lane_prev(q >= 0 ? 1 : 0), hqn = lane_next(q >= 0 ? 1 : 0);
+    const double3 jl{lane_prev(jr.x), lane_prev(jr.y), lane_prev(jr.z)};
+    const double3 jcl{lane_prev(jc.x), lane_prev(jc.y), lane_prev(jc.z)};
+    const double3 jan{lane_next(ja.x), lane_next(ja.y), lane_next(ja.z)};
     double3 z{0.0, 0.0, 0.0};
     if (a.springs) {
         if (hsl) {              // bead i is p2 of spring (i-1, i)

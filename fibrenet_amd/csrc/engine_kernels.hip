@@ -330,8 +330,8 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kCh
 
     // images of beads i + 1 and i - 1 (meaningless in the outermost lanes,
     // whose results no owned lane uses)
-    const P xn = __shfl_down(x, 1, 64), yn = __shfl_down(y, 1, 64), zn = __shfl_down(z, 1, 64);
-    const P xp = __shfl_up(x, 1, 64), yp = __shfl_up(y, 1, 64), zp = __shfl_up(z, 1, 64);
+    const P xn = lane_next(x), yn = lane_next(y), zn = lane_next(z);
+    const P xp = lane_prev(x), yp = lane_prev(y), zp = lane_prev(z);
 
     // spring (i, i+1): its term and its status after this step's spring pass
     V3<T> fr{(T)0, (T)0, (T)0};
@@ -352,9 +352,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kCh
             }
         }
     }
-    const int sl = __shfl_up(hs ? 1 : 0, 1, 64);
-    const int stl = __shfl_up(str, 1, 64);
-    const V3<T> fl{__shfl_up(fr.x, 1, 64), __shfl_up(fr.y, 1, 64), __shfl_up(fr.z, 1, 64)};
+    const int sl = lane_prev(hs ? 1 : 0);
+    const int stl = lane_prev(str);
+    const V3<T> fl{lane_prev(fr.x), lane_prev(fr.y), lane_prev(fr.z)};
 
     // angle (i-1, i, i+1) with this step's HR_FL rule (its springs are the
     // two chain springs of bead i)
@@ -370,9 +370,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kCh
         }
         if (st == 1) angle_force(anp, xp, yp, zp, x, y, z, xn, yn, zn, pa, pc);
     }
-    const int hql = __shfl_up(hq ? 1 : 0, 1, 64), hqn = __shfl_down(hq ? 1 : 0, 1, 64);
-    const V3<T> pcl{__shfl_up(pc.x, 1, 64), __shfl_up(pc.y, 1, 64), __shfl_up(pc.z, 1, 64)};
-    const V3<T> pan{__shfl_down(pa.x, 1, 64), __shfl_down(pa.y, 1, 64), __shfl_down(pa.z, 1, 64)};
+    const int hql = lane_prev(hq ? 1 : 0), hqn = lane_next(hq ? 1 : 0);
+    const V3<T> pcl{lane_prev(pc.x), lane_prev(pc.y), lane_prev(pc.z)};
+    const V3<T> pan{lane_next(pa.x), lane_next(pa.y), lane_next(pa.z)};
 
     if (SPR) {
         if (sl) {               // bead i is p2 of spring (i-1, i)

@@ -652,6 +652,29 @@ struct WaveContactLds {
     ContactOwner<T> ow[64];
 };
 
+// Values of the neighbouring lane through the DPP whole-wave shift by one
+// (a VALU move, no LDS round trip): lane_next(v) is v of lane + 1,
+// lane_prev(v) v of lane - 1; the last (first) lane keeps its own value, as
+// __shfl_down(v, 1) / __shfl_up(v, 1) do.  All 64 lanes must be active.
+constexpr int kDppNext = 0x130;   // wave_shl:1
+constexpr int kDppPrev = 0x138;   // wave_shr:1
+__device__ __forceinline__ int lane_next(int v) { return __builtin_amdgcn_update_dpp(v, v, kDppNext, 0xf, 0xf, false); }
+__device__ __forceinline__ int lane_prev(int v) { return __builtin_amdgcn_update_dpp(v, v, kDppPrev, 0xf, 0xf, false); }
+__device__ __forceinline__ float lane_next(float v) { return __int_as_float(lane_next(__float_as_int(v))); }
+__device__ __forceinline__ float lane_prev(float v) { return __int_as_float(lane_prev(__float_as_int(v))); }
+__device__ __forceinline__ double lane_next(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_next((int)(unsigned)(b & 0xffffffffLL)), hi = lane_next((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double lane_prev(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = lane_prev((int)(unsigned)(b & 0xffffffffLL)), hi = lane_prev((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
 __device__ __forceinline__ void wave_sync_lds()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
