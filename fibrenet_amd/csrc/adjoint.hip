@@ -373,11 +373,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
     const T ri = a.radius[ic];
     const int e0 = a.nb_start[ic], cnt = live ? a.nb_cnt[ic] : 0;
     int incl = cnt;                                   // inclusive scan of the counts over the wave
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-    }
+    incl = wave_incl_scan(incl);
     const int off = incl - cnt;
     const int total = __shfl(incl, 63, 64);
     double g[kAdjPairsLds];
@@ -581,11 +577,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     const T ri = a.radius[ic];
     const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
     int incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-    }
+    incl = wave_incl_scan(incl);
     const int off = incl - cnt;
     const int total = __shfl(incl, 63, 64);
     double g[kAdjPairsLds];

@@ -862,11 +862,7 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *cell_start, i
         }
     }
     int incl = nb;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-    }
+    incl = wave_incl_scan(incl);
     const int off = incl - nb;
     const int total = __shfl(incl, 63, 64);
     for (int base = 0; base < total; base += kFlatChunk) {          // wave-uniform
@@ -1188,11 +1184,7 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     const int cnt = (live && in && e1 <= f.cap) ? e1 - e0 : 0;   // past the capacity: the host reports the run invalid
     const double ri = f.radius[ic], si = f.skin[ic];
     int incl = cnt;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int t = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += t;
-    }
+    incl = wave_incl_scan(incl);
     const int off = incl - cnt;
     const int total = __shfl(incl, 63, 64);
     const Box &b = f.list;
@@ -1268,11 +1260,7 @@ __device__ __forceinline__ int block_excl_scan(int v, int *lds, int &total)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
+    x = wave_incl_scan(x);
     if (lane == 63) lds[wave] = x;
     __syncthreads();
     int off = 0;

@@ -675,6 +675,21 @@ __device__ __forceinline__ double lane_prev(double v)
     return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 
+// Inclusive prefix sum of v over the 64 lanes: Hillis-Steele steps inside
+// each row of 16 lanes (DPP row shifts by 1, 2, 4, 8), then the sum of the
+// rows before (row broadcasts of lanes 15 and 31).  All 64 lanes must be
+// active.
+__device__ __forceinline__ int wave_incl_scan(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);   // row_shr:8
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);   // row_bcast:15 into rows 1 and 3
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 __device__ __forceinline__ void wave_sync_lds()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -715,11 +730,7 @@ __device__ __forceinline__ ContactAhead contacts_ahead(const Args<T, P> &a, int 
 {
     const int lane = threadIdx.x & 63;
     int incl = cnt;                                   // inclusive scan of the counts over the wave
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += y;
-    }
+    incl = wave_incl_scan(incl);
     ContactAhead c;
     c.off = incl - cnt;
     c.total = __shfl(incl, 63, 64);
