@@ -143,7 +143,9 @@ def adjoint_bench(net, precision, options, warmup, steps, every):
     from the warmed-up state (`warmup` steps), where the dynamics are still
     regular: the checkpointing forward over `steps` steps sampling the
     clamp's reaction force 20 times, then the backward sweep (segment
-    recompute, per-step records, adjoint kernel) with dL/d(sample) = 1.  The
+    recompute, per-step records, adjoint kernel) with dL/d(sample) = 1,
+    returning the force-scale derivatives (those of the spring, angle and
+    contact stiffnesses; the per-bead ones stay on the device).  The
     plain forward is timed on the same engine over the next `steps` steps
     (graph-captured, after a capture run), and the ratios are against it.
 
@@ -155,6 +157,7 @@ def adjoint_bench(net, precision, options, warmup, steps, every):
     as in the fp32 one, so later states carry non-finite derivatives."""
     import numpy as np
     from fibrenet_amd import Engine, bind_standard
+    from fibrenet_amd.api import SCALE_OUTPUTS
     group = (np.asarray(net.info["type"]) == 1).astype(np.int32)
     samples = np.unique(np.linspace(steps // 20 - 1, steps - 1, 20).astype(np.int32))
     guf = net.grid_update_freq
@@ -169,7 +172,9 @@ def adjoint_bench(net, precision, options, warmup, steps, every):
         t0 = time.perf_counter()
         eng.adjoint_forward(steps, group, samples, guf, True, True, every)
         t1 = time.perf_counter()
-        g = eng.adjoint_backward(np.ones(len(samples)), len(net.springs), len(net.angles), len(net.pairs))
+        # the force-scale derivatives: the per-bead ones (96 MB here) stay on the device
+        g = eng.adjoint_backward(np.ones(len(samples)), len(net.springs), len(net.angles), len(net.pairs),
+                                 SCALE_OUTPUTS)
         t2 = time.perf_counter()
         eng.run_steps(steps, guf, True, True)          # graph capture
         eng.finish()
@@ -179,7 +184,7 @@ def adjoint_bench(net, precision, options, warmup, steps, every):
     finally:
         eng.close()
     fwd, bwd, plain = (t1 - t0) * 1e3 / steps, (t2 - t1) * 1e3 / steps, (t4 - t3) * 1e3 / steps
-    finite = all(bool(np.isfinite(g[k]).all()) for k in ("spring", "angle", "pair", "position", "velocity_old"))
+    finite = all(bool(np.isfinite(g[k]).all()) for k in SCALE_OUTPUTS)
     return {"steps": steps, "start_step": warmup, "checkpoint_every": every or "auto", "samples": len(samples),
             "plain_forward_ms_per_step": plain, "forward_ms_per_step": fwd, "backward_ms_per_step": bwd,
             "backward_over_forward": bwd / plain, "total_over_forward": (fwd + bwd) / plain,

@@ -173,6 +173,12 @@ class Model:
         return a
 
 
+# What fn_adjoint_backward can return, in its argument order.
+ADJOINT_OUTPUTS = ("spring", "angle", "pair", "position", "velocity_old")
+# The force-scale derivatives alone (what a stiffness fit needs).
+SCALE_OUTPUTS = ("spring", "angle", "pair")
+
+
 class Engine:
     """One engine instance (fn_engine): buffers by id, kernels by name."""
 
@@ -336,20 +342,25 @@ class Engine:
         out = out[:len(st) * nn]
         return out.reshape(nn, len(st)) if self.batched else out
 
-    def adjoint_backward(self, dloss, n_springs, n_angles, n_pairs):
+    def adjoint_backward(self, dloss, n_springs, n_angles, n_pairs, outputs=ADJOINT_OUTPUTS):
         """Derivatives of a loss with dL/d(sample) = dloss through the last
         adjoint_forward run: per-bond and per-pair-type force-scale
-        derivatives and those of the initial positions and velocity_old."""
+        derivatives ("spring", "angle", "pair") and those of the initial
+        positions and velocity_old ("position", "velocity_old").  Only the
+        derivatives named in `outputs` are copied to the host and returned."""
+        bad = set(outputs) - set(ADJOINT_OUTPUTS)
+        if bad:
+            raise ValueError("unknown adjoint outputs %s" % sorted(bad))
         d = np.ascontiguousarray(dloss, np.float64).ravel()
         n = getattr(self, "_adj_beads", 0)
-        out = {"spring": np.zeros(max(n_springs, 1)), "angle": np.zeros(max(n_angles, 1)),
-               "pair": np.zeros(max(n_pairs, 1)), "position": np.zeros((n, 3)), "velocity_old": np.zeros((n, 3))}
-        self._chk(lib().fn_adjoint_backward(self.h, _ptr(d) if d.size else None, _ptr(out["spring"]),
-                                            _ptr(out["angle"]), _ptr(out["pair"]), _ptr(out["position"]),
-                                            _ptr(out["velocity_old"])))
-        out["spring"] = out["spring"][:n_springs]
-        out["angle"] = out["angle"][:n_angles]
-        out["pair"] = out["pair"][:n_pairs]
+        shape = {"spring": (max(n_springs, 1),), "angle": (max(n_angles, 1),), "pair": (max(n_pairs, 1),),
+                 "position": (n, 3), "velocity_old": (n, 3)}
+        out = {k: np.zeros(shape[k]) for k in ADJOINT_OUTPUTS if k in outputs}
+        ptrs = [_ptr(out[k]) if k in out else None for k in ADJOINT_OUTPUTS]
+        self._chk(lib().fn_adjoint_backward(self.h, _ptr(d) if d.size else None, *ptrs))
+        for k, m in (("spring", n_springs), ("angle", n_angles), ("pair", n_pairs)):
+            if k in out:
+                out[k] = out[k][:m]
         return out
 
     def set_timing(self, on):

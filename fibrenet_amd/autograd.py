@@ -12,6 +12,7 @@ import numpy as np
 import torch
 
 from . import api, records as R
+from .api import SCALE_OUTPUTS
 
 
 def scaled_pairs(pairs, s):
@@ -60,9 +61,11 @@ class TensileRun:
         self._upload(scales)
         return self.eng.adjoint_forward(self.n, self.group, self.steps, self.guf, self.spr, self.ang, self.every)
 
-    def backward(self, dloss):
+    def backward(self, dloss, outputs=SCALE_OUTPUTS):
+        """The adjoint's derivatives named in `outputs` (by default the
+        force-scale ones)."""
         net = self.net
-        return self.eng.adjoint_backward(dloss, len(net.springs), len(net.angles), len(net.pairs))
+        return self.eng.adjoint_backward(dloss, len(net.springs), len(net.angles), len(net.pairs), outputs)
 
     def final_beads(self):
         return self.eng.read_buffer(self.ids["bead"], R.BEAD)
@@ -131,7 +134,7 @@ class BatchedTensileRun:
         sigma = 1 of the scaled run (networks x 3)."""
         nn = len(self.nets)
         g = self.ens.adjoint_backward(np.asarray(dloss, np.float64).reshape(nn, -1), int(self.off["spring"][-1]),
-                                      int(self.off["angle"][-1]), int(self.off["pair"][-1]))
+                                      int(self.off["angle"][-1]), int(self.off["pair"][-1]), SCALE_OUTPUTS)
         out = np.zeros((nn, 3))
         for k, key in enumerate(("spring", "angle", "pair")):
             o = self.off[key]

@@ -281,3 +281,19 @@ def test_batch_reload_restarts_every_network(need_gpu):
     for k in range(len(nets)):
         assert a.read(k, "bead").tobytes() == b.read(k, "bead").tobytes()
         assert a.read(k, "spring").tobytes() == b.read(k, "spring").tobytes()
+
+
+def test_backward_outputs_selection(need_gpu):
+    """Asking for the force-scale derivatives alone returns exactly those,
+    bit for bit as in the full set, and unknown names are refused."""
+    net = adjoint_net()
+    w = _weights()
+    eng, ids = engine_for(net, "f64")
+    eng.adjoint_forward(N_STEPS, _group(net), SAMPLES, grid_update_freq=GUF, checkpoint_every=50)
+    full = eng.adjoint_backward(w, len(net.springs), len(net.angles), len(net.pairs))
+    part = eng.adjoint_backward(w, len(net.springs), len(net.angles), len(net.pairs), ("spring", "angle", "pair"))
+    assert sorted(part) == ["angle", "pair", "spring"]
+    for k in part:
+        assert part[k].tobytes() == full[k].tobytes(), k
+    with pytest.raises(ValueError):
+        eng.adjoint_backward(w, len(net.springs), len(net.angles), len(net.pairs), ("stiffness",))
