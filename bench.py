@@ -127,6 +127,44 @@ def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8):
             "grid_update_freq": guf, "generate_s": gen_s}
 
 
+def drift_vs_f64(net, options, marks=(200, 800)):
+    """Drift of the fp32 build against the fp64 build of the same integrator:
+    both engines run the network from the same start (the fp32 one with the
+    measured engine's settings) and their beads are compared at each step
+    count in `marks`: the largest and the rms position difference, and the
+    largest velocity_old and force differences relative to the largest
+    velocity_old and force of the fp64 run.  The marks lie before the
+    workload's launched beads dominate (about step 900)."""
+    import numpy as np
+    from fibrenet_amd import Engine, bind_standard, records as R
+    engs = []
+    try:
+        for prec in ("f32", "f64"):
+            e = Engine(prec)
+            engs.append((e, bind_standard(e, net)))
+        for k, v in options.items():
+            engs[0][0].set_option(k, v)
+        out, done = [], 0
+        for m in marks:
+            for e, _ in engs:
+                e.run_steps(m - done, net.grid_update_freq, True, True)
+            done = m
+            b32, b64 = (e.read_buffer(ids["bead"], R.BEAD) for e, ids in engs)
+            dp = np.linalg.norm(R.vec(b32, "position") - R.vec(b64, "position"), axis=1)
+            dv = np.abs(R.vec(b32, "velocity_old") - R.vec(b64, "velocity_old")).max()
+            df = np.abs(R.vec(b32, "force") - R.vec(b64, "force")).max()
+            out.append({"step": m, "max_position_diff": float(dp.max()),
+                        "rms_position_diff": float(np.sqrt(np.mean(dp ** 2))),
+                        "max_velocity_diff_rel": float(dv / np.abs(R.vec(b64, "velocity_old")).max()),
+                        "max_force_diff_rel": float(df / np.abs(R.vec(b64, "force")).max()),
+                        "breaks_f32": int((engs[0][0].export_breaks(len(net.springs)) >= 0).sum()),
+                        "breaks_f64": int((engs[1][0].export_breaks(len(net.springs)) >= 0).sum())})
+        return out
+    finally:
+        for e, _ in engs:
+            e.close()
+
+
 def bead_state(eng, ids):
     """Regime of the network after the timed steps: non-finite beads and the
     largest speed among the finite ones."""
@@ -442,6 +480,8 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=3000,
                     help="steps of the autotuner, run on a second engine from the warmed-up state (0: none)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-drift", dest="drift", action="store_false",
+                    help="skip the fp32-against-fp64 drift check (tensile_lj / periodic_hs, fp32)")
     ap.add_argument("--copy-bytes", type=float, default=0,
                     help="also run one device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -570,6 +610,13 @@ def main():
         except Exception as exc:          # the plain measurement stands on its own
             adj = {"error": repr(exc)}
 
+    drift = None
+    if args.drift and args.precision == "f32":
+        try:
+            drift = drift_vs_f64(net, {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet")})
+        except Exception as exc:          # the plain measurement stands on its own
+            drift = {"error": repr(exc)}
+
     ens = None
     if args.ensemble_networks > 0:
         try:
@@ -611,7 +658,7 @@ def main():
                   * args.steps / wall / 1e9, "counts": counts, "stats": st, "state_after_timed": state,
                   "generate_s": gen_s,
                   "gpu_over_cpu": (value / cpu["value"]) if cpu else None, "copy_bytes": copy,
-                  "ensemble": ens, "adjoint": adj, "autotune": tune},
+                  "ensemble": ens, "adjoint": adj, "drift_vs_f64": drift, "autotune": tune},
     }
     print(json.dumps(out))
 
