@@ -43,11 +43,24 @@ __device__ __forceinline__ Dual operator-(Dual a, Dual b) { return Dual{a.v - b.
 __device__ __forceinline__ Dual operator-(Dual a) { return Dual{-a.v, -a.d}; }
 __device__ __forceinline__ Dual operator*(Dual a, Dual b) { return Dual{a.v * b.v, a.v * b.d + a.d * b.v}; }
 __device__ __forceinline__ Dual operator*(double s, Dual a) { return Dual{s * a.v, s * a.d}; }
-// a / b through one fp64 reciprocal of b (the quotient within two ulps;
-// every adjoint path shares it).
+// 1/x and 1/sqrt(x) (x > 0 finite) from the hardware estimates refined by
+// one Newton step: within a few ulps, which is all a derivative needs, at a
+// third of the correctly rounded expansions' instructions.
+__device__ __forceinline__ double rcp_d(double x)
+{
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsq_d(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    return y * fma(-0.5 * x * y, y, 1.5);
+}
+
+// a / b through one reciprocal of b (every adjoint path shares it).
 __device__ __forceinline__ Dual operator/(Dual a, Dual b)
 {
-    const double r = 1.0 / b.v;
+    const double r = rcp_d(b.v);
     const double q = a.v * r;
     return Dual{q, (a.d - q * b.d) * r};
 }
@@ -65,8 +78,7 @@ __device__ __forceinline__ Dual datan2(Dual y, Dual x)
 // whose derivative is the exact one.
 __device__ __forceinline__ Dual flen(Dual r2)
 {
-    const double e = sqrt(r2.v);
-    return Dual{(double)native_sqrt(r2.v), e > 0.0 ? r2.d / (2.0 * e) : 0.0};
+    return Dual{(double)native_sqrt(r2.v), r2.v > 0.0 ? r2.d * (0.5 * rsq_d(r2.v)) : 0.0};
 }
 
 struct DV3 {
