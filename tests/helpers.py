@@ -24,6 +24,29 @@ def small_tensile(seed=12, n_fibres=50, n_beads=20, box=(60.0, 60.0, 30.0), weak
     return net
 
 
+def dense_lj(seed=14, n_side=6, n_beads=12, spacing=1.1, lattice=1.15):
+    """A dense block of the tensile network's fibres: n_side x n_side
+    parallel fibres along x on a jittered lattice of 1.15 (no near-coincident
+    beads), all free, in a finite 17.5 x 10 x 10 box.  About 30 listed
+    partners per bead, so a wave's lists run to thousands of entries (several
+    contact passes, the compaction's per-bead writes) and its neighbour search
+    to many candidate passes."""
+    net = G.tensile_lj(n_fibres=n_side * n_side, n_beads=n_beads, box=(17.5, 10.0, 10.0), seed=seed, d_min=0.6)
+    rs = np.random.Generator(np.random.PCG64(seed))
+    yz = np.array([(1.5 + a * lattice, 1.5 + b * lattice) for a in range(n_side) for b in range(n_side)])
+    x0 = 2.0 + rs.uniform(0.0, 1.0, len(yz))
+    img = np.stack([x0[:, None] + spacing * np.arange(n_beads)[None, :],
+                    np.repeat(yz[:, :1], n_beads, 1), np.repeat(yz[:, 1:], n_beads, 1)], axis=2).reshape(-1, 3)
+    img = img + rs.uniform(-0.05, 0.05, img.shape)
+    R.set_vec(net.beads, "position", img)
+    R.set_vec(net.beads, "image_position", img)
+    R.set_vec(net.beads, "velocity", np.zeros_like(img))
+    R.set_vec(net.beads, "velocity_old", np.zeros_like(img))
+    net.info["type"] = 0
+    G.mass_coeffs(net.info, np.array(net.settings, R.SETTINGS).reshape(()), np.ones(len(img)))
+    return net
+
+
 def crosslinked(seed=13, n_links=60):
     """The tensile network plus HR springs between beads of different fibres
     (both index orders) and its angles listed in reverse: bonds outside the

@@ -19,8 +19,8 @@ import numpy as np
 import pytest
 
 from fibrenet_amd import records as R
-from helpers import (crosslinked, engine_for, field_errors, oracle_for, read_beads, run_engine, small_periodic,
-                     small_tensile)
+from helpers import (crosslinked, dense_lj, engine_for, field_errors, oracle_for, read_beads, run_engine,
+                     small_periodic, small_tensile)
 
 pytestmark = pytest.mark.gpu
 
@@ -30,12 +30,13 @@ TOL = {"f64": {f: 1e-9 for f in ("position", "image_position", "velocity", "forc
                "velocity_old": 2e-3, "force_old": 1e-2}}
 
 WORKLOADS = {"periodic_hs": lambda: small_periodic(), "tensile_lj": lambda: small_tensile(),
-             "tensile_lj_breaking": lambda: small_tensile(weak=True), "crosslinked": lambda: crosslinked()}
+             "tensile_lj_breaking": lambda: small_tensile(weak=True), "crosslinked": lambda: crosslinked(),
+             "dense_lj": lambda: dense_lj()}
 
 # Networks whose bonds follow the chain layout (fast path: bonds evaluated in
 # the bead kernel by wave shuffles) and one that does not (bond kernels and
 # incidence slots).
-CHAIN = {"periodic_hs": 1, "tensile_lj": 1, "tensile_lj_breaking": 1, "crosslinked": 0}
+CHAIN = {"periodic_hs": 1, "tensile_lj": 1, "tensile_lj_breaking": 1, "crosslinked": 0, "dense_lj": 1}
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
@@ -381,3 +382,23 @@ def test_autotune_equals_run_steps(need_gpu, workload):
     knobs = {t["knob"] for t in trials}
     assert {"chunk", "staged", "verlet"} <= knobs and all(t["ms_per_step"] > 0 for t in trials)
     assert a.stats()["steps"] == 6100
+
+
+def test_dense_network_takes_the_long_paths(need_gpu):
+    """The dense workload drives the paths the sparse ones leave idle: some
+    64 consecutive beads hold more than 1,024 list entries (the compaction
+    then writes per bead instead of through LDS), every 60 owned beads of a
+    chain wave more than 256 (several contact passes), and the neighbour
+    search tests more than 768 candidates per wave (several passes)."""
+    net = dense_lj()
+    eng, ids = engine_for(net, "f64")
+    run_engine(eng, net, 1, fast=True)
+    p = eng.export_pairs()
+    n = len(net.beads)
+    per_bead = np.bincount(p.ravel(), minlength=n)          # two-sided list length of every bead
+    wave64 = np.add.reduceat(per_bead, np.arange(0, n, 64))
+    chain60 = np.add.reduceat(per_bead, np.arange(0, n, 60))
+    print("list entries per 64 beads: max", wave64.max(), "per chain wave: min", chain60[:-1].min())
+    assert wave64.max() > 1024
+    assert chain60[:-1].min() > 256
+    assert per_bead.mean() > 768 / 64
