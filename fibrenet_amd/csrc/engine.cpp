@@ -470,7 +470,43 @@ class Engine final : public fn_engine {
     {
         adj_.valid = false;
         prep_fast(n, guf, spr, ang);
-        run_chunks(n, guf, spr, ang);
+        presize_rows();
+        if (rows_exact_ > 0 && (!keep_cand_ || rows_cand_ > 0)) {
+            run_chunks(n, guf, spr, ang);
+            return;
+        }
+        // No search-row lengths seen yet for this state (or for kept
+        // candidates): keep the state, so that a run whose search rows or
+        // contact list overflow is repeated with the grown capacities.
+        const std::vector<Part> parts = state_parts(pc_, fc_, false);
+        run_snap_.ensure(parts_bytes(parts));
+        save_parts(parts, run_snap_.p);
+        const int pc = pc_, fc = fc_;
+        const long long steps = steps_, rebuilds = rebuilds_;
+        for (int attempt = 0;; ++attempt) {
+            try {
+                run_chunks(n, guf, spr, ang);
+                return;
+            } catch (const Error &e) {
+                if (e.code != FN_ERR_CAPACITY || attempt == 3) throw;
+            }
+            pc_ = pc;
+            fc_ = fc;
+            steps_ = steps;
+            rebuilds_ = rebuilds;
+            load_parts(state_parts(pc_, fc_, false), run_snap_.p);
+            fnk::fill_int(need_.p, 1, 1, stream_);      // kept candidates are built afresh
+        }
+    }
+
+    // Search rows long enough for the longest rows seen so far (exact lists,
+    // and kept candidates when they are in use), with room to grow.
+    void presize_rows()
+    {
+        const int need = std::max(rows_exact_, keep_cand_ ? rows_cand_ : 0);
+        if (need * 3 / 2 <= K_) return;
+        while (K_ < 2 * need) K_ *= 2;
+        ensure_scratch();
     }
 
     // Checks and device set-up of a fast-path run; spr / ang become the bond
@@ -800,6 +836,7 @@ class Engine final : public fn_engine {
             }
             pairs_.put(pc, stream_);
             cand_dirty_ = true;
+            rows_exact_ = rows_cand_ = 0;
             clear_graphs();
             break;
         }
@@ -845,6 +882,7 @@ class Engine final : public fn_engine {
         field(frc_[1], &fn_bead::force_old);
         HIPCK(hipStreamSynchronize(stream_));
         cand_dirty_ = true;
+        rows_exact_ = rows_cand_ = 0;
     }
 
     void download_beads(fn_bead *o)
@@ -901,6 +939,7 @@ class Engine final : public fn_engine {
             max_skin_ = std::max(max_skin_, q.skin_radius);
         }
         cand_dirty_ = true;
+        rows_exact_ = rows_cand_ = 0;
         clear_graphs();
     }
 
@@ -1478,14 +1517,20 @@ class Engine final : public fn_engine {
 
     void check_capacity()
     {
-        int tot = 0, over = 0;
-        HIPCK(hipMemcpy(&tot, stats_.p + fnk::kStatCandMax, sizeof(int), hipMemcpyDeviceToHost));
-        HIPCK(hipMemcpy(&over, stats_.p + fnk::kStatListOver, sizeof(int), hipMemcpyDeviceToHost));
+        int st[fnk::kStatCount];
+        HIPCK(hipMemcpy(st, stats_.p, sizeof st, hipMemcpyDeviceToHost));
+        const int tot = st[fnk::kStatCandMax], over = st[fnk::kStatListOver];
         last_total_ = std::max(last_total_, tot);
+        // the longest rows the searches needed (counted past the row length)
+        rows_exact_ = std::max(rows_exact_, st[fnk::kStatMaxList]);
+        rows_cand_ = std::max(rows_cand_, st[fnk::kStatMaxCand]);
         if (over > 0) {
+            const int need = std::max(st[fnk::kStatMaxList], st[fnk::kStatMaxCand]), was = K_;
             grow_rows();
-            throw Error(FN_ERR_CAPACITY, fmt("a contact list outgrew its %d-entry search row; the run is invalid "
-                                             "and the row length has been doubled", K_ / 2));
+            while (K_ < 2 * need) K_ *= 2;
+            ensure_scratch();
+            throw Error(FN_ERR_CAPACITY, fmt("a contact list needed %d entries of a %d-entry search row; the run "
+                                             "is invalid and the row length is now %d", need, was, K_));
         }
         if (tot > cap_) {
             grow_list((long long)tot * 2);
@@ -2302,6 +2347,8 @@ class Engine final : public fn_engine {
     bool cellnh_checked_ = false;
     int cap_ = 0, last_total_ = 0;
     int K_ = 64;   // search-row length: entries one bead may collect before compaction
+    int rows_exact_ = 0, rows_cand_ = 0;   // longest exact / candidate rows seen since the state was set (0: none)
+    DArr<unsigned char> run_snap_;         // the state at the start of a guarded run
     int chunk_ = 128;                                // steps per captured graph
     bool chain_on_ = true;                           // use the chain bead kernel where the layout allows
     bool chain_ok_ = false;                          // the bonds follow the chain layout

@@ -402,3 +402,23 @@ def test_dense_network_takes_the_long_paths(need_gpu):
     assert wave64.max() > 1024
     assert chain60[:-1].min() > 256
     assert per_bead.mean() > 768 / 64
+
+
+def test_overflowing_run_is_repeated_with_longer_rows(need_gpu):
+    """Kept candidates of the dense network (about 100 per bead) overflow the
+    initial 64-entry search rows.  The engine keeps the state at the start of
+    a run it has no row lengths for, grows the rows and repeats the run, which
+    then ends bit for bit as a run made with long enough rows from the start."""
+    net = dense_lj()
+    a, ia = engine_for(net, "f64")
+    b, ib = engine_for(net, "f64")
+    for e in (a, b):
+        e.set_option("verlet", 1)
+    run_engine(b, net, 10, fast=True)              # b's rows grow here
+    for kind, rec in (("bead", net.beads), ("spring", net.springs), ("angle", net.angles)):
+        b.write_buffer(ib[kind], rec)              # b back at the start, rows kept
+    run_engine(a, net, 30, fast=True)              # overflows, is repeated
+    run_engine(b, net, 30, fast=True)
+    assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
+    assert np.array_equal(a.export_pairs(), b.export_pairs())
+    assert a.stats()["broken_springs"] == b.stats()["broken_springs"] > 0
