@@ -2268,11 +2268,14 @@ class Engine final : public fn_engine {
                 enqueue_chunk(len, guf, spr, ang, first, last, timing_ ? &rec.ev : nullptr);
                 capturing_ = false;
                 HIPCK(hipStreamEndCapture(stream_, &g));
-                HIPCK(hipGraphInstantiate(&rec.ex, g, nullptr, nullptr, 0));
+                HIPCK(hipGraphInstantiate(&rec.ex[0], g, nullptr, nullptr, 0));
+                HIPCK(hipGraphInstantiate(&rec.ex[1], g, nullptr, nullptr, 0));
                 HIPCK(hipGraphDestroy(g));
                 it = graphs_.emplace(key, rec).first;
             }
-            HIPCK(hipGraphLaunch(it->second.ex, stream_));
+            GraphRec &gr = it->second;
+            HIPCK(hipGraphLaunch(gr.ex[gr.next], stream_));
+            gr.next ^= 1;
             if (timing_) {   // the graph's event nodes hold this replay's times until the next replay
                 HIPCK(hipStreamSynchronize(stream_));
                 accumulate(it->second.ev);
@@ -2299,7 +2302,8 @@ class Engine final : public fn_engine {
     void clear_graphs()
     {
         for (auto &kv : graphs_) {
-            (void)hipGraphExecDestroy(kv.second.ex);
+            for (hipGraphExec_t x : kv.second.ex)
+                if (x) (void)hipGraphExecDestroy(x);
             kv.second.ev.release();
         }
         graphs_.clear();
@@ -2324,8 +2328,13 @@ class Engine final : public fn_engine {
         loose_.clear();
     }
 
+    // A captured chunk, instantiated twice: consecutive replays of one chunk
+    // alternate between the two, so that a replay is submitted while the
+    // previous one still runs (the runtime holds back a second launch of an
+    // executable until its first has finished).
     struct GraphRec {
-        hipGraphExec_t ex = nullptr;
+        hipGraphExec_t ex[2] = {nullptr, nullptr};
+        int next = 0;
         Events ev;
     };
 
