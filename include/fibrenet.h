@@ -127,7 +127,12 @@ int fn_finish(fn_engine *eng);
  * n_steps steps with a rebuild at the first step and every grid_update_freq
  * steps, as fused hipGraph-captured launches.  run_springs / run_angles mirror
  * the caller's springs_num > 0 / angles_num > 0 tests.  Same results as the
- * per-kernel launches; returns after the work is enqueued and checked. */
+ * per-kernel launches; returns after the work is enqueued and checked.  The
+ * contact search rows and list are sized from the longest rows and lists
+ * seen; a run with none seen since the state was written keeps its start
+ * state and is repeated with grown capacities if it overflows them.  Any
+ * other overflow returns FN_ERR_CAPACITY: the run is invalid and the
+ * capacities have been grown for the next one. */
 int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_springs, int run_angles);
 
 /* Launch options of the fast path (no reference counterpart):
