@@ -1436,7 +1436,8 @@ class Engine final : public fn_engine {
     {
         for (;;) {
             if (sync) fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
-            fnk::fill_gated(ncnt_.p, 0, N_, gate, stream_);
+            // every bead has a slot (k_place clamps the ones outside the grid
+            // into it), so the search writes every bead's count
             fnk::search(cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b, ens_net(),
                         boxes_.p, final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, final_list,
                         ncnt_.p, scratch_.p, K_, stats_.p, N_, staged_, flat_, gate, stream_);
@@ -1485,7 +1486,8 @@ class Engine final : public fn_engine {
     {
         const double *p[3] = {pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p};
         if (!keep_cand_) {
-            fnk::set_int(need_.p, 1, nullptr, stream_);
+            // (kept candidates are marked stale by geometry() when they are
+            // switched on, so this path leaves need_ alone)
             enqueue_sort(pc, bg, nullptr);
             search_and_compact(bl, true, false, nullptr);
             return;
