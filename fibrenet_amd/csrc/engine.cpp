@@ -1455,8 +1455,7 @@ class Engine final : public fn_engine {
                 last_total_ = tot;
             }
             fnk::compact(ncnt_.p, nb_start_.p, scratch_.p, K_, N_, final_list ? nb_list_.p : cand_.p, cap_,
-                         final_list ? nb_cnt_.p : nullptr, gate, stream_);
-            fnk::note_build(nb_start_.p + N_, stats_.p, gate, stream_);
+                         final_list ? nb_cnt_.p : nullptr, stats_.p, gate, stream_);
             return;
         }
     }

@@ -81,8 +81,9 @@ void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *p
             const double *skin, const Box &b, const int *net, const Box *boxes, const Stencil &st, double r2,
             bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat, const int *gate,
             hipStream_t s);
+// (stats: counts the build and keeps the largest list total of the run.)
 void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
-             int cap, int *cnt_out, const int *gate, hipStream_t s);
+             int cap, int *cnt_out, int *stats, const int *gate, hipStream_t s);
 // *need = 1 when a bead moved more than sqrt(lim2) since the snapshot.
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
                 const double *rz, int n, const Box &b, double lim2, int *need, hipStream_t s);
@@ -90,8 +91,6 @@ void snapshot(const double *px, const double *py, const double *pz, double *rx, 
               const int *gate, hipStream_t s);
 // The listing rule on the candidates: contact list, counts and cells.
 void filter(const FilterArgs &f, hipStream_t s);
-// Counts a candidate build and keeps the largest candidate total of the run.
-void note_build(const int *total, int *stats, const int *gate, hipStream_t s);
 
 // ---- adjoint (adjoint.hip) ----
 // Operands of one adjoint step: the forward step's positions and images at

@@ -475,13 +475,6 @@ __global__ void k_set_int(int *p, int v, const int *gate)
     *p = v;
 }
 
-__global__ void k_note_build(const int *total, int *stats, const int *gate)
-{
-    if (gated_off(gate)) return;
-    stats[kStatCandBuilds] += 1;
-    stats[kStatCandMax] = max(stats[kStatCandMax], *total);
-}
-
 // FillGrid's cell of a bead (kernel.cl:492-504) in fp64 (-1 outside the grid)
 // into key; the cell of its position clamped into the box into ckey, with the
 // bead's rank among the beads of that cell (first pass of a counting sort).
@@ -1104,9 +1097,14 @@ __device__ __forceinline__ void sort_row(const int *scratch, int i, int m, int n
 }
 
 __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *cand_start, const int *scratch, int K,
-                                                    int n, int *cand, int cap, int *cnt_out, const int *gate)
+                                                    int n, int *cand, int cap, int *cnt_out, int *stats,
+                                                    const int *gate)
 {
     if (gated_off(gate)) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {         // the build's statistics
+        stats[kStatCandBuilds] += 1;
+        stats[kStatCandMax] = max(stats[kStatCandMax], cand_start[n]);
+    }
     __shared__ int stage[kBlock / 64][kCompactStage];
     const int i = blockIdx.x * kBlock + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -1497,9 +1495,10 @@ void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *p
                         n, staged, flat, gate, s);
 }
 void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
-             int cap, int *cnt_out, const int *gate, hipStream_t s)
+             int cap, int *cnt_out, int *stats, const int *gate, hipStream_t s)
 {
-    k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, scratch, K, n, cand, cap, cnt_out, gate);
+    if (n > 0)
+        k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, scratch, K, n, cand, cap, cnt_out, stats, gate);
 }
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
                 const double *rz, int n, const Box &b, double lim2, int *need, hipStream_t s)
@@ -1512,10 +1511,6 @@ void snapshot(const double *px, const double *py, const double *pz, double *rx, 
     k_snapshot<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, gate);
 }
 void filter(const FilterArgs &f, hipStream_t s) { k_filter<<<blocks(f.n), kBlock, 0, s>>>(f); }
-void note_build(const int *total, int *stats, const int *gate, hipStream_t s)
-{
-    k_note_build<<<1, 1, 0, s>>>(total, stats, gate);
-}
 void copy_bounded(const int *src, int *dst, const int *count, int cap, hipStream_t s)
 {
     if (cap > 0) k_copy_bounded<<<std::min(blocks(cap), 4096), kBlock, 0, s>>>(src, dst, count, cap);
