@@ -366,10 +366,14 @@ struct AdjWaveLds {
 template <typename T, typename P>
 __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
 {
-    __shared__ PairC prs[kAdjPairCap];
+    __shared__ PairC prs[kAdjPairsLds];
+    __shared__ PairTab<T> prf[kAdjPairsLds];          // the forward's form, for its mask
     __shared__ AdjWaveLds<T> wl[kBlock / 64];
     const int np = a.n_pairs;
-    for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
+    for (int k = threadIdx.x; k < min(np, kAdjPairsLds); k += kBlock) {
+        prs[k] = a.pairs[k];
+        prf[k] = to_tab<T>(a.pairs[k]);
+    }
     __syncthreads();
     AdjWaveLds<T> &w = wl[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
@@ -419,7 +423,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
                     len = rx * rx + ry * ry + rz * rz;
                 }
                 V3<T> tf;
-                if (contact_force(pr, bf, rx, ry, rz, len, rL, a.radius, j, tf)) {   // the forward's mask
+                if (contact_force(prf[pid], bf, rx, ry, rz, len, rL, a.radius, j, tf)) {   // the forward's mask
                     const double rj = pr.type == 2 ? (double)a.radius[j] : 0.0;
                     const DV3 F = contact_dual(pr, (double)rL, rj,
                                                dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fL, a.fb[j])));
@@ -499,10 +503,14 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
 template <typename T, typename P>
 __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
 {
-    __shared__ PairC prs[kAdjPairCap];
+    __shared__ PairC prs[kAdjPairsLds];
+    __shared__ PairTab<T> prf[kAdjPairsLds];          // the forward's form, for its mask
     __shared__ AdjWaveLds<T> wl[kBlock / 64];
     const int np = a.n_pairs;
-    for (int k = threadIdx.x; k < min(np, kAdjPairCap); k += kBlock) prs[k] = a.pairs[k];
+    for (int k = threadIdx.x; k < min(np, kAdjPairsLds); k += kBlock) {
+        prs[k] = a.pairs[k];
+        prf[k] = to_tab<T>(a.pairs[k]);
+    }
     __syncthreads();
     AdjWaveLds<T> &w = wl[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
@@ -623,7 +631,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
                     len = rx * rx + ry * ry + rz * rz;
                 }
                 V3<T> tf;
-                if (contact_force(pr, bf, rx, ry, rz, len, rL, a.radius, j, tf)) {   // the forward's mask
+                if (contact_force(prf[pid], bf, rx, ry, rz, len, rL, a.radius, j, tf)) {   // the forward's mask
                     const double rj = pr.type == 2 ? (double)a.radius[j] : 0.0;
                     const DV3 F = contact_dual(pr, (double)rL, rj,
                                                dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fL, a.fb[j])));
