@@ -40,33 +40,35 @@
 #include <stdlib.h>
 #include <string.h>
 
-#define OR_GRID_TAIL_SIZE 64    /* kernel.cl:10 */
-#define OR_BEAD_NHOOD_SIZE 256  /* kernel.cl:11 */
+#include "fibrenet_oracle.h"
 
-/* Records restated from kernel.cl:13-156 (AoS, natural alignment). */
-typedef struct { double x, y, z, w; } or_vec3;
-typedef struct { or_vec3 position, image_position, velocity, force, velocity_old, force_old; } or_bead;
-typedef struct {
-    int type, tag, loc_id;
-    double mass, mass_inv, lambda_dt_mass_inv, half_dt_mass_inv, half_dt2_mass_inv;
-    double radius, radius_sq, skin_radius, skin_radius_sq;
-} or_info;
-typedef struct {
-    int type, tag, status, p1, p2;
-    double rest_length, stiffness, damping, c1, c2;
-} or_spring;
-typedef struct {
-    int type, tag, status, b1, b2, b3, s1, s2;
-    double angle, current_angle, c1, c2, c3;
-} or_angle;
-typedef struct { int type, p1, p2; double c1, c2, c3, c4, c5, c6; } or_pair;
-typedef struct {
-    int types_num, bc_type;
-    double wall_k, cutoff, cutoff_sq, unit, dt, half_dt, half_dt2, lambda_dt, gamma, sigma, damping;
-    int cx, cy, cz, cxy;
-    double half_box_sq;
-    or_vec3 step, size, gravity;
-} or_settings;
+/* Work-item execution.  The sequential build runs work-items in order and
+ * adds plainly.  The timing twin (built with -DOR_MT -fopenmp as
+ * liboracle_mt.so) runs each kernel's work-items on all host threads in
+ * chunks of 64 (one OpenCL work-group of the reference's local size,
+ * OpenclManager.cpp:219-285), adds shared forces with the compare-and-swap
+ * loop of kernel.cl:174-193 and takes list slots with atomic increments
+ * (kernel.cl:497-503, 632-637), as a CPU OpenCL device executes kernel.cl. */
+#ifdef OR_MT
+#define OR_PARALLEL_FOR _Pragma("omp parallel for schedule(dynamic, 64)")
+static void or_atomic_add(double *p, double v)
+{
+    unsigned long long *u = (unsigned long long *)p;
+    unsigned long long old = __atomic_load_n(u, __ATOMIC_RELAXED), nw;
+    do {
+        double d;
+        memcpy(&d, &old, sizeof d);
+        d += v;
+        memcpy(&nw, &d, sizeof nw);
+    } while (!__atomic_compare_exchange_n(u, &old, nw, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED));
+}
+#define OR_ADD(p, v) or_atomic_add((p), (v))
+#define OR_FETCH_INC(p) __atomic_fetch_add((p), 1, __ATOMIC_RELAXED)
+#else
+#define OR_PARALLEL_FOR
+#define OR_ADD(p, v) (*(p) += (v))
+#define OR_FETCH_INC(p) ((*(p))++)
+#endif
 
 /* kernel.cl:9 GETADRESS: symmetric upper-triangular index of a type pair. */
 static int or_pair_index(int t1, int t2, int n)
@@ -82,6 +84,7 @@ static double or_native_sqrt(double x) { return (double)sqrtf((float)x); }
 /* kernel.cl:195-265 AdvancePosition. */
 void or_advance_position(or_bead *beads, const or_info *info, const or_settings *s, int n)
 {
+    OR_PARALLEL_FOR
     for (int i = 0; i < n; ++i) {
         or_bead b = beads[i];
         double lam = info[i].lambda_dt_mass_inv;
@@ -133,6 +136,7 @@ void or_advance_position(or_bead *beads, const or_info *info, const or_settings 
  * index at which a spring's status went to 0. */
 void or_resolve_springs(or_bead *beads, or_spring *springs, int n_springs, int step, int *break_step)
 {
+    OR_PARALLEL_FOR
     for (int s = 0; s < n_springs; ++s) {
         or_spring sp = springs[s];
         if (!(sp.type == 0 || sp.type == 1) || sp.status != 1) continue;
@@ -160,15 +164,15 @@ void or_resolve_springs(or_bead *beads, or_spring *springs, int n_springs, int s
             ds.x *= factor;
             ds.y *= factor;
             ds.z *= factor;
-            beads[sp.p1].force.x += ds.x;
-            beads[sp.p1].force.y += ds.y;
-            beads[sp.p1].force.z += ds.z;
+            OR_ADD(&beads[sp.p1].force.x, ds.x);
+            OR_ADD(&beads[sp.p1].force.y, ds.y);
+            OR_ADD(&beads[sp.p1].force.z, ds.z);
             ds.x *= -1.0;
             ds.y *= -1.0;
             ds.z *= -1.0;
-            beads[sp.p2].force.x += ds.x;
-            beads[sp.p2].force.y += ds.y;
-            beads[sp.p2].force.z += ds.z;
+            OR_ADD(&beads[sp.p2].force.x, ds.x);
+            OR_ADD(&beads[sp.p2].force.y, ds.y);
+            OR_ADD(&beads[sp.p2].force.z, ds.z);
         }
     }
 }
@@ -176,6 +180,7 @@ void or_resolve_springs(or_bead *beads, or_spring *springs, int n_springs, int s
 /* kernel.cl:268-363 ResolveAngles. */
 void or_resolve_angles(or_angle *angles, const or_spring *springs, or_bead *beads, int n_angles)
 {
+    OR_PARALLEL_FOR
     for (int a = 0; a < n_angles; ++a) {
         or_angle ab = angles[a];
         if (ab.type == 1) {
@@ -217,21 +222,22 @@ void or_resolve_angles(or_angle *angles, const or_spring *springs, or_bead *bead
         if (ln != 0.0) ln = 1.0 / ln;
         ln *= fc;
         pc.x *= ln; pc.y *= ln; pc.z *= ln;
-        beads[ab.b1].force.x += pa.x;
-        beads[ab.b1].force.y += pa.y;
-        beads[ab.b1].force.z += pa.z;
-        beads[ab.b3].force.x += pc.x;
-        beads[ab.b3].force.y += pc.y;
-        beads[ab.b3].force.z += pc.z;
-        beads[ab.b2].force.x += -pa.x - pc.x;
-        beads[ab.b2].force.y += -pa.y - pc.y;
-        beads[ab.b2].force.z += -pa.z - pc.z;
+        OR_ADD(&beads[ab.b1].force.x, pa.x);
+        OR_ADD(&beads[ab.b1].force.y, pa.y);
+        OR_ADD(&beads[ab.b1].force.z, pa.z);
+        OR_ADD(&beads[ab.b3].force.x, pc.x);
+        OR_ADD(&beads[ab.b3].force.y, pc.y);
+        OR_ADD(&beads[ab.b3].force.z, pc.z);
+        OR_ADD(&beads[ab.b2].force.x, -pa.x - pc.x);
+        OR_ADD(&beads[ab.b2].force.y, -pa.y - pc.y);
+        OR_ADD(&beads[ab.b2].force.z, -pa.z - pc.z);
     }
 }
 
 /* kernel.cl:444-464 AdvanceVelocity. */
 void or_advance_velocity(or_bead *beads, const or_info *info, int n)
 {
+    OR_PARALLEL_FOR
     for (int i = 0; i < n; ++i) {
         double h = info[i].half_dt_mass_inv;
         or_vec3 v = beads[i].velocity_old;
@@ -249,6 +255,7 @@ void or_advance_velocity(or_bead *beads, const or_info *info, int n)
 /* kernel.cl:468-483 ResetBeadEnergy. */
 void or_reset_bead_energy(or_bead *beads, int n)
 {
+    OR_PARALLEL_FOR
     for (int i = 0; i < n; ++i) {
         beads[i].force.x = beads[i].force.y = beads[i].force.z = 0.0;
         beads[i].velocity.x = beads[i].velocity.y = beads[i].velocity.z = 0.0;
@@ -272,13 +279,14 @@ static void or_cell_of(const or_vec3 *p, const or_settings *s, int *cx, int *cy,
  * index of every bead, -1 for beads outside the grid. */
 void or_fill_grid(const or_bead *beads, int *head, int *tail, const or_settings *s, int n, int *cell_of_bead)
 {
+    OR_PARALLEL_FOR
     for (int i = 0; i < n; ++i) {
         int cx, cy, cz;
         or_cell_of(&beads[i].position, s, &cx, &cy, &cz);
         int cell = -1;
         if (cx >= 0 && cx < s->cx && cy >= 0 && cy < s->cy && cz >= 0 && cz < s->cz) {
             cell = cx + cy * s->cx + cz * s->cxy;
-            int old = head[cell]++;
+            int old = OR_FETCH_INC(&head[cell]);
             if (old < OR_GRID_TAIL_SIZE) tail[(size_t)cell * OR_GRID_TAIL_SIZE + old] = i;
         }
         if (cell_of_bead) cell_of_bead[i] = cell;
@@ -315,7 +323,7 @@ static void or_try_pair(int i, int j, const or_bead *beads, const or_info *info,
         take = length < pow(max_radius + min_skin, 2.0);
     }
     if (!take) return;
-    int k = nh_head[i]++;
+    int k = OR_FETCH_INC(&nh_head[i]);
     if (k < OR_BEAD_NHOOD_SIZE) {
         nh_tail[(size_t)i * OR_BEAD_NHOOD_SIZE + k] = j;
         pair_ids[(size_t)i * OR_BEAD_NHOOD_SIZE + k] = pid;
@@ -331,6 +339,7 @@ void or_build_contact_list(const int *head, const int *tail, const int *cell_nh,
                            const or_bead *beads, const or_info *info, int *pair_ids, const or_pair *table,
                            const or_settings *s, int n_cells)
 {
+    OR_PARALLEL_FOR
     for (int c = 0; c < n_cells; ++c) {
         int nc = head[c];
         if (nc <= 0) continue;
@@ -357,6 +366,7 @@ void or_build_contact_list(const int *head, const int *tail, const int *cell_nh,
 void or_resolve_contacts(const int *nh_head, const int *nh_tail, or_bead *beads, const or_info *info,
                          const int *pair_ids, const or_pair *table, const or_settings *s, int n)
 {
+    OR_PARALLEL_FOR
     for (int i = 0; i < n; ++i) {
         int m = nh_head[i];
         if (m <= 0) continue;
@@ -424,16 +434,16 @@ void or_resolve_contacts(const int *nh_head, const int *nh_tail, or_bead *beads,
                 acc.x += fx;
                 acc.y += fy;
                 acc.z += fz;
-                beads[j].force.x += -fx;
-                beads[j].force.y += -fy;
-                beads[j].force.z += -fz;
+                OR_ADD(&beads[j].force.x, -fx);
+                OR_ADD(&beads[j].force.y, -fy);
+                OR_ADD(&beads[j].force.z, -fz);
                 has_contact = 1;
             }
         }
         if (has_contact) {
-            beads[i].force.x += acc.x;
-            beads[i].force.y += acc.y;
-            beads[i].force.z += acc.z;
+            OR_ADD(&beads[i].force.x, acc.x);
+            OR_ADD(&beads[i].force.y, acc.y);
+            OR_ADD(&beads[i].force.z, acc.z);
         }
     }
 }
@@ -479,25 +489,6 @@ void or_grid_neighbours(const or_settings *s, int *nh)
             }
 }
 
-/* Whole state of one oracle run. */
-typedef struct {
-    or_bead *beads;
-    or_info *info;
-    or_spring *springs;
-    or_angle *angles;
-    or_pair *pairs;
-    or_settings settings;
-    int n_beads, n_springs, n_angles, n_cells;
-    int *cell_nh;       /* n_cells*13                     */
-    int *grid_head;     /* n_cells                        */
-    int *grid_tail;     /* n_cells*64                     */
-    int *nh_head;       /* n_beads                        */
-    int *nh_tail;       /* n_beads*256                    */
-    int *pair_ids;      /* n_beads*256                    */
-    int *cell_of_bead;  /* n_beads, last rebuild          */
-    int *break_step;    /* n_springs, -1 = intact         */
-    int step;           /* global step counter            */
-} or_state;
 
 /* SimulationManager.cpp:151-181 RunSimulationLoop: the rebuild counter restarts
  * at zero on every call, so the grid is rebuilt at the first step of each call

@@ -14,34 +14,37 @@ from .loader import grid_neighbours
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# The same restatement run on all host threads (atomic adds, dynamic 64-item
+# chunks): the all-core CPU baseline.  Its sums are order-nondeterministic.
+LIB_MT_PATH = os.path.join(_HERE, "build", "liboracle_mt.so")
 
 
 def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def _lib():
-    if not os.path.exists(LIB_PATH):
+def _load(path):
+    if not os.path.exists(path):
         build()
-    lib = ctypes.CDLL(LIB_PATH)
-    return lib
+    L = ctypes.CDLL(path)
+    assert L.or_sizeof(0) == R.BEAD.itemsize
+    assert L.or_sizeof(1) == R.BEAD_INFO.itemsize
+    assert L.or_sizeof(2) == R.SPRING.itemsize
+    assert L.or_sizeof(3) == R.ANGLE.itemsize
+    assert L.or_sizeof(4) == R.PAIR.itemsize
+    assert L.or_sizeof(5) == R.SETTINGS.itemsize
+    assert L.or_sizeof(6) == ctypes.sizeof(_State)
+    return L
 
 
-_LIB = None
+_LIBS = {}
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        _LIB = _lib()
-        assert _LIB.or_sizeof(0) == R.BEAD.itemsize
-        assert _LIB.or_sizeof(1) == R.BEAD_INFO.itemsize
-        assert _LIB.or_sizeof(2) == R.SPRING.itemsize
-        assert _LIB.or_sizeof(3) == R.ANGLE.itemsize
-        assert _LIB.or_sizeof(4) == R.PAIR.itemsize
-        assert _LIB.or_sizeof(5) == R.SETTINGS.itemsize
-        assert _LIB.or_sizeof(6) == ctypes.sizeof(_State)
-    return _LIB
+def lib(threaded=False):
+    path = LIB_MT_PATH if threaded else LIB_PATH
+    if path not in _LIBS:
+        _LIBS[path] = _load(path)
+    return _LIBS[path]
 
 
 class _State(ctypes.Structure):
@@ -60,9 +63,13 @@ def _p(a):
 
 
 class Oracle:
-    """One sequential fp64 run of the reference step over numpy-held records."""
+    """One fp64 run of the reference step over numpy-held records:
+    sequential (the checker), or with `threaded` every kernel's work-items
+    on all host threads (OMP_NUM_THREADS) with atomic adds (the CPU
+    baseline; summation order, and so the last bits, vary from run to run)."""
 
-    def __init__(self, beads, info, springs, angles, pairs, settings, cell_nh=None):
+    def __init__(self, beads, info, springs, angles, pairs, settings, cell_nh=None, threaded=False):
+        self._lib = lib(threaded)
         self.beads = np.ascontiguousarray(beads.copy())
         self.info = np.ascontiguousarray(info.copy())
         self.springs = np.ascontiguousarray(springs.copy())
@@ -104,48 +111,48 @@ class Oracle:
         return self._st.step
 
     def run(self, n_steps, grid_update_freq=8):
-        rc = lib().or_run_loop(ctypes.byref(self._st), int(n_steps), int(grid_update_freq))
+        rc = self._lib.or_run_loop(ctypes.byref(self._st), int(n_steps), int(grid_update_freq))
         if rc:
             raise ValueError("or_run_loop failed")
 
     def minimize(self, steps, sub_steps, reset, grid_update_freq=8):
-        rc = lib().or_minimize(ctypes.byref(self._st), int(steps), int(sub_steps), int(bool(reset)),
+        rc = self._lib.or_minimize(ctypes.byref(self._st), int(steps), int(sub_steps), int(bool(reset)),
                                int(grid_update_freq))
         if rc:
             raise ValueError("or_minimize failed")
 
     def reset_energy(self):
-        lib().or_reset_bead_energy(_p(self.beads), len(self.beads))
+        self._lib.or_reset_bead_energy(_p(self.beads), len(self.beads))
 
     # -- single kernels, for stage-wise checks --------------------------------------
     def advance_position(self):
-        lib().or_advance_position(_p(self.beads), _p(self.info), _p(self.settings.reshape(1)),
+        self._lib.or_advance_position(_p(self.beads), _p(self.info), _p(self.settings.reshape(1)),
                                   len(self.beads))
 
     def resolve_springs(self, step=0):
-        lib().or_resolve_springs(_p(self.beads), _p(self.springs), len(self.springs), int(step),
+        self._lib.or_resolve_springs(_p(self.beads), _p(self.springs), len(self.springs), int(step),
                                  _p(self.break_step))
 
     def resolve_angles(self):
-        lib().or_resolve_angles(_p(self.angles), _p(self.springs), _p(self.beads), len(self.angles))
+        self._lib.or_resolve_angles(_p(self.angles), _p(self.springs), _p(self.beads), len(self.angles))
 
     def rebuild(self):
         s1 = self.settings.reshape(1)
         self.grid_head[:] = 0
         self.nh_head[:] = 0
-        lib().or_fill_grid(_p(self.beads), _p(self.grid_head), _p(self.grid_tail), _p(s1), len(self.beads),
+        self._lib.or_fill_grid(_p(self.beads), _p(self.grid_head), _p(self.grid_tail), _p(s1), len(self.beads),
                            _p(self.cell_of_bead))
-        lib().or_build_contact_list(_p(self.grid_head), _p(self.grid_tail), _p(self.cell_nh),
+        self._lib.or_build_contact_list(_p(self.grid_head), _p(self.grid_tail), _p(self.cell_nh),
                                     _p(self.nh_head), _p(self.nh_tail), _p(self.beads), _p(self.info),
                                     _p(self.pair_ids), _p(self.pairs), _p(s1), len(self.grid_head))
 
     def resolve_contacts(self):
-        lib().or_resolve_contacts(_p(self.nh_head), _p(self.nh_tail), _p(self.beads), _p(self.info),
+        self._lib.or_resolve_contacts(_p(self.nh_head), _p(self.nh_tail), _p(self.beads), _p(self.info),
                                   _p(self.pair_ids), _p(self.pairs), _p(self.settings.reshape(1)),
                                   len(self.beads))
 
     def advance_velocity(self):
-        lib().or_advance_velocity(_p(self.beads), _p(self.info), len(self.beads))
+        self._lib.or_advance_velocity(_p(self.beads), _p(self.info), len(self.beads))
 
     # -- canonical views -------------------------------------------------------------
     def pair_set(self):

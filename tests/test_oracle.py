@@ -446,6 +446,28 @@ def test_forces_sum_to_zero_and_momentum_is_conserved(kind):
     assert np.allclose(vec(o.beads, "velocity_old").sum(0), p0, atol=1e-9)
 
 
+@pytest.mark.parametrize("kind,bc", [("hs", 0), ("lj", 1)])
+def test_threaded_oracle_matches_sequential(kind, bc):
+    """The all-core CPU baseline (work-items on every thread, compare-and-swap
+    adds) lists the same pairs and breaks the same springs as the sequential
+    oracle, with positions equal up to the summation order of the atomic
+    adds."""
+    b, q, sp, an, pr, s = _random_network(5, n_fib=60, kind=kind, bc=bc)
+    sp["type"] = 1                                  # HR_FL: failure decisions are compared too
+    sp["c1"], sp["c2"] = 1000.0, 20.0
+    seq = Oracle(b, q, sp, an, pr, s)
+    par = Oracle(b, q, sp, an, pr, s, threaded=True)
+    for _ in range(6):
+        seq.run(10, 4)
+        par.run(10, 4)
+        assert np.array_equal(seq.pair_set(), par.pair_set())
+        assert np.array_equal(seq.springs["status"], par.springs["status"])
+        assert np.array_equal(seq.break_step, par.break_step)
+        for f in ("position", "velocity_old", "force"):
+            ref = vec(seq.beads, f)
+            assert np.abs(vec(par.beads, f) - ref).max() <= 1e-9 * max(np.abs(ref).max(), 1.0), f
+
+
 def _brute_pairs(b, q, pr, s, restrict_cells=True):
     """All (i<j) pairs under BuildContactList's predicate (kernel.cl:574-659),
     optionally limited to beads whose cells are equal or adjacent
