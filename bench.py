@@ -14,11 +14,19 @@ The settings that cannot change a result (chain bead kernel, staged search,
 kept contact candidates, graph chunk length) are chosen by the autotuner,
 run on a second engine with the same network from the same warmed-up state.
 
+The network is generated with its fibres at least 2^(1/6) sigma apart
+and relaxed by MinimizeSystemEnergy with reset (8,000 steps, reset every
+400) before anything is timed; the clamp and pull keep their velocities.
+`--workload tensile_run` runs the same network as config 1 asks: 20,000
+steps in 100-step RunSimulationLoop calls with the clamp's reaction force
+read after each, reporting the dynamics along the way.
+
 Prints one JSON line: bead-steps/s of the timed steps, the roofline of the
 dominant kernel (the step kernel with the largest duration, measured with
-HIP event nodes in the captured graphs of the timed region) and a CPU
-baseline (the sequential fp64 oracle on the host, a bounded sample of the
-same network).
+HIP event nodes in the captured graphs of the timed region, against the
+survey's algorithmic bytes of a step) and a CPU baseline (the fp64 oracle
+on all host threads with the reference's atomic adds, and on one core, on a
+bounded sample of the same network).
 """
 import argparse
 import json
@@ -32,9 +40,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def algorithmic_bytes(net, pairs, tb, aliased, chain):
-    """HBM bytes each launch of a step must move at least (every datum read or
-    written once; tb = bytes of a force/velocity component, positions fp64).
+def implementation_bytes(net, pairs, tb, aliased, chain):
+    """HBM bytes each launch of this implementation moves at least (every
+    datum of its own layout read or written once; tb = bytes of a
+    force/velocity component, positions fp64).  Reported beside the survey's
+    algorithmic bytes (survey_bytes), which the roofline uses.
 
     chain layout (one launch per step, the bead kernel evaluating the bonds):
       per bead position read and next position written (48 B), image read
@@ -81,6 +91,28 @@ def algorithmic_bytes(net, pairs, tb, aliased, chain):
     bead = (48 + (0 if aliased else 48) + 12 * tb + 12) * n + (8 + 3 * tb) * s + (12 + 6 * tb) * a + 8 * pairs
     return (dict(spring=spring, angle=angle, bead=bead), rebuild,
             dict(beads=n, springs_hr=s_hr, springs_hr_fl=s_fl, angles=a, pairs=pairs))
+
+
+def survey_bytes(counts, grid_update_freq, chain):
+    """Algorithmic bytes of a step by the survey's formula (fp32 SoA, each
+    array touched once per step; extra-precision state such as the fp64
+    positions is not counted):
+
+      B_step = 100 N + 16 S_HR + 28 S_FL + 20 A + 8 P + (36 N + 8 P) / grid_update_freq
+
+    with N beads, S_HR / S_FL springs by type, A angles and P listed unique
+    contact pairs.  Per kernel: with the chain layout the bead kernel is the
+    whole step apart from the rebuild; otherwise the spring-term kernel
+    takes 16 S_HR + 28 S_FL, the angle-term kernel 20 A and the bead kernel
+    100 N + 8 P.  The rebuild is 36 N + 8 P per rebuild."""
+    n, s_hr, s_fl, a, p = (counts[k] for k in ("beads", "springs_hr", "springs_hr_fl", "angles", "pairs"))
+    bonds = 16 * s_hr + 28 * s_fl
+    if chain:
+        kern = dict(bead=100 * n + bonds + 20 * a + 8 * p)
+    else:
+        kern = dict(spring=bonds, angle=20 * a, bead=100 * n + 8 * p)
+    rebuild = 36 * n + 8 * p
+    return kern, rebuild, sum(kern.values()) + rebuild / grid_update_freq
 
 
 def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8):
@@ -348,6 +380,64 @@ def tensile_fit_bench(args):
     print(json.dumps(out))
 
 
+def tensile_run_bench(args):
+    """Config 1 as its run asks: the relaxed 2,000,000-bead tensile network,
+    20,000 steps as 200 RunSimulationLoop calls of 100 steps (the rebuild
+    cadence restarts with each call, SimulationManager.cpp:151-181), the
+    clamp's reaction force (the x-force summed over the clamped beads,
+    fn_group_force) read after each call.  The timed region holds the calls
+    and the reads; every 2,000 steps the beads are read back outside it
+    for the dynamics (non-finite beads, largest speed, broken springs)."""
+    import numpy as np
+    from fibrenet_amd import Engine, bind_standard, generate as G
+    t0 = time.perf_counter()
+    net = G.tensile_lj(n_fibres=args.fibres or 20000)
+    if args.relax_steps > 0:
+        net = G.relax(net, args.relax_steps, args.relax_substeps)
+    prep_s = time.perf_counter() - t0
+    guf, chunk, total = net.grid_update_freq, 100, args.run_steps
+    group = (np.asarray(net.info["type"]) == 1).astype(np.int32)
+    chosen = None
+    if args.tune_steps > 0:
+        tuner = Engine(args.precision)
+        bind_standard(tuner, net)
+        _, chosen = tuner.autotune(args.tune_steps, guf, True, True)
+        tuner.close()
+    eng = Engine(args.precision)
+    ids = bind_standard(eng, net)
+    for k, v in (chosen or {}).items():
+        eng.set_option(k, v)
+    eng.run_steps(chunk, guf, True, True)            # captures the graphs of a 100-step call
+    eng.write_buffer(ids["bead"], net.beads)
+    eng.write_buffer(ids["spring"], net.springs)
+    eng.finish()
+    reaction, dyn, timed = [], [], 0.0
+    for k in range(total // chunk):
+        t = time.perf_counter()
+        eng.run_steps(chunk, guf, True, True)
+        reaction.append(eng.group_force(group))
+        timed += time.perf_counter() - t
+        done = (k + 1) * chunk
+        if done % 2000 == 0:
+            st = bead_state(eng, ids)
+            st.update(step=done, broken_springs=eng.stats()["broken_springs"])
+            dyn.append(st)
+            print("tensile_run: step %d, %s" % (done, st), file=sys.stderr, flush=True)
+    eng.close()
+    n = len(net.beads)
+    out = {"metric": "bead-steps/s", "value": n * total / timed, "unit": "bead-steps/s", "n_gpus": 1,
+           "steps": total, "warmup": 0, "ms_per_step": timed * 1e3 / total, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32 forces/velocities, f64 positions" if args.precision == "f32" else "f64",
+           "data": "synthetic (generated fibre network, seed %d)" % net.meta["seed"],
+           "config": {"workload": "tensile_run: config 1's run, %d steps in %d-step calls with the clamp's "
+                                  "reaction force read after each, %d beads, relaxed by %d minimisation steps"
+                                  % (total, chunk, n, args.relax_steps), "beads": n},
+           "extra": {"options": chosen, "prepare_s": prep_s, "reaction_force_every_100_steps": reaction,
+                     "dynamics": dyn, "all_finite": all(d["nonfinite_beads"] == 0 for d in dyn)}}
+    print(json.dumps(out))
+
+
 def batched_fit_bench(args):
     """The batched adjoint: 128 tensile networks of uneven size (fibre counts
     uniform in [50, 300], lengths in [30, 60] beads, HR_FL springs with
@@ -448,9 +538,35 @@ def batched_fit_bench(args):
     print(json.dumps(out))
 
 
-def cpu_baseline(net, steps):
+def host_cpu():
+    """Host facts for the CPU baseline: logical CPUs (nproc), the CPUs this
+    process may run on, the model name and the OpenMP thread count."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count()
+    threads = int(os.environ.get("OMP_NUM_THREADS") or allowed or 1)
+    return {"nproc": os.cpu_count(), "allowed_cpus": allowed, "model": model, "omp_threads": threads}
+
+
+def cpu_baseline(net, steps, threaded):
+    """The oracle on the host: `threaded` runs every kernel's work-items on
+    all OpenMP threads with the reference's atomic adds, as a CPU OpenCL
+    device runs kernel.cl; otherwise one core in work-item order.  Ten
+    warm-up steps, then `steps` timed steps (grid rebuilds included)."""
     from oracle.oracle import Oracle
-    orc = Oracle(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, net.cell_nh)
+    orc = Oracle(net.beads, net.info, net.springs, net.angles, net.pairs, net.settings, net.cell_nh,
+                 threaded=threaded)
+    orc.run(10, net.grid_update_freq)
     t = time.perf_counter()
     orc.run(steps, net.grid_update_freq)
     dt = time.perf_counter() - t
@@ -464,7 +580,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--workload", default="tensile_lj",
-                    choices=["tensile_lj", "periodic_hs", "ensemble", "tensile_fit", "batched_fit"])
+                    choices=["tensile_lj", "periodic_hs", "ensemble", "tensile_fit", "batched_fit", "tensile_run"])
+    ap.add_argument("--run-steps", type=int, default=20000, help="tensile_run: steps of the run")
     ap.add_argument("--fit-iterations", type=int, default=None,
                     help="tensile_fit / batched_fit: gradient steps of the fit (30 / 200)")
     ap.add_argument("--fit-networks", type=int, default=128, help="batched_fit: networks")
@@ -473,7 +590,14 @@ def main():
     ap.add_argument("--ensemble-networks", type=int, default=128,
                     help="networks of the ensemble comparison reported beside the plain workload (0: none)")
     ap.add_argument("--fibres", type=int, default=None)
-    ap.add_argument("--cpu-steps", type=int, default=50)
+    ap.add_argument("--cpu-steps", type=int, default=100,
+                    help="timed steps of the all-core CPU baseline (the threaded oracle)")
+    ap.add_argument("--cpu-steps-1core", type=int, default=20,
+                    help="timed steps of the one-core CPU baseline (the sequential oracle)")
+    ap.add_argument("--relax-steps", type=int, default=8000,
+                    help="tensile_lj: MinimizeSystemEnergy steps with reset before the run (0: none)")
+    ap.add_argument("--relax-substeps", type=int, default=400,
+                    help="tensile_lj: steps between the minimisation's energy resets")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the adjoint cost measurement on the same network (0: none)")
     ap.add_argument("--checkpoint-every", type=int, default=50)
@@ -502,6 +626,9 @@ def main():
         args.fit_iterations = 200 if args.fit_iterations is None else args.fit_iterations
         batched_fit_bench(args)
         return
+    if args.workload == "tensile_run":
+        tensile_run_bench(args)
+        return
 
     if args.workload == "ensemble":
         e = ensemble_bench(args.networks, args.steps, args.warmup, args.loop_steps, args.precision)
@@ -519,16 +646,22 @@ def main():
         return
 
     t0 = time.perf_counter()
+    relax_s = None
     if args.workload == "tensile_lj":
         net = G.tensile_lj(n_fibres=args.fibres or 20000)
-        desc = ("tensile_lj: %d fibres x 100 beads (%d beads), finite box 1000x1000x50, HR_FL springs + HR angles"
+        if args.relax_steps > 0:
+            t1 = time.perf_counter()
+            net = G.relax(net, args.relax_steps, args.relax_substeps)
+            relax_s = time.perf_counter() - t1
+        desc = ("tensile_lj: %d fibres x 100 beads (%d beads), finite box 1000x1000x50, placed >= 2^(1/6) sigma "
+                "apart, relaxed by %d minimisation steps (reset every %d), HR_FL springs + HR angles"
                 " + LJ contacts (cutoff 2.5), clamp/pull 0.01, dt 5e-4, grid_update_freq 10"
-                % (net.meta["n_fibres"], len(net.beads)))
+                % (net.meta["n_fibres"], len(net.beads), args.relax_steps, args.relax_substeps))
     else:
         net = G.periodic_hs(n_fibres=args.fibres or 200)
         desc = ("periodic_hs: %d fibres x 50 beads (%d beads), periodic box 100x100x20, HR springs + HR angles"
                 " + penalty contacts, dt 1e-3, grid_update_freq 8" % (net.meta["n_fibres"], len(net.beads)))
-    gen_s = time.perf_counter() - t0
+    gen_s = time.perf_counter() - t0 - (relax_s or 0.0)
 
     eng = Engine(args.precision)
     ids = bind_standard(eng, net)
@@ -578,7 +711,9 @@ def main():
     pairs = st["list_entries"] // 2
     aliased = bool(st["img_aliases_pos"])
     chain = bool(eng.get_option("chain") and eng.get_option("chain_layout"))
-    b_kernel, b_rebuild, counts = algorithmic_bytes(net, pairs, 4 if args.precision == "f32" else 8, aliased, chain)
+    impl_kernel, impl_rebuild, counts = implementation_bytes(net, pairs, 4 if args.precision == "f32" else 8,
+                                                          aliased, chain)
+    b_kernel, b_rebuild, b_step = survey_bytes(counts, net.grid_update_freq, chain)
     n_t = max(tim["steps"], 1)
     kern_ms = {"spring": tim["spring_ms"] / n_t, "angle": tim["angle_ms"] / n_t, "bead": tim["bead_ms"] / n_t}
     names = {"spring": "k_spring_terms", "angle": "k_angle_terms",
@@ -588,8 +723,8 @@ def main():
     dom = max(kern_ms, key=kern_ms.get)
     rebuild_ms = tim["rebuild_ms"] / max(tim["rebuilds"], 1)
     achieved = b_kernel[dom] / (kern_ms[dom] * 1e-3) / 1e9
-    b_step = sum(b_kernel.values())
     value = n * args.steps / wall
+    step_gbs = b_step * args.steps / wall / 1e9
 
     copy = None
     if args.copy_bytes > 0:   # one k_copy_probe launch of known size, for counter calibration
@@ -626,12 +761,19 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        v, dt = cpu_baseline(net, args.cpu_steps)
-        cpu = {"value": v, "unit": "bead-steps/s", "cores": 1, "kind": "port",
-               "sample": "the same %d-bead network, %d steps (%d grid rebuilds) of the sequential fp64 oracle "
-                         "(restatement of kernel.cl), %.1f s" % (n, args.cpu_steps,
-                                                                (args.cpu_steps + net.grid_update_freq - 1)
-                                                                // net.grid_update_freq, dt)}
+        host = host_cpu()
+        v, dt = cpu_baseline(net, args.cpu_steps, True)
+        v1, dt1 = cpu_baseline(net, args.cpu_steps_1core, False)
+        reb = lambda k: (k + net.grid_update_freq - 1) // net.grid_update_freq
+        cpu = {"value": v, "unit": "bead-steps/s", "cores": host["omp_threads"], "kind": "port",
+               "sample": "the same %d-bead network, %d steps (%d grid rebuilds) after 10 warm-up steps: the fp64 "
+                         "oracle (restatement of kernel.cl) with every kernel's work-items on %d OpenMP threads in "
+                         "64-item chunks, shared forces added by compare-and-swap as kernel.cl:174-193 does, one "
+                         "join per kernel; %.1f s" % (n, args.cpu_steps, reb(args.cpu_steps), host["omp_threads"], dt),
+               "host": host,
+               "one_core": {"value": v1, "cores": 1,
+                            "sample": "%d steps (%d grid rebuilds) of the sequential oracle, %.1f s"
+                                      % (args.cpu_steps_1core, reb(args.cpu_steps_1core), dt1)}}
 
     out = {
         "metric": "bead-steps/s", "value": value, "unit": "bead-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -645,19 +787,25 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": names[dom], "algorithmic_bytes_per_launch": b_kernel[dom],
-                     "avg_launch_ms": kern_ms[dom], "launches_timed": tim["steps"]},
+                     "traffic_over_algorithmic": traffic / b_kernel[dom] if traffic else None,
+                     "avg_launch_ms": kern_ms[dom], "launches_timed": tim["steps"],
+                     "bytes_formula": "100 N + 16 S_HR + 28 S_FL + 20 A + 8 P per step, + (36 N + 8 P) per rebuild",
+                     "step_achieved": step_gbs, "step_frac": step_gbs / HBM_PEAK_GBS,
+                     "step_algorithmic_bytes": b_step},
         "cpu_baseline": cpu,
         "extra": {"kernels": {names[k]: {"avg_ms": kern_ms[k], "algorithmic_bytes": b_kernel[k],
+                                          "implementation_bytes": impl_kernel[k],
                                           "GBps": b_kernel[k] / (kern_ms[k] * 1e-3) / 1e9 if kern_ms[k] > 0 else None}
                               for k in kern_ms},
                   "step_kernels_ms": tim["step_ms"] / n_t,
                   "untimed_bead_steps_per_s": n * args.steps / wall_graph,
                   "untimed_ms_per_step": wall_graph * 1e3 / args.steps,
                   "rebuild_ms": rebuild_ms, "rebuild_algorithmic_bytes": b_rebuild,
-                  "step_algorithmic_GBps_incl_rebuild": (b_step + b_rebuild / net.grid_update_freq)
-                  * args.steps / wall / 1e9, "counts": counts, "stats": st, "state_after_timed": state,
-                  "generate_s": gen_s,
-                  "gpu_over_cpu": (value / cpu["value"]) if cpu else None, "copy_bytes": copy,
+                  "rebuild_implementation_bytes": impl_rebuild,
+                  "counts": counts, "stats": st, "state_after_timed": state,
+                  "generate_s": gen_s, "relax_s": relax_s,
+                  "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+                  "gpu_over_cpu_1core": (value / cpu["one_core"]["value"]) if cpu else None, "copy_bytes": copy,
                   "ensemble": ens, "adjoint": adj, "drift_vs_f64": drift, "autotune": tune},
     }
     print(json.dumps(out))

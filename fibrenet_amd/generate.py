@@ -18,7 +18,10 @@ index along the fibre):
                   types; beads with x<5 are clamped (mass 0) and beads with
                   x>995 pulled at +0.01 in x (mass 0, velocity set);
                   dt 5e-4 (2,000,000 beads as 20,000 fibres x 100 in
-                  1000x1000x50 by default)
+                  1000x1000x50 by default); fibres are placed no closer
+                  than the Lennard-Jones minimum 2^(1/6) sigma to one
+                  another, so no pair starts on the repulsive wall
+                  (relax() then drains the attraction's energy)
 Positions are written with 17 significant digits, which round-trips.
 """
 import ctypes
@@ -119,6 +122,22 @@ def hs_pair(a, b, k):
     p = np.zeros((), R.PAIR)
     p["type"], p["p1"], p["p2"], p["c1"] = 2, a, b, k
     return p
+
+
+def st_pair(a, b, t4, t5, t6, t7):
+    """A two-step harmonic ("st") entry from its input tokens
+    `pair st A B t4 t5 t6 t7` (SimulationManager.cpp:922-933): c1 = t5,
+    c2 = t6, c3 = t7 - t6, c4 = t7^2, c5 = t4, c6 = 2 t6 - t7; the force
+    c1 (c2 - r) + c5 (c6 - r) acts while r^2 < c4 (kernel.cl:960-999)."""
+    p = np.zeros((), R.PAIR)
+    p["type"], p["p1"], p["p2"] = 3, a, b
+    p["c1"], p["c2"], p["c3"] = t5, t6, t7 - t6
+    p["c4"], p["c5"], p["c6"] = t7 * t7, t4, 2 * t6 - t7
+    return p
+
+
+# The README example's st tokens (README.md:590-595): t4 2000, t5 500, t6 1.00, t7 1.03.
+README_ST = (2000.0, 500.0, 1.00, 1.03)
 
 
 def pair_index(a, b, T):
@@ -224,7 +243,7 @@ def periodic_hs(n_fibres=200, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B0,
 
 def tensile_lj(n_fibres=20000, n_beads=100, box=(1000.0, 1000.0, 50.0), seed=0xF1B1, dt=5e-4, steps=20000,
                spring_k=100.0, e_mod=1000, angle_k=10.0, eps=1.0, sigma=1.0, rc=2.5, clamp_x=5.0, pull_v=0.01,
-               d_min=0.6, grid_update_freq=10, radius=0.5, mass=1.0):
+               d_min=2.0 ** (1.0 / 6.0), grid_update_freq=10, radius=0.5, mass=1.0):
     net = Network()
     s = make_settings(box, 1, 2.5, rc, dt, 0.5, wall_k=100.0, types=3)
     xyz, tries = place_fibres(seed, n_fibres, n_beads, box, finite=True, d_min=d_min)
@@ -276,10 +295,83 @@ def tensile_lj(n_fibres=20000, n_beads=100, box=(1000.0, 1000.0, 50.0), seed=0xF
     return net
 
 
+def st_network(n_fibres=150, n_beads=30, box=(40.0, 40.0, 20.0), seed=0xF1B5, dt=1e-4, steps=1000,
+               tokens=README_ST, cutoff=1.15, unit=1.25, damping=0.03, spring_k=2000.0, angle_k=50.0,
+               grid_update_freq=8, d_min=0.9):
+    """Fibres with the README example's contact law: two-step harmonic
+    ("st") contacts between one bead type with the README's tokens, cutoff
+    1.15 and grid cell 1.25 (README.md:565-606), damping 0.03, dt 1e-4, in a
+    finite box with walls; HR springs and HR angles along straight fibres
+    placed as in periodic_hs (they cross, so contacts form at every
+    distance below the cutoff)."""
+    net = Network()
+    s = make_settings(box, 1, unit, cutoff, dt, 0.5, wall_k=100.0, damping=damping)
+    xyz, tries = place_fibres(seed, n_fibres, n_beads, box, finite=True, d_min=d_min)
+    pos = xyz.reshape(-1, 3)
+    N = len(pos)
+    net.beads = np.zeros(N, R.BEAD)
+    R.set_vec(net.beads, "position", pos)
+    R.set_vec(net.beads, "image_position", pos)
+    net.info = np.zeros(N, R.BEAD_INFO)
+    net.info["tag"] = np.repeat(np.arange(n_fibres), n_beads)
+    net.info["loc_id"] = np.tile(np.arange(n_beads), n_fibres)
+    net.info["radius"] = 0.5
+    net.info["radius_sq"] = 0.25
+    net.info["skin_radius"] = 1.0
+    net.info["skin_radius_sq"] = 1.0
+    mass_coeffs(net.info, s, 1.0)
+    net.springs, net.angles = _chain_bonds(n_fibres, n_beads, 0, spring_k, 1.0, angle_k=angle_k)
+    net.pairs = np.array([st_pair(0, 0, *tokens)], R.PAIR)
+    net.settings = s
+    net.cell_nh = grid_neighbours(1, int(s["cells_x"]), int(s["cells_y"]), int(s["cells_z"]))
+    net.grid_update_freq = grid_update_freq
+    net.run_steps = steps
+    net.type_names = ["C1"]
+    net.pair_lines = ["pair st C1 C1 %r %r %r %r" % tuple(tokens)]
+    net.input_extra = ["box_type finite", "box_wall_rep 100", "box_unit_cell %r" % unit, "cutoff %r" % cutoff,
+                       "damping %r" % damping, "grid_update_freq %d" % grid_update_freq]
+    net.meta = dict(workload="st_network", n_fibres=n_fibres, n_beads=n_beads, box=list(box), seed=seed,
+                    fibre_tries=tries, dt=dt)
+    return net
+
+
+def relax(net, steps=4000, substeps=200, precision="f64", device=0):
+    """The network after SimulationManager::MinimizeSystemEnergy with reset
+    (SimulationManager.cpp:111-143): ResetBeadEnergy, then steps // substeps
+    runs of `substeps` steps, each followed by ResetBeadEnergy, on the HIP
+    engine.  Returns a copy of `net` holding the relaxed beads and bond
+    statuses.  ResetBeadEnergy also zeroes the velocities of massless beads
+    (kernel.cl:468-483), which would stop a pull; as boundary conditions,
+    the massless beads keep the velocities they had before the
+    minimisation (they do not move during it)."""
+    import copy
+    from .api import Engine, bind_standard
+    out = copy.copy(net)
+    eng = Engine(precision, device)
+    try:
+        ids = bind_standard(eng, net)
+        spr, ang = len(net.springs) > 0, len(net.angles) > 0
+        eng.launch("ResetBeadEnergy")
+        for _ in range(steps // substeps):
+            eng.run_steps(substeps, net.grid_update_freq, spr, ang)
+            eng.launch("ResetBeadEnergy")
+        beads = eng.read_buffer(ids["bead"], R.BEAD).copy()
+        out.springs = eng.read_buffer(ids["spring"], R.SPRING).copy() if spr else net.springs
+        out.angles = eng.read_buffer(ids["angle"], R.ANGLE).copy() if ang else net.angles
+    finally:
+        eng.close()
+    fixed = np.asarray(net.info["mass"]) == 0.0
+    for f in ("velocity", "velocity_old"):
+        beads[f][fixed] = net.beads[f][fixed]
+    out.beads = beads
+    out.meta = dict(net.meta, relax_steps=steps, relax_substeps=substeps, relax_precision=precision)
+    return out
+
+
 def tensile_fit(n_fibres=100, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B3, dt=1e-3, steps=2000, jitter=0.1,
                 clamp_x=5.0, pull_v=0.01, hr_fl=False, spring_k=100.0, angle_k=10.0, contact_k=1000.0, e_mod=1000.0,
                 sigma_f=(50.0, 150.0), damping=None, gravity=(0.0, 0.0, 0.0), unit=2.0, grid_update_freq=10,
-                d_min=0.0, contact="hs", periodic=False):
+                d_min=0.0, contact="hs", periodic=False, st_tokens=(200.0, 50.0, 1.0, 1.03)):
     """A differentiable tensile case: bent fibres (straight chains with a
     random transverse jitter, so that angles and springs start away from
     rest and every stiffness shapes the response) in a finite box with
@@ -289,10 +381,11 @@ def tensile_fit(n_fibres=100, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B3,
     x > box - clamp_x.  Fibres are placed as in periodic_hs (d_min = 0:
     overlaps allowed) unless d_min asks for rejection sampling.  contact
     "lj": Lennard-Jones contacts (eps 1, sigma 1, cutoff 2.5) instead of
-    penalty ones; periodic: a periodic box (images unwrapped, positions
+    penalty ones, "st": two-step harmonic contacts of the tokens st_tokens
+    (cutoff 1.15); periodic: a periodic box (images unwrapped, positions
     wrapped; the clamp and pull groups keep their masses and velocities)."""
     net = Network()
-    cut = 2.5 if contact == "lj" else 1.5
+    cut = {"lj": 2.5, "st": 1.15}.get(contact, 1.5)
     s = make_settings(box, 0 if periodic else 1, max(unit, cut), cut, dt, 0.5, wall_k=100.0, damping=damping,
                       gravity=gravity, types=3)
     xyz, tries = place_fibres(seed, n_fibres, n_beads, (box[0], box[1], box[2]), finite=not periodic, d_min=d_min)
@@ -334,7 +427,8 @@ def tensile_fit(n_fibres=100, n_beads=50, box=(100.0, 100.0, 20.0), seed=0xF1B3,
     net.pairs = np.zeros(T * (T + 1) // 2, R.PAIR)
     for a in range(T):
         for b in range(a, T):
-            net.pairs[pair_index(a, b, T)] = (lj_pair(a, b, 1.0, 1.0, 2.5) if contact == "lj"
+            net.pairs[pair_index(a, b, T)] = (lj_pair(a, b, 1.0, 1.0, 2.5) if contact == "lj" else
+                                              st_pair(a, b, *st_tokens) if contact == "st"
                                               else hs_pair(a, b, contact_k))
     net.settings = s
     net.cell_nh = grid_neighbours(0 if periodic else 1, int(s["cells_x"]), int(s["cells_y"]), int(s["cells_z"]))
