@@ -66,6 +66,17 @@ SIGNATURES = [
     ("fn_sim_buffer_id", _I, [_VP, _I]),
     ("fn_sim_run_loop", _I, [_VP, _I, _I]),
     ("fn_sim_minimize", _I, [_VP, _I]),
+    ("fn_sim_run", _I, [_VP, _I]),
+    ("fn_sim_dump", _I, [_VP, _I]),
+    ("fn_sim_write_restart", _I, [_VP, _CP, _I]),
+    ("fn_sim_set_log_callback", _I, [_VP, _VP, _VP]),
+    ("fn_sim_error", _CP, [_VP]),
+    ("fn_sim_output_file", _CP, [_VP]),
+    ("fn_model_write_restart", _I, [_VP, _VP, _I, _CP]),
+    ("fn_model_write_xyz", _I, [_VP, _VP, _I, _I, _CP]),
+    ("fn_checkpoint_save", _I, [_VP, _CP]),
+    ("fn_checkpoint_load", _I, [_VP, _CP]),
+    ("fn_group_force", _I, [_VP, _VP, _VP]),
     ("fn_gen_fibres", _I, [_c.c_ulonglong, _I, _I, _c.c_double, _c.c_double, _c.c_double, _I, _c.c_double,
                            _c.c_double, _LL, _VP, _c.POINTER(_LL)]),
 ]
@@ -136,6 +147,9 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+XYZ_FIELDS = {"position": 0, "velocity": 1, "force": 2}
+
+
 class Model:
     """Host records parsed from a FibreNet input file (fn_model_*)."""
 
@@ -145,6 +159,7 @@ class Model:
         rc = L.fn_model_load(os.fsencode(input_path), ctypes.byref(h))
         if rc != FN_OK:
             raise FibreNetError(rc, L.fn_model_error().decode())
+        self._h = h
         try:
             n = [ctypes.c_int() for _ in range(5)]
             L.fn_model_counts(h, *[ctypes.byref(x) for x in n])
@@ -162,8 +177,38 @@ class Model:
             self.angles = self._copy(h, 4, R.ANGLE, self.n_angles)
             self.pairs = self._copy(h, 5, R.PAIR, self.n_pairs)
             self.cell_nh = self._copy(h, 8, R.GRID_CELL_NH, self.n_cells)["n"]
-        finally:
-            L.fn_model_free(h)
+        except Exception:
+            self.close()
+            raise
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().fn_model_free(self._h)
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def write_restart(self, beads, progress, path):
+        """DumpRestartData's text of `beads` (this model's bead count of
+        records) into `path` (fn_model_write_restart)."""
+        b = np.ascontiguousarray(beads, R.BEAD)
+        assert len(b) == self.n_beads
+        rc = lib().fn_model_write_restart(self._h, _ptr(b), int(progress), os.fsencode(path))
+        if rc != FN_OK:
+            raise FibreNetError(rc, "fn_model_write_restart failed")
+
+    def write_xyz(self, beads, progress, field, path):
+        """One DumpData frame of `field` ("position", "velocity", "force")
+        appended to `path` (fn_model_write_xyz)."""
+        b = np.ascontiguousarray(beads, R.BEAD)
+        assert len(b) == self.n_beads
+        rc = lib().fn_model_write_xyz(self._h, _ptr(b), int(progress), XYZ_FIELDS[field], os.fsencode(path))
+        if rc != FN_OK:
+            raise FibreNetError(rc, "fn_model_write_xyz failed")
 
     @staticmethod
     def _copy(h, kind, dtype, n):
@@ -366,6 +411,22 @@ class Engine:
     def set_timing(self, on):
         self._chk(lib().fn_set_timing(self.h, int(bool(on))))
 
+    def checkpoint_save(self, path):
+        """Bit-exact binary checkpoint of the dynamic state (fn_checkpoint_save)."""
+        self._chk(lib().fn_checkpoint_save(self.h, os.fsencode(path)))
+
+    def checkpoint_load(self, path):
+        """Restores a checkpoint of the same network (fn_checkpoint_load)."""
+        self._chk(lib().fn_checkpoint_load(self.h, os.fsencode(path)))
+
+    def group_force(self, group):
+        """x-force of the current step summed over the flagged beads, per
+        network (fn_group_force)."""
+        g = np.ascontiguousarray(group, np.int32)
+        out = np.zeros(self.n_networks, np.float64)
+        self._chk(lib().fn_group_force(self.h, _ptr(g), _ptr(out)))
+        return out if self.batched else float(out[0])
+
     def timing(self):
         t = FnTiming()
         self._chk(lib().fn_get_timing(self.h, ctypes.byref(t)))
@@ -477,14 +538,40 @@ class Simulation:
         return lib().fn_sim_buffer_id(self.h, R.KIND[kind])
 
     def run_loop(self, n, fast=True):
-        rc = lib().fn_sim_run_loop(self.h, int(n), int(bool(fast)))
-        if rc != FN_OK:
-            raise FibreNetError(rc, lib().fn_last_error(self.engine.h).decode())
+        self._chk(lib().fn_sim_run_loop(self.h, int(n), int(bool(fast))))
 
     def minimize(self, fast=True):
-        rc = lib().fn_sim_minimize(self.h, int(bool(fast)))
+        self._chk(lib().fn_sim_minimize(self.h, int(bool(fast))))
+
+    def _chk(self, rc):
         if rc != FN_OK:
-            raise FibreNetError(rc, lib().fn_last_error(self.engine.h).decode())
+            raise FibreNetError(rc, lib().fn_sim_error(self.h).decode())
+
+    def run(self, fast=True):
+        """SimulationManager::Run: minimisation, output_freq chunks with
+        their XYZ frames and restart stages, the final restart frame."""
+        self._chk(lib().fn_sim_run(self.h, int(bool(fast))))
+
+    def dump(self, progress):
+        self._chk(lib().fn_sim_dump(self.h, int(progress)))
+
+    def write_restart(self, progress, path=None):
+        self._chk(lib().fn_sim_write_restart(self.h, os.fsencode(path) if path else None, int(progress)))
+
+    @property
+    def output_file(self):
+        return lib().fn_sim_output_file(self.h).decode()
+
+    def set_log(self, fn):
+        """Routes the simulation's (and its engine's) messages to fn(str);
+        None restores stderr."""
+        if fn is None:
+            self._log_cb = None
+            self._chk(lib().fn_sim_set_log_callback(self.h, None, None))
+            return
+        cb = ctypes.CFUNCTYPE(None, ctypes.c_char_p, ctypes.c_void_p)(lambda m, u: fn(m.decode()))
+        self._log_cb = cb                      # keeps the trampoline alive
+        self._chk(lib().fn_sim_set_log_callback(self.h, ctypes.cast(cb, ctypes.c_void_p), None))
 
     def beads(self):
         return self.engine.read_buffer(self.buffer_id("bead"), R.BEAD)

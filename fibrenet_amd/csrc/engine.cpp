@@ -224,6 +224,9 @@ struct fn_engine {
                                  double *samples) = 0;
     virtual void adjoint_backward(const double *dloss, double *g_spring, double *g_angle, double *g_pair,
                                   double *x0b, double *w0b) = 0;
+    virtual void checkpoint_save(const char *path) = 0;
+    virtual void checkpoint_load(const char *path) = 0;
+    virtual void group_force(const int *group, double *out) = 0;
 };
 
 namespace {
@@ -259,6 +262,8 @@ class Engine final : public fn_engine {
         if (rs < 0) throw Error(FN_ERR_ARG, fmt("unknown buffer kind %d", kind));
         if (esize != rs) throw Error(FN_ERR_MISMATCH, fmt("record size %d, kind %d expects %d", esize, kind, rs));
         if (n <= 0) throw Error(FN_ERR_EMPTY, "buffer is empty");
+        if (kind == FN_BUF_BEAD && (long long)n >= (1LL << fnk::kPidShift))
+            throw Error(FN_ERR_UNSUPPORTED, fmt("%d beads: the engine holds fewer than 2^%d", n, fnk::kPidShift));
         const bool role_kind = kind == FN_BUF_BEAD || kind == FN_BUF_BEAD_INFO || kind == FN_BUF_SPRING ||
                                kind == FN_BUF_ANGLE || kind == FN_BUF_PAIR || kind == FN_BUF_GRID_CELL_NH;
         if (role_kind && role_[kind] >= 0)
@@ -1985,6 +1990,111 @@ class Engine final : public fn_engine {
         }
     }
 
+    // ---- checkpoints ----------------------------------------------------------------------
+    // File layout: the header, then the bead records (fp64 AoS, which holds
+    // the fp32 build's values exactly), the spring statuses and break steps,
+    // and the angle statuses.
+    struct CkHeader {
+        char magic[8];
+        int precision, n_beads, n_springs, n_angles;
+        int step_counter, pad;
+        long long steps, rebuilds;
+    };
+    static constexpr char kCkMagic[8] = {'F', 'N', 'C', 'K', 'P', 'T', '0', '1'};
+
+    void checkpoint_save(const char *path) override
+    {
+        if (!path) throw Error(FN_ERR_ARG, "checkpoint path is null");
+        if (N_ <= 0) throw Error(FN_ERR_ARG, "no bead buffer");
+        HIPCK(hipStreamSynchronize(stream_));
+        CkHeader h{};
+        memcpy(h.magic, kCkMagic, sizeof h.magic);
+        h.precision = precision;
+        h.n_beads = N_;
+        h.n_springs = S_;
+        h.n_angles = A_;
+        h.steps = steps_;
+        h.rebuilds = rebuilds_;
+        std::vector<fn_bead> b(N_);
+        download_beads(b.data());
+        std::vector<int> sst, sbr, ast, stp;
+        sp_status_.get(sst, S_, stream_);
+        sp_break_.get(sbr, S_, stream_);
+        an_status_.get(ast, A_, stream_);
+        step_.get(stp, 1, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+        h.step_counter = stp[0];
+        FILE *f = fopen(path, "wb");
+        if (!f) throw Error(FN_ERR_ARG, fmt("unable to open %s for writing", path));
+        bool ok = fwrite(&h, sizeof h, 1, f) == 1 && fwrite(b.data(), sizeof(fn_bead), N_, f) == (size_t)N_;
+        if (S_) ok = ok && fwrite(sst.data(), sizeof(int), S_, f) == (size_t)S_ &&
+                     fwrite(sbr.data(), sizeof(int), S_, f) == (size_t)S_;
+        if (A_) ok = ok && fwrite(ast.data(), sizeof(int), A_, f) == (size_t)A_;
+        ok = (fclose(f) == 0) && ok;
+        if (!ok) throw Error(FN_ERR_ARG, fmt("unable to write the checkpoint %s", path));
+    }
+
+    void checkpoint_load(const char *path) override
+    {
+        if (!path) throw Error(FN_ERR_ARG, "checkpoint path is null");
+        FILE *f = fopen(path, "rb");
+        if (!f) throw Error(FN_ERR_ARG, fmt("unable to open %s", path));
+        std::unique_ptr<FILE, int (*)(FILE *)> guard(f, fclose);
+        CkHeader h{};
+        if (fread(&h, sizeof h, 1, f) != 1 || memcmp(h.magic, kCkMagic, sizeof h.magic) != 0)
+            throw Error(FN_ERR_MISMATCH, fmt("%s is not a checkpoint of this library", path));
+        if (h.precision != precision || h.n_beads != N_ || h.n_springs != S_ || h.n_angles != A_)
+            throw Error(FN_ERR_MISMATCH, fmt("checkpoint of precision %d with %d beads, %d springs, %d angles; the "
+                                             "engine has precision %d with %d, %d, %d", h.precision, h.n_beads,
+                                             h.n_springs, h.n_angles, precision, N_, S_, A_));
+        std::vector<fn_bead> b(N_);
+        std::vector<int> sst(S_), sbr(S_), ast(A_);
+        bool ok = fread(b.data(), sizeof(fn_bead), N_, f) == (size_t)N_;
+        if (S_) ok = ok && fread(sst.data(), sizeof(int), S_, f) == (size_t)S_ &&
+                     fread(sbr.data(), sizeof(int), S_, f) == (size_t)S_;
+        if (A_) ok = ok && fread(ast.data(), sizeof(int), A_, f) == (size_t)A_;
+        if (!ok) throw Error(FN_ERR_MISMATCH, fmt("%s is truncated", path));
+        adj_.valid = false;
+        HIPCK(hipStreamSynchronize(stream_));
+        upload_beads(b.data());
+        if (S_) {
+            sp_status_.put(sst, stream_);
+            sp_break_.put(sbr, stream_);
+        }
+        if (A_) an_status_.put(ast, stream_);
+        std::vector<int> stp(1, h.step_counter);
+        step_.put(stp, stream_);
+        HIPCK(hipStreamSynchronize(stream_));
+        steps_ = h.steps;
+        rebuilds_ = h.rebuilds;
+        grid_valid_ = false;
+    }
+
+    // ---- observables ----------------------------------------------------------------------
+    void group_force(const int *group, double *out) override
+    {
+        if (!group || !out) throw Error(FN_ERR_ARG, "group and output are required");
+        if (N_ <= 0) throw Error(FN_ERR_ARG, "no bead buffer");
+        const int nn = n_networks();
+        std::vector<int> gi, goff(nn + 1, 0);
+        for (int i = 0; i < N_; ++i)
+            if (group[i] != 0) gi.push_back(i);
+        int max_len = 0;
+        for (int k = 0; k < nn; ++k) {
+            const int b1 = ens_boxes_.empty() ? N_ : ens_b0_[k + 1];
+            goff[k + 1] = (int)(std::lower_bound(gi.begin(), gi.end(), b1) - gi.begin());
+            max_len = std::max(max_len, goff[k + 1] - goff[k]);
+        }
+        rf_idx_.put(gi, stream_);
+        rf_off_.put(goff, stream_);
+        rf_part_.ensure(fnk::kSumBlocks * (size_t)nn);
+        rf_out_.ensure(nn);
+        fnk::group_force(frc_[fc_][0].p, rf_idx_.p, rf_off_.p, nn, max_len, rf_part_.p, rf_out_.p, stream_);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(out, rf_out_.p, sizeof(double) * nn, hipMemcpyDeviceToHost, stream_));
+        HIPCK(hipStreamSynchronize(stream_));
+    }
+
     // The adjoint uses the one-pass exact rebuild, whose lists depend on the
     // positions alone, so a recomputed segment lists what the forward listed.
     struct ExactRebuild {
@@ -2404,6 +2514,8 @@ class Engine final : public fn_engine {
     DArr<double> gsp_, gan_, gpr_, gps_;   // scale adjoints (contacts per bead and pair entry, then summed)
     DArr<P> rec_pos_;                      // per step of a segment: positions and images of the force evaluation
     DArr<int> rec_st_, rec_nb_;            // per step: bond statuses; per rebuild: the contact list
+    DArr<int> rf_idx_, rf_off_;            // fn_group_force: flagged beads, their entries per network
+    DArr<double> rf_part_, rf_out_;        // its partial sums and result
     DArr<fnk::PairC> pairs_;
     DArr<int> key_, ckey_, rank_, cell_count_, cell_start_, ncnt_, nb_start_, nb_cnt_, cand_, nb_list_, scan_tmp_,
         stats_, step_, need_;
@@ -2504,6 +2616,10 @@ int fn_register(fn_engine *eng, const char *const *names, int n)
 int fn_buffer_create(fn_engine *eng, int kind, int elem_size, int n_elems, const void *host, int *id)
 {
     return guarded(eng, [&] {
+        // the pair table of one network (an ensemble checks each network's own)
+        if (kind == FN_BUF_PAIR && n_elems > (1 << (31 - fnk::kPidShift)))
+            throw Error(FN_ERR_UNSUPPORTED, fmt("a pair table of %d entries: the engine holds at most %d "
+                                                "(at most 7 bead types)", n_elems, 1 << (31 - fnk::kPidShift)));
         const int r = eng->create_buffer(kind, elem_size, n_elems, host);
         if (id) *id = r;
     });
@@ -2655,6 +2771,21 @@ int fn_adjoint_backward(fn_engine *eng, const double *dloss, double *d_spring, d
                         double *d_pos0, double *d_vel0)
 {
     return guarded(eng, [&] { eng->adjoint_backward(dloss, d_spring, d_angle, d_pair, d_pos0, d_vel0); });
+}
+
+int fn_checkpoint_save(fn_engine *eng, const char *path)
+{
+    return guarded(eng, [&] { eng->checkpoint_save(path); });
+}
+
+int fn_checkpoint_load(fn_engine *eng, const char *path)
+{
+    return guarded(eng, [&] { eng->checkpoint_load(path); });
+}
+
+int fn_group_force(fn_engine *eng, const int *group, double *out)
+{
+    return guarded(eng, [&] { eng->group_force(group, out); });
 }
 
 int fn_get_timing(fn_engine *eng, fn_timing *out)

@@ -1,4 +1,4 @@
-// loader.cpp - FibreNet input/model file loaders and the simulation host.
+// loader.cpp - FibreNet input/model file loaders.
 //
 // The loaders keep the reference's file formats and acceptance rules
 // (SimulationManager.cpp, cited per function): keywords are found by the first
@@ -11,9 +11,9 @@
 // the first upload; the reference reads the device buffers back, edits and
 // writes them, which is the same result for an fp64 device.
 //
-// fn_sim is the SimulationManager counterpart: it uploads through the
-// fibrenet::HipManager facade and binds the kernel arguments exactly as
-// SimulationManager::ReadKernelData does.
+// Paths named in the input file (model, restart, output) are resolved against
+// the input file's directory when relative; the reference opens them relative
+// to its working directory, which is the same when it is started there.
 #include <algorithm>
 #include <cerrno>
 #include <cmath>
@@ -29,7 +29,7 @@
 #include <vector>
 
 #include "fibrenet.h"
-#include "fibrenet_manager.hpp"
+#include "model.h"
 
 namespace {
 
@@ -88,20 +88,6 @@ float stof_(const std::string &t)
 
 }  // namespace
 
-struct fn_model {
-    fn_settings s{};
-    std::map<std::string, int> type_id;
-    std::vector<fn_pair> pairs;
-    std::vector<fn_bead> beads;
-    std::vector<fn_bead_info> info;
-    std::vector<fn_spring> springs;
-    std::vector<fn_angle> angles;
-    std::vector<fn_grid_cell_nh> cellnh;
-    int grid_update_freq = 8;
-    int run = -1, dump = -1, min_steps = 0, min_sub = 0;
-    bool min_reset = false, restart = false;
-    double skin_radius_ratio = 0.5;
-};
 
 namespace {
 
@@ -301,13 +287,13 @@ void load_types_pairs(fn_model &m, const Lines &L)
         if (p.p1 == -1 && p.p2 == -1) throw LoadError("pair list contains undefined interactions");
 }
 
-// SimulationManager.cpp:979-1091 LoadExportInfo (validation only; the XYZ
-// writer is not part of this library).
+// SimulationManager.cpp:979-1091 LoadExportInfo.
 void load_export(fn_model &m, const Lines &L)
 {
     Tokens t;
     if (scan(L, "output_file", t, 0) <= 0) throw LoadError("output file name not specified");
     if (t.size() != 2) throw LoadError("output_file: incorrect number of command parameters");
+    m.output_file = t[1];
     if (scan(L, "output_freq", t, 0) <= 0) throw LoadError("output_freq not specified");
     if (t.size() != 2) throw LoadError("output_freq: incorrect number of command parameters");
     m.dump = stoi_(t[1]);
@@ -326,15 +312,17 @@ void load_export(fn_model &m, const Lines &L)
         }
     }
     if (types.empty()) throw LoadError("list of exported bead types is empty");
-    bool any = false;
+    m.export_types = types;
     ln = 0;
     while ((ln = scan(L, "output_data", t, ln)) != -1) {
         if (t.size() != 2) throw LoadError("output_data: incorrect number of command parameters");
-        if (t[1] != "position" && t[1] != "velocity" && t[1] != "force" && t[1] != "all")
-            throw LoadError("output_data: unknown data type");
-        any = true;
+        if (t[1] == "position") m.export_position = true;
+        else if (t[1] == "velocity") m.export_velocity = true;
+        else if (t[1] == "force") m.export_force = true;
+        else if (t[1] == "all") m.export_position = m.export_velocity = m.export_force = true;
+        else throw LoadError("output_data: unknown data type");
     }
-    if (!any) throw LoadError("export data type not specified");
+    if (!m.export_position && !m.export_velocity && !m.export_force) throw LoadError("export data type not specified");
 }
 
 void mass_coeffs(fn_bead_info &b, const fn_settings &s, double mass)
@@ -592,7 +580,7 @@ void spring_mods(fn_model &m, const Lines &L)
 
 // SimulationManager::ReadInputFile (SimulationManager.cpp:1153-1171) and
 // LoadModelData (SimulationManager.cpp:1315-1364).
-void load_all(fn_model &m, const std::string &path)
+void load_all_(fn_model &m, const std::string &path)
 {
     Lines L;
     if (!read_lines(path, L)) throw LoadError("unable to open input file " + path);
@@ -615,169 +603,13 @@ void load_all(fn_model &m, const std::string &path)
     read_restart(m, L, base);
     bead_mods(m, L);
     spring_mods(m, L);
+    m.output_file = resolve(base, m.output_file);
 }
 
 }  // namespace
 
-// ---- simulation host ------------------------------------------------------------------------------
-struct fn_sim {
-    fn_model m;
-    fibrenet::HipManager mgr;
-    std::string err;
-    int id_bead = -1, id_info = -1, id_spring = -1, id_angle = -1, id_pair = -1;
-    int id_head = -1, id_tail = -1, id_cellnh = -1, id_nh_head = -1, id_nh_tail = -1, id_coeff = -1;
-    int beads_num = 0, springs_num = 0, angles_num = 0;
-};
-
-namespace {
-
-// SimulationManager::ReadKernelData (SimulationManager.cpp:1180-1307).
-bool read_kernel_data(fn_sim &S)
-{
-    auto &M = S.mgr;
-    std::vector<std::string> names = {"AdvancePosition", "ResetGrid", "ResetContactList", "FillGrid",
-                                      "BuildContactList", "ResolveContacts", "AdvanceVelocity",
-                                      "ResetBeadEnergy", "ResolveSprings", "ResolveAngles"};
-    if (!M.RegisterKernel(names)) return false;
-    auto bead = M.GetBuffer(S.id_bead), angle = M.GetBuffer(S.id_angle), spring = M.GetBuffer(S.id_spring);
-    auto info = M.GetBuffer(S.id_info), head = M.GetBuffer(S.id_head), tail = M.GetBuffer(S.id_tail);
-    auto cellnh = M.GetBuffer(S.id_cellnh), nh_head = M.GetBuffer(S.id_nh_head), nh_tail = M.GetBuffer(S.id_nh_tail);
-    auto coeff = M.GetBuffer(S.id_coeff), table = M.GetBuffer(S.id_pair);
-    if (!bead || !head || !tail || !cellnh || !nh_head || !nh_tail || !coeff || !table) return false;
-    const fn_settings &settings = S.m.s;
-    int beads_num = M.GetBufferElements(S.id_bead);
-    int grid_size = M.GetBufferElements(S.id_head);
-    int list_size = M.GetBufferElements(S.id_nh_head);
-    bool ok = true;
-    ok &= M.SetKernelArg("AdvancePosition", 0, *bead);
-    ok &= M.SetKernelArg("AdvancePosition", 1, *info);
-    ok &= M.SetKernelArg("AdvancePosition", 2, settings);
-    ok &= M.SetKernelArg("AdvancePosition", 3, beads_num);
-    M.SetKernelGlobalWorkRange("AdvancePosition", M.GetBufferLength(S.id_bead));
-    if (spring) {
-        int springs_num = M.GetBufferElements(S.id_spring);
-        ok &= M.SetKernelArg("ResolveSprings", 0, *bead);
-        ok &= M.SetKernelArg("ResolveSprings", 1, *spring);
-        ok &= M.SetKernelArg("ResolveSprings", 2, springs_num);
-        M.SetKernelGlobalWorkRange("ResolveSprings", M.GetBufferLength(S.id_spring));
-    }
-    if (angle) {
-        ok &= M.SetKernelArg("ResolveAngles", 0, *angle);
-        ok &= M.SetKernelArg("ResolveAngles", 1, *spring);
-        ok &= M.SetKernelArg("ResolveAngles", 2, *bead);
-        ok &= M.SetKernelArg("ResolveAngles", 3, S.angles_num);
-        M.SetKernelGlobalWorkRange("ResolveAngles", M.GetBufferLength(S.id_angle));
-    }
-    ok &= M.SetKernelArg("ResetGrid", 0, *head);
-    ok &= M.SetKernelArg("ResetGrid", 1, grid_size);
-    M.SetKernelGlobalWorkRange("ResetGrid", M.GetBufferLength(S.id_head));
-    ok &= M.SetKernelArg("ResetContactList", 0, *nh_head);
-    ok &= M.SetKernelArg("ResetContactList", 1, list_size);
-    M.SetKernelGlobalWorkRange("ResetContactList", M.GetBufferLength(S.id_nh_head));
-    ok &= M.SetKernelArg("FillGrid", 0, *bead);
-    ok &= M.SetKernelArg("FillGrid", 1, *head);
-    ok &= M.SetKernelArg("FillGrid", 2, *tail);
-    ok &= M.SetKernelArg("FillGrid", 3, settings);
-    ok &= M.SetKernelArg("FillGrid", 4, beads_num);
-    M.SetKernelGlobalWorkRange("FillGrid", M.GetBufferLength(S.id_bead));
-    ok &= M.SetKernelArg("BuildContactList", 0, *head);
-    ok &= M.SetKernelArg("BuildContactList", 1, *tail);
-    ok &= M.SetKernelArg("BuildContactList", 2, *cellnh);
-    ok &= M.SetKernelArg("BuildContactList", 3, *nh_head);
-    ok &= M.SetKernelArg("BuildContactList", 4, *nh_tail);
-    ok &= M.SetKernelArg("BuildContactList", 5, *bead);
-    ok &= M.SetKernelArg("BuildContactList", 6, *info);
-    ok &= M.SetKernelArg("BuildContactList", 7, *coeff);
-    ok &= M.SetKernelArg("BuildContactList", 8, *table);
-    ok &= M.SetKernelArg("BuildContactList", 9, settings);
-    ok &= M.SetKernelArg("BuildContactList", 10, grid_size);
-    M.SetKernelGlobalWorkRange("BuildContactList", M.GetBufferLength(S.id_head));
-    ok &= M.SetKernelArg("ResolveContacts", 0, *nh_head);
-    ok &= M.SetKernelArg("ResolveContacts", 1, *nh_tail);
-    ok &= M.SetKernelArg("ResolveContacts", 2, *bead);
-    ok &= M.SetKernelArg("ResolveContacts", 3, *info);
-    ok &= M.SetKernelArg("ResolveContacts", 4, *coeff);
-    ok &= M.SetKernelArg("ResolveContacts", 5, *table);
-    ok &= M.SetKernelArg("ResolveContacts", 6, settings);
-    ok &= M.SetKernelArg("ResolveContacts", 7, list_size);
-    M.SetKernelGlobalWorkRange("ResolveContacts", M.GetBufferLength(S.id_nh_head));
-    ok &= M.SetKernelArg("AdvanceVelocity", 0, *bead);
-    ok &= M.SetKernelArg("AdvanceVelocity", 1, *info);
-    ok &= M.SetKernelArg("AdvanceVelocity", 2, beads_num);
-    int padded = M.GetBufferLength(S.id_bead);
-    ok &= M.SetKernelArg("AdvanceVelocity", 3, padded);
-    M.SetKernelGlobalWorkRange("AdvanceVelocity", M.GetBufferLength(S.id_bead));
-    ok &= M.SetKernelArg("ResetBeadEnergy", 0, *bead);
-    ok &= M.SetKernelArg("ResetBeadEnergy", 1, beads_num);
-    M.SetKernelGlobalWorkRange("ResetBeadEnergy", M.GetBufferLength(S.id_bead));
-    return ok;
-}
-
-// Buffer creation in the order and with the sizes of the reference's loaders.
-bool upload(fn_sim &S)
-{
-    auto &M = S.mgr;
-    fn_model &m = S.m;
-    const int C = m.s.cells_x * m.s.cells_y * m.s.cells_z;
-    S.id_head = M.CreateBuffer<int>(C, CL_MEM_READ_WRITE);                       // :2304
-    S.id_tail = M.CreateBuffer<fn_grid_cell>(C, CL_MEM_READ_WRITE);              // :2312
-    std::vector<fn_grid_cell_nh> nh = m.cellnh;
-    S.id_cellnh = M.CreateBuffer<fn_grid_cell_nh>(nh, CL_MEM_READ_ONLY);         // :2319
-    std::vector<fn_pair> pairs = m.pairs;
-    S.id_pair = M.CreateBuffer<fn_pair>(pairs, CL_MEM_READ_ONLY);                // :963
-    std::vector<fn_bead> beads = m.beads;
-    std::vector<fn_bead_info> info = m.info;
-    S.beads_num = (int)beads.size();
-    S.id_bead = M.CreateBuffer<fn_bead>(beads, CL_MEM_READ_WRITE);               // :1461
-    S.id_info = M.CreateBuffer<fn_bead_info>(info, CL_MEM_READ_ONLY);            // :1467
-    S.id_nh_head = M.CreateBuffer<int>(S.beads_num, CL_MEM_READ_WRITE);          // :1473
-    S.id_nh_tail = M.CreateBuffer<fn_particle_nh>(S.beads_num, CL_MEM_READ_WRITE);  // :1478
-    S.id_coeff = M.CreateBuffer<fn_particle_nh>(S.beads_num, CL_MEM_READ_WRITE);    // :1483
-    if (!m.springs.empty()) {
-        std::vector<fn_spring> sp = m.springs;
-        S.springs_num = (int)sp.size();
-        S.id_spring = M.CreateBuffer<fn_spring>(sp, CL_MEM_READ_WRITE);          // :1572
-    }
-    if (!m.angles.empty()) {
-        std::vector<fn_angle> an = m.angles;
-        S.angles_num = (int)an.size();
-        S.id_angle = M.CreateBuffer<fn_angle>(an, CL_MEM_READ_ONLY);             // :1650
-    }
-    return S.id_head >= 0 && S.id_tail >= 0 && S.id_cellnh >= 0 && S.id_pair >= 0 && S.id_bead >= 0 &&
-           S.id_info >= 0 && S.id_nh_head >= 0 && S.id_nh_tail >= 0 && S.id_coeff >= 0 &&
-           (m.springs.empty() || S.id_spring >= 0) && (m.angles.empty() || S.id_angle >= 0);
-}
-
-// SimulationManager::RunSimulationLoop (SimulationManager.cpp:151-181).
-bool run_loop(fn_sim &S, int loop_size, bool fast)
-{
-    if (loop_size <= 0) {
-        S.err = "RunSimulationLoop: loop size should be a positive number";
-        return false;
-    }
-    auto &M = S.mgr;
-    if (fast) return M.RunSteps(loop_size, S.m.grid_update_freq, S.springs_num > 0, S.angles_num > 0);
-    int cnt = 0;
-    for (int step = 0; step < loop_size; ++step) {
-        if (cnt == S.m.grid_update_freq) cnt = 0;
-        if (!M.ExecuteKernel("AdvancePosition")) return false;
-        if (S.springs_num > 0 && !M.ExecuteKernel("ResolveSprings")) return false;
-        if (S.angles_num > 0 && !M.ExecuteKernel("ResolveAngles")) return false;
-        if (cnt == 0) {
-            if (!M.ExecuteKernel("ResetGrid")) return false;
-            if (!M.ExecuteKernel("ResetContactList")) return false;
-            if (!M.ExecuteKernel("FillGrid")) return false;
-            if (!M.ExecuteKernel("BuildContactList")) return false;
-        }
-        if (!M.ExecuteKernel("ResolveContacts")) return false;
-        if (!M.ExecuteKernel("AdvanceVelocity")) return false;
-        if (!M.RunKernels()) return false;
-        ++cnt;
-    }
-    return true;
-}
-
-}  // namespace
+void fnload::load_all(fn_model &m, const std::string &path) { load_all_(m, path); }
+void fnload::set_error(const std::string &message) { g_model_error = message; }
 
 extern "C" {
 
@@ -787,7 +619,7 @@ int fn_model_load(const char *input_path, fn_model **out)
     *out = nullptr;
     fn_model *m = new fn_model();
     try {
-        load_all(*m, input_path);
+        load_all_(*m, input_path);
     } catch (const std::exception &e) {
         g_model_error = e.what();
         delete m;
@@ -844,77 +676,6 @@ int fn_model_copy(const fn_model *m, int kind, void *dst)
     default: return FN_ERR_ARG;
     }
     return FN_OK;
-}
-
-int fn_sim_create(int device, int precision, const char *input_path, fn_sim **out)
-{
-    if (!input_path || !out) return FN_ERR_ARG;
-    *out = nullptr;
-    fn_sim *S = new fn_sim();
-    try {
-        load_all(S->m, input_path);
-    } catch (const std::exception &e) {
-        g_model_error = e.what();
-        delete S;
-        return FN_ERR_LOAD;
-    }
-    std::string log;
-    if (!S->mgr.Initialize([&log](const std::string &s) { log += s + "\n"; }, precision, device)) {
-        g_model_error = "unable to open the HIP device";
-        delete S;
-        return FN_ERR_HIP;
-    }
-    if (!upload(*S) || !read_kernel_data(*S)) {
-        g_model_error = "upload/bind failed: " + log;
-        delete S;
-        return FN_ERR_ARG;
-    }
-    *out = S;
-    return FN_OK;
-}
-
-int fn_sim_destroy(fn_sim *sim)
-{
-    delete sim;
-    return FN_OK;
-}
-
-fn_engine *fn_sim_engine(fn_sim *sim) { return sim ? sim->mgr.engine() : nullptr; }
-
-int fn_sim_buffer_id(fn_sim *sim, int kind)
-{
-    if (!sim) return FN_ERR_ARG;
-    switch (kind) {
-    case FN_BUF_BEAD: return sim->id_bead;
-    case FN_BUF_BEAD_INFO: return sim->id_info;
-    case FN_BUF_SPRING: return sim->id_spring;
-    case FN_BUF_ANGLE: return sim->id_angle;
-    case FN_BUF_PAIR: return sim->id_pair;
-    default: return FN_ERR_ARG;
-    }
-}
-
-int fn_sim_run_loop(fn_sim *sim, int loop_size, int fast)
-{
-    if (!sim) return FN_ERR_ARG;
-    return run_loop(*sim, loop_size, fast != 0) ? FN_OK : FN_ERR_ARG;
-}
-
-// SimulationManager::MinimizeSystemEnergy (SimulationManager.cpp:111-143).
-int fn_sim_minimize(fn_sim *sim, int fast)
-{
-    if (!sim) return FN_ERR_ARG;
-    fn_model &m = sim->m;
-    if (m.min_steps < 0) return FN_ERR_ARG;
-    if (m.min_reset && !sim->mgr.ExecuteKernel("ResetBeadEnergy")) return FN_ERR_ARG;
-    if (m.min_steps == 0) return FN_OK;
-    if (m.min_sub <= 0) return FN_ERR_ARG;   // the reference divides by zero here
-    const int loops = m.min_steps / m.min_sub;
-    for (int p = 0; p < loops; ++p) {
-        if (!run_loop(*sim, m.min_sub, fast != 0)) return FN_ERR_ARG;
-        if (m.min_reset && !sim->mgr.ExecuteKernel("ResetBeadEnergy")) return FN_ERR_ARG;
-    }
-    return sim->mgr.RunKernels() ? FN_OK : FN_ERR_ARG;
 }
 
 }  // extern "C"

@@ -311,6 +311,67 @@ int fn_sim_buffer_id(fn_sim *sim, int kind);
 int fn_sim_run_loop(fn_sim *sim, int loop_size, int fast);
 /* SimulationManager::MinimizeSystemEnergy (SimulationManager.cpp:111-143). */
 int fn_sim_minimize(fn_sim *sim, int fast);
+/* SimulationManager::Run (SimulationManager.cpp:54-103): MinimizeSystemEnergy
+ * when the input asks for minimize_steps > 0, a DumpData frame at step 0,
+ * then RunSimulationLoop chunks of output_freq steps (the last one shorter),
+ * each followed by DumpData, and the final DumpRestartData.  Messages go to
+ * the fn_sim_set_log_callback sink (stderr without one). */
+int fn_sim_run(fn_sim *sim, int fast);
+/* SimulationManager::DumpData (SimulationManager.cpp:1937-2061): appends one
+ * frame of each output_data field of the exported bead types to
+ * <output_file>_coord.xyz / _vel.xyz / _force.xyz (the files are opened for
+ * appending at the first call, as the reference does), and writes
+ * <output_file>_restart.res when progress / run passes the next of 0.25,
+ * 0.5, 0.75, ... (one stage per call, float arithmetic). */
+int fn_sim_dump(fn_sim *sim, int progress);
+/* SimulationManager::DumpRestartData (SimulationManager.cpp:1855-1928) into
+ * `path` (NULL: <output_file>_restart.res). */
+int fn_sim_write_restart(fn_sim *sim, const char *path, int progress);
+/* Log sink of the simulation host (replacing log_stream, log_stream.h:10-67);
+ * it also receives the engine's messages.  NULL restores stderr. */
+int fn_sim_set_log_callback(fn_sim *sim, fn_log_fn fn, void *user);
+/* Message of the last failure of a fn_sim_* call (or of its engine). */
+const char *fn_sim_error(fn_sim *sim);
+/* The output_file name of the input (relative names resolved against the
+ * input file's directory), the stem of the dump and restart files. */
+const char *fn_sim_output_file(fn_sim *sim);
+
+/* ---- output formats (host only; no device is touched) ---- */
+
+/* Fields of an XYZ frame (output_data position / velocity / force). */
+#define FN_OUT_POSITION 0
+#define FN_OUT_VELOCITY 1
+#define FN_OUT_FORCE 2
+/* DumpRestartData's text (SimulationManager.cpp:1894-1924) of `beads` (the
+ * model's bead count of records, e.g. read back by fn_buffer_read) into
+ * `path`, truncated: the bead count, "FRAME progress dt Lx Ly Lz", then per
+ * bead its type name, position, image, velocity and force at the stream's
+ * default 6 significant digits.  The format is lossy as the reference's is:
+ * 6 digits, no velocity_old or force_old (the restart reader sets
+ * velocity_old = velocity and force_old = 0) and no bond statuses. */
+int fn_model_write_restart(const fn_model *m, const fn_bead *beads, int progress, const char *path);
+/* One DumpData frame (SimulationManager.cpp:2002-2048) of `field` for the
+ * beads of the exported types, appended to `path`: their count,
+ * "FRAME progress dt Lx Ly Lz", then "type x y z" per bead, in fixed
+ * notation with 5 decimals. */
+int fn_model_write_xyz(const fn_model *m, const fn_bead *beads, int progress, int field, const char *path);
+
+/* ---- checkpoints and observables (no reference counterpart) ---- */
+
+/* Bit-exact binary checkpoint of the engine's dynamic state, in place of the
+ * lossy restart text: every bead's position, image, predicted velocity,
+ * velocity_old, force and force_old in the engine's own precision, the bond
+ * statuses, the spring break steps and the step counter, plus a header of
+ * the precision and record counts.  fn_checkpoint_load restores it into an
+ * engine holding the same network (same precision, bead, spring and angle
+ * counts); runs then continue bit for bit as the saved engine's would.
+ * FN_ERR_MISMATCH when the file belongs to another network or build. */
+int fn_checkpoint_save(fn_engine *eng, const char *path);
+int fn_checkpoint_load(fn_engine *eng, const char *path);
+/* The x-force of the current step summed over the beads flagged in
+ * group[n_beads] (a clamp's reaction force, as Run's benchmark log records
+ * it), per network: out[n_networks].  A fixed-order reduction on the device. */
+int fn_group_force(fn_engine *eng, const int *group, double *out);
 
 /* ---- synthetic inputs (no reference counterpart) ---- */
 

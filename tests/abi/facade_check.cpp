@@ -2,7 +2,9 @@
 // SimulationManager drives OpenclManager (buffers by id, arguments by index,
 // kernels by name, a finish per step).  Built and run by tests/test_abi.py:
 // without a GPU it must fail cleanly at Initialize; with one it runs a
-// five-bead chain for ten steps and prints the first bead's position.
+// five-bead chain for ten steps of AdvancePosition and ResolveSprings and
+// prints every bead's position, velocity and force, which the GPU test
+// compares with the oracle's same ten steps.
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -74,5 +76,11 @@ int main()
     ok &= m.ReadBuffer(bead, out.data());
     std::printf("%s x0=%.6f\n", ok ? "ok" : "failed", out[0].position.x);
     if (!ok) std::printf("%s", log.c_str());
+    // every bead's position, velocity and force, for the comparison with the oracle
+    for (int i = 0; i < 5 && ok; ++i) {
+        const fn_bead &b = out[i];
+        std::printf("bead %d %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g %.17g\n", i, b.position.x, b.position.y,
+                    b.position.z, b.velocity.x, b.velocity.y, b.velocity.z, b.force.x, b.force.y, b.force.z);
+    }
     return ok ? 0 : 1;
 }

@@ -50,17 +50,61 @@ def test_errors_without_device_are_clean():
     assert b"unable to open" in L.fn_model_error()
 
 
-def test_facade_builds_and_links(tmp_path):
+def _facade(tmp_path):
     exe = str(tmp_path / "facade_check")
     src = os.path.join(ROOT, "tests", "abi", "facade_check.cpp")
     libdir = os.path.dirname(api.LIB_PATH)
     subprocess.check_call(["g++", "-std=c++17", "-Wall", "-Werror", "-I", INC, src, "-o", exe, "-L", libdir,
                            "-lfibrenet_amd", "-Wl,-rpath," + libdir])
-    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    return subprocess.run([exe], capture_output=True, text=True, timeout=120)
+
+
+def test_facade_builds_and_links(tmp_path):
+    r = _facade(tmp_path)
     if "init failed" in r.stdout:
         assert r.returncode == 3          # no device here: a clean failure
     else:
         assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout
+
+
+@pytest.mark.gpu
+def test_facade_run_matches_the_oracle(need_gpu, tmp_path):
+    """The HipManager-driven chain (AdvancePosition and ResolveSprings with a
+    finish every step, as SimulationManager issues them) against the
+    oracle's same ten steps on the same records, in f64."""
+    import numpy as np
+    from oracle import records as OR
+    from oracle.oracle import Oracle
+    r = _facade(tmp_path)
+    assert r.returncode == 0 and r.stdout.startswith("ok"), r.stdout
+    got = np.array([[float(v) for v in l.split()[2:]] for l in r.stdout.split("\n") if l.startswith("bead ")])
+    s = np.zeros((), OR.SETTINGS)
+    s["types_num"], s["bc_type"], s["wall_contact_stiffness"] = 1, 1, 10.0
+    s["cutoff"], s["cutoff_sq"], s["box_unit_cell"] = 1.0, 1.0, 1.0
+    s["dt"], s["half_dt"], s["half_dt2"], s["lambda_dt"] = 1e-3, 5e-4, 5e-7, 5e-4
+    s["cells_x"] = s["cells_y"] = s["cells_z"] = 10
+    s["cells_xy"], s["half_box_sq"] = 100, 25.0
+    for c in "xyz":
+        s["grid_step"][c], s["box_size"][c] = 1.0, 10.0
+    b = np.zeros(5, OR.BEAD)
+    q = np.zeros(5, OR.BEAD_INFO)
+    for i in range(5):
+        for f in ("position", "image_position"):
+            b[f]["x"][i], b[f]["y"][i], b[f]["z"][i] = 2.0 + 1.1 * i, 5.0, 5.0
+    q["loc_id"] = np.arange(5)
+    q["radius"], q["skin_radius"], q["mass"], q["mass_inv"] = 0.5, 1.0, 1.0, 1.0
+    q["lambda_dt_mass_inv"], q["half_dt_mass_inv"], q["half_dt2_mass_inv"] = 5e-4, 5e-4, 5e-7
+    sp = np.zeros(4, OR.SPRING)
+    sp["p1"], sp["p2"], sp["status"], sp["rest_length"], sp["stiffness"] = np.arange(4), np.arange(1, 5), 1, 1.0, 100.0
+    pr = np.zeros(1, OR.PAIR)
+    pr["type"], pr["c1"] = 2, 100.0
+    orc = Oracle(b, q, sp, np.zeros(0, OR.ANGLE), pr, s)
+    for step in range(10):
+        orc.advance_position()
+        orc.resolve_springs(step)
+    want = np.hstack([np.stack([orc.beads[f][c] for c in "xyz"], 1) for f in ("position", "velocity", "force")])
+    assert got.shape == want.shape
+    assert np.all(np.abs(got - want) <= 1e-12 * np.maximum(1.0, np.abs(want))), (got, want)
 
 
 def test_c_header_is_plain_c(tmp_path):
