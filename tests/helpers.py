@@ -143,7 +143,29 @@ def adjoint_xlink(seed=23, n_links=40):
     return net
 
 
+def adjoint_types4(seed=51):
+    """adjoint_net with a fourth bead type (the free beads of even fibres) and
+    penalty contacts of a different stiffness for each of the 10 type pairs:
+    a pair table past the eight entries the one-launch chain adjoint step
+    keeps in LDS, so the adjoint runs its bond kernels and per-bead pair
+    rows."""
+    net = adjoint_net(seed=seed)
+    free = net.info["type"] == 0
+    net.info["type"] = np.where(free & (net.info["tag"] % 2 == 0), 3, net.info["type"])
+    T = 4
+    net.settings = net.settings.copy()
+    net.settings["types_num"] = T
+    net.pairs = np.zeros(T * (T + 1) // 2, R.PAIR)
+    for a in range(T):
+        for b in range(a, T):
+            k = G.pair_index(a, b, T)
+            net.pairs[k] = G.hs_pair(a, b, 600.0 + 100.0 * k)
+    net.type_names = ["C1", "C2", "C3", "C4"]
+    return net
+
+
 ADJ_NETS = {"hs": lambda: adjoint_net(), "hs_fl": lambda: adjoint_net(hr_fl=True),
+            "st": lambda: adjoint_net(seed=43, contact="st"), "types4": lambda: adjoint_types4(),
             "lj": lambda: adjoint_net(seed=41, contact="lj"),
             "periodic": lambda: adjoint_net(seed=41, box=(20.0, 20.0, 8.0), periodic=True),
             "xlink": lambda: adjoint_xlink()}

@@ -44,8 +44,11 @@ def test_values_follow_the_oracle(kind):
     assert abs(vals[-1] - f_last) < 1e-10 * max(1.0, abs(f_last))
 
 
-def test_derivatives_equal_central_differences():
-    net = adjoint_net()
+@pytest.mark.parametrize("kind", ["hs", "st", "lj"])
+def test_derivatives_equal_central_differences(kind):
+    """Along the three scale directions and random position and velocity
+    directions, for penalty, two-step harmonic and Lennard-Jones contacts."""
+    net = ADJ_NETS[kind]()
     N = len(net.beads)
     group, steps = _samples(net)
     rs = np.random.default_rng(5)

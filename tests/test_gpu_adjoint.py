@@ -117,7 +117,8 @@ def test_backward_needs_a_current_forward(need_gpu):
 @pytest.mark.parametrize("prec,kind,tol", [("f64", "hs", 1e-7), ("f64", "hs_fl", 1e-7), ("f32", "hs", 1e-3),
                                            ("f64", "lj", 1e-7), ("f32", "lj", 1e-3), ("f64", "periodic", 1e-7),
                                            ("f32", "periodic", 1e-3), ("f64", "xlink", 1e-7),
-                                           ("f32", "xlink", 1e-3)])
+                                           ("f32", "xlink", 1e-3), ("f64", "st", 1e-7), ("f32", "st", 1e-3),
+                                           ("f64", "types4", 1e-7), ("f32", "types4", 1e-3)])
 def test_matches_dual_number_reference(need_gpu, prec, kind, tol):
     net = ADJ_NETS[kind]()
     w = _weights()

@@ -1,0 +1,252 @@
+"""Parity at the benchmark configurations' own sizes (GPU), and the two-step
+harmonic ("st") contact law.
+
+  config 0  10,000 beads (200 fibres x 50), periodic 100x100x20, penalty
+            contacts: f64 against the oracle over 1,000 steps with cells and
+            pairs exact at all 125 rebuilds; f32 within the f32 tolerances.
+  config 1  2,000,000 beads (20,000 fibres x 100), finite 1000x1000x50,
+            HR_FL springs, LJ contacts: f64 against the oracle over 20 steps
+            (2 rebuilds: cells, pairs, breaks exact); the f32 build from the
+            oracle's state (beads displaced by 0.02) with failure stresses
+            set 5 % either side of each stretched spring's stress at the
+            next step: cells, pairs and the break events of that step exact.
+  config 2  the 512-network ensemble: every network bit for bit its own
+            engine's run, over two calls (three rebuilds).
+  config 4  the 128-network batched adjoint: samples and per-bead derivatives
+            bit for bit each network's own adjoint, bond and pair-entry
+            derivatives to rounding, over a shortened run.
+  st        a network with the README example's st contacts: f64 exact
+            cells and pairs over 1,000 steps, f32 within tolerance; the README
+            example itself through the simulation host against the oracle.
+
+Tolerances are those of test_gpu_parity: f64 fields within 1e-9 of the
+field's largest magnitude; f32 positions within 1e-5, velocities 2e-3,
+forces 1e-2 after 200 steps.
+"""
+import copy
+import os
+
+import numpy as np
+import pytest
+
+from fibrenet_amd import generate as G, records as R
+from helpers import engine_for, field_errors, oracle_for, read_beads, run_engine
+from test_gpu_parity import TOL
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _check_fields(got, ref, precision, what):
+    errs = field_errors(got, ref)
+    print(what, precision, errs, flush=True)
+    for f, e in errs.items():
+        assert e <= TOL[precision][f], (what, f, e)
+
+
+def _exact_over(net, steps, precision="f64"):
+    """Runs engine and oracle one rebuild interval at a time, comparing cells
+    and pair sets after each, then the break steps and the fields."""
+    eng, ids = engine_for(net, precision)
+    orc = oracle_for(net)
+    g = net.grid_update_freq
+    done = 0
+    while done < steps:
+        n = min(g, steps - done)
+        run_engine(eng, net, n, fast=True)
+        orc.run(n, g)
+        done += n
+        assert np.array_equal(eng.export_cells(len(net.beads)), orc.cell_of_bead), done
+        assert np.array_equal(eng.export_pairs(), orc.pair_set()), done
+    if len(net.springs):
+        assert np.array_equal(eng.export_breaks(len(net.springs)), orc.break_step)
+    _check_fields(read_beads(eng, ids), orc.beads, precision, net.meta.get("workload"))
+    eng.close()
+    return orc
+
+
+# ---- config 0 ----------------------------------------------------------------------------------
+
+def test_config0_f64_exact_over_1000_steps(need_gpu):
+    net = G.periodic_hs()
+    assert len(net.beads) == 10000 and int(net.settings["cells_x"]) * int(net.settings["cells_y"]) * \
+        int(net.settings["cells_z"]) == 200000
+    orc = _exact_over(net, 1000)
+    print("config 0 pairs", len(orc.pair_set()))
+
+
+def test_config0_f32_within_tolerance(need_gpu):
+    net = G.periodic_hs()
+    eng, ids = engine_for(net, "f32")
+    run_engine(eng, net, 200)
+    orc = oracle_for(net)
+    orc.run(200, net.grid_update_freq)
+    _check_fields(read_beads(eng, ids), orc.beads, "f32", "config 0")
+
+
+# ---- config 1 ----------------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def config1():
+    net = G.tensile_lj()
+    assert len(net.beads) == 2000000 and len(net.springs) == 1980000
+    orc = oracle_for(net)
+    yield net, orc
+    del orc
+
+
+def test_config1_f64_exact_over_two_rebuilds(need_gpu, config1):
+    net, orc = config1
+    eng, ids = engine_for(net, "f64")
+    for _ in range(2):
+        run_engine(eng, net, 10)
+        orc.run(10, net.grid_update_freq)
+        assert np.array_equal(eng.export_cells(len(net.beads)), orc.cell_of_bead)
+        assert np.array_equal(eng.export_pairs(), orc.pair_set())
+    assert np.array_equal(eng.export_breaks(len(net.springs)), orc.break_step)
+    _check_fields(read_beads(eng, ids), orc.beads, "f64", "config 1")
+    print("config 1 pairs", len(orc.pair_set()), "hashed", int((orc.cell_of_bead >= 0).sum()))
+    eng.close()
+
+
+def test_config1_f32_break_events_exact(need_gpu, config1):
+    """From the oracle's state after 20 steps with every bead displaced by
+    N(0, 0.02) per axis (springs strained by up to a few percent): the
+    springs strained by more than 1e-4 get failure stresses 5 % above or
+    below their stress at the next step, so about half of them break in
+    it.  The f32 engine lists the same cells and pairs at that step's
+    rebuild and breaks the same springs.  (A 1-ulp difference of the
+    float-rounded length moves such a stress by under 0.2 %.)"""
+    from oracle.oracle import Oracle
+    net, orc = config1
+    assert orc.step == 20
+    rs = np.random.default_rng(8)
+    L = np.array([net.settings["box_size"][c] for c in "xyz"])
+    pos = np.clip(R.vec(orc.beads, "position") + rs.normal(0.0, 0.02, (len(net.beads), 3)), 0.0, L)
+    stage = copy.copy(net)
+    stage.beads = orc.beads.copy()
+    R.set_vec(stage.beads, "position", pos)
+    R.set_vec(stage.beads, "image_position", pos)
+    stage.springs = orc.springs.copy()
+    stage.angles = orc.angles.copy()
+    probe = Oracle(stage.beads, net.info, stage.springs, stage.angles, net.pairs, net.settings, net.cell_nh)
+    probe.advance_position()
+    img = R.vec(probe.beads, "image_position")
+    del probe
+    d = img[net.springs["p1"]] - img[net.springs["p2"]]
+    l = np.sqrt((np.einsum("ij,ij->i", d, d)).astype(np.float32)).astype(np.float64)
+    stress = np.abs(net.springs["c1"] * (net.springs["rest_length"] - l) / l)
+    strained = (np.abs(net.springs["rest_length"] - l) > 1e-4) & (stage.springs["status"] == 1)
+    factor = np.where(np.arange(len(l)) % 2 == 0, 1.05, 0.95)
+    stage.springs["c2"] = np.where(strained, np.float32(stress * factor).astype(np.float64), 1e9)
+    ref = Oracle(stage.beads, stage.info, stage.springs, stage.angles, stage.pairs, stage.settings, stage.cell_nh)
+    ref.run(1, net.grid_update_freq)
+    expect = int((ref.break_step == 0).sum())
+    print("config 1 stage: strained springs", int(strained.sum()), "breaking", expect, flush=True)
+    assert expect > 100000
+    eng, ids = engine_for(stage, "f32")
+    run_engine(eng, stage, 1)
+    assert np.array_equal(eng.export_cells(len(net.beads)), ref.cell_of_bead)
+    assert np.array_equal(eng.export_pairs(), ref.pair_set())
+    assert np.array_equal(eng.export_breaks(len(net.springs)), ref.break_step)
+    assert np.array_equal(eng.read_buffer(ids["spring"], R.SPRING)["status"], ref.springs["status"])
+    assert np.array_equal(eng.read_buffer(ids["angle"], R.ANGLE)["status"], ref.angles["status"])
+    eng.close()
+
+
+# ---- config 2 ----------------------------------------------------------------------------------
+
+def test_config2_ensemble_of_512_equals_plain_runs(need_gpu):
+    from fibrenet_amd import Ensemble
+    nets = G.ensemble(512)
+    beads = sum(len(n.beads) for n in nets)
+    print("config 2 beads", beads, flush=True)
+    guf = 8
+    calls = (12, 9)
+    ens = Ensemble(nets, "f32")
+    for n in calls:
+        ens.run_steps(n, guf, True, True)
+    for k, net in enumerate(nets):
+        net.cell_nh = G.grid_neighbours(0, int(net.settings["cells_x"]), int(net.settings["cells_y"]),
+                                        int(net.settings["cells_z"]))
+        eng, ids = engine_for(net, "f32")
+        for n in calls:
+            eng.run_steps(n, guf, True, True)
+        assert ens.read(k, "bead").tobytes() == read_beads(eng, ids).tobytes(), k
+        eng.close()
+    ens.close()
+
+
+# ---- config 4 ----------------------------------------------------------------------------------
+
+def test_config4_batched_adjoint_of_128_equals_single_networks(need_gpu):
+    from fibrenet_amd import Ensemble
+    nets = G.tensile_batch(128)
+    steps, guf = 60, 10
+    samples = np.array([19, 39, 59], np.int32)
+    group = lambda n: (np.asarray(n.info["type"]) == 1).astype(np.int32)
+    rs = np.random.default_rng(4)
+    w = rs.normal(size=(len(nets), len(samples)))
+    ens = Ensemble(nets, "f32")
+    s = ens.adjoint_forward(steps, np.concatenate([group(n) for n in nets]), samples, grid_update_freq=guf,
+                            checkpoint_every=20)
+    off = {k: ens.offsets(k) for k in ("bead", "spring", "angle", "pair")}
+    g = ens.adjoint_backward(w, int(off["spring"][-1]), int(off["angle"][-1]), int(off["pair"][-1]))
+    print("config 4 beads", int(off["bead"][-1]), flush=True)
+    scale = {key: max(float(np.max(np.abs(g[key]))), 1e-300) for key in ("spring", "angle", "pair")}
+    for k, net in enumerate(nets):
+        eng, ids = engine_for(net, "f32")
+        sk = eng.adjoint_forward(steps, group(net), samples, grid_update_freq=guf, checkpoint_every=20)
+        gk = eng.adjoint_backward(w[k], len(net.springs), len(net.angles), len(net.pairs))
+        b0, b1 = off["bead"][k], off["bead"][k + 1]
+        assert np.allclose(s[k], sk, rtol=1e-12, atol=1e-12 * np.max(np.abs(sk))), k
+        assert g["position"][b0:b1].tobytes() == gk["position"].tobytes(), k
+        assert g["velocity_old"][b0:b1].tobytes() == gk["velocity_old"].tobytes(), k
+        for key in ("spring", "angle", "pair"):
+            # a pair entry's derivative is a sum over the network's contacts that
+            # can cancel to rounding noise, so its absolute bar is the batch's scale
+            o0, o1 = off[key][k], off[key][k + 1]
+            assert np.allclose(g[key][o0:o1], gk[key], rtol=1e-10, atol=1e-12 * scale[key]), (k, key)
+        eng.close()
+    ens.close()
+
+
+# ---- st contacts ---------------------------------------------------------------------------------
+
+def test_st_network_f64_exact_over_1000_steps(need_gpu):
+    net = G.st_network()
+    assert set(net.pairs["type"]) == {3}
+    orc = _exact_over(net, 1000)
+    b = orc.beads
+    r = np.linalg.norm(R.vec(b, "position")[orc.pair_set()[:, 0]] - R.vec(b, "position")[orc.pair_set()[:, 1]],
+                       axis=1)
+    acting = int((r * r < net.pairs["c4"][0]).sum())
+    print("st pairs", len(r), "within the force range", acting)
+    assert acting > 50
+
+
+def test_st_network_f32_within_tolerance(need_gpu):
+    net = G.st_network()
+    eng, ids = engine_for(net, "f32")
+    run_engine(eng, net, 200)
+    orc = oracle_for(net)
+    orc.run(200, net.grid_update_freq)
+    _check_fields(read_beads(eng, ids), orc.beads, "f32", "st")
+
+
+@pytest.mark.parametrize("fast", [True, False])
+def test_readme_example_through_the_simulation_host(need_gpu, fast):
+    """The README's input and model files (st pairs, damping, HR springs and
+    angles, a finite box without wall stiffness) loaded by fn_sim and run
+    200 steps; the oracle runs the same records."""
+    from fibrenet_amd import Model, Simulation
+    path = os.path.join(GOLD, "readme_input.txt")
+    sim = Simulation(path, "f64")
+    sim.run_loop(200, fast=fast)
+    got = sim.beads()
+    m = Model(path)
+    orc = oracle_for(m)
+    orc.run(200, m.grid_update_freq)
+    _check_fields(got, orc.beads, "f64", "README example")
+    assert np.abs(R.vec(got, "position") - R.vec(m.beads, "position")).max() > 1e-6   # it moved

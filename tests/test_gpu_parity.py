@@ -422,3 +422,15 @@ def test_overflowing_run_is_repeated_with_longer_rows(need_gpu):
     assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
     assert np.array_equal(a.export_pairs(), b.export_pairs())
     assert a.stats()["broken_springs"] == b.stats()["broken_springs"] > 0
+
+
+def test_more_than_seven_bead_types_are_refused_at_upload(need_gpu):
+    """The contact-list entries carry the pair index in 5 bits: a pair table
+    of more than 32 entries (8 or more bead types) is refused when it is
+    created, with the limit in the message."""
+    from fibrenet_amd import Engine, FibreNetError
+    eng = Engine("f64")
+    with pytest.raises(FibreNetError) as e:
+        eng.create_buffer("pair", np.zeros(36, R.PAIR))
+    assert e.value.code == -5 and "at most 7 bead types" in str(e.value)
+    assert eng.create_buffer("pair", np.zeros(28, R.PAIR)) >= 0
