@@ -77,6 +77,7 @@ SIGNATURES = [
     ("fn_checkpoint_save", _I, [_VP, _CP]),
     ("fn_checkpoint_load", _I, [_VP, _CP]),
     ("fn_group_force", _I, [_VP, _VP, _VP]),
+    ("fn_group_force_noalloc", _I, [_VP, _VP, _VP]),
     ("fn_gen_fibres", _I, [_c.c_ulonglong, _I, _I, _c.c_double, _c.c_double, _c.c_double, _I, _c.c_double,
                            _c.c_double, _LL, _VP, _c.POINTER(_LL)]),
 ]

@@ -227,6 +227,7 @@ struct fn_engine {
     virtual void checkpoint_save(const char *path) = 0;
     virtual void checkpoint_load(const char *path) = 0;
     virtual void group_force(const int *group, double *out) = 0;
+    virtual void group_force_noalloc(const int *group, double *out) = 0;
 };
 
 namespace {
@@ -2095,6 +2096,53 @@ class Engine final : public fn_engine {
         HIPCK(hipStreamSynchronize(stream_));
     }
 
+    // fn_group_force with every device buffer in the fixed workspace
+    // (fnk::scratch_workspace) and no allocation: the flagged indices go to
+    // the device through a stack buffer, synchronously, a chunk at a time.
+    void group_force_noalloc(const int *group, double *out) override
+    {
+        if (!group || !out) throw Error(FN_ERR_ARG, "group and output are required");
+        if (N_ <= 0) throw Error(FN_ERR_ARG, "no bead buffer");
+        const int nn = n_networks();
+        if (nn > fnk::kScratchNets)
+            throw Error(FN_ERR_CAPACITY, fmt("%d networks: the fixed workspace holds %d", nn, fnk::kScratchNets));
+        int total = 0;
+        for (int i = 0; i < N_; ++i) total += group[i] != 0;
+        if (total > fnk::kScratchGroup)
+            throw Error(FN_ERR_CAPACITY, fmt("%d flagged beads: the fixed workspace holds %d", total,
+                                             fnk::kScratchGroup));
+        fnk::ScratchWorkspace w;
+        HIPCK(fnk::scratch_workspace(device, &w));
+        constexpr int kChunk = 4096;
+        int buf[kChunk];
+        int off[fnk::kScratchNets + 1];
+        int nb = 0, written = 0, max_len = 0;
+        auto flush = [&] {
+            if (nb == 0) return;
+            HIPCK(hipMemcpyAsync(w.idx + written, buf, sizeof(int) * nb, hipMemcpyHostToDevice, stream_));
+            HIPCK(hipStreamSynchronize(stream_));
+            written += nb;
+            nb = 0;
+        };
+        off[0] = 0;
+        for (int k = 0; k < nn; ++k) {
+            const int b0 = ens_boxes_.empty() ? 0 : ens_b0_[k], b1 = ens_boxes_.empty() ? N_ : ens_b0_[k + 1];
+            for (int i = b0; i < b1; ++i)
+                if (group[i] != 0) {
+                    buf[nb++] = i;
+                    if (nb == kChunk) flush();
+                }
+            off[k + 1] = written + nb;
+            max_len = std::max(max_len, off[k + 1] - off[k]);
+        }
+        flush();
+        HIPCK(hipMemcpyAsync(w.off, off, sizeof(int) * (nn + 1), hipMemcpyHostToDevice, stream_));
+        fnk::group_force(frc_[fc_][0].p, w.idx, w.off, nn, max_len, w.part, w.out, stream_);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(out, w.out, sizeof(double) * nn, hipMemcpyDeviceToHost, stream_));
+        HIPCK(hipStreamSynchronize(stream_));
+    }
+
     // The adjoint uses the one-pass exact rebuild, whose lists depend on the
     // positions alone, so a recomputed segment lists what the forward listed.
     struct ExactRebuild {
@@ -2786,6 +2834,11 @@ int fn_checkpoint_load(fn_engine *eng, const char *path)
 int fn_group_force(fn_engine *eng, const int *group, double *out)
 {
     return guarded(eng, [&] { eng->group_force(group, out); });
+}
+
+int fn_group_force_noalloc(fn_engine *eng, const int *group, double *out)
+{
+    return guarded(eng, [&] { eng->group_force_noalloc(group, out); });
 }
 
 int fn_get_timing(fn_engine *eng, fn_timing *out)

@@ -154,6 +154,22 @@ template <typename T>
 void group_force(const T *fx, const int *idx, const int *off, int n_net, int max_len, double *part, double *out,
                  hipStream_t s);
 
+// The fixed device workspace of fn_group_force_noalloc (noalloc.hip): room
+// for kScratchGroup flagged beads over at most kScratchNets networks, with
+// the partial sums and results of the reduction; no allocation at any call.
+constexpr int kScratchGroup = 1 << 20;
+constexpr int kScratchNets = 1024;
+struct ScratchWorkspace {
+    int *idx;       // kScratchGroup flagged bead indices
+    int *off;       // kScratchNets + 1 entry offsets per network
+    double *part;   // kSumBlocks * kScratchNets partial sums
+    double *out;    // kScratchNets results
+};
+// The workspace of `device`, which must be current: its address is looked up
+// at the first call on a device and kept.
+hipError_t scratch_workspace(int device, ScratchWorkspace *w);
+size_t scratch_bytes();
+
 // cs_st[i] = sp_status[ch_spring[i]], ca_st[i] = an_status[ch_angle[i]] (0 without a bond).
 void chain_status(const int *ch_spring, const int *ch_angle, const int *sp_status, const int *an_status, int *cs_st,
                   int *ca_st, int n, hipStream_t s);

@@ -372,6 +372,18 @@ int fn_checkpoint_load(fn_engine *eng, const char *path);
  * group[n_beads] (a clamp's reaction force, as Run's benchmark log records
  * it), per network: out[n_networks].  A fixed-order reduction on the device. */
 int fn_group_force(fn_engine *eng, const int *group, double *out);
+/* fn_group_force for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc); it
+ * has no counterpart in the reference.  Same arguments and the same result,
+ * bit for bit: the same reduction kernels, with the flagged bead indices, the
+ * per-network offsets, the partial sums and the result in a fixed device
+ * workspace that the library holds per GPU (about 6 MiB, for at most 2^20
+ * flagged beads over at most 1,024 networks; FN_ERR_CAPACITY beyond that).
+ * No hipMalloc, hipFree or other allocating device call is made.  Every call
+ * on one GPU shares that GPU's workspace, so calls that overlap on one GPU
+ * (from several host threads) race; the workspace keeps the last call's
+ * contents, and every call overwrites what it reads. */
+int fn_group_force_noalloc(fn_engine *eng, const int *group, double *out);
 
 /* ---- synthetic inputs (no reference counterpart) ---- */
 
