@@ -51,8 +51,9 @@ struct Launch {
     static void reset_energy(const Args<T, P> &a, int n_active, hipStream_t s);
     static void drift_first(const Args<T, P> &a, hipStream_t s);
     // the bead kernel of the fast path: after spring_terms / angle_terms, or
-    // with chain (a.ch_spring / a.ch_angle layout) evaluating the bonds itself
-    static void bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, bool chain,
+    // with chain (a.ch_spring / a.ch_angle layout) evaluating the bonds itself;
+    // block: threads per block (128, 256 or 512; the results do not depend on it)
+    static void bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, bool chain, int block,
                           hipStream_t s);
 };
 
@@ -157,8 +158,8 @@ void group_force(const T *fx, const int *idx, const int *off, int n_net, int max
 // The fixed device workspace of fn_group_force_noalloc (noalloc.hip): room
 // for kScratchGroup flagged beads over at most kScratchNets networks, with
 // the partial sums and results of the reduction; no allocation at any call.
-constexpr int kScratchGroup = 1 << 20;
-constexpr int kScratchNets = 1024;
+constexpr int kScratchGroup = 1 << 17;
+constexpr int kScratchNets = 256;
 struct ScratchWorkspace {
     int *idx;       // kScratchGroup flagged bead indices
     int *off;       // kScratchNets + 1 entry offsets per network
