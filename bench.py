@@ -115,15 +115,22 @@ def survey_bytes(counts, grid_update_freq, chain):
     return kern, rebuild, sum(kern.values()) + rebuild / grid_update_freq
 
 
-def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8):
+def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8, tune_steps=0):
     """Batched ensemble of config-2 networks against looping the plain engine
-    over the same networks: bead-steps/s of each (Sum N_i per step)."""
+    over the same networks: bead-steps/s of each (Sum N_i per step).  With
+    tune_steps, the autotuner first runs that many steps on the ensemble
+    (the settings it chooses, block size and chunk length among them, also
+    go to the looped engines)."""
     from fibrenet_amd import Engine, Ensemble, bind_standard, generate as G
     t0 = time.perf_counter()
     nets = G.ensemble(n_networks)
     gen_s = time.perf_counter() - t0
     n_tot = sum(len(n.beads) for n in nets)
     ens = Ensemble(nets, precision)
+    tune = None
+    if tune_steps > 0:
+        trials, chosen = ens.autotune(tune_steps, guf, True, True)
+        tune = {"steps": tune_steps, "chosen": chosen, "trials": trials}
     ens.run_steps(warmup, guf, True, True)
     ens.run_steps(steps, guf, True, True)          # captures the graphs of this run length
     ens.finish()
@@ -139,6 +146,9 @@ def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8):
                                         int(net.settings["cells_z"]))
         e = Engine(precision)
         bind_standard(e, net)
+        for k in ("block", "chunk"):
+            if tune:
+                e.set_option(k, tune["chosen"][k])
         e.run_steps(loop_steps, guf, True, True)
         engines.append(e)
     for e in engines:
@@ -156,7 +166,7 @@ def ensemble_bench(n_networks, steps, warmup, loop_steps, precision, guf=8):
     return {"networks": n_networks, "beads": n_tot, "steps": steps, "ms_per_step": wall * 1e3 / steps,
             "bead_steps_per_s": v_ens, "loop_steps": loop_steps, "loop_bead_steps_per_s": v_loop,
             "speedup_over_loop": v_ens / v_loop, "listed_pairs": st["list_entries"] // 2,
-            "grid_update_freq": guf, "generate_s": gen_s}
+            "grid_update_freq": guf, "generate_s": gen_s, "autotune": tune}
 
 
 def drift_vs_f64(net, options, marks=(200, 800)):
@@ -603,6 +613,8 @@ def main():
     ap.add_argument("--checkpoint-every", type=int, default=50)
     ap.add_argument("--tune-steps", type=int, default=3000,
                     help="steps of the autotuner, run on a second engine from the warmed-up state (0: none)")
+    ap.add_argument("--set", action="append", default=[], metavar="OPTION=VALUE",
+                    help="engine option applied to the measured engine after the autotuner (repeatable)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-drift", dest="drift", action="store_false",
                     help="skip the fp32-against-fp64 drift check (tensile_lj / periodic_hs, fp32)")
@@ -631,7 +643,8 @@ def main():
         return
 
     if args.workload == "ensemble":
-        e = ensemble_bench(args.networks, args.steps, args.warmup, args.loop_steps, args.precision)
+        e = ensemble_bench(args.networks, args.steps, args.warmup, args.loop_steps, args.precision,
+                           tune_steps=args.tune_steps)
         out = {"metric": "bead-steps/s (batched ensemble)", "value": e["bead_steps_per_s"], "unit": "bead-steps/s",
                "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": e["ms_per_step"],
                "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
@@ -683,6 +696,9 @@ def main():
         for k, v in chosen.items():
             eng.set_option(k, v)
         tune = {"steps": args.tune_steps, "chosen": chosen, "trials": trials}
+    for kv in args.set:
+        k, v = kv.split("=")
+        eng.set_option(k, int(v))
     print("bench: %s, warmup %d done, timing %d steps" % (desc, args.warmup, args.steps), file=sys.stderr,
           flush=True)
 
@@ -740,7 +756,7 @@ def main():
     adj = None
     if args.adjoint_steps > 0:
         try:
-            opts = {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet")}
+            opts = {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet", "block")}
             adj = adjoint_bench(net, args.precision, opts, args.warmup, args.adjoint_steps, args.checkpoint_every)
         except Exception as exc:          # the plain measurement stands on its own
             adj = {"error": repr(exc)}
@@ -748,14 +764,15 @@ def main():
     drift = None
     if args.drift and args.precision == "f32":
         try:
-            drift = drift_vs_f64(net, {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet")})
+            drift = drift_vs_f64(net, {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet", "block")})
         except Exception as exc:          # the plain measurement stands on its own
             drift = {"error": repr(exc)}
 
     ens = None
     if args.ensemble_networks > 0:
         try:
-            ens = ensemble_bench(args.ensemble_networks, 400, 40, args.loop_steps, args.precision)
+            ens = ensemble_bench(args.ensemble_networks, 400, 40, args.loop_steps, args.precision,
+                                 tune_steps=min(args.tune_steps, 1200))
         except Exception as exc:          # the plain measurement stands on its own
             ens = {"error": repr(exc)}
 

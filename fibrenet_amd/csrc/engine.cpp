@@ -692,6 +692,11 @@ class Engine final : public fn_engine {
             skin_permille_ = value;
         }
         else if (name == "graphs") graphs_on_ = value != 0;
+        else if (name == "block") {
+            if (value != 128 && value != 256 && value != 512)
+                throw Error(FN_ERR_ARG, "block is 128, 256 or 512 threads");
+            block_ = value;
+        }
         else throw Error(FN_ERR_ARG, "unknown option " + name);
         clear_graphs();
     }
@@ -713,6 +718,7 @@ class Engine final : public fn_engine {
             return chain_ok_ && cmp_ok_ ? 1 : 0;
         }
         if (name == "graphs") return graphs_on_ ? 1 : 0;
+        if (name == "block") return block_;
         throw Error(FN_ERR_ARG, "unknown option " + name);
     }
 
@@ -1598,7 +1604,7 @@ class Engine final : public fn_engine {
             mark(sv);
             if (ang && !chain) fnk::Launch<T, P>::angle_terms(a, A_, stream_);
             mark(sv);
-            fnk::Launch<T, P>::bead_step(a, spr, ang, !is_last, is_last, chain, stream_);
+            fnk::Launch<T, P>::bead_step(a, spr, ang, !is_last, is_last, chain, block_, stream_);
             mark(sv);
             if (hook) (*hook)(k, pc, 1 - fc);
             fc ^= 1;
@@ -1871,8 +1877,9 @@ class Engine final : public fn_engine {
             knobs.push_back({FN_TUNE_STAGED, {0, 1}});
             knobs.push_back({FN_TUNE_VERLET, {verlet_, 0, 1}});
         }
+        knobs.push_back({FN_TUNE_BLOCK, {block_, 256, 128, 512}});
         knobs.push_back({FN_TUNE_CHUNK, chunks});       // last: its trials are the longest
-        static const char *names[] = {"chunk", "chain", "staged", "verlet"};
+        static const char *names[] = {"chunk", "chain", "staged", "verlet", "block"};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIPCK(hipEventCreate(&e0));
         HIPCK(hipEventCreate(&e1));
@@ -2143,18 +2150,26 @@ class Engine final : public fn_engine {
         HIPCK(hipStreamSynchronize(stream_));
     }
 
-    // The adjoint uses the one-pass exact rebuild, whose lists depend on the
-    // positions alone, so a recomputed segment lists what the forward listed.
-    struct ExactRebuild {
+    // The rebuild of the adjoint's runs: kept candidates when they are
+    // switched on (verlet 1), the one-pass exact rebuild otherwise.  Either
+    // way the lists depend on the positions alone.  With kept candidates,
+    // every segment's first rebuild builds them afresh, in the forward run
+    // and in the segment's recomputation alike, so both build them at the
+    // same steps from the same positions (and need the same capacities).
+    struct AdjointRebuild {
         Engine &e;
         int saved;
-        explicit ExactRebuild(Engine &en) : e(en), saved(en.verlet_) { e.verlet_ = 0; }
-        ~ExactRebuild()
+        AdjointRebuild(Engine &en, bool keep) : e(en), saved(en.verlet_) { e.verlet_ = keep ? 1 : 0; }
+        ~AdjointRebuild()
         {
             e.verlet_ = saved;
             e.cand_dirty_ = true;
         }
     };
+    void fresh_candidates()
+    {
+        if (keep_cand_) fnk::fill_int(need_.p, 1, 1, stream_);
+    }
 
     void adjoint_forward(const fn_adjoint_spec &sp, const int *group, const int *steps, int ns,
                          double *samples) override
@@ -2173,7 +2188,8 @@ class Engine final : public fn_engine {
             if (steps[k] < 0 || steps[k] >= n || (k > 0 && steps[k] <= steps[k - 1]))
                 throw Error(FN_ERR_ARG, "sample steps must increase strictly within [0, n_steps)");
         bool spr = sp.run_springs != 0, ang = sp.run_angles != 0;
-        ExactRebuild exact(*this);
+        adj_.keep = verlet_ == 1;
+        AdjointRebuild rebuild_mode(*this, adj_.keep);
         prep_fast(n, guf, spr, ang);
         std::vector<int> g(N_, 0), gi;
         if (group)
@@ -2217,6 +2233,7 @@ class Engine final : public fn_engine {
                 }
             };
             const bool first = seg == 0, last = seg == nseg - 1;
+            fresh_candidates();
             enqueue_chunk(len, guf, spr, ang, first, last, nullptr, &hook);
             HIPCK(hipGetLastError());
             flip(len, first, last);
@@ -2276,7 +2293,7 @@ class Engine final : public fn_engine {
         auto seed_at = [&](int step) -> const double * {
             return (step >= 0 && row_of[step] >= 0) ? seed_.p + (size_t)row_of[step] * nn : nullptr;
         };
-        ExactRebuild exact(*this);
+        AdjointRebuild rebuild_mode(*this, adj_.keep);
         if (ens) {
             prepare_ensemble_run(spr, ang);
         } else {
@@ -2329,6 +2346,7 @@ class Engine final : public fn_engine {
             pc_ = adj_.ck_pc[seg];
             fc_ = adj_.ck_fc[seg];
             load_parts(state_parts(pc_, fc_, false), ck_.p + adj_.cb * seg);
+            fresh_candidates();
             const StepHook hook = [&](int k, int pr, int) {
                 const double *p[3] = {pos_[pr][0].p, pos_[pr][1].p, pos_[pr][2].p};
                 const double *im[3] = {img_[pr][0].p, img_[pr][1].p, img_[pr][2].p};
@@ -2408,7 +2426,7 @@ class Engine final : public fn_engine {
     }
 
     struct AdjRun {
-        bool valid = false, spr = false, ang = false;
+        bool valid = false, spr = false, ang = false, keep = false;
         int n = 0, guf = 1, every = 1, nseg = 0, n_spr = 0, n_ang = 0, cap = 0, fin_pc = 0, fin_fc = 0, n_net = 1;
         size_t cb = 0;
         std::vector<int> ck_pc, ck_fc, steps;
@@ -2518,6 +2536,7 @@ class Engine final : public fn_engine {
     int rows_exact_ = 0, rows_cand_ = 0;   // longest exact / candidate rows seen since the state was set (0: none)
     DArr<unsigned char> run_snap_;         // the state at the start of a guarded run
     int chunk_ = 128;                                // steps per captured graph
+    int block_ = 256;                                // threads per block of the bead kernels
     bool chain_on_ = true;                           // use the chain bead kernel where the layout allows
     bool chain_ok_ = false;                          // the bonds follow the chain layout
     int verlet_ = 2;                                 // candidates kept between rebuilds: 0 never, 1 always, 2 auto

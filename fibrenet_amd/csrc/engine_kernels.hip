@@ -225,13 +225,13 @@ __device__ __forceinline__ void bead_end(const Args<T, P> &a, const Box &bx, int
 // drift.  Reads f of the previous step from f*, writes this step's force to
 // g*.  The terms and their summation order are those of the per-kernel
 // path, so both paths agree bit for bit.
-template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS>
-__global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
+template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS, int BLK>
+__global__ void __launch_bounds__(BLK) k_bead_step(Args<T, P> a)
 {
     __shared__ PairTab<T> prs[kPairCap];
-    __shared__ WaveContactLds<T> cl[kBlock / 64];
-    const int i = xcd_block() * kBlock + threadIdx.x;
-    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = to_tab<T>(a.pairs[k]);
+    __shared__ WaveContactLds<T> cl[BLK / 64];
+    const int i = xcd_block() * BLK + threadIdx.x;
+    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += BLK) prs[k] = to_tab<T>(a.pairs[k]);
     __syncthreads();
     const bool live = i < a.n;
     const int ic = live ? i : a.n - 1;      // lanes past the end follow along for the wave-wide contacts
@@ -268,15 +268,17 @@ __global__ void __launch_bounds__(kBlock) k_bead_step(Args<T, P> a)
 template <typename T, bool ENS>
 constexpr int kChainWaves = (sizeof(T) == sizeof(float) && !ENS) ? 5 : 1;
 
-template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kChainWaves<T, ENS>, 8))) k_bead_step_chain(Args<T, P> a)
+template <typename T, typename P, bool SPR, bool ANG, bool NEXT, bool LAST, bool ENS, int BLK>
+__global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChainWaves<T, ENS>, 8))) k_bead_step_chain(Args<T, P> a)
 {
     __shared__ PairTab<T> prs[kPairCap];
-    __shared__ WaveContactLds<T> cl[kBlock / 64];
-    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += kBlock) prs[k] = to_tab<T>(a.pairs[k]);
+    __shared__ WaveContactLds<T> cl[BLK / 64];
+    for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += BLK) prs[k] = to_tab<T>(a.pairs[k]);
     __syncthreads();
     const int lane = threadIdx.x & 63;
-    const int i = (xcd_block() * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kChainOwned + lane - kHalo;
+    // a wave's beads depend on its index over the whole grid only, so every
+    // block size covers each bead with the same wave and the same lanes
+    const int i = (xcd_block() * (BLK / 64) + (int)(threadIdx.x >> 6)) * kChainOwned + lane - kHalo;
     const bool in = i >= 0 && i < a.n;
     const bool own = in && lane >= kHalo && lane < 64 - kHalo;
     const int ic = in ? i : (i < 0 ? 0 : a.n - 1);
@@ -1392,34 +1394,36 @@ void Launch<T, P>::drift_first(const Args<T, P> &a, hipStream_t s)
     if (a.net) k_drift_first<T, P, true><<<blocks(a.n), kBlock, 0, s>>>(a);
     else k_drift_first<T, P, false><<<blocks(a.n), kBlock, 0, s>>>(a);
 }
-template <typename T, typename P, int M>
+template <typename T, typename P, int M, int BLK>
 static void launch_fused_mask(const Args<T, P> &a, hipStream_t s)
 {
     constexpr bool spr = (M & 1) != 0, ang = (M & 2) != 0, next = (M & 4) != 0, last = (M & 8) != 0;
     constexpr bool ens = (M & 32) != 0;
     if (M & 16) {
-        constexpr int per_block = kChainOwned * (kBlock / 64);
-        k_bead_step_chain<T, P, spr, ang, next, last, ens><<<(a.n + per_block - 1) / per_block, kBlock, 0, s>>>(a);
+        constexpr int per_block = kChainOwned * (BLK / 64);
+        k_bead_step_chain<T, P, spr, ang, next, last, ens, BLK><<<(a.n + per_block - 1) / per_block, BLK, 0, s>>>(a);
     } else {
-        k_bead_step<T, P, spr, ang, next, last, ens><<<blocks(a.n), kBlock, 0, s>>>(a);
+        k_bead_step<T, P, spr, ang, next, last, ens, BLK><<<(a.n + BLK - 1) / BLK, BLK, 0, s>>>(a);
     }
 }
 
-template <typename T, typename P, int... M>
+template <typename T, typename P, int BLK, int... M>
 static void dispatch_fused(const Args<T, P> &a, int mask, hipStream_t s, std::integer_sequence<int, M...>)
 {
     using Fn = void (*)(const Args<T, P> &, hipStream_t);
-    static constexpr Fn table[] = {&launch_fused_mask<T, P, M>...};
+    static constexpr Fn table[] = {&launch_fused_mask<T, P, M, BLK>...};
     table[mask](a, s);
 }
 
 template <typename T, typename P>
-void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, bool chain,
+void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next, bool last, bool chain, int block,
                              hipStream_t s)
 {
     const int mask =
         (spr ? 1 : 0) | (ang ? 2 : 0) | (next ? 4 : 0) | (last ? 8 : 0) | (chain ? 16 : 0) | (a.net ? 32 : 0);
-    dispatch_fused<T, P>(a, mask, s, std::make_integer_sequence<int, 64>{});
+    if (block == 128) dispatch_fused<T, P, 128>(a, mask, s, std::make_integer_sequence<int, 64>{});
+    else if (block == 512) dispatch_fused<T, P, 512>(a, mask, s, std::make_integer_sequence<int, 64>{});
+    else dispatch_fused<T, P, 256>(a, mask, s, std::make_integer_sequence<int, 64>{});
 }
 template struct Launch<double, double>;
 template struct Launch<float, double>;

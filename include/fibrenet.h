@@ -150,6 +150,7 @@ int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_spri
  *   "staged" 1/0  neighbour search reads each block's neighbourhood from LDS
  *   "skin"   n    the skin in permille of the largest listing distance (300)
  *   "graphs" 1/0  launch through captured hipGraphs or directly
+ *   "block"  n    threads per block of the bead kernels: 128, 256 or 512
  * All options give bit-identical results; they change only the speed. */
 int fn_set_option(fn_engine *eng, const char *name, int value);
 int fn_get_option(fn_engine *eng, const char *name, int *value);
@@ -212,14 +213,15 @@ int fn_batch_load(fn_engine *eng, int n_networks, const fn_network *nets);
  * Runs n_steps steps exactly as fn_run_steps does (bitwise the same
  * results) while timing the settings that cannot change a result, one knob
  * at a time: the chain bead kernel, the LDS-staged search, kept contact
- * candidates and the chunk length of the captured graphs (options "chain",
- * "staged", "verlet", "chunk"; in that order).  Each value runs one chunk to
+ * candidates, the bead kernels' block size and the chunk length of the
+ * captured graphs (options "chain", "staged", "verlet", "block", "chunk"; in
+ * that order; an ensemble tunes "chain", "block" and "chunk").  Each value runs one chunk to
  * capture its graphs and one timed chunk; the fastest value of a knob is
  * kept before the next knob is tried, and the chosen settings stay set
  * (fn_get_option reads them).  A run too short for every trial tries what
  * fits.  trials (capacity entries, may be NULL) receives each timing;
  * *n_trials their number. */
-enum { FN_TUNE_CHUNK = 0, FN_TUNE_CHAIN = 1, FN_TUNE_STAGED = 2, FN_TUNE_VERLET = 3 };
+enum { FN_TUNE_CHUNK = 0, FN_TUNE_CHAIN = 1, FN_TUNE_STAGED = 2, FN_TUNE_VERLET = 3, FN_TUNE_BLOCK = 4 };
 typedef struct fn_tune_trial {
     int knob;               /* FN_TUNE_*                                       */
     int value;              /* the option's value during the timed chunk       */
@@ -377,8 +379,8 @@ int fn_group_force(fn_engine *eng, const int *group, double *out);
  * has no counterpart in the reference.  Same arguments and the same result,
  * bit for bit: the same reduction kernels, with the flagged bead indices, the
  * per-network offsets, the partial sums and the result in a fixed device
- * workspace that the library holds per GPU (about 6 MiB, for at most 2^20
- * flagged beads over at most 1,024 networks; FN_ERR_CAPACITY beyond that).
+ * workspace that the library holds per GPU (about 1 MiB, for at most 2^17
+ * flagged beads over at most 256 networks; FN_ERR_CAPACITY beyond that).
  * No hipMalloc, hipFree or other allocating device call is made.  Every call
  * on one GPU shares that GPU's workspace, so calls that overlap on one GPU
  * (from several host threads) race; the workspace keeps the last call's

@@ -365,23 +365,49 @@ def test_graphs_equal_direct_launches(need_gpu, workload, precision):
     assert all(o == out[0] for o in out[1:])
 
 
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("workload", ["tensile_lj_breaking", "crosslinked", "dense_lj"])
+def test_block_sizes_equal(need_gpu, workload, precision):
+    """The bead kernels' block size (128, 256, 512 threads; the chain kernel
+    and the slot kernel) changes nothing: every wave covers the same beads."""
+    net = WORKLOADS[workload]()
+    spr, ang = len(net.springs) > 0, len(net.angles) > 0
+    out = []
+    for block in (256, 128, 512):
+        eng, ids = engine_for(net, precision)
+        eng.set_option("block", block)
+        for n in (37, 120):
+            eng.run_steps(n, net.grid_update_freq, spr, ang)
+        out.append((read_beads(eng, ids).tobytes(), eng.export_breaks(len(net.springs)).tobytes(),
+                    eng.export_pairs().tobytes()))
+    assert all(o == out[0] for o in out[1:])
+
+
+def test_block_option_is_checked(need_gpu):
+    from fibrenet_amd import FibreNetError
+    eng, _ = engine_for(small_periodic(), "f32")
+    with pytest.raises(FibreNetError):
+        eng.set_option("block", 192)
+    assert eng.get_option("block") == 256
+
+
 @pytest.mark.parametrize("workload", ["tensile_lj_breaking", "periodic_hs"])
 def test_autotune_equals_run_steps(need_gpu, workload):
     """The autotuner's run is a plain run bit for bit, whatever it tries."""
     net = WORKLOADS[workload]()
     spr, ang = len(net.springs) > 0, len(net.angles) > 0
     a, ia = engine_for(net, "f32")
-    trials, chosen = a.autotune(6000, net.grid_update_freq, spr, ang)   # room for every trial
+    trials, chosen = a.autotune(8000, net.grid_update_freq, spr, ang)   # room for every trial
     a.run_steps(100, net.grid_update_freq, spr, ang)        # continues with the chosen settings
     b, ib = engine_for(net, "f32")
-    b.run_steps(6000, net.grid_update_freq, spr, ang)
+    b.run_steps(8000, net.grid_update_freq, spr, ang)
     b.run_steps(100, net.grid_update_freq, spr, ang)
     assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
     assert np.array_equal(a.export_breaks(len(net.springs)), b.export_breaks(len(net.springs)))
     assert np.array_equal(a.export_pairs(), b.export_pairs())
     knobs = {t["knob"] for t in trials}
-    assert {"chunk", "staged", "verlet"} <= knobs and all(t["ms_per_step"] > 0 for t in trials)
-    assert a.stats()["steps"] == 6100
+    assert {"chunk", "staged", "verlet", "block"} <= knobs and all(t["ms_per_step"] > 0 for t in trials)
+    assert a.stats()["steps"] == 8100
 
 
 def test_dense_network_takes_the_long_paths(need_gpu):
