@@ -23,8 +23,8 @@ read after each, reporting the dynamics along the way.
 
 Prints one JSON line: bead-steps/s of the timed steps, the roofline of the
 dominant kernel (the step kernel with the largest duration, measured with
-HIP event nodes in the captured graphs of the timed region, against the
-survey's algorithmic bytes of a step) and a CPU baseline (the fp64 oracle
+HIP event nodes in the captured graphs of a second run of the same length,
+against the survey's algorithmic bytes of a step) and a CPU baseline (the fp64 oracle
 on all host threads with the reference's atomic adds, and on one core, on a
 bounded sample of the same network).
 """
@@ -611,7 +611,7 @@ def main():
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the adjoint cost measurement on the same network (0: none)")
     ap.add_argument("--checkpoint-every", type=int, default=50)
-    ap.add_argument("--tune-steps", type=int, default=3000,
+    ap.add_argument("--tune-steps", type=int, default=5000,
                     help="steps of the autotuner, run on a second engine from the warmed-up state (0: none)")
     ap.add_argument("--set", action="append", default=[], metavar="OPTION=VALUE",
                     help="engine option applied to the measured engine after the autotuner (repeatable)")
@@ -702,26 +702,28 @@ def main():
     print("bench: %s, warmup %d done, timing %d steps" % (desc, args.warmup, args.steps), file=sys.stderr,
           flush=True)
 
-    # Timed region: the hipGraph-captured chunks with HIP event nodes around
-    # every step kernel and rebuild (read after each chunk).  A first run of
-    # the same length captures the graphs outside the timed region.
-    eng.set_timing(True)
-    eng.run_steps(args.steps, net.grid_update_freq, spr, ang)
-    eng.set_timing(True)                                          # clears the sums, keeps the graphs
-    eng.finish()
-    t = time.perf_counter()
-    eng.run_steps(args.steps, net.grid_update_freq, spr, ang)   # returns after the stream has drained
-    wall = time.perf_counter() - t
-    tim = eng.timing()
-    st = eng.stats()
-    state = bead_state(eng, ids)
-    eng.set_timing(False)
-    # The same again without timing (no event nodes, no synchronisation per chunk).
+    # Timed region: K steps in the hipGraph-captured chunks, bracketed by
+    # stream synchronisations (run_steps returns after the stream has
+    # drained).  A first run of the same length captures the graphs outside
+    # the timed region.
     eng.run_steps(args.steps, net.grid_update_freq, spr, ang)   # capture
     eng.finish()
     t = time.perf_counter()
     eng.run_steps(args.steps, net.grid_update_freq, spr, ang)
-    wall_graph = time.perf_counter() - t
+    wall = time.perf_counter() - t
+    # Kernel durations: the same run length again with HIP event nodes around
+    # every step kernel and rebuild in the graphs (read after each chunk).
+    eng.set_timing(True)
+    eng.run_steps(args.steps, net.grid_update_freq, spr, ang)   # capture with event nodes
+    eng.set_timing(True)                                          # clears the sums, keeps the graphs
+    eng.finish()
+    t = time.perf_counter()
+    eng.run_steps(args.steps, net.grid_update_freq, spr, ang)
+    wall_events = time.perf_counter() - t
+    tim = eng.timing()
+    st = eng.stats()
+    state = bead_state(eng, ids)
+    eng.set_timing(False)
 
     n = len(net.beads)
     pairs = st["list_entries"] // 2
@@ -756,7 +758,7 @@ def main():
     adj = None
     if args.adjoint_steps > 0:
         try:
-            opts = {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet", "block")}
+            opts = {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet", "block", "skin")}
             adj = adjoint_bench(net, args.precision, opts, args.warmup, args.adjoint_steps, args.checkpoint_every)
         except Exception as exc:          # the plain measurement stands on its own
             adj = {"error": repr(exc)}
@@ -764,7 +766,7 @@ def main():
     drift = None
     if args.drift and args.precision == "f32":
         try:
-            drift = drift_vs_f64(net, {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet", "block")})
+            drift = drift_vs_f64(net, {k: eng.get_option(k) for k in ("chunk", "chain", "staged", "verlet", "block", "skin")})
         except Exception as exc:          # the plain measurement stands on its own
             drift = {"error": repr(exc)}
 
@@ -815,8 +817,8 @@ def main():
                                           "GBps": b_kernel[k] / (kern_ms[k] * 1e-3) / 1e9 if kern_ms[k] > 0 else None}
                               for k in kern_ms},
                   "step_kernels_ms": tim["step_ms"] / n_t,
-                  "untimed_bead_steps_per_s": n * args.steps / wall_graph,
-                  "untimed_ms_per_step": wall_graph * 1e3 / args.steps,
+                  "event_timed_bead_steps_per_s": n * args.steps / wall_events,
+                  "event_timed_ms_per_step": wall_events * 1e3 / args.steps,
                   "rebuild_ms": rebuild_ms, "rebuild_algorithmic_bytes": b_rebuild,
                   "rebuild_implementation_bytes": impl_rebuild,
                   "counts": counts, "stats": st, "state_after_timed": state,

@@ -112,7 +112,7 @@ class FnTuneTrial(ctypes.Structure):
     _fields_ = [("knob", _I), ("value", _I), ("ms_per_step", _c.c_double)]
 
 
-TUNE_KNOBS = ("chunk", "chain", "staged", "verlet", "block")
+TUNE_KNOBS = ("chunk", "chain", "staged", "verlet", "block", "skin")
 
 
 class FnAdjointSpec(ctypes.Structure):

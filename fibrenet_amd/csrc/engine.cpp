@@ -940,6 +940,7 @@ class Engine final : public fn_engine {
             sk[i] = q.skin_radius;
         }
         lam_.put(l, stream_); hlf_.put(h, stream_); hlf2_.put(h2, stream_); rad_.put(r, stream_);
+        build_mass_sets(l, h, h2, r);
         type_.put(ty, stream_); tag_.put(tg, stream_); loc_.put(lc, stream_);
         rad_d_.put(rd, stream_); skin_d_.put(sk, stream_);
         HIPCK(hipStreamSynchronize(stream_));
@@ -1081,6 +1082,33 @@ class Engine final : public fn_engine {
         build_chain_sets(cs, ca);
     }
 
+    // The beads' mass coefficients and radii as a one-byte code per bead into
+    // their distinct sets, when there are at most kMassSets of them (the
+    // chain kernel reads the code in place of four values per bead).
+    void build_mass_sets(const std::vector<T> &l, const std::vector<T> &h, const std::vector<T> &h2,
+                         const std::vector<T> &r)
+    {
+        ms_ok_ = false;
+        std::map<std::tuple<T, T, T, T>, int> key;
+        std::vector<fnk::MassSet<T>> sets;
+        std::vector<unsigned char> code(N_);
+        for (int i = 0; i < N_; ++i) {
+            const auto k = std::make_tuple(l[i], h[i], h2[i], r[i]);
+            auto it = key.find(k);
+            if (it == key.end()) {
+                if ((int)sets.size() == fnk::kMassSets) return;
+                it = key.emplace(k, (int)sets.size()).first;
+                sets.push_back(fnk::MassSet<T>{l[i], h[i], h2[i], r[i]});
+            }
+            code[i] = (unsigned char)it->second;
+        }
+        ms_code_.put(code, stream_);
+        ms_sets_.put(sets, stream_);
+        n_ms_ = (int)sets.size();
+        HIPCK(hipStreamSynchronize(stream_));
+        ms_ok_ = true;
+    }
+
     // The chain layout's bond parameters by bead: a code per bead into the
     // distinct (type, l0, E, k) spring sets and (type, theta0, k) angle sets,
     // the failure stress as a float where every one is exact in float.
@@ -1196,6 +1224,9 @@ class Engine final : public fn_engine {
             a.cs_set = cs_set_.p; a.ca_set = ca_set_.p; a.cs_c2 = cs_c2_.p;
             a.cs_st = cs_st_.p; a.ca_st = ca_st_.p;
             a.sp_sets = sp_sets_.p; a.an_sets = an_sets_.p;
+        }
+        if (ms_ok_ && ens_boxes_.empty()) {
+            a.ms_code = ms_code_.p; a.ms_sets = ms_sets_.p; a.n_ms = n_ms_;
         }
         a.net = ens_net();
         a.boxes = boxes_.p;
@@ -1380,6 +1411,8 @@ class Engine final : public fn_engine {
         const double lmin = std::min(l.box_size.x, std::min(l.box_size.y, l.box_size.z));
         bool keep = want && skin > 0.0;
         if (l.bc_type == 0 && !(reach + skin < 0.5 * lmin)) keep = false;
+        // the filter's cell records hold 21 bits per axis
+        if (std::max(g.cells_x, std::max(g.cells_y, g.cells_z)) > (int)fnk::kCellMask) keep = false;
         if (!keep) skin = 0.0;
         const double r = reach + skin;
         fnk::Stencil st{};
@@ -1433,6 +1466,7 @@ class Engine final : public fn_engine {
     void ensure_scratch()
     {
         so_bead_.ensure(N_);
+        cellbead_.ensure(N_);
         if (scratch_.n < (size_t)N_ * K_) {
             clear_graphs();
             scratch_.alloc((size_t)N_ * K_);
@@ -1476,7 +1510,7 @@ class Engine final : public fn_engine {
     void enqueue_filter(int pc, const fnk::Box &bg, const fnk::Box &bl)
     {
         fnk::FilterArgs f{};
-        f.px = pos_[pc][0].p; f.py = pos_[pc][1].p; f.pz = pos_[pc][2].p;
+        f.cb = cellbead_.p;
         f.n = N_;
         f.grid = bg;
         f.list = bl;
@@ -1484,7 +1518,7 @@ class Engine final : public fn_engine {
         f.pairs = pairs_.p;
         f.n_pairs = (int)h_pairs_.size();
         f.radius = rad_d_.p; f.skin = skin_d_.p;
-        f.list_out = nb_list_.p; f.cnt = nb_cnt_.p; f.key = key_.p; f.stats = stats_.p;
+        f.list_out = nb_list_.p; f.cnt = nb_cnt_.p; f.stats = stats_.p;
         fnk::filter(f, stream_);
     }
 
@@ -1503,7 +1537,8 @@ class Engine final : public fn_engine {
             search_and_compact(bl, true, false, nullptr);
             return;
         }
-        fnk::disp_check(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, bl, lim2_, need_.p, stream_);
+        fnk::disp_check(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, bl, bg, lim2_, need_.p, cellbead_.p,
+                        key_.p, stream_);
         enqueue_sort(pc, bg, need_.p);
         search_and_compact(bl, false, false, need_.p);
         fnk::snapshot(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, need_.p, stream_);
@@ -1876,10 +1911,13 @@ class Engine final : public fn_engine {
         if (!ens) {
             knobs.push_back({FN_TUNE_STAGED, {0, 1}});
             knobs.push_back({FN_TUNE_VERLET, {verlet_, 0, 1}});
+            // the kept candidates' skin (used while verlet keeps them): a thinner
+            // skin filters fewer candidates and rebuilds them more often
+            knobs.push_back({FN_TUNE_SKIN, {skin_permille_, 300, 150, 75}});
         }
         knobs.push_back({FN_TUNE_BLOCK, {block_, 256, 128, 512}});
         knobs.push_back({FN_TUNE_CHUNK, chunks});       // last: its trials are the longest
-        static const char *names[] = {"chunk", "chain", "staged", "verlet", "block"};
+        static const char *names[] = {"chunk", "chain", "staged", "verlet", "block", "skin"};
         hipEvent_t e0 = nullptr, e1 = nullptr;
         HIPCK(hipEventCreate(&e0));
         HIPCK(hipEventCreate(&e1));
@@ -2564,6 +2602,10 @@ class Engine final : public fn_engine {
     DArr<fnk::V3<T>> sp_term_, an_term_;   // bond terms of the current step at the incidence slots
     DArr<int> ch_spring_, ch_angle_;       // chain layout: spring (i, i+1), angle centred on i
     bool cmp_ok_ = false;                  // the chain layout's bond parameters by bead are valid
+    bool ms_ok_ = false;                   // the beads' mass sets are valid
+    int n_ms_ = 0;
+    DArr<unsigned char> ms_code_;          // per bead: its mass set
+    DArr<fnk::MassSet<T>> ms_sets_;
     DArr<unsigned short> cs_set_, ca_set_; // per bead: parameter set of spring (i, i+1), of the angle at i
     DArr<float> cs_c2_;                    // per bead: failure stress of spring (i, i+1)
     DArr<int> cs_st_, ca_st_;              // per bead: the two statuses (copies)
@@ -2598,6 +2640,7 @@ class Engine final : public fn_engine {
     DArr<fnk::Box> boxes_;
     DArr<int> net_;
     DArr<fnk::SortedBead> so_bead_;
+    DArr<fnk::CellBead> cellbead_;   // the filter's positions and cells (kept candidates)
     DArr<int> scratch_;
     double reach2_ = 0.0;
 

@@ -22,9 +22,19 @@ struct Stencil {
     int all[3];
 };
 
+// A bead as the rebuild filter reads it, one 32-byte record per bead written
+// by the displacement check: position and FillGrid's cell coordinates
+// (cx | cy << 21 | cz << 42; -1 outside the grid).
+struct alignas(32) CellBead {
+    double x, y, z;
+    long long cell;
+};
+constexpr int kCellBits = 21;
+constexpr long long kCellMask = (1LL << kCellBits) - 1;
+
 // Operands of the rebuild filter.
 struct FilterArgs {
-    const double *px, *py, *pz;     // positions
+    const CellBead *cb;             // positions and cells (disp_check)
     int n;
     Box grid;                        // FillGrid's settings (cells, cell size)
     Box list;                        // BuildContactList's settings (box, minimum image, cutoff)
@@ -33,7 +43,7 @@ struct FilterArgs {
     const PairC *pairs;
     int n_pairs;
     const double *radius, *skin;
-    int *list_out, *cnt, *key, *stats;
+    int *list_out, *cnt, *stats;
 };
 
 template <typename T, typename P>
@@ -85,9 +95,12 @@ void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *p
 // (stats: counts the build and keeps the largest list total of the run.)
 void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
              int cap, int *cnt_out, int *stats, const int *gate, hipStream_t s);
-// *need = 1 when a bead moved more than sqrt(lim2) since the snapshot.
+// *need = 1 when a bead moved more than sqrt(lim2) since the snapshot; also
+// writes every bead's CellBead and FillGrid's cell index (key, -1 outside
+// the grid) for the filter and the exports.
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
-                const double *rz, int n, const Box &b, double lim2, int *need, hipStream_t s);
+                const double *rz, int n, const Box &b, const Box &grid, double lim2, int *need, CellBead *cb,
+                int *key, hipStream_t s);
 void snapshot(const double *px, const double *py, const double *pz, double *rx, double *ry, double *rz, int n,
               const int *gate, hipStream_t s);
 // The listing rule on the candidates: contact list, counts and cells.

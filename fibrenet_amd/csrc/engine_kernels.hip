@@ -178,16 +178,25 @@ struct BeadState {
     T fpx, fpy, fpz, wx, wy, wz, lam;
     V3<T> vp;
     P px, py, pz;
+    int ms;                 // the bead's mass set (with a mass-set table), else -1
 };
 
 // Loads bead i and returns its base force: the predicted velocity of this
-// step (kernel.cl:200-205) gives damping, plus gravity and walls.
+// step (kernel.cl:200-205) gives damping, plus gravity and walls.  With ms
+// (the mass-set table in LDS), lambda*dt/m comes from the bead's set.
 template <typename T, typename P>
-__device__ __forceinline__ V3<T> bead_begin(const Args<T, P> &a, const Box &bx, int i, BeadState<T, P> &b)
+__device__ __forceinline__ V3<T> bead_begin(const Args<T, P> &a, const Box &bx, int i, BeadState<T, P> &b,
+                                            const MassSet<T> *ms = nullptr)
 {
     b.fpx = a.fx[i]; b.fpy = a.fy[i]; b.fpz = a.fz[i];
     b.wx = a.wx[i]; b.wy = a.wy[i]; b.wz = a.wz[i];
-    b.lam = a.lam[i];
+    if (ms) {
+        b.ms = a.ms_code[i];
+        b.lam = ms[b.ms].lam;
+    } else {
+        b.ms = -1;
+        b.lam = a.lam[i];
+    }
     b.vp.x = tfma(b.fpx, b.lam, b.wx);
     b.vp.y = tfma(b.fpy, b.lam, b.wy);
     b.vp.z = tfma(b.fpz, b.lam, b.wz);
@@ -200,10 +209,10 @@ __device__ __forceinline__ V3<T> bead_begin(const Args<T, P> &a, const Box &bx, 
 // this step for read-back.
 template <typename T, typename P, bool NEXT, bool LAST>
 __device__ __forceinline__ void bead_end(const Args<T, P> &a, const Box &bx, int i, const BeadState<T, P> &b,
-                                         const V3<T> &f, P ix, P iy, P iz)
+                                         const V3<T> &f, P ix, P iy, P iz, const MassSet<T> *ms = nullptr)
 {
     a.gx[i] = f.x; a.gy[i] = f.y; a.gz[i] = f.z;
-    const T h = a.hlf[i];
+    const T h = ms ? ms[b.ms].hlf : a.hlf[i];
     T sx = f.x, sy = f.y, sz = f.z;
     sx += b.fpx;
     sy += b.fpy;
@@ -214,7 +223,7 @@ __device__ __forceinline__ void bead_end(const Args<T, P> &a, const Box &bx, int
     if (NEXT) {
         P px = b.px, py = b.py, pz = b.pz;
         V3<T> vn;
-        drift(bx, f.x, f.y, f.z, nwx, nwy, nwz, b.lam, a.hlf2[i], px, py, pz, ix, iy, iz, vn);
+        drift(bx, f.x, f.y, f.z, nwx, nwy, nwz, b.lam, ms ? ms[b.ms].hlf2 : a.hlf2[i], px, py, pz, ix, iy, iz, vn);
         a.qx[i] = px; a.qy[i] = py; a.qz[i] = pz;
         if (a.jx != a.qx) { a.jx[i] = ix; a.jy[i] = iy; a.jz[i] = iz; }
     }
@@ -273,7 +282,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
 {
     __shared__ PairTab<T> prs[kPairCap];
     __shared__ WaveContactLds<T> cl[BLK / 64];
+    __shared__ MassSet<T> mss[kMassSets];
     for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += BLK) prs[k] = to_tab<T>(a.pairs[k]);
+    const MassSet<T> *ms = nullptr;
+    if (a.ms_code) {                     // block-uniform
+        for (int k = threadIdx.x; k < a.n_ms; k += BLK) mss[k] = a.ms_sets[k];
+        ms = mss;
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63;
     // a wave's beads depend on its index over the whole grid only, so every
@@ -292,9 +307,9 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
     }
     const Box &bx = drift_box<ENS>(a, ic);
     BeadState<T, P> b;
-    V3<T> f = bead_begin(a, bx, ic, b);
+    V3<T> f = bead_begin(a, bx, ic, b, ms);
     const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
-    const T r1 = a.radius[ic];
+    const T r1 = ms ? ms[b.ms].r : a.radius[ic];
     // the chain bonds of bead i: spring (i, i+1) and the angle centred on i,
     // their parameters by bead (parameter sets) or through the bond indices
     bool hs = false, hq = false;
@@ -407,7 +422,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
     }
     contacts_finish<T, P, ENS>(a, ca, e0, cnt, r1, b.px, b.py, b.pz, f, prs, w, ENS ? a.net[ic] : 0);
     if (!own) return;
-    bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, x, y, z);
+    bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, x, y, z, ms);
 }
 
 __global__ void k_add_int(int *p, int v) { *p += v; }
@@ -1133,18 +1148,29 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *c
 // moved).
 __global__ void __launch_bounds__(kBlock) k_disp(const double *px, const double *py, const double *pz,
                                                  const double *rx, const double *ry, const double *rz, int n, Box b,
-                                                 double lim2, int *need)
+                                                 Box g, double lim2, int *need, CellBead *cb, int *key)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     bool moved = false;
     if (i < n) {
-        double dx = px[i] - rx[i], dy = py[i] - ry[i], dz = pz[i] - rz[i];
+        const double x = px[i], y = py[i], z = pz[i];
+        double dx = x - rx[i], dy = y - ry[i], dz = z - rz[i];
         if (b.bc == 0) {
             dx = dx > 0.5 * b.lx ? dx - b.lx : (dx < -0.5 * b.lx ? dx + b.lx : dx);
             dy = dy > 0.5 * b.ly ? dy - b.ly : (dy < -0.5 * b.ly ? dy + b.ly : dy);
             dz = dz > 0.5 * b.lz ? dz - b.lz : (dz < -0.5 * b.lz ? dz + b.lz : dz);
         }
         moved = !(dx * dx + dy * dy + dz * dz <= lim2);
+        // FillGrid's cell (kernel.cl:493-499), once per bead for the filter
+        const int cx = (int)floor(x / g.sx), cy = (int)floor(y / g.sy), cz = (int)floor(z / g.sz);
+        const bool in = cx >= 0 && cx < g.cx && cy >= 0 && cy < g.cy && cz >= 0 && cz < g.cz;
+        CellBead r;
+        r.x = x;
+        r.y = y;
+        r.z = z;
+        r.cell = in ? ((long long)cx | ((long long)cy << kCellBits) | ((long long)cz << (2 * kCellBits))) : -1LL;
+        cb[i] = r;
+        key[i] = in ? cx + cy * g.cx + cz * g.cxy : -1;
     }
     if (__any(moved) && (threadIdx.x & 63) == 0) *need = 1;
 }
@@ -1175,10 +1201,11 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     const bool live = i < f.n;
     const int ic = live ? i : f.n - 1;
     const Box &g = f.grid;
-    const double x = f.px[ic], y = f.py[ic], z = f.pz[ic];
-    const int cx = (int)floor(x / g.sx), cy = (int)floor(y / g.sy), cz = (int)floor(z / g.sz);
-    const bool in = cx >= 0 && cx < g.cx && cy >= 0 && cy < g.cy && cz >= 0 && cz < g.cz;
-    if (live) f.key[i] = in ? cx + cy * g.cx + cz * g.cxy : -1;
+    const CellBead me = f.cb[ic];
+    const double x = me.x, y = me.y, z = me.z;
+    const bool in = me.cell >= 0;
+    const int cx = (int)(me.cell & kCellMask), cy = (int)((me.cell >> kCellBits) & kCellMask),
+              cz = (int)(me.cell >> (2 * kCellBits));
     const int e0 = f.cand_start[ic];
     const int e1 = f.cand_start[ic + 1];
     const int cnt = (live && in && e1 <= f.cap) ? e1 - e0 : 0;   // past the capacity: the host reports the run invalid
@@ -1210,10 +1237,12 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
                 // entries of an overflowed build may be stale: never follow one outside the arrays
                 const bool valid = j < f.n && (ent >> kPidShift) < f.n_pairs;
                 const int jj = valid ? j : 0;
-                const double xj = f.px[jj], yj = f.py[jj], zj = f.pz[jj];
-                const int jx = (int)floor(xj / g.sx), jy = (int)floor(yj / g.sy), jz = (int)floor(zj / g.sz);
+                const CellBead o = f.cb[jj];
+                const double xj = o.x, yj = o.y, zj = o.z;
+                const bool jin = o.cell >= 0;
+                const int jx = (int)(o.cell & kCellMask), jy = (int)((o.cell >> kCellBits) & kCellMask),
+                          jz = (int)(o.cell >> (2 * kCellBits));
                 const int ax = abs(jx - cxL), ay = abs(jy - cyL), az = abs(jz - czL);
-                const bool jin = jx >= 0 && jx < g.cx && jy >= 0 && jy < g.cy && jz >= 0 && jz < g.cz;
                 const bool adj = (ax <= 1 || (per && ax == g.cx - 1)) && (ay <= 1 || (per && ay == g.cy - 1)) &&
                                  (az <= 1 || (per && az == g.cz - 1));
                 if (valid && jin && adj) {
@@ -1505,9 +1534,10 @@ void compact(const int *cnt, const int *cand_start, const int *scratch, int K, i
         k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, scratch, K, n, cand, cap, cnt_out, stats, gate);
 }
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
-                const double *rz, int n, const Box &b, double lim2, int *need, hipStream_t s)
+                const double *rz, int n, const Box &b, const Box &grid, double lim2, int *need, CellBead *cb,
+                int *key, hipStream_t s)
 {
-    k_disp<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, b, lim2, need);
+    k_disp<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, b, grid, lim2, need, cb, key);
 }
 void snapshot(const double *px, const double *py, const double *pz, double *rx, double *ry, double *rz, int n,
               const int *gate, hipStream_t s)

@@ -129,6 +129,14 @@ struct AngleSet {
 };
 constexpr unsigned short kNoSet = 0xFFFF;
 
+// A distinct set of a bead's mass coefficients and radius, as the engine's T
+// holds them: lambda*dt/m, (dt/2)/m, (dt^2/2)/m and r.
+template <typename T>
+struct MassSet {
+    T lam, hlf, hlf2, r;
+};
+constexpr int kMassSets = 256;   // a one-byte code per bead
+
 template <typename T, typename P>
 struct Args {
     int n;                                  // beads
@@ -175,6 +183,11 @@ struct Args {
     int *cs_st, *ca_st;
     const SpringSet *sp_sets;
     const AngleSet *an_sets;
+    // Each bead's mass coefficients and radius as a one-byte code into at
+    // most kMassSets distinct sets (null: read the per-bead arrays).
+    const unsigned char *ms_code;
+    const MassSet<T> *ms_sets;
+    int n_ms;
     // contacts: bead i's partners are nb_list[nb_start[i] .. nb_start[i] + nb_cnt[i])
     const int *nb_start, *nb_cnt, *nb_list;
     const PairC *pairs;

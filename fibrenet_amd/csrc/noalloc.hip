@@ -28,11 +28,8 @@ unsigned char *g_ptr[kMaxDevices] = {};
 }  // namespace
 
 // One copy per GPU, held until the process ends.  It is initialised data
-// (one nonzero byte), so the code object carries it as file bytes: a loaded
-// code object's size then equals its size in the file.  Left uninitialised it
-// was a megabytes-long zero-fill segment behind a code object of about 1.5 KB
-// of file, and rocprofv3's kernel tracing ended in a segmentation fault (a
-// read running off a mapping) while such a code object was loaded.
+// (one nonzero byte), so the code object carries it as file bytes and its
+// loaded size equals its size in the file.
 static __device__ __align__(256) unsigned char scratch[kScratchBytes] = {1};
 
 namespace fnk {

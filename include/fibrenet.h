@@ -213,15 +213,16 @@ int fn_batch_load(fn_engine *eng, int n_networks, const fn_network *nets);
  * Runs n_steps steps exactly as fn_run_steps does (bitwise the same
  * results) while timing the settings that cannot change a result, one knob
  * at a time: the chain bead kernel, the LDS-staged search, kept contact
- * candidates, the bead kernels' block size and the chunk length of the
- * captured graphs (options "chain", "staged", "verlet", "block", "chunk"; in
- * that order; an ensemble tunes "chain", "block" and "chunk").  Each value runs one chunk to
+ * candidates and their skin, the bead kernels' block size and the chunk
+ * length of the captured graphs (options "chain", "staged", "verlet",
+ * "skin", "block", "chunk"; in that order; an ensemble tunes "chain",
+ * "block" and "chunk").  Each value runs one chunk to
  * capture its graphs and one timed chunk; the fastest value of a knob is
  * kept before the next knob is tried, and the chosen settings stay set
  * (fn_get_option reads them).  A run too short for every trial tries what
  * fits.  trials (capacity entries, may be NULL) receives each timing;
  * *n_trials their number. */
-enum { FN_TUNE_CHUNK = 0, FN_TUNE_CHAIN = 1, FN_TUNE_STAGED = 2, FN_TUNE_VERLET = 3, FN_TUNE_BLOCK = 4 };
+enum { FN_TUNE_CHUNK = 0, FN_TUNE_CHAIN = 1, FN_TUNE_STAGED = 2, FN_TUNE_VERLET = 3, FN_TUNE_BLOCK = 4, FN_TUNE_SKIN = 5 };
 typedef struct fn_tune_trial {
     int knob;               /* FN_TUNE_*                                       */
     int value;              /* the option's value during the timed chunk       */

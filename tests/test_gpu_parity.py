@@ -397,17 +397,17 @@ def test_autotune_equals_run_steps(need_gpu, workload):
     net = WORKLOADS[workload]()
     spr, ang = len(net.springs) > 0, len(net.angles) > 0
     a, ia = engine_for(net, "f32")
-    trials, chosen = a.autotune(8000, net.grid_update_freq, spr, ang)   # room for every trial
+    trials, chosen = a.autotune(10000, net.grid_update_freq, spr, ang)   # room for every trial
     a.run_steps(100, net.grid_update_freq, spr, ang)        # continues with the chosen settings
     b, ib = engine_for(net, "f32")
-    b.run_steps(8000, net.grid_update_freq, spr, ang)
+    b.run_steps(10000, net.grid_update_freq, spr, ang)
     b.run_steps(100, net.grid_update_freq, spr, ang)
     assert read_beads(a, ia).tobytes() == read_beads(b, ib).tobytes()
     assert np.array_equal(a.export_breaks(len(net.springs)), b.export_breaks(len(net.springs)))
     assert np.array_equal(a.export_pairs(), b.export_pairs())
     knobs = {t["knob"] for t in trials}
-    assert {"chunk", "staged", "verlet", "block"} <= knobs and all(t["ms_per_step"] > 0 for t in trials)
-    assert a.stats()["steps"] == 8100
+    assert {"chunk", "staged", "verlet", "skin", "block"} <= knobs and all(t["ms_per_step"] > 0 for t in trials)
+    assert a.stats()["steps"] == 10100
 
 
 def test_dense_network_takes_the_long_paths(need_gpu):
