@@ -400,7 +400,6 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
     for (int base = 0; base < total; base += kWaveChunk) {           // wave-uniform
         const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
         mark_owner(off, cnt, base, w.own);
-        wave_sync_lds();
         for (int r = 0; r * 64 < n; ++r) {
             const int k = r * 64 + lane;
             int L;
@@ -608,7 +607,6 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     for (int base = 0; base < total; base += kWaveChunk) {           // wave-uniform
         const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
         mark_owner(off, cnt, base, w.own);
-        wave_sync_lds();
         for (int r = 0; r * 64 < n; ++r) {
             const int k = r * 64 + lane;
             int L;

@@ -1411,8 +1411,10 @@ class Engine final : public fn_engine {
         const double lmin = std::min(l.box_size.x, std::min(l.box_size.y, l.box_size.z));
         bool keep = want && skin > 0.0;
         if (l.bc_type == 0 && !(reach + skin < 0.5 * lmin)) keep = false;
-        // the filter's cell records hold 21 bits per axis
-        if (std::max(g.cells_x, std::max(g.cells_y, g.cells_z)) > (int)fnk::kCellMask) keep = false;
+        // the filter's cell records pack the three cell coordinates into 31 bits
+        const int bx = bits_for(g.cells_x), by = bits_for(g.cells_y), bz = bits_for(g.cells_z);
+        if (bx + by + bz > 31) keep = false;
+        pack_ = fnk::CellPack{bx, bx + by};
         if (!keep) skin = 0.0;
         const double r = reach + skin;
         fnk::Stencil st{};
@@ -1440,6 +1442,14 @@ class Engine final : public fn_engine {
         lim2_ = lim * lim;
         stencil_ = st;
         stencil_exact_ = ex;
+    }
+
+    // Bits that hold 0 .. n - 1 (at least one).
+    static int bits_for(int n)
+    {
+        int b = 1;
+        while (b < 31 && (1LL << b) < (long long)n) ++b;
+        return b;
     }
 
     // verlet = 2: keeping the candidates pays only while most rebuilds find
@@ -1511,6 +1521,18 @@ class Engine final : public fn_engine {
     {
         fnk::FilterArgs f{};
         f.cb = cellbead_.p;
+        f.px = pos_[pc][0].p; f.py = pos_[pc][1].p; f.pz = pos_[pc][2].p;
+        f.pack = pack_;
+        // a bead inside the grid has |coordinate| < cells * step on every
+        // axis; rounding two of them to fp32 moves their difference by at
+        // most 2 * 2^-24 of that
+        const double extent = std::max(bg.cx * bg.sx, std::max(bg.cy * bg.sy, bg.cz * bg.sz));
+        f.delta = std::ldexp(extent, -23) * (1.0 + 1e-9) + 1e-300;
+        f.cutoff = std::sqrt(bl.cutoff_sq);
+        {
+            const double rh = std::sqrt(bl.hbs);
+            f.hbs_guard = bl.hbs - (12.0 * f.delta * (2.0 * rh + f.delta) + 4e-14 * bl.hbs);
+        }
         f.n = N_;
         f.grid = bg;
         f.list = bl;
@@ -1537,8 +1559,8 @@ class Engine final : public fn_engine {
             search_and_compact(bl, true, false, nullptr);
             return;
         }
-        fnk::disp_check(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, bl, bg, lim2_, need_.p, cellbead_.p,
-                        key_.p, stream_);
+        fnk::disp_check(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, bl, bg, pack_, lim2_, need_.p,
+                        cellbead_.p, key_.p, stream_);
         enqueue_sort(pc, bg, need_.p);
         search_and_compact(bl, false, false, need_.p);
         fnk::snapshot(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, need_.p, stream_);
@@ -2640,7 +2662,8 @@ class Engine final : public fn_engine {
     DArr<fnk::Box> boxes_;
     DArr<int> net_;
     DArr<fnk::SortedBead> so_bead_;
-    DArr<fnk::CellBead> cellbead_;   // the filter's positions and cells (kept candidates)
+    DArr<fnk::CellBead> cellbead_;   // the filter's rounded positions and cells (kept candidates)
+    fnk::CellPack pack_{10, 20};       // bit offsets of the cell coordinates in cellbead_
     DArr<int> scratch_;
     double reach2_ = 0.0;
 

@@ -22,19 +22,28 @@ struct Stencil {
     int all[3];
 };
 
-// A bead as the rebuild filter reads it, one 32-byte record per bead written
-// by the displacement check: position and FillGrid's cell coordinates
-// (cx | cy << 21 | cz << 42; -1 outside the grid).
-struct alignas(32) CellBead {
-    double x, y, z;
-    long long cell;
+// A bead as the rebuild filter reads a partner, one 16-byte record per bead
+// written by the displacement check: its position rounded to fp32 and
+// FillGrid's cell coordinates packed as cx | cy << sy | cz << sz (-1 outside
+// the grid).
+struct alignas(16) CellBead {
+    float x, y, z;
+    int cell;
 };
-constexpr int kCellBits = 21;
-constexpr long long kCellMask = (1LL << kCellBits) - 1;
+
+// Bit offsets of cy and cz in CellBead::cell (cx takes sy bits, cy sz - sy).
+struct CellPack {
+    int sy, sz;
+};
 
 // Operands of the rebuild filter.
 struct FilterArgs {
-    const CellBead *cb;             // positions and cells (disp_check)
+    const CellBead *cb;             // rounded positions and cells (disp_check)
+    const double *px, *py, *pz;     // the positions themselves
+    CellPack pack;
+    double delta;                   // bound on the error of a difference of two rounded coordinates
+    double cutoff;                  // sqrt of list.cutoff_sq
+    double hbs_guard;               // below it, no axis of a pair is wrapped, exactly or rounded
     int n;
     Box grid;                        // FillGrid's settings (cells, cell size)
     Box list;                        // BuildContactList's settings (box, minimum image, cutoff)
@@ -99,8 +108,8 @@ void compact(const int *cnt, const int *cand_start, const int *scratch, int K, i
 // writes every bead's CellBead and FillGrid's cell index (key, -1 outside
 // the grid) for the filter and the exports.
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
-                const double *rz, int n, const Box &b, const Box &grid, double lim2, int *need, CellBead *cb,
-                int *key, hipStream_t s);
+                const double *rz, int n, const Box &b, const Box &grid, CellPack pack, double lim2, int *need,
+                CellBead *cb, int *key, hipStream_t s);
 void snapshot(const double *px, const double *py, const double *pz, double *rx, double *ry, double *rz, int n,
               const int *gate, hipStream_t s);
 // The listing rule on the candidates: contact list, counts and cells.

@@ -223,7 +223,8 @@ __device__ __forceinline__ void bead_end(const Args<T, P> &a, const Box &bx, int
     if (NEXT) {
         P px = b.px, py = b.py, pz = b.pz;
         V3<T> vn;
-        drift(bx, f.x, f.y, f.z, nwx, nwy, nwz, b.lam, ms ? ms[b.ms].hlf2 : a.hlf2[i], px, py, pz, ix, iy, iz, vn);
+        drift(bx, f.x, f.y, f.z, nwx, nwy, nwz, ms ? ms[b.ms].lam : b.lam, ms ? ms[b.ms].hlf2 : a.hlf2[i], px, py,
+              pz, ix, iy, iz, vn);
         a.qx[i] = px; a.qy[i] = py; a.qz[i] = pz;
         if (a.jx != a.qx) { a.jx[i] = ix; a.jy[i] = iy; a.jz[i] = iz; }
     }
@@ -422,7 +423,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
     }
     contacts_finish<T, P, ENS>(a, ca, e0, cnt, r1, b.px, b.py, b.pz, f, prs, w, ENS ? a.net[ic] : 0);
     if (!own) return;
-    bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, x, y, z, ms);
+    // the bead's position and image again (cache hits) rather than holding
+    // twelve registers through the contact passes
+    b.px = a.px[i]; b.py = a.py[i]; b.pz = a.pz[i];
+    bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, a.ix[i], a.iy[i], a.iz[i], ms);
 }
 
 __global__ void k_add_int(int *p, int v) { *p += v; }
@@ -1148,7 +1152,8 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *c
 // moved).
 __global__ void __launch_bounds__(kBlock) k_disp(const double *px, const double *py, const double *pz,
                                                  const double *rx, const double *ry, const double *rz, int n, Box b,
-                                                 Box g, double lim2, int *need, CellBead *cb, int *key)
+                                                 Box g, CellPack pack, double lim2, int *need, CellBead *cb,
+                                                 int *key)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     bool moved = false;
@@ -1164,12 +1169,7 @@ __global__ void __launch_bounds__(kBlock) k_disp(const double *px, const double 
         // FillGrid's cell (kernel.cl:493-499), once per bead for the filter
         const int cx = (int)floor(x / g.sx), cy = (int)floor(y / g.sy), cz = (int)floor(z / g.sz);
         const bool in = cx >= 0 && cx < g.cx && cy >= 0 && cy < g.cy && cz >= 0 && cz < g.cz;
-        CellBead r;
-        r.x = x;
-        r.y = y;
-        r.z = z;
-        r.cell = in ? ((long long)cx | ((long long)cy << kCellBits) | ((long long)cz << (2 * kCellBits))) : -1LL;
-        cb[i] = r;
+        cb[i] = CellBead{(float)x, (float)y, (float)z, in ? (cx | (cy << pack.sy) | (cz << pack.sz)) : -1};
         key[i] = in ? cx + cy * g.cx + cz * g.cxy : -1;
     }
     if (__any(moved) && (threadIdx.x & 63) == 0) *need = 1;
@@ -1186,98 +1186,169 @@ __global__ void __launch_bounds__(kBlock) k_snapshot(const double *px, const dou
     rz[i] = pz[i];
 }
 
-// The exact listing rule applied to the candidates: writes FillGrid's cell of
-// every bead to key, bead i's listed partners (in candidate order, so
-// ascending) to list[cand_start[i] ...] and their number to cnt[i].  The
-// candidates of a wave are evaluated round-robin by all its lanes (each lane
-// every 64th entry), the verdicts pass through LDS, and each bead then
-// writes its own accepted entries in order.
+// The exact listing rule applied to the candidates: bead i's listed partners
+// (in candidate order, so ascending) to list[cand_start[i] ...] and their
+// number to cnt[i].  The candidates of a wave are evaluated round-robin by
+// all its lanes (each lane every 64th entry) against the owning bead's
+// record in LDS, the verdicts pass through LDS, a running count over them
+// gives every accepted entry its place in its bead's list, and the wave
+// writes the pass's accepted entries with consecutive lanes on consecutive
+// entries.
+//
+// A partner is read as its 16-byte CellBead.  With d the rounded coordinate
+// differences, |d - exact| <= delta per axis, the squared distance len from
+// them differs from the fp64 one by at most 2 sqrt(3) delta sqrt(len_exact)
+// + 3 delta^2 (+ its own rounding).  So, for a threshold T = root_t^2 with
+// root_t >= 2 delta: len > 4T means beyond T exactly too, and otherwise a
+// pair more than 12 delta (2 root_t + delta) + 4e-14 T from T is decided by
+// len; the few within that band (and, in a periodic box, pairs that may wrap)
+// are decided by the fp64 positions.  The verdicts are those of the fp64
+// rule.
+
+// A bead of a wave as the filter reads it: position, cell, radius and skin,
+// eb = cand_start - (its first entry in the wave's entries), so that entry k
+// of the wave is candidate eb + k, and ob, which places the pass's accepted
+// entries: the one with p accepted entries of the pass before it goes to
+// list[ob + p].
+struct alignas(16) FilterOwner {
+    double x, y, z;
+    int cx, cy, cz, eb;
+    double r, s;
+    int ob, pad;
+};
+
+__device__ __forceinline__ bool needs_fp64(double len, double t, double root_t, double delta)
+{
+    if (root_t < 2.0 * delta) return true;
+    return len <= 4.0 * t && fabs(len - t) <= 12.0 * delta * (2.0 * root_t + delta) + 4e-14 * t;
+}
+
 __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
 {
-    __shared__ int verdict[kBlock / 64][kWaveChunk];
-    int *acc = verdict[threadIdx.x >> 6];
+    __shared__ __align__(16) int verdict[kBlock / 64][kWaveChunk];
+    __shared__ __align__(16) int before[kBlock / 64][kWaveChunk + 4];   // accepted entries of the pass before each slot
+    __shared__ unsigned owner_words[kBlock / 64][kWaveChunk / 4];
+    __shared__ FilterOwner fo[kBlock];
+    const int wv = threadIdx.x >> 6;
+    int *acc = verdict[wv];
+    int *pre = before[wv];
+    unsigned char *own = reinterpret_cast<unsigned char *>(owner_words[wv]);
+    FilterOwner *ow = fo + wv * 64;
     const int lane = threadIdx.x & 63;
     const int i = xcd_block() * kBlock + threadIdx.x;
     const bool live = i < f.n;
     const int ic = live ? i : f.n - 1;
     const Box &g = f.grid;
-    const CellBead me = f.cb[ic];
-    const double x = me.x, y = me.y, z = me.z;
-    const bool in = me.cell >= 0;
-    const int cx = (int)(me.cell & kCellMask), cy = (int)((me.cell >> kCellBits) & kCellMask),
-              cz = (int)(me.cell >> (2 * kCellBits));
+    const int mcell = f.cb[ic].cell;
+    const bool in = mcell >= 0;
     const int e0 = f.cand_start[ic];
     const int e1 = f.cand_start[ic + 1];
     const int cnt = (live && in && e1 <= f.cap) ? e1 - e0 : 0;   // past the capacity: the host reports the run invalid
-    const double ri = f.radius[ic], si = f.skin[ic];
     int incl = cnt;
     incl = wave_incl_scan(incl);
     const int off = incl - cnt;
     const int total = __shfl(incl, 63, 64);
-    const Box &b = f.list;
-    const bool per = b.bc == 0;
+    const int my = (1 << (f.pack.sz - f.pack.sy)) - 1, mx = (1 << f.pack.sy) - 1;
     int kept = 0;
-    for (int base = 0; base < total; base += kWaveChunk) {
-        for (int r = 0; r < kWaveChunk; r += 64) {
-            const int k = base + r + lane;
-            int L = 0;
+    if (total > 0) {                                     // wave-uniform
+        fo[threadIdx.x] = FilterOwner{f.px[ic], f.py[ic], f.pz[ic], mcell & mx, (mcell >> f.pack.sy) & my,
+                                      mcell >> f.pack.sz, e0 - off, f.radius[ic], f.skin[ic], 0, 0};
+        const Box &b = f.list;
+        const bool per = b.bc == 0;
+        for (int base = 0; base < total; base += kWaveChunk) {
+            mark_owner(off, cnt, base, own);             // also publishes the records
+            const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
+            // every candidate of the pass, then every partner record, before
+            // any verdict: the loads of the pass are in flight together
+            int ent[kChunkRounds];
 #pragma unroll
-            for (int step = 32; step > 0; step >>= 1) {
-                const int cand = L + step;
-                if (cand < 64 && __shfl(off, cand & 63, 64) <= k) L = cand;
+            for (int r = 0; r < kChunkRounds; ++r) {
+                const int k = r * 64 + lane;
+                ent[r] = k < n ? f.cand[ow[own[k]].eb + base + k] : 0;
             }
-            const int e = __shfl(e0, L, 64) + (k - __shfl(off, L, 64));
-            const double xL = __shfl(x, L, 64), yL = __shfl(y, L, 64), zL = __shfl(z, L, 64);
-            const int cxL = __shfl(cx, L, 64), cyL = __shfl(cy, L, 64), czL = __shfl(cz, L, 64);
-            const double riL = __shfl(ri, L, 64), siL = __shfl(si, L, 64);
-            int v = -1;
-            if (k < total) {
-                const int ent = f.cand[e];
-                const int j = ent & kIdxMask;
+            CellBead q[kChunkRounds];
+#pragma unroll
+            for (int r = 0; r < kChunkRounds; ++r) {
+                const int j = ent[r] & kIdxMask;
                 // entries of an overflowed build may be stale: never follow one outside the arrays
-                const bool valid = j < f.n && (ent >> kPidShift) < f.n_pairs;
-                const int jj = valid ? j : 0;
-                const CellBead o = f.cb[jj];
-                const double xj = o.x, yj = o.y, zj = o.z;
-                const bool jin = o.cell >= 0;
-                const int jx = (int)(o.cell & kCellMask), jy = (int)((o.cell >> kCellBits) & kCellMask),
-                          jz = (int)(o.cell >> (2 * kCellBits));
-                const int ax = abs(jx - cxL), ay = abs(jy - cyL), az = abs(jz - czL);
-                const bool adj = (ax <= 1 || (per && ax == g.cx - 1)) && (ay <= 1 || (per && ay == g.cy - 1)) &&
-                                 (az <= 1 || (per && az == g.cz - 1));
-                if (valid && jin && adj) {
-                    const double len = list_len2(xL - xj, yL - yj, zL - zj, b);
-                    const int pt = f.pairs[ent >> kPidShift].type;
-                    bool take = false;
-                    if (pt == 1 || pt == 3) take = len < b.cutoff_sq;
-                    if (pt == 2) {                         // kernel.cl:632-659
-                        const double rj = f.radius[j], sj = f.skin[j];
-                        const double min_skin = (siL > sj) ? sj : siL;
-                        const double max_r = (riL > rj) ? riL : rj;
-                        const double th = max_r + min_skin;
-                        take = len < th * th;
-                    }
-                    if (take) v = ent;
-                }
+                const bool valid = r * 64 + lane < n && j < f.n && (unsigned)(ent[r] >> kPidShift) < (unsigned)f.n_pairs;
+                if (!valid) ent[r] = -1;
+                q[r] = f.cb[valid ? j : 0];
             }
-            acc[r + lane] = v;
+#pragma unroll
+            for (int r = 0; r < kChunkRounds; ++r) {
+                if (r * 64 >= n) break;                  // wave-uniform
+                int v = -1;
+                if (ent[r] >= 0 && q[r].cell >= 0) {
+                    const FilterOwner &o = ow[own[r * 64 + lane]];
+                    const int j = ent[r] & kIdxMask;
+                    const int qc = q[r].cell;
+                    const int ax = abs((qc & mx) - o.cx), ay = abs(((qc >> f.pack.sy) & my) - o.cy),
+                              az = abs((qc >> f.pack.sz) - o.cz);
+                    const bool adj = (ax <= 1 || (per && ax == g.cx - 1)) && (ay <= 1 || (per && ay == g.cy - 1)) &&
+                                     (az <= 1 || (per && az == g.cz - 1));
+                    if (adj) {
+                        const int pt = f.pairs[ent[r] >> kPidShift].type;
+                        double t = -1.0, root_t = 0.0;
+                        if (pt == 1 || pt == 3) {
+                            t = b.cutoff_sq;
+                            root_t = f.cutoff;
+                        } else if (pt == 2) {            // kernel.cl:632-659
+                            const double rj = f.radius[j], sj = f.skin[j];
+                            const double min_skin = (o.s > sj) ? sj : o.s;
+                            const double max_r = (o.r > rj) ? o.r : rj;
+                            root_t = max_r + min_skin;
+                            t = root_t * root_t;
+                        }
+                        if (t >= 0.0) {
+                            const double dx = (double)(float)o.x - (double)q[r].x;
+                            const double dy = (double)(float)o.y - (double)q[r].y;
+                            const double dz = (double)(float)o.z - (double)q[r].z;
+                            const double len = dx * dx + dy * dy + dz * dz;
+                            bool take = len < t;
+                            if (needs_fp64(len, t, root_t, f.delta) || (per && !(len < f.hbs_guard))) {
+                                const double le = list_len2(o.x - f.px[j], o.y - f.py[j], o.z - f.pz[j], b);
+                                take = le < t;
+                            }
+                            if (take) v = ent[r];
+                        }
+                    }
+                }
+                acc[r * 64 + lane] = v;
+            }
+            wave_sync_lds();
+            // accepted entries before each slot: a lane's four slots, then
+            // the lanes before it
+            const int4 v4 = *reinterpret_cast<const int4 *>(acc + 4 * lane);
+            const int t0 = 4 * lane;
+            const int a0 = (t0 < n && v4.x >= 0) ? 1 : 0, a1 = (t0 + 1 < n && v4.y >= 0) ? 1 : 0;
+            const int a2 = (t0 + 2 < n && v4.z >= 0) ? 1 : 0, a3 = (t0 + 3 < n && v4.w >= 0) ? 1 : 0;
+            const int sum4 = a0 + a1 + a2 + a3;
+            const int incl4 = wave_incl_scan(sum4);
+            const int ex = incl4 - sum4;
+            *reinterpret_cast<int4 *>(pre + t0) = make_int4(ex, ex + a0, ex + a0 + a1, ex + a0 + a1 + a2);
+            if (lane == 63) pre[kWaveChunk] = incl4;
+            wave_sync_lds();
+            // this bead's place for the pass and its count so far
+            {
+                const int s0 = min(max(off - base, 0), n), s1 = min(max(off + cnt - base, 0), n);
+                ow[lane].ob = e0 + kept - pre[s0];
+                kept += pre[s1] - pre[s0];
+            }
+            wave_sync_lds();
+#pragma unroll
+            for (int r = 0; r < kChunkRounds; ++r) {
+                if (r * 64 >= n) break;                  // wave-uniform
+                const int t = r * 64 + lane;
+                const int v = acc[t];
+                if (t < n && v >= 0) f.list_out[ow[own[t]].ob + pre[t]] = v;
+            }
+            wave_sync_lds();
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int k0 = off > base ? off : base;
-        const int k1 = (off + cnt < base + kWaveChunk) ? off + cnt : base + kWaveChunk;
-        for (int k = k0; k < k1; ++k) {
-            const int v = acc[k - base];
-            if (v >= 0) f.list_out[e0 + kept++] = v;
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
     if (live) f.cnt[i] = kept;
-    const int mx = wave_max(kept);
-    if (lane == 0) raise_max(&f.stats[kStatMaxList], mx);
+    const int mxk = wave_max(kept);
+    if (lane == 0) raise_max(&f.stats[kStatMaxList], mxk);
 }
 
 // ---- device-wide exclusive scan (n ints -> n+1 offsets) ----------------------------
@@ -1534,10 +1605,10 @@ void compact(const int *cnt, const int *cand_start, const int *scratch, int K, i
         k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, scratch, K, n, cand, cap, cnt_out, stats, gate);
 }
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
-                const double *rz, int n, const Box &b, const Box &grid, double lim2, int *need, CellBead *cb,
-                int *key, hipStream_t s)
+                const double *rz, int n, const Box &b, const Box &grid, CellPack pack, double lim2, int *need,
+                CellBead *cb, int *key, hipStream_t s)
 {
-    k_disp<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, b, grid, lim2, need, cb, key);
+    k_disp<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, b, grid, pack, lim2, need, cb, key);
 }
 void snapshot(const double *px, const double *py, const double *pz, double *rx, double *ry, double *rz, int n,
               const int *gate, hipStream_t s)

@@ -703,6 +703,19 @@ __device__ __forceinline__ int wave_incl_scan(int v)
     return v;
 }
 
+// Inclusive maximum of v over the 64 lanes (v >= 0), by the same DPP steps
+// as wave_incl_scan.  All 64 lanes must be active.
+__device__ __forceinline__ int wave_incl_max(int v)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false));   // row_shr:1
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false));   // row_shr:2
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false));   // row_shr:4
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false));   // row_shr:8
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false));   // row_bcast:15 into rows 1 and 3
+    v = max(v, __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false));   // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 __device__ __forceinline__ void wave_sync_lds()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -724,13 +737,38 @@ __device__ __forceinline__ int owner_entry(int k, bool ok, int e0, int off, cons
     return __shfl(e0, L, 64) + (k - __shfl(off, L, 64));
 }
 
-// Marks in `own` the entries of [base, base + kWaveChunk) that this lane owns.
+// Writes to `own` (kWaveChunk bytes, 4-byte aligned, in LDS) the lane that
+// owns each entry of [base, base + kWaveChunk), where lane L owns entries
+// [off_L, off_L + cnt_L) and the offsets are the exclusive scan of the counts
+// (every lane of the wave calls it).  Each lane with entries marks the slot
+// its entries start at with L + 1, and a running maximum over the slots (a
+// lane's four slots, then the lanes by DPP) carries each mark to the slots
+// after it; the owner of slot base itself, when its entries start before
+// base, is the last lane that starts at or before base.  A fixed number of
+// steps, whatever the counts.  Ends with the map visible to the wave.
 __device__ __forceinline__ void mark_owner(int off, int cnt, int base, unsigned char *own)
 {
     const int lane = threadIdx.x & 63;
-    const int k0 = off > base ? off : base;
-    const int k1 = (off + cnt < base + kWaveChunk) ? off + cnt : base + kWaveChunk;
-    for (int k = k0; k < k1; ++k) own[k - base] = (unsigned char)lane;
+    unsigned *words = reinterpret_cast<unsigned *>(own);
+    const unsigned long long before = __ballot(cnt > 0 && off <= base);
+    const int first = before ? 64 - __clzll(before) : 0;       // its lane + 1 (0: none)
+    words[lane] = 0u;
+    wave_sync_lds();
+    const int s = off - base;
+    if (cnt > 0 && s > 0 && s < kWaveChunk) own[s] = (unsigned char)(lane + 1);
+    wave_sync_lds();
+    const unsigned q = words[lane];                              // slots 4 lane .. 4 lane + 3
+    const int m0 = (int)(q & 255u);
+    const int m1 = max(m0, (int)((q >> 8) & 255u));
+    const int m2 = max(m1, (int)((q >> 16) & 255u));
+    const int m3 = max(m2, (int)(q >> 24));
+    int carry = lane_prev(wave_incl_max(m3));                    // the lanes before
+    if (lane == 0) carry = 0;
+    carry = max(carry, first);
+    const unsigned o0 = (unsigned)(max(carry, m0) - 1) & 255u, o1 = (unsigned)(max(carry, m1) - 1) & 255u;
+    const unsigned o2 = (unsigned)(max(carry, m2) - 1) & 255u, o3 = (unsigned)(max(carry, m3) - 1) & 255u;
+    words[lane] = o0 | (o1 << 8) | (o2 << 16) | (o3 << 24);
+    wave_sync_lds();
 }
 
 // Starts a wave's contacts (every lane of the wave calls it; lane L's bead
@@ -748,7 +786,6 @@ __device__ __forceinline__ ContactAhead contacts_ahead(const Args<T, P> &a, int 
     c.off = incl - cnt;
     c.total = __shfl(incl, 63, 64);
     mark_owner(c.off, cnt, 0, w.own);
-    wave_sync_lds();
     const int n0 = c.total < kWaveChunk ? c.total : kWaveChunk;
 #pragma unroll
     for (int r = 0; r < kChunkRounds; ++r) {
@@ -854,7 +891,6 @@ __device__ __forceinline__ void contacts_finish(const Args<T, P> &a, const Conta
     for (int base = kWaveChunk; base < c.total; base += kWaveChunk) {
         const int n = (c.total - base) < kWaveChunk ? c.total - base : kWaveChunk;
         mark_owner(c.off, cnt, base, w.own);
-        wave_sync_lds();
         int ent[kChunkRounds];
 #pragma unroll
         for (int r = 0; r < kChunkRounds; ++r) {

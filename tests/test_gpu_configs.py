@@ -175,6 +175,8 @@ def test_config2_ensemble_of_512_equals_plain_runs(need_gpu):
             eng.run_steps(n, guf, True, True)
         assert ens.read(k, "bead").tobytes() == read_beads(eng, ids).tobytes(), k
         eng.close()
+        if k % 64 == 63:
+            print("config 2: %d networks equal" % (k + 1), flush=True)
     ens.close()
 
 
@@ -209,6 +211,8 @@ def test_config4_batched_adjoint_of_128_equals_single_networks(need_gpu):
             o0, o1 = off[key][k], off[key][k + 1]
             assert np.allclose(g[key][o0:o1], gk[key], rtol=1e-10, atol=1e-12 * scale[key]), (k, key)
         eng.close()
+        if k % 32 == 31:
+            print("config 4: %d networks equal" % (k + 1), flush=True)
     ens.close()
 
 

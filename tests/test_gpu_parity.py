@@ -18,7 +18,7 @@ import copy
 import numpy as np
 import pytest
 
-from fibrenet_amd import records as R
+from fibrenet_amd import generate as G, records as R
 from helpers import (crosslinked, dense_lj, engine_for, field_errors, oracle_for, read_beads, run_engine,
                      small_periodic, small_tensile)
 
@@ -302,6 +302,65 @@ def test_candidate_list_is_exact(need_gpu, workload, precision):
     print(workload, precision, "candidate builds", builds, "of", rebuilds, "rebuilds")
     assert 1 <= builds < rebuilds
     assert b.stats()["candidate_builds"] >= 1
+
+
+def _threshold_pairs(etas, seed=7):
+    """Two-bead fibres far out in the 1000 x 1000 x 50 tensile box, in pairs
+    whose first beads (and second beads) lie at squared distance
+    cutoff_sq + eta: at x near 990 an fp32-rounded coordinate is off by up to
+    3e-5, enough to put such pairs on the wrong side of the cutoff."""
+    net = G.tensile_lj(n_fibres=2 * len(etas), n_beads=2, box=(1000.0, 1000.0, 50.0), seed=seed, d_min=0.6)
+    s = np.array(net.settings, R.SETTINGS).reshape(())
+    rs = np.random.Generator(np.random.PCG64(seed))
+    pos = []
+    for k, eta in enumerate(etas):
+        a = np.array([990.0 + rs.uniform(0.0, 0.1), 20.0 + 8.0 * k + rs.uniform(0.0, 0.1), 25.0 + rs.uniform(0.0, 0.1)])
+        u = rs.normal(size=3)
+        u[2] = 0.0
+        u /= np.linalg.norm(u)
+        b = a + np.sqrt(float(s["cutoff_sq"]) + eta) * u
+        up = np.array([0.0, 0.0, 1.0])
+        pos += [a, a + up, b, b + up]
+    pos = np.array(pos)
+    R.set_vec(net.beads, "position", pos)
+    R.set_vec(net.beads, "image_position", pos)
+    R.set_vec(net.beads, "velocity", np.zeros_like(pos))
+    R.set_vec(net.beads, "velocity_old", np.zeros_like(pos))
+    net.info["type"] = 0
+    return net, pos, float(s["cutoff_sq"])
+
+
+def test_filter_decides_pairs_at_the_cutoff_in_fp64(need_gpu):
+    """Kept candidates are filtered on fp32-rounded positions with an fp64
+    recheck near the threshold: pairs within 1e-3 of the cutoff, where
+    rounding alone misjudges some, are listed exactly as the one-pass fp64
+    rebuild lists them."""
+    etas = [s * e for e in (1e-3, 3e-4, 1e-4, 3e-5, 1e-5, 1e-6, 1e-8, 1e-10) for s in (-1.0, 1.0)] + [0.0]
+    net, pos, cut2 = _threshold_pairs(etas)
+    # rounding to fp32 flips at least one of these decisions
+    p32 = pos.astype(np.float32).astype(np.float64)
+    flips = 0
+    for k in range(len(etas)):
+        for o in (0, 1):
+            i, j = 4 * k + o, 4 * k + 2 + o
+            exact = np.sum((pos[i] - pos[j]) ** 2) < cut2
+            rounded = np.sum((p32[i] - p32[j]) ** 2) < cut2
+            flips += exact != rounded
+    assert flips >= 1
+    for precision in ("f64", "f32"):
+        a, _ = engine_for(net, precision)
+        b, _ = engine_for(net, precision)
+        a.set_option("verlet", 1)
+        b.set_option("verlet", 0)
+        run_engine(a, net, 1, fast=True)
+        run_engine(b, net, 1, fast=True)
+        pa, pb = a.export_pairs(), b.export_pairs()
+        assert len(pb) > 0 and np.array_equal(pa, pb), precision
+        listed = {tuple(sorted(p)) for p in pb.tolist()}
+        for k in range(len(etas)):
+            for o in (0, 1):
+                i, j = 4 * k + o, 4 * k + 2 + o
+                assert ((i, j) in listed) == bool(np.sum((pos[i] - pos[j]) ** 2) < cut2), (precision, etas[k], o)
 
 
 def _ensemble_members():
