@@ -822,6 +822,7 @@ def main():
                   "rebuild_ms": rebuild_ms, "rebuild_algorithmic_bytes": b_rebuild,
                   "rebuild_implementation_bytes": impl_rebuild,
                   "counts": counts, "stats": st, "state_after_timed": state,
+                  "graphs": eng.get_option("graphs"),
                   "generate_s": gen_s, "relax_s": relax_s,
                   "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
                   "gpu_over_cpu_1core": (value / cpu["one_core"]["value"]) if cpu else None, "copy_bytes": copy,

@@ -247,6 +247,8 @@ class Engine final : public fn_engine {
         fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
         for (int k = 0; k < K_COUNT; ++k) kernels_[kKernelNames[k]] = KernelRec();
         for (auto &r : role_) r = -1;
+        const char *g = std::getenv("FIBRENET_GRAPHS");
+        if (g && std::string(g) == "0") graphs_on_ = false;
     }
     ~Engine() override
     {
@@ -1109,52 +1111,51 @@ class Engine final : public fn_engine {
         ms_ok_ = true;
     }
 
-    // The chain layout's bond parameters by bead: a code per bead into the
-    // distinct (type, l0, E, k) spring sets and (type, theta0, k) angle sets,
-    // the failure stress as a float where every one is exact in float.
+    // The chain layout's bond parameters by bead: one word per bead with the
+    // codes of its spring's distinct (type, l0, E, failure stress, k) set and
+    // its angle's (type, theta0, k) set, when each kind has at most kNoSet
+    // sets; the status bits are filled by chain_status before every chunk.
     void build_chain_sets(const std::vector<int> &cs, const std::vector<int> &ca)
     {
         cmp_ok_ = false;
         if (!chain_ok_) return;
         const int n = N_;
-        std::map<std::tuple<int, double, double, double>, int> sk;
+        std::map<std::tuple<int, double, double, double, double>, int> sk;
         std::map<std::tuple<int, double, double>, int> ak;
         std::vector<fnk::SpringSet> ss;
         std::vector<fnk::AngleSet> as;
-        std::vector<unsigned short> scode(n, fnk::kNoSet), acode(n, fnk::kNoSet);
-        std::vector<float> c2(n, 0.0f);
+        std::vector<unsigned> code(n, fnk::kNoSet | (fnk::kNoSet << 16));
         for (int i = 0; i < n; ++i) {
+            unsigned sc = fnk::kNoSet, ac = fnk::kNoSet;
             if (cs[i] >= 0) {
                 const fn_spring &q = h_springs_[cs[i]];
-                const auto key = std::make_tuple(q.type, q.rest_length, q.c1, (double)(T)q.stiffness);
+                if (q.status != 0 && q.status != 1) return;   // a status bit holds 0 or 1 only
+                const auto key = std::make_tuple(q.type, q.rest_length, q.c1, q.c2, (double)(T)q.stiffness);
                 auto it = sk.find(key);
                 if (it == sk.end()) {
+                    if ((unsigned)ss.size() >= fnk::kNoSet) return;
                     it = sk.emplace(key, (int)ss.size()).first;
-                    ss.push_back(fnk::SpringSet{q.type, q.rest_length, q.c1, (double)(T)q.stiffness});
+                    ss.push_back(fnk::SpringSet{q.type, q.rest_length, q.c1, q.c2, (double)(T)q.stiffness});
                 }
-                if ((double)(float)q.c2 != q.c2 || it->second >= fnk::kNoSet) return;
-                scode[i] = (unsigned short)it->second;
-                c2[i] = (float)q.c2;
+                sc = (unsigned)it->second;
             }
             if (ca[i] >= 0) {
                 const fn_angle &g = h_angles_[ca[i]];
+                if (g.status != 0 && g.status != 1) return;
                 const auto key = std::make_tuple(g.type, (double)(T)g.angle, (double)(T)g.c1);
                 auto it = ak.find(key);
                 if (it == ak.end()) {
+                    if ((unsigned)as.size() >= fnk::kNoSet) return;
                     it = ak.emplace(key, (int)as.size()).first;
                     as.push_back(fnk::AngleSet{g.type, (double)(T)g.angle, (double)(T)g.c1});
                 }
-                if (it->second >= fnk::kNoSet) return;
-                acode[i] = (unsigned short)it->second;
+                ac = (unsigned)it->second;
             }
+            code[i] = sc | (ac << 16);
         }
-        cs_set_.put(scode, stream_);
-        ca_set_.put(acode, stream_);
-        cs_c2_.put(c2, stream_);
+        bond_code_.put(code, stream_);
         sp_sets_.put(ss.empty() ? std::vector<fnk::SpringSet>(1) : ss, stream_);
         an_sets_.put(as.empty() ? std::vector<fnk::AngleSet>(1) : as, stream_);
-        cs_st_.ensure(n);
-        ca_st_.ensure(n);
         HIPCK(hipStreamSynchronize(stream_));
         cmp_ok_ = true;
     }
@@ -1221,8 +1222,7 @@ class Engine final : public fn_engine {
         a.sp_slot = sp_slot_.p; a.an_slot = an_slot_.p;
         a.ch_spring = ch_spring_.p; a.ch_angle = ch_angle_.p;
         if (chain_ok_ && cmp_ok_) {
-            a.cs_set = cs_set_.p; a.ca_set = ca_set_.p; a.cs_c2 = cs_c2_.p;
-            a.cs_st = cs_st_.p; a.ca_st = ca_st_.p;
+            a.bond_code = bond_code_.p;
             a.sp_sets = sp_sets_.p; a.an_sets = an_sets_.p;
         }
         if (ms_ok_ && ens_boxes_.empty()) {
@@ -1630,7 +1630,7 @@ class Engine final : public fn_engine {
         int pc = pc_, fc = fc_;
         const bool chain = use_chain();
         if (chain && cmp_ok_)        // the chain kernel's status copies, whatever ran before
-            fnk::chain_status(ch_spring_.p, ch_angle_.p, sp_status_.p, an_status_.p, cs_st_.p, ca_st_.p, N_, stream_);
+            fnk::chain_status(ch_spring_.p, ch_angle_.p, sp_status_.p, an_status_.p, bond_code_.p, N_, stream_);
         if (first) {
             fnk::Args<T, P> a = args_base();
             a.drift = bd;
@@ -2628,9 +2628,7 @@ class Engine final : public fn_engine {
     int n_ms_ = 0;
     DArr<unsigned char> ms_code_;          // per bead: its mass set
     DArr<fnk::MassSet<T>> ms_sets_;
-    DArr<unsigned short> cs_set_, ca_set_; // per bead: parameter set of spring (i, i+1), of the angle at i
-    DArr<float> cs_c2_;                    // per bead: failure stress of spring (i, i+1)
-    DArr<int> cs_st_, ca_st_;              // per bead: the two statuses (copies)
+    DArr<unsigned> bond_code_;             // per bead: the sets and statuses of spring (i, i+1) and angle at i
     DArr<fnk::SpringSet> sp_sets_;
     DArr<fnk::AngleSet> an_sets_;
     AdjRun adj_;                           // the last adjoint forward run

@@ -193,9 +193,10 @@ struct ScratchWorkspace {
 hipError_t scratch_workspace(int device, ScratchWorkspace *w);
 size_t scratch_bytes();
 
-// cs_st[i] = sp_status[ch_spring[i]], ca_st[i] = an_status[ch_angle[i]] (0 without a bond).
-void chain_status(const int *ch_spring, const int *ch_angle, const int *sp_status, const int *an_status, int *cs_st,
-                  int *ca_st, int n, hipStream_t s);
+// The status bits of bond_code[i] from sp_status[ch_spring[i]] and
+// an_status[ch_angle[i]] (off without a bond).
+void chain_status(const int *ch_spring, const int *ch_angle, const int *sp_status, const int *an_status,
+                  unsigned *bond_code, int n, hipStream_t s);
 void fill_int(int *p, int v, int n, hipStream_t s);
 void fill_gated(int *p, int v, int n, const int *gate, hipStream_t s);
 void set_int(int *p, int v, const int *gate, hipStream_t s);

@@ -243,6 +243,7 @@ __global__ void __launch_bounds__(BLK) k_bead_step(Args<T, P> a)
     const int i = xcd_block() * BLK + threadIdx.x;
     for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += BLK) prs[k] = to_tab<T>(a.pairs[k]);
     __syncthreads();
+    const bool lj_uni = !ENS && lj_uniform_wave(prs, min(min(a.n_pairs, kPairCap), a.force.types * (a.force.types + 1) / 2));
     const bool live = i < a.n;
     const int ic = live ? i : a.n - 1;      // lanes past the end follow along for the wave-wide contacts
     const int e0 = a.nb_start[ic], cnt = live ? a.nb_cnt[ic] : 0;
@@ -254,7 +255,7 @@ __global__ void __launch_bounds__(BLK) k_bead_step(Args<T, P> a)
     V3<T> f = bead_begin(a, bx, ic, b);
     if (SPR) add_terms(a.sp_term, a.sp_start[ic], a.sp_start[ic + 1], f);
     if (ANG) add_terms(a.an_term, a.an_start[ic], a.an_start[ic + 1], f);
-    contacts_finish<T, P, ENS>(a, ca, e0, cnt, r1, b.px, b.py, b.pz, f, prs, w, ENS ? a.net[ic] : 0);
+    contacts_finish<T, P, ENS>(a, ca, e0, cnt, r1, b.px, b.py, b.pz, f, prs, w, ENS ? a.net[ic] : 0, lj_uni);
     if (!live) return;
     if (NEXT) bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, a.ix[i], a.iy[i], a.iz[i]);
     else bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, (P)0, (P)0, (P)0);
@@ -283,7 +284,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
 {
     __shared__ PairTab<T> prs[kPairCap];
     __shared__ WaveContactLds<T> cl[BLK / 64];
-    __shared__ MassSet<T> mss[kMassSets];
+    // the mass sets in the launch's dynamic LDS, a.n_ms of them (sized at
+    // launch, so that a network with few sets keeps its LDS small)
+    extern __shared__ __align__(16) unsigned char chain_dyn_lds[];
+    MassSet<T> *mss = reinterpret_cast<MassSet<T> *>(chain_dyn_lds);
     for (int k = threadIdx.x; k < min(a.n_pairs, kPairCap); k += BLK) prs[k] = to_tab<T>(a.pairs[k]);
     const MassSet<T> *ms = nullptr;
     if (a.ms_code) {                     // block-uniform
@@ -291,6 +295,7 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
         ms = mss;
     }
     __syncthreads();
+    const bool lj_uni = !ENS && lj_uniform_wave(prs, min(min(a.n_pairs, kPairCap), a.force.types * (a.force.types + 1) / 2));
     const int lane = threadIdx.x & 63;
     // a wave's beads depend on its index over the whole grid only, so every
     // block size covers each bead with the same wave and the same lanes
@@ -299,9 +304,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
     const bool own = in && lane >= kHalo && lane < 64 - kHalo;
     const int ic = in ? i : (i < 0 ? 0 : a.n - 1);
     // every load that depends on nothing else first: the bead's image, state,
-    // list offsets, and the indices of its chain bonds
+    // list offsets, and the indices of its chain bonds; where one array holds
+    // both positions and images (a finite box), the image is the position
+    const bool alias = a.ix == a.px;     // grid-uniform
     P x = (P)0, y = (P)0, z = (P)0;
-    if (in) {
+    if (in && !alias) {
         x = a.ix[i];
         y = a.iy[i];
         z = a.iz[i];
@@ -309,6 +316,11 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
     const Box &bx = drift_box<ENS>(a, ic);
     BeadState<T, P> b;
     V3<T> f = bead_begin(a, bx, ic, b, ms);
+    if (in && alias) {
+        x = b.px;
+        y = b.py;
+        z = b.pz;
+    }
     const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
     const T r1 = ms ? ms[b.ms].r : a.radius[ic];
     // the chain bonds of bead i: spring (i, i+1) and the angle centred on i,
@@ -317,21 +329,23 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
     int sr = -1, q = -1;
     SpringP<T> spp{};
     AngleP<T> anp{};
-    if (a.cs_set) {
+    unsigned code = kNoSet | (kNoSet << 16);
+    if (a.bond_code) {
         if (in && (SPR || ANG)) {
-            const unsigned code = a.cs_set[i];
-            if (code != kNoSet) {
-                const SpringSet ss = a.sp_sets[code];
+            code = a.bond_code[i];
+            const unsigned sc = code & kNoSet;
+            if (sc != kNoSet) {
+                const SpringSet ss = a.sp_sets[sc];
                 hs = true;
-                spp = SpringP<T>{ss.ty, a.cs_st[i], ss.l0, ss.c1, (double)a.cs_c2[i], (T)ss.k};
+                spp = SpringP<T>{ss.ty, (code & kSpringOn) ? 1 : 0, ss.l0, ss.c1, ss.c2, (T)ss.k};
             }
         }
         if (in && ANG) {
-            const unsigned code = a.ca_set[i];
-            if (code != kNoSet) {
-                const AngleSet as = a.an_sets[code];
+            const unsigned ac = (code >> 16) & kNoSet;
+            if (ac != kNoSet) {
+                const AngleSet as = a.an_sets[ac];
                 hq = true;
-                anp = AngleP<T>{a.ca_st[i], as.ty, (T)as.th0, (T)as.k};
+                anp = AngleP<T>{(code & kAngleOn) ? 1 : 0, as.ty, (T)as.th0, (T)as.k};
             }
         }
     } else {
@@ -362,10 +376,10 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
             if (broke) {
                 str = 0;
                 if (own) {
-                    const int s = a.cs_set ? a.ch_spring[i] : sr;
+                    const int s = a.bond_code ? a.ch_spring[i] : sr;
                     a.sp_status[s] = 0;
                     a.sp_break[s] = *a.step + a.step_off;
-                    if (a.cs_set) a.cs_st[i] = 0;
+                    if (a.bond_code) a.bond_code[i] = (code &= ~kSpringOn);
                 }
             }
         }
@@ -382,8 +396,8 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
         if (st == 1 && anp.ty == 1 && (stl == 0 || str == 0)) {
             st = 0;
             if (own) {
-                a.an_status[a.cs_set ? a.ch_angle[i] : q] = 0;
-                if (a.cs_set) a.ca_st[i] = 0;
+                a.an_status[a.bond_code ? a.ch_angle[i] : q] = 0;
+                if (a.bond_code) a.bond_code[i] = (code &= ~kAngleOn);
             }
         }
         if (st == 1) angle_force(anp, xp, yp, zp, x, y, z, xn, yn, zn, pa, pc);
@@ -421,12 +435,13 @@ __global__ void __launch_bounds__(BLK) __attribute__((amdgpu_waves_per_eu(kChain
             f.z += pan.z;
         }
     }
-    contacts_finish<T, P, ENS>(a, ca, e0, cnt, r1, b.px, b.py, b.pz, f, prs, w, ENS ? a.net[ic] : 0);
+    contacts_finish<T, P, ENS>(a, ca, e0, cnt, r1, b.px, b.py, b.pz, f, prs, w, ENS ? a.net[ic] : 0, lj_uni);
     if (!own) return;
     // the bead's position and image again (cache hits) rather than holding
     // twelve registers through the contact passes
     b.px = a.px[i]; b.py = a.py[i]; b.pz = a.pz[i];
-    bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, a.ix[i], a.iy[i], a.iz[i], ms);
+    if (alias) bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, b.px, b.py, b.pz, ms);
+    else bead_end<T, P, NEXT, LAST>(a, bx, i, b, f, a.ix[i], a.iy[i], a.iz[i], ms);
 }
 
 __global__ void k_add_int(int *p, int v) { *p += v; }
@@ -439,16 +454,19 @@ __global__ void __launch_bounds__(kBlock) k_copy_bounded(const int *src, int *ds
     for (int k = blockIdx.x * kBlock + threadIdx.x; k < m; k += gridDim.x * kBlock) dst[k] = src[k];
 }
 
-// The chain layout's status copies from the bond-ordered statuses.
+// The status bits of the chain layout's bond words from the bond-ordered
+// statuses (set fields kept).
 __global__ void __launch_bounds__(kBlock) k_chain_status(const int *ch_spring, const int *ch_angle,
-                                                         const int *sp_status, const int *an_status, int *cs_st,
-                                                         int *ca_st, int n)
+                                                         const int *sp_status, const int *an_status,
+                                                         unsigned *bond_code, int n)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= n) return;
     const int s = ch_spring[i], q = ch_angle[i];
-    cs_st[i] = s >= 0 ? sp_status[s] : 0;
-    ca_st[i] = q >= 0 ? an_status[q] : 0;
+    unsigned c = bond_code[i] & ~(kSpringOn | kAngleOn);
+    if (s >= 0 && sp_status[s] == 1) c |= kSpringOn;
+    if (q >= 0 && an_status[q] == 1) c |= kAngleOn;
+    bond_code[i] = c;
 }
 
 __global__ void __launch_bounds__(kBlock) k_fill_int(int *p, int v, int n)
@@ -1501,7 +1519,8 @@ static void launch_fused_mask(const Args<T, P> &a, hipStream_t s)
     constexpr bool ens = (M & 32) != 0;
     if (M & 16) {
         constexpr int per_block = kChainOwned * (BLK / 64);
-        k_bead_step_chain<T, P, spr, ang, next, last, ens, BLK><<<(a.n + per_block - 1) / per_block, BLK, 0, s>>>(a);
+        const size_t dyn = a.ms_code ? sizeof(MassSet<T>) * (size_t)a.n_ms : 0;
+        k_bead_step_chain<T, P, spr, ang, next, last, ens, BLK><<<(a.n + per_block - 1) / per_block, BLK, dyn, s>>>(a);
     } else {
         k_bead_step<T, P, spr, ang, next, last, ens, BLK><<<(a.n + BLK - 1) / BLK, BLK, 0, s>>>(a);
     }
@@ -1528,10 +1547,10 @@ void Launch<T, P>::bead_step(const Args<T, P> &a, bool spr, bool ang, bool next,
 template struct Launch<double, double>;
 template struct Launch<float, double>;
 
-void chain_status(const int *ch_spring, const int *ch_angle, const int *sp_status, const int *an_status, int *cs_st,
-                  int *ca_st, int n, hipStream_t s)
+void chain_status(const int *ch_spring, const int *ch_angle, const int *sp_status, const int *an_status,
+                  unsigned *bond_code, int n, hipStream_t s)
 {
-    if (n > 0) k_chain_status<<<blocks(n), kBlock, 0, s>>>(ch_spring, ch_angle, sp_status, an_status, cs_st, ca_st, n);
+    if (n > 0) k_chain_status<<<blocks(n), kBlock, 0, s>>>(ch_spring, ch_angle, sp_status, an_status, bond_code, n);
 }
 void fill_int(int *p, int v, int n, hipStream_t s)
 {

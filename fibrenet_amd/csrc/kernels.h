@@ -117,17 +117,22 @@ template <typename T>
 struct V3 { T x, y, z; };
 
 // A distinct parameter set of the chain layout's springs (type, rest length,
-// E, stiffness as the engine's T widened to fp64) and angles (type, rest
-// angle and stiffness as T widened); kNoSet marks a bead without that bond.
+// E, failure stress, stiffness as the engine's T widened to fp64) and angles
+// (type, rest angle and stiffness as T widened).
 struct SpringSet {
     int ty;
-    double l0, c1, k;
+    double l0, c1, c2, k;
 };
 struct AngleSet {
     int ty;
     double th0, k;
 };
-constexpr unsigned short kNoSet = 0xFFFF;
+// A bead's chain bonds in one word: bits 0-14 the set of spring (i, i+1),
+// bit 15 that spring's status == 1; bits 16-30 the set of the angle centred
+// on i, bit 31 its status == 1.  kNoSet in a set field: no such bond.
+constexpr unsigned kNoSet = 0x7FFF;
+constexpr unsigned kSpringOn = 1u << 15;
+constexpr unsigned kAngleOn = 1u << 31;
 
 // A distinct set of a bead's mass coefficients and radius, as the engine's T
 // holds them: lambda*dt/m, (dt/2)/m, (dt^2/2)/m and r.
@@ -173,14 +178,12 @@ struct Args {
     // spring (p1 i, p2 i+1), ch_angle[i] the angle (b1 i-1, b2 i, b3 i+1),
     // -1 where there is none.
     const int *ch_spring, *ch_angle;
-    // The chain layout's bond parameters by bead, when the bonds have few
-    // distinct parameter sets (null otherwise): the set of spring (i, i+1)
-    // and of the angle centred on i, the spring's failure stress (exact as a
-    // float), and copies of both statuses (refreshed from sp_status /
-    // an_status at the start of every chunk, written with them on a change).
-    const unsigned short *cs_set, *ca_set;
-    const float *cs_c2;
-    int *cs_st, *ca_st;
+    // The chain layout's bond parameters by bead, when the bonds have at
+    // most kNoSet distinct parameter sets of each kind (null otherwise): one
+    // word per bead (kNoSet above), whose status bits copy sp_status /
+    // an_status (refreshed at the start of every chunk, written with them on
+    // a change).
+    unsigned *bond_code;
     const SpringSet *sp_sets;
     const AngleSet *an_sets;
     // Each bead's mass coefficients and radius as a one-byte code into at
@@ -656,11 +659,18 @@ struct alignas(16) ContactOwner {
     int net;
 };
 
+// One contact term in LDS, padded to 16 (fp32) or 32 (fp64) bytes so that a
+// lane reads or writes it with one LDS instruction.
+template <typename T>
+struct alignas(4 * sizeof(T)) TermSlot {
+    T x, y, z, pad;
+};
+
 // A wave's LDS for its contacts: the terms of one pass, the lane that owns
 // each entry of it, and every lane's bead.
 template <typename T>
 struct WaveContactLds {
-    V3<T> term[kWaveChunk];
+    TermSlot<T> term[kWaveChunk];
     unsigned char own[kWaveChunk];
     ContactOwner<T> ow[64];
 };
@@ -817,6 +827,54 @@ __device__ __forceinline__ bool contact_term_at(const Args<T, P> &a, const Box &
     return contact_force(pr, b, rx, ry, rz, len, r1, a.radius, j, t);
 }
 
+// contact_term_at for a pair table whose addressed entries are all one
+// Lennard-Jones entry pr (fp32 build): the same operations in the same
+// order, with the entry's tests folded into one mask instead of branches.
+template <typename P>
+__device__ __forceinline__ V3<float> lj_uniform_term(const PairF &pr, const Box &b, P x1, P y1, P z1, P xj, P yj,
+                                                     P zj, bool valid)
+{
+    P rx = x1 - xj, ry = y1 - yj, rz = z1 - zj;
+    P len = rx * rx + ry * ry + rz * rz;
+    if (len > (P)b.hbs) {
+        if (rx * rx > (P)b.hbs) rx += (rx > (P)0) ? (P)-b.lx : (P)b.lx;
+        if (ry * ry > (P)b.hbs) ry += (ry > (P)0) ? (P)-b.ly : (P)b.ly;
+        if (rz * rz > (P)b.hbs) rz += (rz > (P)0) ? (P)-b.lz : (P)b.lz;
+        len = rx * rx + ry * ry + rz * rz;
+    }
+    const float r2 = (float)len;
+    const float r = hw_sqrt(r2);
+    const bool on = valid && (double)len < b.cutoff_sq && r < pr.c3f;
+    const float rr = hw_rcp(r2);
+    const float fr2 = pr.lj_s2 * rr;
+    const float fr6 = fr2 * fr2 * fr2;
+    float coef = pr.lj_e48 * fr6 * (fr6 - 0.5f) * rr;
+    coef -= pr.c4f;
+    float inv = r * rr;
+    if (r == 0.0f) inv = 0.0f;
+    V3<float> t;
+    t.x = on ? coef * ((float)rx * inv) : -0.0f;
+    t.y = on ? coef * ((float)ry * inv) : -0.0f;
+    t.z = on ? coef * ((float)rz * inv) : -0.0f;
+    return t;
+}
+
+// Whether the entries [0, na) of a pair table in LDS (na <= 64) are all
+// one Lennard-Jones entry as the fp32 contact term reads it; wave-uniform,
+// every lane of the wave calls it.  The fp64 build has no such path.
+__device__ __forceinline__ bool lj_uniform_wave(const PairF *tab, int na)
+{
+    const int lane = threadIdx.x & 63;
+    const PairF p0 = tab[0];
+    bool ok = true;
+    if (lane < na) {
+        const PairF p = tab[lane];
+        ok = p.type == p0.type && p.lj_s2 == p0.lj_s2 && p.lj_e48 == p0.lj_e48 && p.c3f == p0.c3f && p.c4f == p0.c4f;
+    }
+    return na >= 1 && p0.type == 1 && __ballot(!ok) == 0;
+}
+__device__ __forceinline__ bool lj_uniform_wave(const PairC *, int) { return false; }
+
 // One pass of a wave's contacts: entries [base, base + n) with their list
 // entries ent[r] at k = base + r * 64 + lane, owner lanes in w.own and the
 // owners' beads in w.ow; the terms go to w.term (an entry without force
@@ -824,7 +882,8 @@ __device__ __forceinline__ bool contact_term_at(const Args<T, P> &a, const Box &
 // partner position of the pass is loaded before the terms are evaluated.
 template <typename T, typename P, bool ENS, typename E>
 __device__ __forceinline__ void contacts_pass(const Args<T, P> &a, const int (&ent)[kChunkRounds], int base, int n,
-                                              int off, int cnt, V3<T> &f, const E *table, WaveContactLds<T> &w)
+                                              int off, int cnt, V3<T> &f, const E *table, WaveContactLds<T> &w,
+                                              bool lj_uni)
 {
     const int lane = threadIdx.x & 63;
     P xj[kChunkRounds], yj[kChunkRounds], zj[kChunkRounds];
@@ -838,9 +897,25 @@ __device__ __forceinline__ void contacts_pass(const Args<T, P> &a, const int (&e
             zj[r] = a.pz[j];
         }
     }
+    bool done = false;
+    if constexpr (!ENS && sizeof(T) == sizeof(float)) {
+        if (lj_uni) {                                  // block-uniform
+            const PairF pr = table[0];
+#pragma unroll
+            for (int r = 0; r < kChunkRounds; ++r) {
+                if (r * 64 >= n) break;               // wave-uniform
+                const int k = r * 64 + lane;
+                const int L = k < n ? w.own[k] : 0;
+                const ContactOwner<T> o = w.ow[L];
+                const V3<float> t = lj_uniform_term(pr, a.force, (P)o.x, (P)o.y, (P)o.z, xj[r], yj[r], zj[r], k < n);
+                w.term[k] = TermSlot<T>{t.x, t.y, t.z, (T)0};
+            }
+            done = true;
+        }
+    }
 #pragma unroll
     for (int r = 0; r < kChunkRounds; ++r) {
-        if (r * 64 >= n) break;                       // wave-uniform
+        if (done || r * 64 >= n) break;               // block- / wave-uniform
         const int k = r * 64 + lane;
         const int L = k < n ? w.own[k] : 0;
         const ContactOwner<T> o = w.ow[L];
@@ -855,16 +930,25 @@ __device__ __forceinline__ void contacts_pass(const Args<T, P> &a, const int (&e
                                           t))
                 t = V3<T>{(T)-0.0, (T)-0.0, (T)-0.0};
         }
-        w.term[k] = t;
+        w.term[k] = TermSlot<T>{t.x, t.y, t.z, (T)0};
     }
     wave_sync_lds();
+    // Each lane adds its entries [k0, k1) in list order, four per iteration:
+    // the four reads are issued together, and a slot past k1 adds -0.0,
+    // which leaves every sum (including +0.0) unchanged bit for bit.
     const int k0 = off > base ? off : base;
     const int k1 = (off + cnt < base + n) ? off + cnt : base + n;
-    for (int k = k0; k < k1; ++k) {
-        const V3<T> t = w.term[k - base];
-        f.x += t.x;
-        f.y += t.y;
-        f.z += t.z;
+    for (int k = k0; k < k1; k += 4) {
+        TermSlot<T> t[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) t[u] = w.term[min(k + u - base, kWaveChunk - 1)];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const bool on = k + u < k1;
+            f.x += on ? t[u].x : (T)-0.0;
+            f.y += on ? t[u].y : (T)-0.0;
+            f.z += on ? t[u].z : (T)-0.0;
+        }
     }
     wave_sync_lds();
 }
@@ -876,18 +960,20 @@ __device__ __forceinline__ void contacts_pass(const Args<T, P> &a, const int (&e
 // entries / 64) rounds instead of its longest list; each bead then adds its
 // own terms in list order, which equals the per-bead loop (gather_contacts)
 // bit for bit.  ENS: the lane's bead belongs to network `netw` (its Box and
-// pair table); otherwise a.force and `table`.
+// pair table); otherwise a.force and `table`.  lj_uni (block-uniform): every
+// entry the lists address is table[0], a Lennard-Jones entry.
 template <typename T, typename P, bool ENS, typename E>
 __device__ __forceinline__ void contacts_finish(const Args<T, P> &a, const ContactAhead &c, int e0, int cnt, T r1,
                                                 P x1, P y1, P z1, V3<T> &f, const E *table, WaveContactLds<T> &w,
-                                                int netw = 0)
+                                                int netw = 0, bool lj_uni = false)
 {
     static_assert(sizeof(P) == sizeof(double), "positions are fp64 in both builds");
     const int lane = threadIdx.x & 63;
     if (c.total == 0) return;                         // wave-uniform
     w.ow[lane] = ContactOwner<T>{(double)x1, (double)y1, (double)z1, r1, netw};
     wave_sync_lds();
-    contacts_pass<T, P, ENS>(a, c.ent, 0, c.total < kWaveChunk ? c.total : kWaveChunk, c.off, cnt, f, table, w);
+    contacts_pass<T, P, ENS>(a, c.ent, 0, c.total < kWaveChunk ? c.total : kWaveChunk, c.off, cnt, f, table, w,
+                             lj_uni);
     for (int base = kWaveChunk; base < c.total; base += kWaveChunk) {
         const int n = (c.total - base) < kWaveChunk ? c.total - base : kWaveChunk;
         mark_owner(c.off, cnt, base, w.own);
@@ -899,7 +985,7 @@ __device__ __forceinline__ void contacts_finish(const Args<T, P> &a, const Conta
             const int e = owner_entry(k, k < n, e0, c.off - base, w.own, L);
             ent[r] = k < n ? a.nb_list[e] : 0;
         }
-        contacts_pass<T, P, ENS>(a, ent, base, n, c.off, cnt, f, table, w);
+        contacts_pass<T, P, ENS>(a, ent, base, n, c.off, cnt, f, table, w, lj_uni);
     }
 }
 

@@ -149,7 +149,10 @@ int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_spri
  *                 saves candidate builds)
  *   "staged" 1/0  neighbour search reads each block's neighbourhood from LDS
  *   "skin"   n    the skin in permille of the largest listing distance (300)
- *   "graphs" 1/0  launch through captured hipGraphs or directly
+ *   "graphs" 1/0  launch through captured hipGraphs or directly; a new
+ *                 engine starts with 0 when the environment variable
+ *                 FIBRENET_GRAPHS is "0" (e.g. under a profiler that cannot
+ *                 follow graph launches), else with 1
  *   "block"  n    threads per block of the bead kernels: 128, 256 or 512
  * All options give bit-identical results; they change only the speed. */
 int fn_set_option(fn_engine *eng, const char *name, int value);
