@@ -512,8 +512,10 @@ class Engine final : public fn_engine {
     void presize_rows()
     {
         const int need = std::max(rows_exact_, keep_cand_ ? rows_cand_ : 0);
-        if (need * 3 / 2 <= K_) return;
-        while (K_ < 2 * need) K_ *= 2;
+        if (need * 3LL / 2 <= K_) return;
+        long long k = K_;
+        while (k < 2LL * need) k *= 2;
+        set_rows(k);
         ensure_scratch();
     }
 
@@ -1491,7 +1493,10 @@ class Engine final : public fn_engine {
     void search_and_compact(const fnk::Box &b, bool final_list, bool sync, const int *gate)
     {
         for (;;) {
-            if (sync) fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
+            if (sync) {
+                fnk::fill_int(stats_.p + fnk::kStatListOver, 0, 1, stream_);
+                fnk::fill_int(stats_.p + fnk::kStatDense, 0, 1, stream_);
+            }
             // every bead has a slot (k_place clamps the ones outside the grid
             // into it), so the search writes every bead's count
             fnk::search(cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b, ens_net(),
@@ -1499,10 +1504,14 @@ class Engine final : public fn_engine {
                         ncnt_.p, scratch_.p, K_, stats_.p, N_, staged_, flat_, gate, stream_);
             fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, gate, stream_);
             if (sync) {
-                int tot = 0, over = 0;
+                int tot = 0, over = 0, dense = 0;
                 HIPCK(hipMemcpyAsync(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
                 HIPCK(hipMemcpyAsync(&over, stats_.p + fnk::kStatListOver, sizeof(int), hipMemcpyDeviceToHost, stream_));
+                HIPCK(hipMemcpyAsync(&dense, stats_.p + fnk::kStatDense, sizeof(int), hipMemcpyDeviceToHost, stream_));
                 HIPCK(hipStreamSynchronize(stream_));
+                if (dense > 0)
+                    throw Error(FN_ERR_UNSUPPORTED, fmt("%d beads lay in grid cells holding more than %d beads (the "
+                                                     "reference keeps 64 per cell)", dense, fnk::kMaxCellBeads));
                 if (over > 0) {
                     grow_rows();
                     continue;
@@ -1570,8 +1579,17 @@ class Engine final : public fn_engine {
 
     void grow_rows()
     {
-        K_ *= 2;
+        set_rows(2 * K_);
         ensure_scratch();
+    }
+
+    // Search rows of k entries: the rows of all beads are one array whose
+    // offsets are ints, so N * k stays below 2^31.
+    void set_rows(long long k)
+    {
+        if ((long long)N_ * k > (1LL << 31) - 1)
+            throw Error(FN_ERR_CAPACITY, fmt("search rows of %lld entries for %d beads exceed 2^31 entries", k, N_));
+        K_ = (int)k;
     }
 
     // BuildContactList of the per-kernel path, after FillGrid's sort: the
@@ -1590,14 +1608,18 @@ class Engine final : public fn_engine {
         int st[fnk::kStatCount];
         HIPCK(hipMemcpy(st, stats_.p, sizeof st, hipMemcpyDeviceToHost));
         const int tot = st[fnk::kStatCandMax], over = st[fnk::kStatListOver];
+        if (st[fnk::kStatDense] > 0)
+            throw Error(FN_ERR_UNSUPPORTED, fmt("%d beads lay in grid cells holding more than %d beads (the reference keeps "
+                                             "64 per cell); the run is invalid", st[fnk::kStatDense], fnk::kMaxCellBeads));
         last_total_ = std::max(last_total_, tot);
         // the longest rows the searches needed (counted past the row length)
         rows_exact_ = std::max(rows_exact_, st[fnk::kStatMaxList]);
         rows_cand_ = std::max(rows_cand_, st[fnk::kStatMaxCand]);
         if (over > 0) {
             const int need = std::max(st[fnk::kStatMaxList], st[fnk::kStatMaxCand]), was = K_;
-            grow_rows();
-            while (K_ < 2 * need) K_ *= 2;
+            long long k = 2LL * K_;
+            while (k < 2LL * need) k *= 2;
+            set_rows(k);
             ensure_scratch();
             throw Error(FN_ERR_CAPACITY, fmt("a contact list needed %d entries of a %d-entry search row; the run "
                                              "is invalid and the row length is now %d", need, was, K_));

@@ -8,11 +8,17 @@ namespace fnk {
 
 // Slots of the device statistics array: cell occupancy (at the last sort),
 // the longest listed and candidate lists, search rows that overflowed, the
-// candidate builds of the run and their largest total.
+// candidate builds of the run and their largest total, and the beads not
+// searched because their cell held more than kMaxCellBeads beads.
 enum {
     kStatMaxCell = 0, kStatCellsOver = 1, kStatMaxList = 2, kStatListOver = 3, kStatMaxCand = 4,
-    kStatCandBuilds = 5, kStatCandMax = 6, kStatCount = 7
+    kStatCandBuilds = 5, kStatCandMax = 6, kStatDense = 7, kStatCount = 8
 };
+
+// Beads one grid cell may hold for the searches (the reference keeps 64 per
+// cell and drops the rest; beyond this bound a run is refused instead of
+// searching a collapsed neighbourhood for minutes).
+constexpr int kMaxCellBeads = 4096;
 
 // Cells visited around a bead's cell by the candidate search, per axis: the
 // half-width w, or every cell of the axis once (a periodic axis with fewer

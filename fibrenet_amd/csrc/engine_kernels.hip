@@ -514,6 +514,16 @@ __global__ void k_set_int(int *p, int v, const int *gate)
     *p = v;
 }
 
+// FillGrid's cell coordinate floor(v / step) (kernel.cl:493-495) for a grid
+// of n cells on the axis: the cell itself inside [0, n), -1 below the grid
+// or for NaN, n above it; converted to int only in range.
+__device__ __forceinline__ int cell_coord(double v, double step, int n)
+{
+    const double c = floor(v / step);
+    if (c >= 0.0 && c < (double)n) return (int)c;
+    return c >= (double)n ? n : -1;
+}
+
 // FillGrid's cell of a bead (kernel.cl:492-504) in fp64 (-1 outside the grid)
 // into key; the cell of its position clamped into the box into ckey, with the
 // bead's rank among the beads of that cell (first pass of a counting sort).
@@ -530,9 +540,9 @@ __global__ void __launch_bounds__(kBlock) k_hash(const double *px, const double 
     int c = -1;                                   // -1: no bead on this lane
     if (i < n) {
         const Box &b = net_box<ENS>(bs, boxes, net, i);
-        const int cx = (int)floor(px[i] / b.sx);
-        const int cy = (int)floor(py[i] / b.sy);
-        const int cz = (int)floor(pz[i] / b.sz);
+        const int cx = cell_coord(px[i], b.sx, b.cx);
+        const int cy = cell_coord(py[i], b.sy, b.cy);
+        const int cz = cell_coord(pz[i], b.sz, b.cz);
         const bool in = cx >= 0 && cx < b.cx && cy >= 0 && cy < b.cy && cz >= 0 && cz < b.cz;
         key[i] = in ? b.cell_base + cx + cy * b.cx + cz * b.cxy : -1;
         const int qx = min(max(cx, 0), b.cx - 1), qy = min(max(cy, 0), b.cy - 1), qz = min(max(cz, 0), b.cz - 1);
@@ -721,6 +731,7 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_
     if (gated_off(gate)) return;
     const int s = xcd_block() * kBlock + threadIdx.x;   // neighbouring slot ranges share one XCD's L2
     int k = 0, occ = 0;
+    bool dense = false;
     if (s < cell_start[n_cells]) {
         const SortedBead me = so.bead[s];
         const int i = me.bead;
@@ -729,6 +740,7 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_
         const int code = me.cell;
         const int cg = slot_cell(code);
         if (cell_start[cg] == s) occ = cell_start[cg + 1] - s;   // the cell's first slot reports its occupancy
+        dense = cell_start[cg + 1] - cell_start[cg] > kMaxCellBeads;
         const int c = cg - b.cell_base;                          // cell within the bead's network
         const int cz = c / b.cxy;
         const int cy = (c - cz * b.cxy) / b.cx;
@@ -739,7 +751,9 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_
         const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, st.w[1], st.all[1], per},
             az{cz, b.cz, st.w[2], st.all[2], per};
         const bool row_run = !st.all[0] && cx - st.w[0] >= 0 && cx + st.w[0] < b.cx;
-        if (EXACT && code >= 0 && row_run && st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !st.all[1] &&
+        if (dense) {
+            // refused: the host reports the run invalid
+        } else if (EXACT && code >= 0 && row_run && st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !st.all[1] &&
             !st.all[2]) {
             // the reference's 3x3x3 stencil with x-rows inside the grid: the
             // nine row ranges in one round trip, then the rows
@@ -778,17 +792,19 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_
                 }
             }
         }
-        cnt[i] = k;
+        cnt[i] = min(k, K);   // the row holds at most K; the count past it goes to the statistics
     }
     const int mx = wave_max(k);
     const int over = wave_sum(k > K ? 1 : 0);
     const int mocc = wave_max(occ);
     const int big = wave_sum(occ > 64 ? 1 : 0);
+    const int nd = wave_sum(dense ? 1 : 0);
     if ((threadIdx.x & 63) == 0) {
         raise_max(&stats[EXACT ? kStatMaxList : kStatMaxCand], mx);
         if (over > 0) atomicAdd(&stats[kStatListOver], over);
         raise_max(&stats[kStatMaxCell], mocc);
         if (big > 0) atomicAdd(&stats[kStatCellsOver], big);
+        if (nd > 0) atomicAdd(&stats[kStatDense], nd);
     }
 }
 
@@ -870,18 +886,19 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *cell_start, i
     const int n_slots = cell_start[n_cells];
     const bool live = s < n_slots;
     int i = 0, code = -1, occ = 0, cx = 0, cy = 0, cz = 0;
-    bool elig = false;
+    bool elig = false, dense = false;
     if (live) {
         const SortedBead me = so.bead[s];
         i = me.bead;
         code = me.cell;
         const int cg = slot_cell(code);
         if (cell_start[cg] == s) occ = cell_start[cg + 1] - s;
+        dense = cell_start[cg + 1] - cell_start[cg] > kMaxCellBeads;
         const int c = cg - b.cell_base;
         cz = c / b.cxy;
         cy = (c - cz * b.cxy) / b.cx;
         cx = c - cz * b.cxy - cy * b.cx;
-        elig = !(EXACT && code < 0);
+        elig = !(EXACT && code < 0) && !dense;
         w.own[lane] = FlatOwner{me.x, me.y, me.z, me.tag, me.loc, me.type, s, i, 0};
     }
     w.cnt[lane] = 0;
@@ -891,6 +908,11 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *cell_start, i
             int lo0, hi0, lo1, hi1;
             stencil_row(cell_start, b, cx, cy, cz, q, lo0, hi0, lo1, hi1);
             nb += (hi0 - lo0) + (hi1 - lo1);
+        }
+        if (nb > (1 << 24)) {          // keeps the wave's total below 2^30
+            nb = 0;
+            elig = false;
+            dense = true;
         }
     }
     int incl = nb;
@@ -953,16 +975,18 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *cell_start, i
         wave_sync_lds();
     }
     const int kc = w.cnt[lane];
-    if (live) cnt[i] = kc;
+    if (live) cnt[i] = min(kc, K);
     const int mx = wave_max(kc);
     const int over = wave_sum(kc > K ? 1 : 0);
     const int mocc = wave_max(occ);
     const int big = wave_sum(occ > 64 ? 1 : 0);
+    const int nd = wave_sum(dense ? 1 : 0);
     if (lane == 0) {
         raise_max(&stats[EXACT ? kStatMaxList : kStatMaxCand], mx);
         if (over > 0) atomicAdd(&stats[kStatListOver], over);
         raise_max(&stats[kStatMaxCell], mocc);
         if (big > 0) atomicAdd(&stats[kStatCellsOver], big);
+        if (nd > 0) atomicAdd(&stats[kStatDense], nd);
     }
 }
 
@@ -1029,12 +1053,14 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *cell_s
 
     const int s = s0 + threadIdx.x;
     int k = 0, occ = 0;
+    bool dense = false;
     if (s < s_end) {
         const SortedBead me = so.bead[s];
         const int i = me.bead;
         const int code = me.cell;
         const int c = slot_cell(code);
         if (cell_start[c] == s) occ = cell_start[c + 1] - s;
+        dense = cell_start[c + 1] - cell_start[c] > kMaxCellBeads;
         const int cz = c / b.cxy;
         const int cy = (c - cz * b.cxy) / b.cx;
         const int cx = c - cz * b.cxy - cy * b.cx;
@@ -1044,7 +1070,7 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *cell_s
         const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, wy, st.all[1], per},
             az{cz, b.cz, wz, st.all[2], per};
         const bool row_run = !st.all[0] && cx - st.w[0] >= 0 && cx + st.w[0] < b.cx;
-        for (int dz = az.first(); dz <= az.last() && !(EXACT && code < 0); ++dz) {
+        for (int dz = az.first(); dz <= az.last() && !(EXACT && code < 0) && !dense; ++dz) {
             const int z = az.at(dz);
             if (z < 0) continue;
             for (int dy = ay.first(); dy <= ay.last(); ++dy) {
@@ -1074,17 +1100,19 @@ __global__ void __launch_bounds__(kStageBlock) k_search_staged(const int *cell_s
                 }
             }
         }
-        cnt[i] = k;
+        cnt[i] = min(k, K);   // the row holds at most K; the count past it goes to the statistics
     }
     const int mx = wave_max(k);
     const int over = wave_sum(k > K ? 1 : 0);
     const int mocc = wave_max(occ);
     const int big = wave_sum(occ > 64 ? 1 : 0);
+    const int nd = wave_sum(dense ? 1 : 0);
     if ((threadIdx.x & 63) == 0) {
         raise_max(&stats[EXACT ? kStatMaxList : kStatMaxCand], mx);
         if (over > 0) atomicAdd(&stats[kStatListOver], over);
         raise_max(&stats[kStatMaxCell], mocc);
         if (big > 0) atomicAdd(&stats[kStatCellsOver], big);
+        if (nd > 0) atomicAdd(&stats[kStatDense], nd);
     }
 }
 
@@ -1151,7 +1179,7 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *c
     const bool live = i < n;
     int m = live ? min(cnt[i], K) : 0;
     const int base = live ? cand_start[i] : 0;
-    if (base + m > cap) m = 0;   // capacity exceeded: the host reports the run invalid
+    if (base < 0 || m > cap - base) m = 0;   // capacity exceeded: the host reports the run invalid
     if (live && cnt_out) cnt_out[i] = m;
     const int wbase = __shfl(base, 0, 64);             // lane 0 holds a bead
     const int span = wave_max(m > 0 ? base + m : wbase) - wbase;   // every written entry lies below cap
@@ -1185,7 +1213,7 @@ __global__ void __launch_bounds__(kBlock) k_disp(const double *px, const double 
         }
         moved = !(dx * dx + dy * dy + dz * dz <= lim2);
         // FillGrid's cell (kernel.cl:493-499), once per bead for the filter
-        const int cx = (int)floor(x / g.sx), cy = (int)floor(y / g.sy), cz = (int)floor(z / g.sz);
+        const int cx = cell_coord(x, g.sx, g.cx), cy = cell_coord(y, g.sy, g.cy), cz = cell_coord(z, g.sz, g.cz);
         const bool in = cx >= 0 && cx < g.cx && cy >= 0 && cy < g.cy && cz >= 0 && cz < g.cz;
         cb[i] = CellBead{(float)x, (float)y, (float)z, in ? (cx | (cy << pack.sy) | (cz << pack.sz)) : -1};
         key[i] = in ? cx + cy * g.cx + cz * g.cxy : -1;
@@ -1261,7 +1289,8 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     const bool in = mcell >= 0;
     const int e0 = f.cand_start[ic];
     const int e1 = f.cand_start[ic + 1];
-    const int cnt = (live && in && e1 <= f.cap) ? e1 - e0 : 0;   // past the capacity: the host reports the run invalid
+    // past the capacity: the host reports the run invalid
+    const int cnt = (live && in && e0 >= 0 && e1 >= e0 && e1 <= f.cap) ? e1 - e0 : 0;
     int incl = cnt;
     incl = wave_incl_scan(incl);
     const int off = incl - cnt;

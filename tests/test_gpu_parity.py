@@ -102,20 +102,28 @@ def test_chain_kernel_equals_slot_path(need_gpu, workload, precision):
 
 def _inexact_failure_stress():
     net = small_tensile(weak=True)
-    net.springs["c2"] = net.springs["c2"] + 0.1          # not exact in float
+    net.springs["c2"] = net.springs["c2"] + 0.1          # not exact in float: still parameter sets
+    return net
+
+
+def _status_two():
+    net = small_tensile(weak=True)
+    net.springs["status"][7] = 2                          # inactive, but not a 0/1 status bit
     return net
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-@pytest.mark.parametrize("case", ["sets", "indexed"])
+@pytest.mark.parametrize("case", ["sets", "sets_inexact", "indexed"])
 def test_chain_parameter_sets_and_indexed_loads_equal_slot_path(need_gpu, case, precision):
-    """The chain kernel reads bond parameters through per-bead set codes when
-    every failure stress is exact in float, else through the bond indices;
+    """The chain kernel reads bond parameters through one word per bead (set
+    codes and status bits) when every bond status is 0 or 1 and each kind has
+    at most 32767 distinct parameter sets, else through the bond indices;
     both give the slot path's bytes."""
-    net = small_tensile(weak=True) if case == "sets" else _inexact_failure_stress()
+    net = {"sets": lambda: small_tensile(weak=True), "sets_inexact": _inexact_failure_stress,
+           "indexed": _status_two}[case]()
     a, ia = engine_for(net, precision)
     b, ib = engine_for(net, precision)
-    assert a.get_option("chain_sets") == (1 if case == "sets" else 0)
+    assert a.get_option("chain_sets") == (0 if case == "indexed" else 1)
     b.set_option("chain", 0)
     for n in (5, 200):
         run_engine(a, net, n, fast=True)
