@@ -1297,6 +1297,7 @@ class Engine final : public fn_engine {
                 C_ = C;
                 cell_count_.alloc((size_t)C + 1);
                 cell_start_.alloc((size_t)C + 1);
+                fnk::fill_int(cell_count_.p, 0, C + 1, stream_);
             }
             const int nb = std::max(fnk::scan_blocks_needed(std::max(C_, N_)), 1) + 1;
             if ((int)scan_tmp_.n < nb) {
@@ -1368,13 +1369,13 @@ class Engine final : public fn_engine {
     // Counting sort of the beads (positions of buffer pc) into cells.
     void enqueue_sort(int pc, const fnk::Box &b, const int *gate)
     {
-        fnk::fill_gated(cell_count_.p, 0, C_ + 1, gate, stream_);
+        // (the counts are zero: allocated so, and every placement zeroes them)
         fnk::hash(pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, N_, b, ens_net(), boxes_.p, key_.p, ckey_.p, rank_.p,
                   cell_count_.p, gate,
                   stream_);
         fnk::exclusive_scan(cell_count_.p, C_, cell_start_.p, scan_tmp_.p, gate, stream_);
         fnk::place(key_.p, ckey_.p, rank_.p, cell_start_.p, pos_[pc][0].p, pos_[pc][1].p, pos_[pc][2].p, type_.p, tag_.p,
-                   loc_.p, N_, sorted_view(), gate, stream_);
+                   loc_.p, N_, sorted_view(), cell_count_.p, C_ + 1, gate, stream_);
     }
 
     fnk::Sorted sorted_view()
@@ -1490,7 +1491,8 @@ class Engine final : public fn_engine {
     // otherwise the candidates within reach + skin into the candidate list.
     // Gated by `gate`.  sync (per-kernel path, ungated): the host checks the
     // row and list capacities before compacting and grows them as needed.
-    void search_and_compact(const fnk::Box &b, bool final_list, bool sync, const int *gate)
+    void search_and_compact(const fnk::Box &b, bool final_list, bool sync, const int *gate,
+                            const fnk::Snapshot &snap = fnk::Snapshot{})
     {
         for (;;) {
             if (sync) {
@@ -1520,7 +1522,7 @@ class Engine final : public fn_engine {
                 last_total_ = tot;
             }
             fnk::compact(ncnt_.p, nb_start_.p, scratch_.p, K_, N_, final_list ? nb_list_.p : cand_.p, cap_,
-                         final_list ? nb_cnt_.p : nullptr, stats_.p, gate, stream_);
+                         final_list ? nb_cnt_.p : nullptr, stats_.p, snap, gate, stream_);
             return;
         }
     }
@@ -1550,6 +1552,7 @@ class Engine final : public fn_engine {
         f.n_pairs = (int)h_pairs_.size();
         f.radius = rad_d_.p; f.skin = skin_d_.p;
         f.list_out = nb_list_.p; f.cnt = nb_cnt_.p; f.stats = stats_.p;
+        f.need = need_.p;
         fnk::filter(f, stream_);
     }
 
@@ -1570,10 +1573,10 @@ class Engine final : public fn_engine {
         }
         fnk::disp_check(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, bl, bg, pack_, lim2_, need_.p,
                         cellbead_.p, key_.p, stream_);
+        // gated by the flag: the build, whose compaction also records the
+        // positions it was made at; the filter clears the flag
         enqueue_sort(pc, bg, need_.p);
-        search_and_compact(bl, false, false, need_.p);
-        fnk::snapshot(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, need_.p, stream_);
-        fnk::set_int(need_.p, 0, need_.p, stream_);
+        search_and_compact(bl, false, false, need_.p, fnk::Snapshot{p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p});
         enqueue_filter(pc, bg, bl);
     }
 
@@ -1812,6 +1815,7 @@ class Engine final : public fn_engine {
         adopt_ensemble(pk);
         cell_count_.alloc((size_t)C_ + 1);
         cell_start_.alloc((size_t)C_ + 1);
+        fnk::fill_int(cell_count_.p, 0, C_ + 1, stream_);
         HIPCK(hipStreamSynchronize(stream_));
         clear_graphs();
     }

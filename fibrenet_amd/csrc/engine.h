@@ -59,6 +59,7 @@ struct FilterArgs {
     int n_pairs;
     const double *radius, *skin;
     int *list_out, *cnt, *stats;
+    int *need;                       // the candidate-build flag, cleared by the filter (null: none)
 };
 
 template <typename T, typename P>
@@ -94,9 +95,11 @@ struct Launch {
 // or NULL for one network described by b)
 void hash(const double *px, const double *py, const double *pz, int n, const Box &b, const int *net,
           const Box *boxes, int *key, int *ckey, int *rank, int *count, const int *gate, hipStream_t s);
+// (count: the n_count cell counts the offsets were scanned from, zeroed for
+// the next sort.)
 void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
            const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n,
-           const Sorted &so, const int *gate, hipStream_t s);
+           const Sorted &so, int *count, int n_count, const int *gate, hipStream_t s);
 // Partners over the stencil into scratch rows of K entries per bead: the
 // candidates within sqrt(r2), or (exact) the pairs BuildContactList lists;
 // staged: the neighbourhood of each block read through LDS; flat: the 3x3x3
@@ -107,9 +110,16 @@ void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *p
             const double *skin, const Box &b, const int *net, const Box *boxes, const Stencil &st, double r2,
             bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat, const int *gate,
             hipStream_t s);
-// (stats: counts the build and keeps the largest list total of the run.)
+// Positions to copy into the displacement check's reference when the
+// candidates are built (rx null: none).
+struct Snapshot {
+    const double *px, *py, *pz;
+    double *rx, *ry, *rz;
+};
+// (stats: counts the build and keeps the largest list total of the run;
+// snap: the reference positions of the kept candidates.)
 void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
-             int cap, int *cnt_out, int *stats, const int *gate, hipStream_t s);
+             int cap, int *cnt_out, int *stats, const Snapshot &snap, const int *gate, hipStream_t s);
 // *need = 1 when a bead moved more than sqrt(lim2) since the snapshot; also
 // writes every bead's CellBead and FillGrid's cell index (key, -1 outside
 // the grid) for the filter and the exports.

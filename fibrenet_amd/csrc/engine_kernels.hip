@@ -571,10 +571,13 @@ __global__ void __launch_bounds__(kBlock) k_hash(const double *px, const double 
 __global__ void __launch_bounds__(kBlock) k_place(const int *key, const int *ckey, const int *rank,
                                                   const int *cell_start, const double *px, const double *py,
                                                   const double *pz, const int *type, const int *tag, const int *loc,
-                                                  int n, Sorted so, const int *gate)
+                                                  int n, Sorted so, int *count, int n_count, const int *gate)
 {
     if (gated_off(gate)) return;
     const int i = blockIdx.x * kBlock + threadIdx.x;
+    // the cell counts are spent (the offsets are scanned): zero them for the
+    // next sort, whose hash adds into them
+    for (int c = i; c < n_count; c += gridDim.x * kBlock) count[c] = 0;
     if (i >= n) return;
     const int c = ckey[i];
     const int s = cell_start[c] + rank[i];
@@ -1165,12 +1168,21 @@ __device__ __forceinline__ void sort_row(const int *scratch, int i, int m, int n
 
 __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *cand_start, const int *scratch, int K,
                                                     int n, int *cand, int cap, int *cnt_out, int *stats,
-                                                    const int *gate)
+                                                    Snapshot snap, const int *gate)
 {
     if (gated_off(gate)) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) {         // the build's statistics
         stats[kStatCandBuilds] += 1;
         stats[kStatCandMax] = max(stats[kStatCandMax], cand_start[n]);
+    }
+    {
+        // the positions the candidates were built at (the displacement check's reference)
+        const int b = blockIdx.x * kBlock + threadIdx.x;
+        if (snap.rx && b < n) {
+            snap.rx[b] = snap.px[b];
+            snap.ry[b] = snap.py[b];
+            snap.rz[b] = snap.pz[b];
+        }
     }
     __shared__ int stage[kBlock / 64][kCompactStage];
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1396,6 +1408,8 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     if (live) f.cnt[i] = kept;
     const int mxk = wave_max(kept);
     if (lane == 0) raise_max(&f.stats[kStatMaxList], mxk);
+    // the candidates are current again: no kernel of this rebuild reads the flag after the compaction
+    if (f.need && blockIdx.x == 0 && threadIdx.x == 0) *f.need = 0;
 }
 
 // ---- device-wide exclusive scan (n ints -> n+1 offsets) ----------------------------
@@ -1610,9 +1624,10 @@ void hash(const double *px, const double *py, const double *pz, int n, const Box
 }
 void place(const int *key, const int *ckey, const int *rank, const int *cell_start, const double *px,
            const double *py, const double *pz, const int *type, const int *tag, const int *loc, int n,
-           const Sorted &so, const int *gate, hipStream_t s)
+           const Sorted &so, int *count, int n_count, const int *gate, hipStream_t s)
 {
-    k_place<<<blocks(n), kBlock, 0, s>>>(key, ckey, rank, cell_start, px, py, pz, type, tag, loc, n, so, gate);
+    k_place<<<blocks(std::max(n, 1)), kBlock, 0, s>>>(key, ckey, rank, cell_start, px, py, pz, type, tag, loc, n, so,
+                                                       count, n_count, gate);
 }
 template <bool EXACT>
 static void search_t(const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs, const double *radius,
@@ -1647,10 +1662,10 @@ void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *p
                         n, staged, flat, gate, s);
 }
 void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
-             int cap, int *cnt_out, int *stats, const int *gate, hipStream_t s)
+             int cap, int *cnt_out, int *stats, const Snapshot &snap, const int *gate, hipStream_t s)
 {
     if (n > 0)
-        k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, scratch, K, n, cand, cap, cnt_out, stats, gate);
+        k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, scratch, K, n, cand, cap, cnt_out, stats, snap, gate);
 }
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
                 const double *rz, int n, const Box &b, const Box &grid, CellPack pack, double lim2, int *need,
