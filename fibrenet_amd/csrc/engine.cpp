@@ -1263,6 +1263,7 @@ class Engine final : public fn_engine {
     void ensure_grid(const fn_settings *s)
     {
         nb_start_.ensure((size_t)N_ + 1);
+        cstart_.ensure((size_t)N_ + 1);
         nb_cnt_.ensure(N_);
         ncnt_.ensure(N_);
         key_.ensure(N_);
@@ -1273,6 +1274,7 @@ class Engine final : public fn_engine {
         if (!nb_list_.p) grow_list(std::max<long long>(32LL * N_, 1 << 20));
         if (!nb_init_) {
             fnk::fill_int(nb_start_.p, 0, N_ + 1, stream_);
+            fnk::fill_int(cstart_.p, 0, N_ + 1, stream_);
             fnk::fill_int(nb_cnt_.p, 0, N_, stream_);
             fnk::fill_int(key_.p, -1, N_, stream_);
             fnk::fill_int(need_.p, 1, 1, stream_);
@@ -1504,10 +1506,12 @@ class Engine final : public fn_engine {
             fnk::search(cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b, ens_net(),
                         boxes_.p, final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, final_list,
                         ncnt_.p, scratch_.p, K_, stats_.p, N_, staged_, flat_, gate, stream_);
-            fnk::exclusive_scan(ncnt_.p, N_, nb_start_.p, scan_tmp_.p, gate, stream_);
+            // offsets: the contact list's (exact) or the candidates' CSR
+            int *starts = final_list ? nb_start_.p : cstart_.p;
+            fnk::exclusive_scan(ncnt_.p, N_, starts, scan_tmp_.p, gate, stream_);
             if (sync) {
                 int tot = 0, over = 0, dense = 0;
-                HIPCK(hipMemcpyAsync(&tot, nb_start_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+                HIPCK(hipMemcpyAsync(&tot, starts + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
                 HIPCK(hipMemcpyAsync(&over, stats_.p + fnk::kStatListOver, sizeof(int), hipMemcpyDeviceToHost, stream_));
                 HIPCK(hipMemcpyAsync(&dense, stats_.p + fnk::kStatDense, sizeof(int), hipMemcpyDeviceToHost, stream_));
                 HIPCK(hipStreamSynchronize(stream_));
@@ -1521,7 +1525,7 @@ class Engine final : public fn_engine {
                 if (tot > cap_) grow_list((long long)tot * 2);
                 last_total_ = tot;
             }
-            fnk::compact(ncnt_.p, nb_start_.p, scratch_.p, K_, N_, final_list ? nb_list_.p : cand_.p, cap_,
+            fnk::compact(ncnt_.p, starts, scratch_.p, K_, N_, final_list ? nb_list_.p : cand_.p, cap_,
                          final_list ? nb_cnt_.p : nullptr, stats_.p, snap, gate, stream_);
             return;
         }
@@ -1547,7 +1551,8 @@ class Engine final : public fn_engine {
         f.n = N_;
         f.grid = bg;
         f.list = bl;
-        f.cand_start = nb_start_.p; f.cand = cand_.p; f.cap = cap_;
+        f.cand_start = cstart_.p; f.cand = cand_.p; f.cap = cap_;
+        f.list_start = nb_start_.p;
         f.pairs = pairs_.p;
         f.n_pairs = (int)h_pairs_.size();
         f.radius = rad_d_.p; f.skin = skin_d_.p;
@@ -2672,6 +2677,7 @@ class Engine final : public fn_engine {
     DArr<int> rf_idx_, rf_off_;            // fn_group_force: flagged beads, their entries per network
     DArr<double> rf_part_, rf_out_;        // its partial sums and result
     DArr<fnk::PairC> pairs_;
+    DArr<int> cstart_;   // offsets of the kept candidates' rows (nb_start_: of the contact list)
     DArr<int> key_, ckey_, rank_, cell_count_, cell_start_, ncnt_, nb_start_, nb_cnt_, cand_, nb_list_, scan_tmp_,
         stats_, step_, need_;
     DArr<double> ref_[3];   // positions at the last candidate build

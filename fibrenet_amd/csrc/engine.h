@@ -54,6 +54,7 @@ struct FilterArgs {
     Box grid;                        // FillGrid's settings (cells, cell size)
     Box list;                        // BuildContactList's settings (box, minimum image, cutoff)
     const int *cand_start, *cand;    // candidate CSR
+    int *list_start;                 // out: where each bead's list starts (n + 1 entries)
     int cap;                         // entries reserved in cand and list_out
     const PairC *pairs;
     int n_pairs;

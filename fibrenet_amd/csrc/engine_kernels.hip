@@ -1287,6 +1287,9 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     __shared__ __align__(16) int before[kBlock / 64][kWaveChunk + 4];   // accepted entries of the pass before each slot
     __shared__ unsigned owner_words[kBlock / 64][kWaveChunk / 4];
     __shared__ FilterOwner fo[kBlock];
+    __shared__ int ptype[kPairCap];          // the pair table's types (entries a list addresses: at most kPairCap)
+    for (int k = threadIdx.x; k < kPairCap; k += kBlock) ptype[k] = k < f.n_pairs ? f.pairs[k].type : 0;
+    __syncthreads();
     const int wv = threadIdx.x >> 6;
     int *acc = verdict[wv];
     int *pre = before[wv];
@@ -1308,7 +1311,10 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     const int off = incl - cnt;
     const int total = __shfl(incl, 63, 64);
     const int my = (1 << (f.pack.sz - f.pack.sy)) - 1, mx = (1 << f.pack.sy) - 1;
-    int kept = 0;
+    // the wave's accepted entries go end to end from the start of its
+    // candidates (dense: the bead kernels read a wave's lists as one run)
+    const int wb = __shfl(e0, 0, 64);
+    int kept = 0, wkept = 0;
     if (total > 0) {                                     // wave-uniform
         fo[threadIdx.x] = FilterOwner{f.px[ic], f.py[ic], f.pz[ic], mcell & mx, (mcell >> f.pack.sy) & my,
                                       mcell >> f.pack.sz, e0 - off, f.radius[ic], f.skin[ic], 0, 0};
@@ -1347,7 +1353,8 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
                     const bool adj = (ax <= 1 || (per && ax == g.cx - 1)) && (ay <= 1 || (per && ay == g.cy - 1)) &&
                                      (az <= 1 || (per && az == g.cz - 1));
                     if (adj) {
-                        const int pt = f.pairs[ent[r] >> kPidShift].type;
+                        const int pid = ent[r] >> kPidShift;
+                        const int pt = pid < kPairCap ? ptype[pid] : f.pairs[pid].type;
                         double t = -1.0, root_t = 0.0;
                         if (pt == 1 || pt == 3) {
                             t = b.cutoff_sq;
@@ -1388,24 +1395,31 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
             *reinterpret_cast<int4 *>(pre + t0) = make_int4(ex, ex + a0, ex + a0 + a1, ex + a0 + a1 + a2);
             if (lane == 63) pre[kWaveChunk] = incl4;
             wave_sync_lds();
-            // this bead's place for the pass and its count so far
+            // this bead's count so far
             {
                 const int s0 = min(max(off - base, 0), n), s1 = min(max(off + cnt - base, 0), n);
-                ow[lane].ob = e0 + kept - pre[s0];
                 kept += pre[s1] - pre[s0];
             }
-            wave_sync_lds();
 #pragma unroll
             for (int r = 0; r < kChunkRounds; ++r) {
                 if (r * 64 >= n) break;                  // wave-uniform
                 const int t = r * 64 + lane;
                 const int v = acc[t];
-                if (t < n && v >= 0) f.list_out[ow[own[t]].ob + pre[t]] = v;
+                if (t < n && v >= 0) f.list_out[wb + wkept + pre[t]] = v;
             }
+            wkept += pre[kWaveChunk];
             wave_sync_lds();
         }
     }
-    if (live) f.cnt[i] = kept;
+    {
+        // each bead's list starts after the wave's earlier beads' lists
+        const int ik = wave_incl_scan(kept);
+        if (live) {
+            f.cnt[i] = kept;
+            f.list_start[i] = wb + ik - kept;
+        }
+        if (i == f.n - 1) f.list_start[f.n] = f.cand_start[f.n];   // the extent the lists lie in
+    }
     const int mxk = wave_max(kept);
     if (lane == 0) raise_max(&f.stats[kStatMaxList], mxk);
     // the candidates are current again: no kernel of this rebuild reads the flag after the compaction

@@ -157,27 +157,39 @@ def test_config1_f32_break_events_exact(need_gpu, config1):
 
 # ---- config 2 ----------------------------------------------------------------------------------
 
-def test_config2_ensemble_of_512_equals_plain_runs(need_gpu):
+_C2_CALLS, _C2_GUF = (12, 9), 8
+
+
+@pytest.fixture(scope="module")
+def config2_runs(need_gpu):
+    """The 512-network ensemble after two RunSimulationLoop calls: the
+    networks and the ensemble's bead records of each."""
     from fibrenet_amd import Ensemble
     nets = G.ensemble(512)
-    beads = sum(len(n.beads) for n in nets)
-    print("config 2 beads", beads, flush=True)
-    guf = 8
-    calls = (12, 9)
+    print("config 2 beads", sum(len(n.beads) for n in nets), flush=True)
     ens = Ensemble(nets, "f32")
-    for n in calls:
-        ens.run_steps(n, guf, True, True)
-    for k, net in enumerate(nets):
+    for n in _C2_CALLS:
+        ens.run_steps(n, _C2_GUF, True, True)
+    got = [ens.read(k, "bead").copy() for k in range(len(nets))]
+    ens.close()
+    return nets, got
+
+
+@pytest.mark.parametrize("part", range(4))
+def test_config2_ensemble_of_512_equals_plain_runs(need_gpu, config2_runs, part):
+    """Each quarter of the 512 networks run alone equals its share of the
+    ensemble bit for bit (a quarter per test keeps every test short)."""
+    nets, got = config2_runs
+    for k in range(part * 128, (part + 1) * 128):
+        net = nets[k]
         net.cell_nh = G.grid_neighbours(0, int(net.settings["cells_x"]), int(net.settings["cells_y"]),
                                         int(net.settings["cells_z"]))
         eng, ids = engine_for(net, "f32")
-        for n in calls:
-            eng.run_steps(n, guf, True, True)
-        assert ens.read(k, "bead").tobytes() == read_beads(eng, ids).tobytes(), k
+        eng.set_option("graphs", 0)          # direct launches: the same bytes without the graph captures
+        for n in _C2_CALLS:
+            eng.run_steps(n, _C2_GUF, True, True)
+        assert got[k].tobytes() == read_beads(eng, ids).tobytes(), k
         eng.close()
-        if k % 64 == 63:
-            print("config 2: %d networks equal" % (k + 1), flush=True)
-    ens.close()
 
 
 # ---- config 4 ----------------------------------------------------------------------------------
