@@ -490,6 +490,37 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
     }
 }
 
+// g[p] += v for the pair entry p of a lane's run of equal entries.
+__device__ __forceinline__ void add_pair(double (&g)[kAdjPairsLds], int p, double v)
+{
+#pragma unroll
+    for (int q = 0; q < kAdjPairsLds; ++q)
+        if (q == p) g[q] += v;
+}
+
+// The wave's sums of g[0..7] over its 64 lanes, in 10 exchanges instead of
+// 8 x 6: each exchange hands half of the remaining entries to the partner
+// lane, so that after the exchanges across lane bits 5, 4 and 3 a lane
+// holds entry p = lane >> 3 summed over 8 lanes, and the exchanges across
+// bits 2, 1 and 0 complete it.  Lane 8p returns the total of entry p.
+static_assert(kAdjPairsLds == 8, "wave_pair_sums halves 8 entries three times");
+__device__ __forceinline__ double wave_pair_sums(const double (&g)[kAdjPairsLds], int lane)
+{
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    double h4[4], h2[2];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+        h4[m] = (b5 ? g[m + 4] : g[m]) + __shfl_xor(b5 ? g[m] : g[m + 4], 32, 64);
+#pragma unroll
+    for (int m = 0; m < 2; ++m)
+        h2[m] = (b4 ? h4[m + 2] : h4[m]) + __shfl_xor(b4 ? h4[m] : h4[m + 2], 16, 64);
+    double v = (b3 ? h2[1] : h2[0]) + __shfl_xor(b3 ? h2[0] : h2[1], 8, 64);
+    v += __shfl_xor(v, 4, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 1, 64);
+    return v;
+}
+
 // The adjoint step of a single network in the chain layout with at most
 // kAdjPairsLds pair entries, in one launch: each lane evaluates the spring
 // to its right and the angle centred on it as k_adj_springs / k_adj_angles
@@ -499,8 +530,20 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_flat(AdjArgs<T, P> a)
 // adjoints equal k_adj_springs + k_adj_angles + k_adjoint_step_flat bit for
 // bit.  Contacts, walls and the update as in k_adjoint_step_flat; a wave
 // owns kChainOwned beads with kHalo lanes of neighbours on each side.
+//
+// A lane issues the global loads of its bead before its first global store
+// (the compiler keeps loads behind stores through other pointers): the
+// bead's own values, then its bonds' parameters and scale adjoints at once,
+// and in the contact passes a partner's position and force adjoint
+// together.  Spring (i, i+1) and the angle at i belong to bead i alone.
+//
+// Four waves per SIMD, as the LDS allows in the fp32 build (one 8-byte
+// register spill); the fp64 build's larger pair table leaves three, so the
+// target is not met there and the compiler's note about it is silenced.
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wpass-failed"
 template <typename T, typename P>
-__global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4, 4))) k_adjoint_step_chain(AdjArgs<T, P> a)
 {
     __shared__ PairC prs[kAdjPairsLds];
     __shared__ PairTab<T> prf[kAdjPairsLds];          // the forward's form, for its mask
@@ -510,7 +553,6 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
         prs[k] = a.pairs[k];
         prf[k] = to_tab<T>(a.pairs[k]);
     }
-    __syncthreads();
     AdjWaveLds<T> &w = wl[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const int i = (blockIdx.x * (kBlock / 64) + (int)(threadIdx.x >> 6)) * kChainOwned + lane - kHalo;
@@ -519,8 +561,61 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     const int ic = in ? i : (i < 0 ? 0 : a.n - 1);
     const double3 fb = a.fb[ic];
     const P xi = a.ix[ic], yi = a.iy[ic], zi = a.iz[ic];
+    const bool alias = a.ix == a.px;                  // images are the positions
+    const P pxi = alias ? xi : a.px[ic], pyi = alias ? yi : a.py[ic], pzi = alias ? zi : a.pz[ic];
+    const T ri = a.radius[ic];
+    const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
     const int s = (in && a.springs) ? a.ch_spring[i] : -1;
     const int q = (in && a.angles) ? a.ch_angle[i] : -1;
+    const unsigned code = (in && a.bond_code) ? a.bond_code[i] : 0u;
+    double3 X{0.0, 0.0, 0.0}, W{0.0, 0.0, 0.0};
+    T lam = (T)0, h = (T)0, h2 = (T)0;
+    bool grp = false;
+    if (own) {
+        X = a.xb[i];
+        W = a.wb[i];
+        lam = a.lam[i];
+        h = a.hlf[i];
+        h2 = a.hlf2[i];
+        grp = a.seed_prev && a.group[i];
+    }
+    // the bonds: spring (i, i+1) and the angle at i
+    const bool s_ok = s >= 0 && s < a.n_springs, q_ok = q >= 0 && q < a.n_angles;
+    int s_on = 0, s_ty = -1, q_on = 0, q_ty = 0;
+    double s_l0 = 0.0, gs = 0.0, ga = 0.0;
+    T s_k = (T)0, q_th0 = (T)0, q_k = (T)0;
+    if (s_ok) {
+        s_on = a.sp_on[s];
+        if (a.bond_code) {                            // the sets hold the bond arrays' values exactly
+            const SpringSet &ss = a.sp_sets[code & kNoSet];
+            s_ty = ss.ty;
+            s_k = (T)ss.k;
+            s_l0 = ss.l0;
+        } else {
+            s_ty = a.sp_type[s];
+            s_k = a.sp_k[s];
+            s_l0 = a.sp_l0[s];
+        }
+        if (own) gs = a.g_spring[s];
+    }
+    if (q_ok) {
+        q_on = a.an_on[q];
+        if (a.bond_code) {
+            const AngleSet &as = a.an_sets[(code >> 16) & kNoSet];
+            q_ty = as.ty;
+            q_th0 = (T)as.th0;
+            q_k = (T)as.k;
+        } else {
+            q_ty = a.an_type[q];
+            q_th0 = a.an_th0[q];
+            q_k = a.an_k[q];
+        }
+        if (own) ga = a.g_angle[q];
+    }
+    // LDS: the pair table, and the lane's bead for the contact passes
+    w.ow[lane] = AdjOwner{(double)pxi, (double)pyi, (double)pzi, fb.x, fb.y, fb.z};
+    w.orad[lane] = ri;
+    __syncthreads();
     // neighbours i + 1 and i - 1 (meaningless in the outermost lanes, whose
     // results no owned lane uses)
     const P xn = lane_next(xi), yn = lane_next(yi), zn = lane_next(zi);
@@ -529,32 +624,34 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     const double3 fbp{lane_prev(fb.x), lane_prev(fb.y), lane_prev(fb.z)};
     // spring (i, i+1): J Fb on p1 = i (p2 receives its negation)
     double3 jr{0.0, 0.0, 0.0};
-    if (s >= 0 && s < a.n_springs && a.sp_on[s] == 1 && (a.sp_type[s] == 0 || a.sp_type[s] == 1)) {
+    const bool s_act = s_on == 1 && (s_ty == 0 || s_ty == 1);
+    if (s_act) {
         const double3 u = sub3(fb, fbn);
         const double3 d{(double)(xi - xn), (double)(yi - yn), (double)(zi - zn)};
-        const DV3 F = spring_dual((double)a.sp_k[s], a.sp_l0[s], dvec(d, u));
+        const DV3 F = spring_dual((double)s_k, s_l0, dvec(d, u));
         jr = double3{F.x.d, F.y.d, F.z.d};
-        if (own) a.g_spring[s] += dot_v(u, F);
+        gs += dot_v(u, F);
     }
     // angle (i-1, i, i+1): J Fb on b1 (ja), b3 (jc) and b2 (-ja - jc)
     double3 ja{0.0, 0.0, 0.0}, jc{0.0, 0.0, 0.0};
-    if (q >= 0 && q < a.n_angles && a.an_on[q] == 1) {
+    if (q_on == 1) {
         const double3 x2{(double)xi, (double)yi, (double)zi};
         const double3 ba{(double)xp - x2.x, (double)yp - x2.y, (double)zp - x2.z};
         const double3 bc{(double)xn - x2.x, (double)yn - x2.y, (double)zn - x2.z};
         const double3 da = sub3(fbp, fb), dc = sub3(fbn, fb);
         DV3 pa, pc;
-        angle_dual((double)a.an_k[q], (double)a.an_th0[q], dvec(ba, da), dvec(bc, dc), pa, pc);
+        angle_dual((double)q_k, (double)q_th0, dvec(ba, da), dvec(bc, dc), pa, pc);
         ja = double3{pa.x.d, pa.y.d, pa.z.d};
         jc = double3{pc.x.d, pc.y.d, pc.z.d};
         if (own) {
             V3<T> fa, fc;                             // the forward's own values
-            angle_force(AngleP<T>{1, a.an_type[q], a.an_th0[q], a.an_k[q]}, xp, yp, zp, xi, yi, zi, xn, yn, zn, fa,
-                        fc);
-            a.g_angle[q] += da.x * (double)fa.x + da.y * (double)fa.y + da.z * (double)fa.z +
-                            dc.x * (double)fc.x + dc.y * (double)fc.y + dc.z * (double)fc.z;
+            angle_force(AngleP<T>{1, q_ty, q_th0, q_k}, xp, yp, zp, xi, yi, zi, xn, yn, zn, fa, fc);
+            ga += da.x * (double)fa.x + da.y * (double)fa.y + da.z * (double)fa.z +
+                  dc.x * (double)fc.x + dc.y * (double)fc.y + dc.z * (double)fc.z;
         }
     }
+    if (own && s_act) a.g_spring[s] = gs;
+    if (own && q_on == 1) a.g_angle[q] = ga;
     const int hsl = lane_prev(s >= 0 ? 1 : 0);
     const int hql = lane_prev(q >= 0 ? 1 : 0), hqn = lane_next(q >= 0 ? 1 : 0);
     const double3 jl{lane_prev(jr.x), lane_prev(jr.y), lane_prev(jr.z)};
@@ -592,9 +689,6 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     }
     // contacts, flattened over the wave as in k_adjoint_step_flat
     const Box &bf = a.force;
-    const P pxi = a.px[ic], pyi = a.py[ic], pzi = a.pz[ic];
-    const T ri = a.radius[ic];
-    const int e0 = a.nb_start[ic], cnt = own ? a.nb_cnt[ic] : 0;
     int incl = cnt;
     incl = wave_incl_scan(incl);
     const int off = incl - cnt;
@@ -602,8 +696,8 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     double g[kAdjPairsLds];
 #pragma unroll
     for (int p = 0; p < kAdjPairsLds; ++p) g[p] = 0.0;
-    w.ow[lane] = AdjOwner{(double)pxi, (double)pyi, (double)pzi, fb.x, fb.y, fb.z};
-    w.orad[lane] = ri;
+    int gp = -1;                // the pair entry of the lane's current run of terms, and its sum
+    double gacc = 0.0;
     for (int base = 0; base < total; base += kWaveChunk) {           // wave-uniform
         const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
         mark_owner(off, cnt, base, w.own);
@@ -619,8 +713,10 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
             if (k < n) {
                 const int ent = a.nb_list[e];
                 const int j = ent & kIdxMask, pid = ent >> kPidShift;
+                const P pxj = a.px[j], pyj = a.py[j], pzj = a.pz[j];
+                const double3 fbj = a.fb[j];
                 const PairC pr = prs[pid];
-                P rx = xL - a.px[j], ry = yL - a.py[j], rz = zL - a.pz[j];
+                P rx = xL - pxj, ry = yL - pyj, rz = zL - pzj;
                 P len = rx * rx + ry * ry + rz * rz;
                 if (len > (P)bf.hbs) {           // ResolveContacts' minimum image (kernel.cl:844-870)
                     if (rx * rx > (P)bf.hbs) rx += (rx > (P)0) ? (P)-bf.lx : (P)bf.lx;
@@ -632,12 +728,15 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
                 if (contact_force(prf[pid], bf, rx, ry, rz, len, rL, a.radius, j, tf)) {   // the forward's mask
                     const double rj = pr.type == 2 ? (double)a.radius[j] : 0.0;
                     const DV3 F = contact_dual(pr, (double)rL, rj,
-                                               dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fL, a.fb[j])));
+                                               dvec(double3{(double)rx, (double)ry, (double)rz}, sub3(fL, fbj)));
                     t = double3{F.x.d, F.y.d, F.z.d};
                     const double gv = dot_v(fL, F);
-#pragma unroll
-                    for (int p = 0; p < kAdjPairsLds; ++p)
-                        if (p == pid) g[p] += gv;
+                    if (pid != gp) {
+                        if (gp >= 0) add_pair(g, gp, gacc);
+                        gp = pid;
+                        gacc = 0.0;
+                    }
+                    gacc += gv;
                 }
             }
             if (k < n) w.term[k] = t;
@@ -662,30 +761,26 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
             if (pyi < (P)0 || pyi > (P)bd.ly) z.y += k * fb.y;
             if (pzi < (P)0 || pzi > (P)bd.lz) z.z += k * fb.z;
         }
-        double3 X = a.xb[i];
         X.x += z.x;
         X.y += z.y;
         X.z += z.z;
-        const double lam = a.lam[i], h = a.hlf[i], h2 = a.hlf2[i];
         const double dt = (double)(T)bd.dt, D = (double)(T)bd.damping;
         const double3 vb{D * fb.x, D * fb.y, D * fb.z};
-        double3 W = a.wb[i];
-        const double3 fprev{h * W.x + lam * vb.x + h2 * X.x, h * W.y + lam * vb.y + h2 * X.y,
-                            h * W.z + lam * vb.z + h2 * X.z};
+        const double hd = h, lamd = lam, h2d = h2;
+        const double3 fprev{hd * W.x + lamd * vb.x + h2d * X.x, hd * W.y + lamd * vb.y + h2d * X.y,
+                            hd * W.z + lamd * vb.z + h2d * X.z};
         W.x += vb.x + dt * X.x;
         W.y += vb.y + dt * X.y;
         W.z += vb.z + dt * X.z;
         a.xb[i] = X;
         a.wb[i] = W;
-        const double seed = (a.seed_prev && a.group[i]) ? a.seed_prev[0] : 0.0;
-        a.fb_next[i] = double3{fprev.x + h * W.x + seed, fprev.y + h * W.y, fprev.z + h * W.z};
+        const double seed = grp ? a.seed_prev[0] : 0.0;
+        a.fb_next[i] = double3{fprev.x + hd * W.x + seed, fprev.y + hd * W.y, fprev.z + hd * W.z};
     }
-#pragma unroll
-    for (int p = 0; p < kAdjPairsLds; ++p) {
-        double v = g[p];
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-        if (lane == 0) w.g[p] = v;
+    if (gp >= 0) add_pair(g, gp, gacc);
+    {
+        const double v = wave_pair_sums(g, lane);
+        if ((lane & 7) == 0) w.g[lane >> 3] = v;
     }
     __syncthreads();
     if ((int)threadIdx.x < np) {
@@ -695,12 +790,16 @@ __global__ void __launch_bounds__(kBlock) k_adjoint_step_chain(AdjArgs<T, P> a)
     }
 }
 
+#pragma clang diagnostic pop
+
 // Blocks of the chain-layout adjoint kernel for n beads.
 static inline int chain_adj_blocks(int n)
 {
     constexpr int per_block = kChainOwned * (kBlock / 64);
     return (n + per_block - 1) / per_block;
 }
+
+bool adjoint_one_launch(bool chain, bool batched, int np) { return chain && !batched && np <= kAdjPairsLds; }
 
 int adjoint_pair_rows(int n, int np, bool batched, bool chain)
 {
@@ -710,11 +809,11 @@ int adjoint_pair_rows(int n, int np, bool batched, bool chain)
 
 // Positions (and images, when ix is not null) and bond statuses of one step
 // into its record: rec_pos = x, y, z[, ix, iy, iz] (n each), rec_st = spring
-// then angle statuses.
+// then angle flags (1 where the status is 1: the bond acts in the step).
 __global__ void __launch_bounds__(kBlock) k_record_step(const double *px, const double *py, const double *pz,
                                                         const double *ix, const double *iy, const double *iz,
                                                         const int *sp_st, int ns, const int *an_st, int na, int n,
-                                                        double *rec_pos, int *rec_st)
+                                                        double *rec_pos, unsigned char *rec_st)
 {
     const int k = blockIdx.x * kBlock + threadIdx.x;
     if (k < n) {
@@ -727,12 +826,12 @@ __global__ void __launch_bounds__(kBlock) k_record_step(const double *px, const 
             rec_pos[5 * (size_t)n + k] = iz[k];
         }
     }
-    if (k < ns) rec_st[k] = sp_st[k];
-    if (k < na) rec_st[ns + k] = an_st[k];
+    if (k < ns) rec_st[k] = sp_st[k] == 1 ? 1 : 0;
+    if (k < na) rec_st[ns + k] = an_st[k] == 1 ? 1 : 0;
 }
 
 void record_step(const double *const p[3], const double *const im[3], const int *sp_st, int ns, const int *an_st,
-                 int na, int n, double *rec_pos, int *rec_st, hipStream_t s)
+                 int na, int n, double *rec_pos, unsigned char *rec_st, hipStream_t s)
 {
     const int m = std::max(n, std::max(ns, na));
     if (m > 0)

@@ -693,6 +693,7 @@ class Engine final : public fn_engine {
         }
         else if (name == "skin") {
             if (value < 0 || value > 1000) throw Error(FN_ERR_ARG, "skin must lie in [0, 1000] permille of the reach");
+            if (value != skin_permille_) adj_.valid = false;   // its recomputation keeps the run's candidates
             skin_permille_ = value;
         }
         else if (name == "graphs") graphs_on_ = value != 0;
@@ -2262,6 +2263,43 @@ class Engine final : public fn_engine {
         if (keep_cand_) fnk::fill_int(need_.p, 1, 1, stream_);
     }
 
+    // Kept candidates at a checkpoint of an adjoint run: their CSR and the
+    // positions they were built at.  Between rebuilds the build flag is 0
+    // (every rebuild's filter clears it), so it is not part of the copy.
+    struct CandSnap {
+        DArr<int> start, list;
+        DArr<double> ref[3];
+        int count = 0;
+    };
+    void d2d(void *dst, const void *src, size_t bytes)
+    {
+        if (bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_));
+    }
+    void snapshot_candidates()
+    {
+        int total = 0;
+        HIPCK(hipMemcpyAsync(&total, cstart_.p + N_, sizeof(int), hipMemcpyDeviceToHost, stream_));
+        HIPCK(hipStreamSynchronize(stream_));
+        auto c = std::make_unique<CandSnap>();
+        c->count = std::max(0, std::min(total, cap_));
+        c->start.alloc((size_t)N_ + 1);
+        c->list.alloc(std::max(c->count, 1));
+        d2d(c->start.p, cstart_.p, sizeof(int) * ((size_t)N_ + 1));
+        d2d(c->list.p, cand_.p, sizeof(int) * (size_t)c->count);
+        for (int q = 0; q < 3; ++q) {
+            c->ref[q].alloc(N_);
+            d2d(c->ref[q].p, ref_[q].p, sizeof(double) * N_);
+        }
+        cand_ck_.push_back(std::move(c));
+    }
+    void restore_candidates(const CandSnap &c)
+    {
+        d2d(cstart_.p, c.start.p, sizeof(int) * ((size_t)N_ + 1));
+        d2d(cand_.p, c.list.p, sizeof(int) * (size_t)c.count);
+        for (int q = 0; q < 3; ++q) d2d(ref_[q].p, c.ref[q].p, sizeof(double) * N_);
+        fnk::fill_int(need_.p, 0, 1, stream_);
+    }
+
     void adjoint_forward(const fn_adjoint_spec &sp, const int *group, const int *steps, int ns,
                          double *samples) override
     {
@@ -2308,31 +2346,66 @@ class Engine final : public fn_engine {
         samp_.ensure(std::max((size_t)ns * nn, (size_t)1));
         adj_.ck_pc.assign(nseg, 0);
         adj_.ck_fc.assign(nseg, 0);
-        if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
-        fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
-        int si = 0;
-        for (int seg = 0; seg < nseg; ++seg) {
-            const int c = seg * every, len = std::min(every, n - c);
-            adj_.ck_pc[seg] = pc_;
-            adj_.ck_fc[seg] = fc_;
-            save_parts(state_parts(pc_, fc_, false), ck_.p + cb * seg);
-            const StepHook hook = [&](int k, int, int fw) {
-                if (si < ns && steps[si] == c + k) {
-                    fnk::group_force(frc_[fw][0].p, group_idx_.p, group_off_.p, nn, max_len, part_.p,
-                                     samp_.p + (size_t)si * nn, stream_);
-                    ++si;
+        // Kept candidates carry over from one segment to the next; the
+        // recomputation of a segment restores the ones its start held (a copy
+        // per candidate build), so both runs build them at the same steps from
+        // the same positions.  Only the first segment builds them afresh.
+        adj_.ck_cand.assign(nseg, -1);
+        presize_rows();
+        // A run whose search rows or lists overflow is invalid: it is run
+        // again from its start (the first checkpoint) with the grown
+        // capacities, as run_steps does.
+        for (int attempt = 0;; ++attempt) {
+            cand_ck_.clear();
+            int snap_builds = -1;
+            if ((long long)last_total_ * 3 / 2 + 1024 > (long long)cap_) grow_list((long long)last_total_ * 2 + 1024);
+            fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
+            int si = 0;
+            for (int seg = 0; seg < nseg; ++seg) {
+                const int c = seg * every, len = std::min(every, n - c);
+                if (seg > 0 || attempt == 0) {
+                    adj_.ck_pc[seg] = pc_;
+                    adj_.ck_fc[seg] = fc_;
+                    save_parts(state_parts(pc_, fc_, false), ck_.p + cb * seg);
                 }
-            };
-            const bool first = seg == 0, last = seg == nseg - 1;
-            fresh_candidates();
-            enqueue_chunk(len, guf, spr, ang, first, last, nullptr, &hook);
-            HIPCK(hipGetLastError());
-            flip(len, first, last);
+                if (seg == 0 || !keep_cand_) {
+                    fresh_candidates();
+                } else {
+                    int builds = 0;
+                    HIPCK(hipMemcpyAsync(&builds, stats_.p + fnk::kStatCandBuilds, sizeof(int),
+                                         hipMemcpyDeviceToHost, stream_));
+                    HIPCK(hipStreamSynchronize(stream_));
+                    if (builds != snap_builds) {
+                        snapshot_candidates();
+                        snap_builds = builds;
+                    }
+                    adj_.ck_cand[seg] = (int)cand_ck_.size() - 1;
+                }
+                const StepHook hook = [&](int k, int, int fw) {
+                    if (si < ns && steps[si] == c + k) {
+                        fnk::group_force(frc_[fw][0].p, group_idx_.p, group_off_.p, nn, max_len, part_.p,
+                                         samp_.p + (size_t)si * nn, stream_);
+                        ++si;
+                    }
+                };
+                const bool first = seg == 0, last = seg == nseg - 1;
+                enqueue_chunk(len, guf, spr, ang, first, last, nullptr, &hook);
+                HIPCK(hipGetLastError());
+                flip(len, first, last);
+            }
+            HIPCK(hipStreamSynchronize(stream_));
+            try {
+                check_capacity();
+                break;
+            } catch (const Error &e) {
+                if (e.code != FN_ERR_CAPACITY || attempt == 3) throw;
+            }
+            pc_ = adj_.ck_pc[0];
+            fc_ = adj_.ck_fc[0];
+            load_parts(state_parts(pc_, fc_, false), ck_.p);
         }
         steps_ += n;
         rebuilds_ += (n + guf - 1) / guf;
-        HIPCK(hipStreamSynchronize(stream_));
-        check_capacity();
         const std::vector<Part> fin = state_parts(pc_, fc_, true);
         fin_.ensure(parts_bytes(fin));
         save_parts(fin, fin_.p);
@@ -2411,8 +2484,10 @@ class Engine final : public fn_engine {
         const int prow = fnk::adjoint_pair_rows(N_, cols, ens, chain_adj);
         gpr_.ensure(std::max<size_t>((size_t)prow * cols, 1));
         gps_.ensure(std::max(np, 1));
-        sp_jv_.ensure(std::max<size_t>(2 * (size_t)S_, 1));
-        an_jv_.ensure(std::max<size_t>(3 * (size_t)A_, 1));
+        if (!fnk::adjoint_one_launch(chain_adj, ens, cols)) {
+            sp_jv_.ensure(std::max<size_t>(2 * (size_t)S_, 1));
+            an_jv_.ensure(std::max<size_t>(3 * (size_t)A_, 1));
+        }
         HIPCK(hipMemsetAsync(xb_.p, 0, sizeof(double3) * N, stream_));
         HIPCK(hipMemsetAsync(wb_.p, 0, sizeof(double3) * N, stream_));
         HIPCK(hipMemsetAsync(gsp_.p, 0, sizeof(double) * gsp_.n, stream_));
@@ -2427,9 +2502,6 @@ class Engine final : public fn_engine {
         rec_pos_.ensure((size_t)every * nimg * N);
         rec_st_.ensure(std::max<size_t>((size_t)every * SA, 1));
         rec_nb_.ensure((size_t)nr * lrow);
-        auto d2d = [&](void *dst, const void *src, size_t bytes) {
-            if (bytes) HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, stream_));
-        };
         fnk::adjoint_seed<T>(fb_[0].p, wb_.p, hlf_.p, group_.p, seed_at(n - 1), ens_net(), N_, stream_);
         int cur = 0;
         for (int seg = nseg - 1; seg >= 0; --seg) {
@@ -2437,7 +2509,10 @@ class Engine final : public fn_engine {
             pc_ = adj_.ck_pc[seg];
             fc_ = adj_.ck_fc[seg];
             load_parts(state_parts(pc_, fc_, false), ck_.p + adj_.cb * seg);
-            fresh_candidates();
+            if (adj_.ck_cand[seg] < 0)
+                fresh_candidates();
+            else
+                restore_candidates(*cand_ck_[adj_.ck_cand[seg]]);
             const StepHook hook = [&](int k, int pr, int) {
                 const double *p[3] = {pos_[pr][0].p, pos_[pr][1].p, pos_[pr][2].p};
                 const double *im[3] = {img_[pr][0].p, img_[pr][1].p, img_[pr][2].p};
@@ -2454,7 +2529,7 @@ class Engine final : public fn_engine {
             for (int k = len - 1; k >= 0; --k) {
                 const P *rp = rec_pos_.p + (size_t)k * nimg * N;
                 const P *ri = alias_ ? rp : rp + 3 * N;
-                const int *rs = rec_st_.p + (size_t)k * SA;
+                const unsigned char *rs = rec_st_.p + (size_t)k * SA;
                 const int *rl = rec_nb_.p + (size_t)(k / guf) * lrow;
                 fnk::AdjArgs<T, P> a{};
                 a.n = N_;
@@ -2486,6 +2561,11 @@ class Engine final : public fn_engine {
                 if (chain_adj) {
                     a.ch_spring = ch_spring_.p;
                     a.ch_angle = ch_angle_.p;
+                    if (cmp_ok_) {
+                        a.bond_code = bond_code_.p;
+                        a.sp_sets = sp_sets_.p;
+                        a.an_sets = an_sets_.p;
+                    }
                 }
                 fnk::adjoint_step(a, stream_);
                 cur ^= 1;
@@ -2520,7 +2600,7 @@ class Engine final : public fn_engine {
         bool valid = false, spr = false, ang = false, keep = false;
         int n = 0, guf = 1, every = 1, nseg = 0, n_spr = 0, n_ang = 0, cap = 0, fin_pc = 0, fin_fc = 0, n_net = 1;
         size_t cb = 0;
-        std::vector<int> ck_pc, ck_fc, steps;
+        std::vector<int> ck_pc, ck_fc, ck_cand, steps;   // ck_cand: kept candidates' copy (-1: built afresh)
     };
 
     void run_chunk(int len, int guf, bool spr, bool ang, bool first, bool last)
@@ -2664,6 +2744,7 @@ class Engine final : public fn_engine {
     DArr<fnk::AngleSet> an_sets_;
     AdjRun adj_;                           // the last adjoint forward run
     DArr<unsigned char> ck_, fin_;         // its checkpoints (one per segment) and end state
+    std::vector<std::unique_ptr<CandSnap>> cand_ck_;   // and the kept candidates its segments start from
     DArr<int> group_, group_idx_;          // sampled beads: flags, ascending indices
     DArr<int> group_off_, net_off_, pair_off_;   // per network: group entries, beads, pair entries
     DArr<double> seed_;                    // dL/d(sample) rows of the sampled steps
@@ -2673,7 +2754,8 @@ class Engine final : public fn_engine {
     DArr<double3> fb_[2], xb_, wb_;        // adjoint state
     DArr<double> gsp_, gan_, gpr_, gps_;   // scale adjoints (contacts per bead and pair entry, then summed)
     DArr<P> rec_pos_;                      // per step of a segment: positions and images of the force evaluation
-    DArr<int> rec_st_, rec_nb_;            // per step: bond statuses; per rebuild: the contact list
+    DArr<unsigned char> rec_st_;           // per step: the bonds that act
+    DArr<int> rec_nb_;                     // per rebuild: the contact list
     DArr<int> rf_idx_, rf_off_;            // fn_group_force: flagged beads, their entries per network
     DArr<double> rf_part_, rf_out_;        // its partial sums and result
     DArr<fnk::PairC> pairs_;

@@ -145,10 +145,11 @@ struct AdjArgs {
     const Box *boxes;
     const P *px, *py, *pz, *ix, *iy, *iz;
     const T *lam, *hlf, *hlf2, *radius;
-    const int *sp_start, *sp_slot, *sp_p1, *sp_p2, *sp_type, *sp_on;
+    const int *sp_start, *sp_slot, *sp_p1, *sp_p2, *sp_type;
+    const unsigned char *sp_on, *an_on;   // per bond: 1 where it acted in the step (status 1), else 0
     const double *sp_l0;
     const T *sp_k;
-    const int *an_start, *an_slot, *an_b1, *an_b2, *an_b3, *an_type, *an_on;
+    const int *an_start, *an_slot, *an_b1, *an_b2, *an_b3, *an_type;
     const T *an_th0, *an_k;
     const int *nb_start, *nb_cnt, *nb_list;
     const PairC *pairs;
@@ -163,6 +164,9 @@ struct AdjArgs {
     double *g_pair;               // and of the contact pair entries: adjoint_pair_rows rows of n_pairs
                                   // (batched: one row per bead, entries local to its network)
     const int *ch_spring, *ch_angle;   // chain layout (single network): spring (i, i+1), angle at i; or null
+    const unsigned *bond_code;         // chain layout: the parameter sets of those bonds (statuses from sp_on /
+    const SpringSet *sp_sets;          // an_on); or null: the parameters by bond index
+    const AngleSet *an_sets;
 };
 // One adjoint step: the bond terms (when a.springs / a.angles), then the
 // bead update.
@@ -175,6 +179,9 @@ void adjoint_seed(double3 *fb, const double3 *wb, const T *hlf, const int *group
 // Rows of the contact scale adjoints g_pair (row * n_pairs + entry) that the
 // adjoint step accumulates for n beads (chain: bonds in the chain layout).
 int adjoint_pair_rows(int n, int np, bool batched, bool chain);
+// Whether adjoint_step takes the bonds in its one chain-layout launch (so
+// that sp_jv / an_jv are not used).
+bool adjoint_one_launch(bool chain, bool batched, int np);
 // Batched ensemble: out[pair_base[k] + t] = sum of g[i * cols + t] over the
 // beads i in [bead_off[k], bead_off[k + 1]), t < pair_base[k + 1] - pair_base[k].
 void sum_network_columns(const double *g, int cols, const int *bead_off, const int *pair_base, int n_net,
@@ -182,7 +189,7 @@ void sum_network_columns(const double *g, int cols, const int *bead_off, const i
 // One step's positions (images when im is not null) and bond statuses into
 // its record (see k_record_step).
 void record_step(const double *const p[3], const double *const im[3], const int *sp_st, int ns, const int *an_st,
-                 int na, int n, double *rec_pos, int *rec_st, hipStream_t s);
+                 int na, int n, double *rec_pos, unsigned char *rec_st, hipStream_t s);
 // out[t] = sum over the n rows of g[row * np + t]; part holds kSumBlocks * np
 // doubles.  Deterministic (fixed partition and order).
 constexpr int kSumBlocks = 256;

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 import torch
 
-from fibrenet_amd import records as R
+from fibrenet_amd import FibreNetError, records as R
 from fibrenet_amd.autograd import TensileRun, tensile_samples
 from helpers import ADJ_NETS, adjoint_net, dual_run, engine_for, read_beads
 
@@ -303,24 +303,43 @@ def test_backward_outputs_selection(need_gpu):
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 @pytest.mark.parametrize("kind", ["hs_fl", "lj"])
 def test_kept_candidates_give_the_same_gradients(need_gpu, prec, kind):
-    """With kept contact candidates (verlet 1) the adjoint's forward run and
-    its recomputed segments rebuild candidates at the same steps and filter
-    them to the one-pass rebuild's lists: samples, gradients and the end
-    state equal the exact rebuild's bit for bit."""
+    """With kept contact candidates (verlet 1) the adjoint's forward run
+    carries its candidates from segment to segment and each recomputed
+    segment restores the ones its start held, so both rebuild candidates at
+    the same steps and filter them to the one-pass rebuild's lists: samples,
+    gradients and the end state equal the exact rebuild's bit for bit, also
+    with the widest skin, whose candidates outlive several segments."""
     net = ADJ_NETS[kind]()
     w = _weights()
     out = []
-    for verlet in (0, 1):
+    for verlet, skin in ((0, None), (1, None), (1, 1000)):
         eng, ids = engine_for(net, prec)
         eng.set_option("verlet", verlet)
+        if skin is not None:
+            eng.set_option("skin", skin)
         s = eng.adjoint_forward(N_STEPS, _group(net), SAMPLES, grid_update_freq=GUF, checkpoint_every=50)
         builds = eng.stats()["candidate_builds"]
         g = eng.adjoint_backward(w, len(net.springs), len(net.angles), len(net.pairs))
         out.append((s.tobytes(), {k: v.tobytes() for k, v in g.items()}, read_beads(eng, ids).tobytes()))
         if verlet == 1:
-            assert builds >= N_STEPS // 50, builds
+            assert builds >= 1, builds
         assert eng.get_option("verlet") == verlet
-    assert out[0][0] == out[1][0]
-    for k in out[0][1]:
-        assert out[0][1][k] == out[1][1][k], k
-    assert out[0][2] == out[1][2]
+    for o in out[1:]:
+        assert out[0][0] == o[0]
+        for k in out[0][1]:
+            assert out[0][1][k] == o[1][k], k
+        assert out[0][2] == o[2]
+
+
+def test_skin_change_invalidates_the_adjoint_run(need_gpu):
+    """The recomputation restores the forward run's candidates, which hold
+    for its skin only: a new skin leaves no run to differentiate."""
+    net = ADJ_NETS["lj"]()
+    eng, _ = engine_for(net, "f32")
+    eng.set_option("verlet", 1)
+    eng.adjoint_forward(60, _group(net), SAMPLES[:3], grid_update_freq=GUF, checkpoint_every=20)
+    eng.set_option("skin", eng.get_option("skin"))          # unchanged: the run stays
+    eng.adjoint_backward(_weights()[:3], len(net.springs), len(net.angles), len(net.pairs))
+    eng.set_option("skin", eng.get_option("skin") + 100)
+    with pytest.raises(FibreNetError):
+        eng.adjoint_backward(_weights()[:3], len(net.springs), len(net.angles), len(net.pairs))
