@@ -229,12 +229,10 @@ def adjoint_bench(net, precision, options, warmup, steps, every):
     plain forward is timed on the same engine over the next `steps` steps
     (graph-captured, after a capture run), and the ratios are against it.
 
-    The randomly placed fibres of this workload cross one another, and with
-    Lennard-Jones contacts the overlapping beads are launched: beads that
-    leave the finite box are not hashed (kernel.cl:494-499), lose their
-    contacts and can come back overlapping other beads.  From about step 900
-    the largest speed grows from ~60 to 1e6 and beyond, in the fp64 build
-    as in the fp32 one, so later states carry non-finite derivatives."""
+    The first backward call allocates the records of a segment (positions,
+    bond flags and contact lists of `every` steps, about 3 GB here); the
+    repeat figures time a second backward over the same forward run, as in
+    each iteration after the first of a fitting loop."""
     import numpy as np
     from fibrenet_amd import Engine, bind_standard
     from fibrenet_amd.api import SCALE_OUTPUTS
@@ -256,6 +254,11 @@ def adjoint_bench(net, precision, options, warmup, steps, every):
         g = eng.adjoint_backward(np.ones(len(samples)), len(net.springs), len(net.angles), len(net.pairs),
                                  SCALE_OUTPUTS)
         t2 = time.perf_counter()
+        # the same backward again: its records and buffers are allocated now,
+        # as in every call after the first of a fitting loop
+        eng.adjoint_backward(np.ones(len(samples)), len(net.springs), len(net.angles), len(net.pairs),
+                             SCALE_OUTPUTS)
+        t2b = time.perf_counter()
         eng.run_steps(steps, guf, True, True)          # graph capture
         eng.finish()
         t3 = time.perf_counter()
@@ -264,10 +267,12 @@ def adjoint_bench(net, precision, options, warmup, steps, every):
     finally:
         eng.close()
     fwd, bwd, plain = (t1 - t0) * 1e3 / steps, (t2 - t1) * 1e3 / steps, (t4 - t3) * 1e3 / steps
+    bwd2 = (t2b - t2) * 1e3 / steps
     finite = all(bool(np.isfinite(g[k]).all()) for k in SCALE_OUTPUTS)
     return {"steps": steps, "start_step": warmup, "checkpoint_every": every or "auto", "samples": len(samples),
             "plain_forward_ms_per_step": plain, "forward_ms_per_step": fwd, "backward_ms_per_step": bwd,
             "backward_over_forward": bwd / plain, "total_over_forward": (fwd + bwd) / plain,
+            "repeat_backward_ms_per_step": bwd2, "repeat_backward_over_forward": bwd2 / plain,
             "gradients_finite": finite,
             "d_spring_scale_sum": float(g["spring"].sum()), "d_angle_scale_sum": float(g["angle"].sum()),
             "d_contact_scale": [float(x) for x in g["pair"]]}
