@@ -238,8 +238,11 @@ int fn_autotune(fn_engine *eng, int n_steps, int grid_update_freq, int run_sprin
  * Reverse-mode derivative of a fast-path run with respect to the force
  * scales of every spring, angle and contact pair type and to the initial
  * positions and velocity_old.  fn_adjoint_forward runs the steps exactly as
- * fn_run_steps does (same results; the rebuild is the exact one-pass one)
- * and records, per sample step, the x-force summed over the flagged beads
+ * fn_run_steps does (same results; with kept candidates, option "verlet" 1,
+ * they are built afresh at its first step and kept across its segments; a
+ * run whose search rows or contact list overflow is repeated from its start
+ * with the grown capacities) and records, per sample step, the x-force
+ * summed over the flagged beads
  * (a clamp's reaction force) together with a checkpoint of the state every
  * checkpoint_every steps.  fn_adjoint_backward takes dL/d(sample) and
  * recomputes each segment from its checkpoint (bitwise the same steps, so
@@ -252,7 +255,7 @@ int fn_autotune(fn_engine *eng, int n_steps, int grid_update_freq, int run_sprin
  * (network-major), d_pair is indexed like the concatenated pair tables, and
  * forward and backward are each one launch per kernel for all networks.
  * Any call that changes the engine's state in between (runs, writes,
- * launches) makes fn_adjoint_backward fail until the next
+ * launches, a new "skin") makes fn_adjoint_backward fail until the next
  * fn_adjoint_forward. */
 typedef struct fn_adjoint_spec {
     int n_steps;            /* as fn_run_steps                                 */
