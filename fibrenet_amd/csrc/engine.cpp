@@ -243,6 +243,7 @@ class Engine final : public fn_engine {
         HIPCK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
         step_.alloc(1);
         stats_.alloc(fnk::kStatCount);
+        HIPCK(hipHostMalloc((void **)&h_stats_, sizeof(int) * fnk::kStatCount, hipHostMallocDefault));
         fnk::fill_int(step_.p, 0, 1, stream_);
         fnk::fill_int(stats_.p, 0, fnk::kStatCount, stream_);
         for (int k = 0; k < K_COUNT; ++k) kernels_[kKernelNames[k]] = KernelRec();
@@ -255,6 +256,7 @@ class Engine final : public fn_engine {
         clear_graphs();
         if (stream_) (void)hipStreamSynchronize(stream_);
         if (stream_) (void)hipStreamDestroy(stream_);
+        if (h_stats_) (void)hipHostFree(h_stats_);
     }
 
     // ---- buffers ------------------------------------------------------------------------
@@ -575,9 +577,11 @@ class Engine final : public fn_engine {
         steps_ += n;
         const int rb = (n + guf - 1) / guf;
         rebuilds_ += rb;
+        // the run's statistics come back with the stream's last work
+        HIPCK(hipMemcpyAsync(h_stats_, stats_.p, sizeof(int) * fnk::kStatCount, hipMemcpyDeviceToHost, stream_));
         HIPCK(hipStreamSynchronize(stream_));
         collect_timing();
-        check_capacity();
+        check_capacity(h_stats_);
         tune_verlet(rb);
     }
 
@@ -1474,8 +1478,7 @@ class Engine final : public fn_engine {
             return;
         }
         if (rebuilds < 4) return;
-        int builds = 0;
-        HIPCK(hipMemcpy(&builds, stats_.p + fnk::kStatCandBuilds, sizeof(int), hipMemcpyDeviceToHost));
+        const int builds = h_stats_[fnk::kStatCandBuilds];   // read back at the end of the run
         if (2 * builds >= rebuilds) verlet_trial_ = false;
     }
 
@@ -1616,6 +1619,11 @@ class Engine final : public fn_engine {
     {
         int st[fnk::kStatCount];
         HIPCK(hipMemcpy(st, stats_.p, sizeof st, hipMemcpyDeviceToHost));
+        check_capacity(st);
+    }
+    // The same from statistics already read back (st: kStatCount values).
+    void check_capacity(const int *st)
+    {
         const int tot = st[fnk::kStatCandMax], over = st[fnk::kStatListOver];
         if (st[fnk::kStatDense] > 0)
             throw Error(FN_ERR_UNSUPPORTED, fmt("%d beads lay in grid cells holding more than %d beads (the reference keeps "
@@ -2717,6 +2725,7 @@ class Engine final : public fn_engine {
     bool flat_ = true;                               // 3x3x3 neighbour search flattened over each wave
     int skin_permille_ = 300;                        // skin as a fraction of the listing reach
     bool keep_cand_ = false, cand_dirty_ = true;
+    int *h_stats_ = nullptr;                         // pinned copy of the statistics at the end of a run
     double cand_r2_ = -1.0, lim2_ = 0.0;
     fnk::Stencil stencil_{}, stencil_exact_{};
     bool graphs_on_ = true;
