@@ -602,8 +602,9 @@ def main():
     ap.add_argument("--fit-networks", type=int, default=128, help="batched_fit: networks")
     ap.add_argument("--networks", type=int, default=512, help="ensemble: number of networks")
     ap.add_argument("--loop-steps", type=int, default=40, help="ensemble: steps of the looped plain engine")
-    ap.add_argument("--ensemble-networks", type=int, default=128,
-                    help="networks of the ensemble comparison reported beside the plain workload (0: none)")
+    ap.add_argument("--ensemble-networks", type=int, default=512,
+                    help="networks of the ensemble comparison reported beside the plain workload (0: none; "
+                         "512: the batched-ensemble configuration)")
     ap.add_argument("--fibres", type=int, default=None)
     ap.add_argument("--cpu-steps", type=int, default=100,
                     help="timed steps of the all-core CPU baseline (the threaded oracle)")

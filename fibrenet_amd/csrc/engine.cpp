@@ -1545,12 +1545,16 @@ class Engine final : public fn_engine {
         // a bead inside the grid has |coordinate| < cells * step on every
         // axis; rounding two of them to fp32 moves their difference by at
         // most 2 * 2^-24 of that
-        const double extent = std::max(bg.cx * bg.sx, std::max(bg.cy * bg.sy, bg.cz * bg.sz));
+        // (an ensemble: the bound of its largest grid, the lowest guard)
+        double extent = 0.0;
+        for (const fnk::Box &eb : ens_boxes_.empty() ? std::vector<fnk::Box>{bg} : ens_boxes_)
+            extent = std::max(extent, std::max(eb.cx * eb.sx, std::max(eb.cy * eb.sy, eb.cz * eb.sz)));
         f.delta = std::ldexp(extent, -23) * (1.0 + 1e-9) + 1e-300;
         f.cutoff = std::sqrt(bl.cutoff_sq);
-        {
-            const double rh = std::sqrt(bl.hbs);
-            f.hbs_guard = bl.hbs - (12.0 * f.delta * (2.0 * rh + f.delta) + 4e-14 * bl.hbs);
+        f.hbs_guard = 1e300;
+        for (const fnk::Box &eb : ens_boxes_.empty() ? std::vector<fnk::Box>{bl} : ens_boxes_) {
+            const double rh = std::sqrt(eb.hbs);
+            f.hbs_guard = std::min(f.hbs_guard, eb.hbs - (12.0 * f.delta * (2.0 * rh + f.delta) + 4e-14 * eb.hbs));
         }
         f.n = N_;
         f.grid = bg;
@@ -1559,6 +1563,12 @@ class Engine final : public fn_engine {
         f.list_start = nb_start_.p;
         f.pairs = pairs_.p;
         f.n_pairs = (int)h_pairs_.size();
+        if (!ens_boxes_.empty()) {        // entries are local to their network's table
+            f.net = ens_net();
+            f.boxes = boxes_.p;
+            f.n_pairs = 0;
+            for (const fnk::Box &eb : ens_boxes_) f.n_pairs = std::max(f.n_pairs, eb.types * (eb.types + 1) / 2);
+        }
         f.radius = rad_d_.p; f.skin = skin_d_.p;
         f.list_out = nb_list_.p; f.cnt = nb_cnt_.p; f.stats = stats_.p;
         f.need = need_.p;
@@ -1580,8 +1590,8 @@ class Engine final : public fn_engine {
             search_and_compact(bl, true, false, nullptr);
             return;
         }
-        fnk::disp_check(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, bl, bg, pack_, lim2_, need_.p,
-                        cellbead_.p, key_.p, stream_);
+        fnk::disp_check(p[0], p[1], p[2], ref_[0].p, ref_[1].p, ref_[2].p, N_, bl, bg, ens_net(), boxes_.p, pack_,
+                        lim2_, need_.p, cellbead_.p, key_.p, stream_);
         // gated by the flag: the build, whose compaction also records the
         // positions it was made at; the filter clears the flag
         enqueue_sort(pc, bg, need_.p);
@@ -1891,9 +1901,49 @@ class Engine final : public fn_engine {
         r2 = reach2(any);
         fnk::Stencil ex{};
         for (int k = 0; k < 3; ++k) ex.w[k] = 1;
-        if (r2 != reach2_ || keep_cand_) clear_graphs();
+        // kept candidates as geometry() decides them for one network, with
+        // one skin and candidate reach for all networks (from the largest
+        // listing reach) and a stencil wide enough for the finest grid; the
+        // search visits a periodic grid narrower than it whole
+        const double reach = std::sqrt(r2);
+        const bool want = verlet_ == 1 || (verlet_ == 2 && verlet_trial_);
+        double skin = want ? reach * skin_permille_ / 1000.0 : 0.0;
+        bool keep = want && skin > 0.0;
+        int mx = 1, my = 1, mz = 1;
+        for (const fnk::Box &eb : ens_boxes_) {
+            const double lmin = std::min(eb.lx, std::min(eb.ly, eb.lz));
+            if (eb.bc == 0 && !(reach + skin < 0.5 * lmin)) keep = false;
+            mx = std::max(mx, eb.cx);
+            my = std::max(my, eb.cy);
+            mz = std::max(mz, eb.cz);
+        }
+        const int bx = bits_for(mx), by = bits_for(my), bz = bits_for(mz);
+        if (bx + by + bz > 31) keep = false;
+        pack_ = fnk::CellPack{bx, bx + by};
+        if (!keep) skin = 0.0;
+        const double r = reach + skin;
+        fnk::Stencil st{};
+        for (int k = 0; k < 3; ++k) {
+            int w = 1;
+            for (const fnk::Box &eb : ens_boxes_) {
+                const double step = k == 0 ? eb.sx : (k == 1 ? eb.sy : eb.sz);
+                if (keep && step > 0.0) w = std::max(w, (int)std::ceil(r / step * (1.0 + 1e-12)));
+            }
+            st.w[k] = w;
+            st.all[k] = 0;
+        }
+        const double cand_r2 = r * r * (1.0 + 1e-12);
+        if (r2 != reach2_ || cand_r2 != cand_r2_ || keep != keep_cand_ || memcmp(&st, &stencil_, sizeof st) != 0 ||
+            cand_dirty_) {
+            clear_graphs();
+            fnk::fill_int(need_.p, 1, 1, stream_);
+            cand_dirty_ = false;
+        }
         reach2_ = r2;
-        keep_cand_ = false;
+        cand_r2_ = cand_r2;
+        keep_cand_ = keep;
+        lim2_ = (0.45 * skin) * (0.45 * skin);
+        stencil_ = st;
         stencil_exact_ = ex;
     }
 
@@ -1970,13 +2020,11 @@ class Engine final : public fn_engine {
             if (std::find(chunks.begin(), chunks.end(), l) == chunks.end()) chunks.push_back(l);
         }
         if (chain_ok_ && n_spr_ == S_ && n_ang_ == A_) knobs.push_back({FN_TUNE_CHAIN, {1, 0}});
-        if (!ens) {
-            knobs.push_back({FN_TUNE_STAGED, {0, 1}});
-            knobs.push_back({FN_TUNE_VERLET, {verlet_, 0, 1}});
-            // the kept candidates' skin (used while verlet keeps them): a thinner
-            // skin filters fewer candidates and rebuilds them more often
-            knobs.push_back({FN_TUNE_SKIN, {skin_permille_, 300, 150, 75}});
-        }
+        if (!ens) knobs.push_back({FN_TUNE_STAGED, {0, 1}});
+        knobs.push_back({FN_TUNE_VERLET, {verlet_, 0, 1}});
+        // the kept candidates' skin (used while verlet keeps them): a thinner
+        // skin filters fewer candidates and rebuilds them more often
+        knobs.push_back({FN_TUNE_SKIN, {skin_permille_, 300, 150, 75}});
         knobs.push_back({FN_TUNE_BLOCK, {block_, 256, 128, 512}});
         knobs.push_back({FN_TUNE_CHUNK, chunks});       // last: its trials are the longest
         static const char *names[] = {"chunk", "chain", "staged", "verlet", "block", "skin"};
@@ -2000,7 +2048,8 @@ class Engine final : public fn_engine {
         };
         auto apply = [&](int knob, int v) {
             set_option(names[knob], v);
-            if (!ens) geometry(settings_of(K_FILL_GRID), settings_of(K_BUILD_LIST));
+            if (ens) prepare_ensemble_run(spr, ang);
+            else geometry(settings_of(K_FILL_GRID), settings_of(K_BUILD_LIST));
         };
         try {
             for (auto &kv : knobs) {

@@ -61,6 +61,8 @@ struct FilterArgs {
     const double *radius, *skin;
     int *list_out, *cnt, *stats;
     int *need;                       // the candidate-build flag, cleared by the filter (null: none)
+    const int *net;                  // ensemble: each bead's network, whose Box (boxes[net]) holds its box,
+    const Box *boxes;                // grid, cutoff and pair table (at pair_base); null: grid and list above
 };
 
 template <typename T, typename P>
@@ -123,10 +125,12 @@ void compact(const int *cnt, const int *cand_start, const int *scratch, int K, i
              int cap, int *cnt_out, int *stats, const Snapshot &snap, const int *gate, hipStream_t s);
 // *need = 1 when a bead moved more than sqrt(lim2) since the snapshot; also
 // writes every bead's CellBead and FillGrid's cell index (key, -1 outside
-// the grid) for the filter and the exports.
+// the grid) for the filter and the exports.  With net (an ensemble) each
+// bead's network Box stands for both b and grid: CellBead holds the cell
+// within the network, key the ensemble's cell index.
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
-                const double *rz, int n, const Box &b, const Box &grid, CellPack pack, double lim2, int *need,
-                CellBead *cb, int *key, hipStream_t s);
+                const double *rz, int n, const Box &b, const Box &grid, const int *net, const Box *boxes,
+                CellPack pack, double lim2, int *need, CellBead *cb, int *key, hipStream_t s);
 void snapshot(const double *px, const double *py, const double *pz, double *rx, double *ry, double *rz, int n,
               const int *gate, hipStream_t s);
 // The listing rule on the candidates: contact list, counts and cells.

@@ -751,13 +751,18 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_
         const double ri = EXACT ? radius[i] : 0.0, si = EXACT ? skin[i] : 0.0;
         const SearchRow row{scratch, i, cell_start[n_cells], K};
         const int per = b.bc == 0 ? 1 : 0;
-        const AxisRange ax{cx, b.cx, st.w[0], st.all[0], per}, ay{cy, b.cy, st.w[1], st.all[1], per},
-            az{cz, b.cz, st.w[2], st.all[2], per};
-        const bool row_run = !st.all[0] && cx - st.w[0] >= 0 && cx + st.w[0] < b.cx;
+        // an ensemble's networks have grids of their own: a periodic axis
+        // shorter than the stencil is visited whole (once per cell)
+        const int all0 = st.all[0] | ((ENS && per && b.cx < 2 * st.w[0] + 1) ? 1 : 0);
+        const int all1 = st.all[1] | ((ENS && per && b.cy < 2 * st.w[1] + 1) ? 1 : 0);
+        const int all2 = st.all[2] | ((ENS && per && b.cz < 2 * st.w[2] + 1) ? 1 : 0);
+        const AxisRange ax{cx, b.cx, st.w[0], all0, per}, ay{cy, b.cy, st.w[1], all1, per},
+            az{cz, b.cz, st.w[2], all2, per};
+        const bool row_run = !all0 && cx - st.w[0] >= 0 && cx + st.w[0] < b.cx;
         if (dense) {
             // refused: the host reports the run invalid
-        } else if (EXACT && code >= 0 && row_run && st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !st.all[1] &&
-            !st.all[2]) {
+        } else if (EXACT && code >= 0 && row_run && st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !all1 &&
+            !all2) {
             // the reference's 3x3x3 stencil with x-rows inside the grid: the
             // nine row ranges in one round trip, then the rows
             int lo[9], hi[9];
@@ -1208,14 +1213,18 @@ __global__ void __launch_bounds__(kBlock) k_compact(const int *cnt, const int *c
 // Sets *need when some bead has moved more than sqrt(lim2) from its position
 // at the last candidate build (minimum image in a periodic box; NaN counts as
 // moved).
+template <bool ENS>
 __global__ void __launch_bounds__(kBlock) k_disp(const double *px, const double *py, const double *pz,
-                                                 const double *rx, const double *ry, const double *rz, int n, Box b,
-                                                 Box g, CellPack pack, double lim2, int *need, CellBead *cb,
-                                                 int *key)
+                                                 const double *rx, const double *ry, const double *rz, int n, Box bs,
+                                                 Box gs, const int *net, const Box *boxes, CellPack pack, double lim2,
+                                                 int *need, CellBead *cb, int *key)
 {
     const int i = blockIdx.x * kBlock + threadIdx.x;
     bool moved = false;
     if (i < n) {
+        // an ensemble's bead: its network's box and grid (cells local to it in cb)
+        const Box &b = net_box<ENS>(bs, boxes, net, i);
+        const Box &g = ENS ? b : gs;
         const double x = px[i], y = py[i], z = pz[i];
         double dx = x - rx[i], dy = y - ry[i], dz = z - rz[i];
         if (b.bc == 0) {
@@ -1228,7 +1237,7 @@ __global__ void __launch_bounds__(kBlock) k_disp(const double *px, const double 
         const int cx = cell_coord(x, g.sx, g.cx), cy = cell_coord(y, g.sy, g.cy), cz = cell_coord(z, g.sz, g.cz);
         const bool in = cx >= 0 && cx < g.cx && cy >= 0 && cy < g.cy && cz >= 0 && cz < g.cz;
         cb[i] = CellBead{(float)x, (float)y, (float)z, in ? (cx | (cy << pack.sy) | (cz << pack.sz)) : -1};
-        key[i] = in ? cx + cy * g.cx + cz * g.cxy : -1;
+        key[i] = in ? (ENS ? g.cell_base : 0) + cx + cy * g.cx + cz * g.cxy : -1;
     }
     if (__any(moved) && (threadIdx.x & 63) == 0) *need = 1;
 }
@@ -1272,7 +1281,7 @@ struct alignas(16) FilterOwner {
     double x, y, z;
     int cx, cy, cz, eb;
     double r, s;
-    int ob, pad;
+    int ob, net;                // net: the bead's network in an ensemble
 };
 
 __device__ __forceinline__ bool needs_fp64(double len, double t, double root_t, double delta)
@@ -1281,6 +1290,7 @@ __device__ __forceinline__ bool needs_fp64(double len, double t, double root_t, 
     return len <= 4.0 * t && fabs(len - t) <= 12.0 * delta * (2.0 * root_t + delta) + 4e-14 * t;
 }
 
+template <bool ENS>
 __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
 {
     __shared__ __align__(16) int verdict[kBlock / 64][kWaveChunk];
@@ -1288,7 +1298,8 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     __shared__ unsigned owner_words[kBlock / 64][kWaveChunk / 4];
     __shared__ FilterOwner fo[kBlock];
     __shared__ int ptype[kPairCap];          // the pair table's types (entries a list addresses: at most kPairCap)
-    for (int k = threadIdx.x; k < kPairCap; k += kBlock) ptype[k] = k < f.n_pairs ? f.pairs[k].type : 0;
+    if (!ENS)
+        for (int k = threadIdx.x; k < kPairCap; k += kBlock) ptype[k] = k < f.n_pairs ? f.pairs[k].type : 0;
     __syncthreads();
     const int wv = threadIdx.x >> 6;
     int *acc = verdict[wv];
@@ -1299,7 +1310,6 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     const int i = xcd_block() * kBlock + threadIdx.x;
     const bool live = i < f.n;
     const int ic = live ? i : f.n - 1;
-    const Box &g = f.grid;
     const int mcell = f.cb[ic].cell;
     const bool in = mcell >= 0;
     const int e0 = f.cand_start[ic];
@@ -1317,9 +1327,8 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     int kept = 0, wkept = 0;
     if (total > 0) {                                     // wave-uniform
         fo[threadIdx.x] = FilterOwner{f.px[ic], f.py[ic], f.pz[ic], mcell & mx, (mcell >> f.pack.sy) & my,
-                                      mcell >> f.pack.sz, e0 - off, f.radius[ic], f.skin[ic], 0, 0};
-        const Box &b = f.list;
-        const bool per = b.bc == 0;
+                                      mcell >> f.pack.sz, e0 - off, f.radius[ic], f.skin[ic], 0,
+                                      ENS ? f.net[ic] : 0};
         for (int base = 0; base < total; base += kWaveChunk) {
             mark_owner(off, cnt, base, own);             // also publishes the records
             const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
@@ -1346,19 +1355,25 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
                 int v = -1;
                 if (ent[r] >= 0 && q[r].cell >= 0) {
                     const FilterOwner &o = ow[own[r * 64 + lane]];
+                    // the owner's box, grid and pair table (an ensemble's: its network's)
+                    const Box &b = ENS ? f.boxes[o.net] : f.list;
+                    const Box &g = ENS ? b : f.grid;
+                    const bool per = b.bc == 0;
                     const int j = ent[r] & kIdxMask;
                     const int qc = q[r].cell;
                     const int ax = abs((qc & mx) - o.cx), ay = abs(((qc >> f.pack.sy) & my) - o.cy),
                               az = abs((qc >> f.pack.sz) - o.cz);
-                    const bool adj = (ax <= 1 || (per && ax == g.cx - 1)) && (ay <= 1 || (per && ay == g.cy - 1)) &&
-                                     (az <= 1 || (per && az == g.cz - 1));
+                    bool adj = (ax <= 1 || (per && ax == g.cx - 1)) && (ay <= 1 || (per && ay == g.cy - 1)) &&
+                               (az <= 1 || (per && az == g.cz - 1));
+                    const int pid = ent[r] >> kPidShift;
+                    if (ENS && pid >= b.types * (b.types + 1) / 2) adj = false;   // not an entry of its table
                     if (adj) {
-                        const int pid = ent[r] >> kPidShift;
-                        const int pt = pid < kPairCap ? ptype[pid] : f.pairs[pid].type;
+                        const int pt = ENS ? f.pairs[b.pair_base + pid].type
+                                           : (pid < kPairCap ? ptype[pid] : f.pairs[pid].type);
                         double t = -1.0, root_t = 0.0;
                         if (pt == 1 || pt == 3) {
                             t = b.cutoff_sq;
-                            root_t = f.cutoff;
+                            root_t = ENS ? sqrt(b.cutoff_sq) : f.cutoff;
                         } else if (pt == 2) {            // kernel.cl:632-659
                             const double rj = f.radius[j], sj = f.skin[j];
                             const double min_skin = (o.s > sj) ? sj : o.s;
@@ -1682,17 +1697,26 @@ void compact(const int *cnt, const int *cand_start, const int *scratch, int K, i
         k_compact<<<blocks(n), kBlock, 0, s>>>(cnt, cand_start, scratch, K, n, cand, cap, cnt_out, stats, snap, gate);
 }
 void disp_check(const double *px, const double *py, const double *pz, const double *rx, const double *ry,
-                const double *rz, int n, const Box &b, const Box &grid, CellPack pack, double lim2, int *need,
-                CellBead *cb, int *key, hipStream_t s)
+                const double *rz, int n, const Box &b, const Box &grid, const int *net, const Box *boxes,
+                CellPack pack, double lim2, int *need, CellBead *cb, int *key, hipStream_t s)
 {
-    k_disp<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, b, grid, pack, lim2, need, cb, key);
+    if (net)
+        k_disp<true><<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, b, grid, net, boxes, pack, lim2, need,
+                                                  cb, key);
+    else
+        k_disp<false><<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, b, grid, nullptr, nullptr, pack, lim2,
+                                                   need, cb, key);
 }
 void snapshot(const double *px, const double *py, const double *pz, double *rx, double *ry, double *rz, int n,
               const int *gate, hipStream_t s)
 {
     k_snapshot<<<blocks(n), kBlock, 0, s>>>(px, py, pz, rx, ry, rz, n, gate);
 }
-void filter(const FilterArgs &f, hipStream_t s) { k_filter<<<blocks(f.n), kBlock, 0, s>>>(f); }
+void filter(const FilterArgs &f, hipStream_t s)
+{
+    if (f.net) k_filter<true><<<blocks(f.n), kBlock, 0, s>>>(f);
+    else k_filter<false><<<blocks(f.n), kBlock, 0, s>>>(f);
+}
 void copy_bounded(const int *src, int *dst, const int *count, int cap, hipStream_t s)
 {
     if (cap > 0) k_copy_bounded<<<std::min(blocks(cap), 4096), kBlock, 0, s>>>(src, dst, count, cap);

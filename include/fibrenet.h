@@ -146,7 +146,10 @@ int fn_run_steps(fn_engine *eng, int n_steps, int grid_update_freq, int run_spri
  *                 between rebuilds while every bead stays within skin/2 of
  *                 its position at their build: never, always, or auto
  *                 (default: every 16th run tries it and keeps it while it
- *                 saves candidate builds)
+ *                 saves candidate builds).  An ensemble keeps them with one
+ *                 skin for all networks (from the largest listing
+ *                 distance), and only while that reach stays below half of
+ *                 every periodic network's box
  *   "staged" 1/0  neighbour search reads each block's neighbourhood from LDS
  *   "skin"   n    the skin in permille of the largest listing distance (300)
  *   "graphs" 1/0  launch through captured hipGraphs or directly; a new
@@ -218,8 +221,8 @@ int fn_batch_load(fn_engine *eng, int n_networks, const fn_network *nets);
  * at a time: the chain bead kernel, the LDS-staged search, kept contact
  * candidates and their skin, the bead kernels' block size and the chunk
  * length of the captured graphs (options "chain", "staged", "verlet",
- * "skin", "block", "chunk"; in that order; an ensemble tunes "chain",
- * "block" and "chunk").  Each value runs one chunk to
+ * "skin", "block", "chunk"; in that order; an ensemble tunes all but
+ * "staged").  Each value runs one chunk to
  * capture its graphs and one timed chunk; the fastest value of a knob is
  * kept before the next knob is tried, and the chosen settings stay set
  * (fn_get_option reads them).  A run too short for every trial tries what

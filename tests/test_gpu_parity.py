@@ -403,6 +403,48 @@ def test_ensemble_equals_plain_runs(need_gpu, precision):
     assert st["steps"] == 65 and st["list_entries"] > 0
 
 
+def _kept_members(group):
+    """Ensembles whose networks can keep candidates under one skin: periodic
+    penalty networks (listing reach 1.5, the smallest box half 3), or the
+    finite Lennard-Jones tensile and crosslinked networks."""
+    from fibrenet_amd import generate as G
+    if group == "periodic":
+        return [G.periodic_hs(n_fibres=f, n_beads=b, box=box, seed=40 + f, spring_k=k)
+                for f, b, box, k in ((12, 15, (16.0, 16.0, 8.0), 60.0), (40, 22, (30.0, 30.0, 15.0), 150.0),
+                                     (5, 9, (9.0, 9.0, 6.0), 100.0))]
+    return [small_tensile(weak=True), crosslinked()]
+
+
+@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("group,skin", [("periodic", 300), ("periodic", 500), ("finite", 300), ("finite", 1000)])
+def test_ensemble_kept_candidates_equal_plain_runs(need_gpu, precision, group, skin):
+    """With kept candidates (verlet 1) the ensemble's networks share one
+    skin and candidate stencil; each bead's displacement check, cell record
+    and filter use its own network's box, grid and pair table.  The lists
+    equal the one-pass rebuild's, so every network still reproduces its own
+    plain run bit for bit.  At skin 500 the stencil spans seven cells, more
+    than the 6-cell periodic axis of the smallest network, which the search
+    then visits whole."""
+    from fibrenet_amd import Ensemble
+    nets = _kept_members(group)
+    guf = 8
+    ens = Ensemble(nets, precision)
+    ens.set_option("verlet", 1)
+    ens.set_option("skin", skin)
+    for n in (5, 60, 40):
+        ens.run_steps(n, guf, True, True)
+        if n == 5:                 # the first run builds the candidates
+            st = ens.stats()
+            assert st["candidate_builds"] >= 1
+            assert st["candidate_entries"] >= st["list_entries"] > 0
+    for k, net in enumerate(nets):
+        eng, ids = engine_for(net, precision)
+        for n in (5, 60, 40):
+            eng.run_steps(n, guf, len(net.springs) > 0, len(net.angles) > 0)
+        assert ens.read(k, "bead").tobytes() == read_beads(eng, ids).tobytes(), k
+        assert np.array_equal(ens.read(k, "spring")["status"], eng.read_buffer(ids["spring"], R.SPRING)["status"])
+
+
 def test_ensemble_rejects_bad_networks(need_gpu):
     from fibrenet_amd import Ensemble, FibreNetError
     net = _ensemble_members()[0]
