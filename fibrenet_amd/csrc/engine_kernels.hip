@@ -738,7 +738,10 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_
     if (s < cell_start[n_cells]) {
         const SortedBead me = so.bead[s];
         const int i = me.bead;
-        const Box &b = net_box<ENS>(bs, boxes, net, i);
+        // an ensemble's network Box by value: the scans read its fields for
+        // every candidate, which through a reference would be loads that the
+        // row stores keep in the loop
+        const Box b = net_box<ENS>(bs, boxes, net, i);
         const PairC *pairs = pairs_all + b.pair_base;
         const int code = me.cell;
         const int cg = slot_cell(code);
