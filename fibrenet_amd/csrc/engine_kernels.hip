@@ -722,6 +722,40 @@ struct AxisRange {
     }
 };
 
+// Slot ranges of stencil row q (dz = q / 3 - 1, dy = q % 3 - 1) around cell
+// (cx, cy, cz) for the 3x3x3 stencil with no axis visited whole: the cells
+// cx - 1 .. cx + 1 of that row as one slot range, or two where a periodic
+// row wraps (a finite box clips the row and skips rows outside the grid).
+__device__ __forceinline__ void stencil_row(const int *cell_start, const Box &b, int cx, int cy, int cz, int q,
+                                            int &lo0, int &hi0, int &lo1, int &hi1)
+{
+    lo0 = hi0 = lo1 = hi1 = 0;
+    const bool per = b.bc == 0;
+    int z = cz + q / 3 - 1, y = cy + q % 3 - 1;
+    if (per) {
+        z = z < 0 ? z + b.cz : (z >= b.cz ? z - b.cz : z);
+        y = y < 0 ? y + b.cy : (y >= b.cy ? y - b.cy : y);
+    } else if (z < 0 || z >= b.cz || y < 0 || y >= b.cy) {
+        return;
+    }
+    const int rb = b.cell_base + y * b.cx + z * b.cxy;
+    if (!per || (cx >= 1 && cx + 1 < b.cx)) {
+        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < b.cx ? cx + 1 : b.cx - 1;
+        lo0 = cell_start[rb + x0];
+        hi0 = cell_start[rb + x1 + 1];
+    } else if (cx == 0) {                      // cells cx - 1 = b.cx - 1, then 0 and 1
+        lo0 = cell_start[rb + b.cx - 1];
+        hi0 = cell_start[rb + b.cx];
+        lo1 = cell_start[rb];
+        hi1 = cell_start[rb + 2];
+    } else {                                   // cx = b.cx - 1: cells b.cx - 2, b.cx - 1, then 0
+        lo0 = cell_start[rb + b.cx - 2];
+        hi0 = cell_start[rb + b.cx];
+        lo1 = cell_start[rb];
+        hi1 = cell_start[rb + 1];
+    }
+}
+
 // Candidate search of the bead in slot s over the stencil's cells; the cells
 // of one x-row are one contiguous slot range unless the row wraps or is
 // clipped.  Up to K entries go to the bead's scratch row; cnt[bead] = count.
@@ -764,24 +798,17 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_
         const bool row_run = !all0 && cx - st.w[0] >= 0 && cx + st.w[0] < b.cx;
         if (dense) {
             // refused: the host reports the run invalid
-        } else if (EXACT && code >= 0 && row_run && st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !all1 &&
-            !all2) {
-            // the reference's 3x3x3 stencil with x-rows inside the grid: the
-            // nine row ranges in one round trip, then the rows
-            int lo[9], hi[9];
+        } else if (EXACT && code >= 0 && st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !all0 && !all1 && !all2) {
+            // the reference's 3x3x3 stencil: the nine rows' slot ranges (two
+            // where a periodic row wraps) in one round trip, then the rows
+            int lo[9], hi[9], lo2[9], hi2[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) stencil_row(cell_start, b, cx, cy, cz, q, lo[q], hi[q], lo2[q], hi2[q]);
 #pragma unroll
             for (int q = 0; q < 9; ++q) {
-                const int z = az.at(q / 3 - 1), y = ay.at(q % 3 - 1);
-                lo[q] = hi[q] = 0;
-                if (y >= 0 && z >= 0) {
-                    const int rb = b.cell_base + y * b.cx + z * b.cxy + cx;
-                    lo[q] = cell_start[rb - 1];
-                    hi[q] = cell_start[rb + 2];
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 9; ++q)
                 scan_range<EXACT>(lo[q], hi[q], s, me, ri, si, so.bead, 0, pairs, radius, skin, b, r2, row, k);
+                scan_range<EXACT>(lo2[q], hi2[q], s, me, ri, si, so.bead, 0, pairs, radius, skin, b, r2, row, k);
+            }
         } else
         for (int dz = az.first(); dz <= az.last() && !(EXACT && code < 0); ++dz) {
             const int z = az.at(dz);
@@ -816,40 +843,6 @@ __global__ void __launch_bounds__(kBlock) k_search(const int *cell_start, int n_
         raise_max(&stats[kStatMaxCell], mocc);
         if (big > 0) atomicAdd(&stats[kStatCellsOver], big);
         if (nd > 0) atomicAdd(&stats[kStatDense], nd);
-    }
-}
-
-// Slot ranges of stencil row q (dz = q / 3 - 1, dy = q % 3 - 1) around cell
-// (cx, cy, cz) for the 3x3x3 stencil with no axis visited whole: the cells
-// cx - 1 .. cx + 1 of that row as one slot range, or two where a periodic
-// row wraps (a finite box clips the row and skips rows outside the grid).
-__device__ __forceinline__ void stencil_row(const int *cell_start, const Box &b, int cx, int cy, int cz, int q,
-                                            int &lo0, int &hi0, int &lo1, int &hi1)
-{
-    lo0 = hi0 = lo1 = hi1 = 0;
-    const bool per = b.bc == 0;
-    int z = cz + q / 3 - 1, y = cy + q % 3 - 1;
-    if (per) {
-        z = z < 0 ? z + b.cz : (z >= b.cz ? z - b.cz : z);
-        y = y < 0 ? y + b.cy : (y >= b.cy ? y - b.cy : y);
-    } else if (z < 0 || z >= b.cz || y < 0 || y >= b.cy) {
-        return;
-    }
-    const int rb = b.cell_base + y * b.cx + z * b.cxy;
-    if (!per || (cx >= 1 && cx + 1 < b.cx)) {
-        const int x0 = cx > 0 ? cx - 1 : 0, x1 = cx + 1 < b.cx ? cx + 1 : b.cx - 1;
-        lo0 = cell_start[rb + x0];
-        hi0 = cell_start[rb + x1 + 1];
-    } else if (cx == 0) {                      // cells cx - 1 = b.cx - 1, then 0 and 1
-        lo0 = cell_start[rb + b.cx - 1];
-        hi0 = cell_start[rb + b.cx];
-        lo1 = cell_start[rb];
-        hi1 = cell_start[rb + 2];
-    } else {                                   // cx = b.cx - 1: cells b.cx - 2, b.cx - 1, then 0
-        lo0 = cell_start[rb + b.cx - 2];
-        hi0 = cell_start[rb + b.cx];
-        lo1 = cell_start[rb];
-        hi1 = cell_start[rb + 1];
     }
 }
 
