@@ -2675,6 +2675,11 @@ class Engine final : public fn_engine {
                 HIPCK(hipStreamEndCapture(stream_, &g));
                 HIPCK(hipGraphInstantiate(&rec.ex[0], g, nullptr, nullptr, 0));
                 HIPCK(hipGraphInstantiate(&rec.ex[1], g, nullptr, nullptr, 0));
+                // both copies go to the device now, so that neither one's
+                // first replay waits for its upload (where the runtime cannot,
+                // the first replay uploads it: no error is kept)
+                for (hipGraphExec_t x : rec.ex)
+                    if (hipGraphUpload(x, stream_) != hipSuccess) (void)hipGetLastError();
                 HIPCK(hipGraphDestroy(g));
                 it = graphs_.emplace(key, rec).first;
             }
