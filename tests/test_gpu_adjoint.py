@@ -268,6 +268,34 @@ def test_batched_adjoint_equals_single_networks(need_gpu, prec):
             assert np.allclose(g[key][o0:o1], gk[key], rtol=1e-10, atol=1e-12 * scale), key
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_batched_adjoint_with_kept_candidates(need_gpu, prec):
+    """The ensemble's adjoint with kept candidates (verlet 1, one skin for
+    all networks, candidates carried across segments and restored for each
+    recomputed segment) gives the one-pass rebuild's samples and
+    derivatives bit for bit."""
+    from fibrenet_amd import Ensemble
+    nets = _batch_members()
+    rs = np.random.default_rng(4)
+    w = rs.normal(size=(len(nets), len(SAMPLES)))
+    group = np.concatenate([_group(n) for n in nets])
+    out = []
+    for verlet in (0, 1):
+        ens = Ensemble(nets, prec)
+        ens.set_option("verlet", verlet)
+        off = {k: ens.offsets(k) for k in ("spring", "angle", "pair")}
+        s = ens.adjoint_forward(N_STEPS, group, SAMPLES, grid_update_freq=GUF, checkpoint_every=50)
+        builds = ens.stats()["candidate_builds"]
+        g = ens.adjoint_backward(w, int(off["spring"][-1]), int(off["angle"][-1]), int(off["pair"][-1]))
+        if verlet == 1:
+            assert builds >= 1, builds
+        out.append((s.tobytes(), {k: v.tobytes() for k, v in g.items()}))
+        ens.close()
+    assert out[0][0] == out[1][0]
+    for k in out[0][1]:
+        assert out[0][1][k] == out[1][1][k], k
+
+
 def test_batch_reload_restarts_every_network(need_gpu):
     """fn_batch_load puts new records in place: a run from them equals a run
     of a freshly created ensemble bit for bit."""
