@@ -1508,7 +1508,8 @@ class Engine final : public fn_engine {
             // every bead has a slot (k_place clamps the ones outside the grid
             // into it), so the search writes every bead's count
             fnk::search(cell_start_.p, C_, sorted_view(), pairs_.p, rad_d_.p, skin_d_.p, b, ens_net(),
-                        boxes_.p, final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, final_list,
+                        boxes_.p, final_list ? stencil_exact_ : stencil_, final_list ? reach2_ : cand_r2_, ens_grids3_,
+                        final_list,
                         ncnt_.p, scratch_.p, K_, stats_.p, N_, staged_, flat_, gate, stream_);
             // offsets: the contact list's (exact) or the candidates' CSR
             int *starts = final_list ? nb_start_.p : cstart_.p;
@@ -1875,6 +1876,8 @@ class Engine final : public fn_engine {
         boxes_.put(pk.boxes, stream_);
         net_.put(pk.net, stream_);
         C_ = (int)pk.cells;
+        ens_grids3_ = true;
+        for (const fnk::Box &eb : ens_boxes_) ens_grids3_ = ens_grids3_ && eb.cx >= 3 && eb.cy >= 3 && eb.cz >= 3;
     }
 
     void prepare_ensemble_run(bool spr, bool ang)
@@ -2834,6 +2837,7 @@ class Engine final : public fn_engine {
     std::vector<fn_settings> ens_sets_;
     std::vector<int> ens_b0_, ens_s0_, ens_a0_;
     bool ens_finite_ = false;
+    bool ens_grids3_ = false;    // every network's grid has at least three cells per axis
     DArr<fnk::Box> boxes_;
     DArr<int> net_;
     DArr<fnk::SortedBead> so_bead_;

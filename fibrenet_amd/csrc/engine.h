@@ -106,13 +106,15 @@ void place(const int *key, const int *ckey, const int *rank, const int *cell_sta
 // Partners over the stencil into scratch rows of K entries per bead: the
 // candidates within sqrt(r2), or (exact) the pairs BuildContactList lists;
 // staged: the neighbourhood of each block read through LDS; flat: the 3x3x3
-// stencil's candidates tested by whole waves (k_search_flat).  Then
+// stencil's candidates tested by whole waves (k_search_flat; for an
+// ensemble, k_search_ens_wave when ens_grids3: every network's grid has at
+// least three cells per axis).  Then
 // compaction into a bead-indexed CSR sorted per bead (with cnt_out: the final
 // list, its lengths and the listed total).
 void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs, const double *radius,
             const double *skin, const Box &b, const int *net, const Box *boxes, const Stencil &st, double r2,
-            bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat, const int *gate,
-            hipStream_t s);
+            bool ens_grids3, bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat,
+            const int *gate, hipStream_t s);
 // Positions to copy into the displacement check's reference when the
 // candidates are built (rx null: none).
 struct Snapshot {

@@ -994,6 +994,134 @@ __global__ void __launch_bounds__(kBlock) k_search_flat(const int *cell_start, i
     }
 }
 
+// The exact 3x3x3 search of an ensemble, flattened over the wave: each lane
+// takes its bead's nine stencil rows (one or two slot ranges each, wrapped
+// by its own network's periodic grid) into registers once, the wave lays
+// all their candidate slots end to end in LDS, and every lane tests every
+// 64th candidate against its owner with the owner's network Box.  In the
+// sparse grids of an ensemble a bead has about one candidate in 18 ranges,
+// so the per-bead search ran each range's loop for a few active lanes.
+// Every network's grid has at least three cells per axis (the host checks).
+template <bool EXACT>
+__global__ void __launch_bounds__(kBlock) k_search_ens_wave(const int *cell_start, int n_cells, Sorted so,
+                                                            const PairC *pairs_all, const double *radius,
+                                                            const double *skin, const int *net, const Box *boxes,
+                                                            double r2, int *cnt, int *scratch, int K, int *stats,
+                                                            const int *gate)
+{
+    if (gated_off(gate)) return;
+    __shared__ WaveSearchLds wl[kBlock / 64];
+    WaveSearchLds &w = wl[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int s = xcd_block() * kBlock + threadIdx.x;
+    const int n_slots = cell_start[n_cells];
+    const bool live = s < n_slots;
+    int i = 0, occ = 0, nb = 0;
+    bool dense = false;
+    int lo[9], hi[9], lo2[9], hi2[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) lo[q] = hi[q] = lo2[q] = hi2[q] = 0;
+    if (live) {
+        const SortedBead me = so.bead[s];
+        i = me.bead;
+        const int nw = net[i];
+        const Box b = boxes[nw];
+        const int code = me.cell;
+        const int cg = slot_cell(code);
+        if (cell_start[cg] == s) occ = cell_start[cg + 1] - s;
+        dense = cell_start[cg + 1] - cell_start[cg] > kMaxCellBeads;
+        const int c = cg - b.cell_base;
+        const int cz = c / b.cxy;
+        const int cy = (c - cz * b.cxy) / b.cx;
+        const int cx = c - cz * b.cxy - cy * b.cx;
+        w.own[lane] = FlatOwner{me.x, me.y, me.z, me.tag, me.loc, me.type, s, i, nw};
+        if (!(EXACT && code < 0) && !dense) {
+#pragma unroll
+            for (int q = 0; q < 9; ++q) {
+                stencil_row(cell_start, b, cx, cy, cz, q, lo[q], hi[q], lo2[q], hi2[q]);
+                nb += (hi[q] - lo[q]) + (hi2[q] - lo2[q]);
+            }
+            if (nb > (1 << 24)) {          // keeps the wave's total below 2^30
+                nb = 0;
+                dense = true;
+#pragma unroll
+                for (int q = 0; q < 9; ++q) lo[q] = hi[q] = lo2[q] = hi2[q] = 0;
+            }
+        }
+    }
+    w.cnt[lane] = 0;
+    int incl = nb;
+    incl = wave_incl_scan(incl);
+    const int off = incl - nb;
+    const int total = __shfl(incl, 63, 64);
+    for (int base = 0; base < total; base += kFlatChunk) {          // wave-uniform
+        if (nb > 0 && off < base + kFlatChunk && off + nb > base) {
+            int pos = off;
+#pragma unroll
+            for (int q = 0; q < 18; ++q) {
+                const int a = q < 9 ? lo[q] : lo2[q - 9], e = q < 9 ? hi[q] : hi2[q - 9];
+                const int len = e - a;
+                const int a0 = pos > base ? pos : base;
+                const int a1 = (pos + len < base + kFlatChunk) ? pos + len : base + kFlatChunk;
+                for (int k = a0; k < a1; ++k)
+                    w.cand[k - base] = (int)((unsigned)(a + (k - pos)) | ((unsigned)lane << kPidShift));
+                pos += len;
+            }
+        }
+        wave_sync_lds();
+        const int n = (total - base) < kFlatChunk ? total - base : kFlatChunk;
+        for (int r0 = 0; r0 < n; r0 += 64) {
+            const int k = r0 + lane;
+            if (k < n) {
+                const int e = w.cand[k];
+                const int t = e & kIdxMask, L = (int)((unsigned)e >> kPidShift);
+                const SortedBead o = so.bead[t];
+                const FlatOwner me = w.own[L];
+                const Box &b = boxes[me.pad];             // the owner's network
+                const double len = list_len2(me.x - o.x, me.y - o.y, me.z - o.z, b);
+                bool take = len < r2 && t != me.slot && !(me.tag == o.tag && abs(me.loc - o.loc) <= 1);
+                int pid = 0;
+                if (take) {
+                    pid = pair_index(me.type, o.type, b.types);
+                    const int pt = pairs_all[b.pair_base + pid].type;
+                    take = pt >= 1 && pt <= 3;
+                    if (EXACT && take) {
+                        take = o.cell >= 0;               // FillGrid does not hash a bead outside the grid
+                        if (take && (pt == 1 || pt == 3)) take = len < b.cutoff_sq;
+                        if (take && pt == 2) {
+                            const double ri = radius[me.bead], si = skin[me.bead];
+                            const double rj = radius[o.bead], sj = skin[o.bead];
+                            const double min_skin = (si > sj) ? sj : si;
+                            const double max_r = (ri > rj) ? ri : rj;
+                            const double th = max_r + min_skin;
+                            take = len < th * th;
+                        }
+                    }
+                }
+                if (take) {
+                    const int p = atomicAdd(&w.cnt[L], 1);
+                    if (p < K) scratch[row_at(me.bead, p, n_slots, K)] = o.bead | (pid << kPidShift);
+                }
+            }
+        }
+        wave_sync_lds();
+    }
+    const int kc = w.cnt[lane];
+    if (live) cnt[i] = min(kc, K);
+    const int mx = wave_max(kc);
+    const int over = wave_sum(kc > K ? 1 : 0);
+    const int mocc = wave_max(occ);
+    const int big = wave_sum(occ > 64 ? 1 : 0);
+    const int nd = wave_sum(dense ? 1 : 0);
+    if (lane == 0) {
+        raise_max(&stats[EXACT ? kStatMaxList : kStatMaxCand], mx);
+        if (over > 0) atomicAdd(&stats[kStatListOver], over);
+        raise_max(&stats[kStatMaxCell], mocc);
+        if (big > 0) atomicAdd(&stats[kStatCellsOver], big);
+        if (nd > 0) atomicAdd(&stats[kStatDense], nd);
+    }
+}
+
 // The same search with the block's neighbourhood staged in LDS.  A block of
 // kStageBlock consecutive slots covers cells [c_lo, c_hi]; for every (dy, dz)
 // of the stencil the cells its beads can reach in that row offset form one
@@ -1657,13 +1785,16 @@ void place(const int *key, const int *ckey, const int *rank, const int *cell_sta
 template <bool EXACT>
 static void search_t(const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs, const double *radius,
                      const double *skin, const Box &b, const int *net, const Box *boxes, const Stencil &st, double r2,
-                     int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat, const int *gate,
-                     hipStream_t s)
+                     bool ens_grids3, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat,
+                     const int *gate, hipStream_t s)
 {
     const bool stencil3 = st.w[0] == 1 && st.w[1] == 1 && st.w[2] == 1 && !st.all[0] && !st.all[1] && !st.all[2];
     if (!net && flat && !staged && stencil3)
         k_search_flat<EXACT><<<blocks(n), kBlock, 0, s>>>(cell_start, n_cells, so, pairs, radius, skin, b, r2, cnt,
                                                           scratch, K, stats, gate);
+    else if (net && flat && stencil3 && ens_grids3)
+        k_search_ens_wave<EXACT><<<blocks(n), kBlock, 0, s>>>(cell_start, n_cells, so, pairs, radius, skin, net,
+                                                              boxes, r2, cnt, scratch, K, stats, gate);
     else if (net)
         k_search<EXACT, true><<<blocks(n), kBlock, 0, s>>>(cell_start, n_cells, so, pairs, radius, skin, b, net,
                                                            boxes, st, r2, cnt, scratch, K, stats, gate);
@@ -1676,14 +1807,15 @@ static void search_t(const int *cell_start, int n_cells, const Sorted &so, const
 }
 void search(const int *cell_start, int n_cells, const Sorted &so, const PairC *pairs, const double *radius,
             const double *skin, const Box &b, const int *net, const Box *boxes, const Stencil &st, double r2,
-            bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat, const int *gate,
-            hipStream_t s)
+            bool ens_grids3, bool exact, int *cnt, int *scratch, int K, int *stats, int n, bool staged, bool flat,
+            const int *gate, hipStream_t s)
 {
     if (exact)
-        search_t<true>(cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K, stats,
-                       n, staged, flat, gate, s);
+        search_t<true>(cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, ens_grids3, cnt, scratch,
+                       K, stats, n, staged, flat, gate, s);
     else
-        search_t<false>(cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, cnt, scratch, K, stats,
+        search_t<false>(cell_start, n_cells, so, pairs, radius, skin, b, net, boxes, st, r2, ens_grids3, cnt, scratch,
+                        K, stats,
                         n, staged, flat, gate, s);
 }
 void compact(const int *cnt, const int *cand_start, const int *scratch, int K, int n, int *cand,
