@@ -382,14 +382,17 @@ def _ensemble_members():
 
 
 @pytest.mark.parametrize("precision", ["f64", "f32"])
-def test_ensemble_equals_plain_runs(need_gpu, precision):
+@pytest.mark.parametrize("flat", [1, 0])
+def test_ensemble_equals_plain_runs(need_gpu, precision, flat):
     """a batched ensemble reproduces each network's own run bit for bit:
     same boxes, pair tables and integrator constants per network, contacts
-    only inside a network, the same terms in the same order."""
+    only inside a network, the same terms in the same order.  flat: the
+    exact search flattened over the wave (k_search_ens_wave) or per bead."""
     from fibrenet_amd import Ensemble
     nets = _ensemble_members()
     guf = 8
     ens = Ensemble(nets, precision)
+    ens.set_option("flat_search", flat)
     for n in (5, 60):
         ens.run_steps(n, guf, True, True)
     for k, net in enumerate(nets):
