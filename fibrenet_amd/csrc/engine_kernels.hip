@@ -1396,16 +1396,16 @@ __global__ void __launch_bounds__(kBlock) k_snapshot(const double *px, const dou
 // are decided by the fp64 positions.  The verdicts are those of the fp64
 // rule.
 
-// A bead of a wave as the filter reads it: position, cell, radius and skin,
-// eb = cand_start - (its first entry in the wave's entries), so that entry k
-// of the wave is candidate eb + k, and ob, which places the pass's accepted
-// entries: the one with p accepted entries of the pass before it goes to
-// list[ob + p].
+// A bead of a wave as the filter reads it: its rounded position (the
+// CellBead's), its index, its cell and eb = cand_start - (its first entry in
+// the wave's entries), so that entry k of the wave is candidate eb + k.  Its
+// fp64 position, radius and skin are read from the bead arrays only by the
+// verdicts that need them (the fp64 decision, a penalty pair).
 struct alignas(16) FilterOwner {
-    double x, y, z;
+    float x, y, z;
+    int idx;
     int cx, cy, cz, eb;
-    double r, s;
-    int ob, net;                // net: the bead's network in an ensemble
+    int net;                    // the bead's network in an ensemble
 };
 
 __device__ __forceinline__ bool needs_fp64(double len, double t, double root_t, double delta)
@@ -1434,7 +1434,8 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     const int i = xcd_block() * kBlock + threadIdx.x;
     const bool live = i < f.n;
     const int ic = live ? i : f.n - 1;
-    const int mcell = f.cb[ic].cell;
+    const CellBead mine = f.cb[ic];
+    const int mcell = mine.cell;
     const bool in = mcell >= 0;
     const int e0 = f.cand_start[ic];
     const int e1 = f.cand_start[ic + 1];
@@ -1450,9 +1451,8 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
     const int wb = __shfl(e0, 0, 64);
     int kept = 0, wkept = 0;
     if (total > 0) {                                     // wave-uniform
-        fo[threadIdx.x] = FilterOwner{f.px[ic], f.py[ic], f.pz[ic], mcell & mx, (mcell >> f.pack.sy) & my,
-                                      mcell >> f.pack.sz, e0 - off, f.radius[ic], f.skin[ic], 0,
-                                      ENS ? f.net[ic] : 0};
+        fo[threadIdx.x] = FilterOwner{mine.x, mine.y, mine.z, ic, mcell & mx, (mcell >> f.pack.sy) & my,
+                                      mcell >> f.pack.sz, e0 - off, ENS ? f.net[ic] : 0};
         for (int base = 0; base < total; base += kWaveChunk) {
             mark_owner(off, cnt, base, own);             // also publishes the records
             const int n = (total - base) < kWaveChunk ? total - base : kWaveChunk;
@@ -1499,20 +1499,23 @@ __global__ void __launch_bounds__(kBlock) k_filter(FilterArgs f)
                             t = b.cutoff_sq;
                             root_t = ENS ? sqrt(b.cutoff_sq) : f.cutoff;
                         } else if (pt == 2) {            // kernel.cl:632-659
+                            const double ri = f.radius[o.idx], si = f.skin[o.idx];
                             const double rj = f.radius[j], sj = f.skin[j];
-                            const double min_skin = (o.s > sj) ? sj : o.s;
-                            const double max_r = (o.r > rj) ? o.r : rj;
+                            const double min_skin = (si > sj) ? sj : si;
+                            const double max_r = (ri > rj) ? ri : rj;
                             root_t = max_r + min_skin;
                             t = root_t * root_t;
                         }
                         if (t >= 0.0) {
-                            const double dx = (double)(float)o.x - (double)q[r].x;
-                            const double dy = (double)(float)o.y - (double)q[r].y;
-                            const double dz = (double)(float)o.z - (double)q[r].z;
+                            const double dx = (double)o.x - (double)q[r].x;
+                            const double dy = (double)o.y - (double)q[r].y;
+                            const double dz = (double)o.z - (double)q[r].z;
                             const double len = dx * dx + dy * dy + dz * dz;
                             bool take = len < t;
                             if (needs_fp64(len, t, root_t, f.delta) || (per && !(len < f.hbs_guard))) {
-                                const double le = list_len2(o.x - f.px[j], o.y - f.py[j], o.z - f.pz[j], b);
+                                const int oi = o.idx;
+                                const double le =
+                                    list_len2(f.px[oi] - f.px[j], f.py[oi] - f.py[j], f.pz[oi] - f.pz[j], b);
                                 take = le < t;
                             }
                             if (take) v = ent[r];
